@@ -1,0 +1,114 @@
+// C ABI of libdsplat_hip.so: argument validation and dispatch to the launch units.
+#include <math.h>
+
+#include "dsplat_internal.h"
+
+namespace dsplat {
+
+// Zero fill without hipMemsetAsync, so a caller may capture these calls in a HIP graph.
+__global__ void zero_fill(float* __restrict__ p, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+hipError_t fill_zero(float* p, size_t n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(zero_fill, dim3((unsigned)blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t allow_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    if (bytes > 160 * 1024) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+bool make_chirp(const dsplat_fmcw* cfg, Chirp* out) {
+    if (!cfg) return false;
+    const int S = cfg->samples, P = cfg->pad, N = cfg->n_fft, K = cfg->num_bins;
+    if (S < 1 || !(P == 1 || P == 2 || P == 4 || P == 8) || N != S * P || K != N / 2) return false;
+    if (N < 32 || K > 512) return false;
+    if (!(cfg->k0_per_m > 0) || !(cfg->c0_m_s > 0) || !(cfg->lambda_m > 0)) return false;
+    Chirp c;
+    c.K = K;
+    c.N = N;
+    c.S = S;
+    c.P = P;
+    c.Kp = (K + 15) / 16 * 16;
+    c.k0pm = cfg->k0_per_m;
+    c.fc = cfg->fc_hz;
+    c.slope = cfg->slope_hz_s;
+    c.t0 = cfg->t_start_s;
+    c.c0 = cfg->c0_m_s;
+    c.lam = (float)cfg->lambda_m;
+    c.fresnel_n = (float)cfg->fresnel_n;
+    c.inv_S = (float)(1.0 / S);
+    c.pi_P = (float)(M_PI / P);
+    c.pi_N = (float)(M_PI / N);
+    c.h = (float)(M_PI * (S - 1) / N);
+    c.k0pm_f = (float)cfg->k0_per_m;
+    // carrier rate 2 pi 2 / c0 (fc + K (t0 - 2 r / c0)) = rate0 - rate1 r (scattering.slang:47-50)
+    c.rate0 = (float)(4.0 * M_PI / cfg->c0_m_s * (cfg->fc_hz + cfg->slope_hz_s * cfg->t_start_s));
+    c.rate1 = (float)(4.0 * M_PI / cfg->c0_m_s * cfg->slope_hz_s * 2.0 / cfg->c0_m_s);
+    const double a = M_PI / P, b = M_PI / N;
+    c.c2 = (float)((b * b - a * a) / 6.0);
+    c.c4 = (float)((3 * a * a * a * a + 7 * b * b * b * b - 10 * a * a * b * b) / 360.0);
+    *out = c;
+    return true;
+}
+
+}  // namespace dsplat
+
+using namespace dsplat;
+
+extern "C" {
+
+const char* dsplat_version(void) { return "dsplat-hip 0.1.0 gfx950"; }
+
+const char* dsplat_error_string(int code) { return hipGetErrorString((hipError_t)code); }
+
+int dsplat_surfel_render(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                         const float* centers, const float* normals, const float* weights, int32_t num_surfels,
+                         float* out, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0) return hipErrorInvalidValue;
+    if (num_apertures == 0) return hipSuccess;
+    if (!positions || !out || (num_surfels > 0 && (!centers || !normals || !weights))) return hipErrorInvalidValue;
+    if (num_surfels == 0) {
+        return fill_zero(out, 2 * (size_t)num_apertures * c.K, (hipStream_t)stream);
+    }
+    return launch_render(c, positions, num_apertures, centers, normals, reinterpret_cast<const float2*>(weights),
+                         num_surfels, reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
+size_t dsplat_surfel_vjp_workspace(int32_t num_apertures, int32_t num_surfels) {
+    if (num_apertures <= 0 || num_surfels <= 0) return 0;
+    const VjpShape v = vjp_shape(num_apertures, num_surfels);
+    return sizeof(float) * 8 * (size_t)num_surfels * v.nblk;
+}
+
+int dsplat_surfel_vjp(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* centers,
+                      const float* normals, const float* weights, int32_t num_surfels, const float* upstream,
+                      float* grad_centers, float* grad_normals, float* grad_weights, void* workspace,
+                      size_t workspace_bytes, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0) return hipErrorInvalidValue;
+    if (num_surfels == 0) return hipSuccess;
+    hipStream_t st = (hipStream_t)stream;
+    if (!grad_centers || !grad_normals || !grad_weights) return hipErrorInvalidValue;
+    if (num_apertures == 0) {
+        hipError_t e = fill_zero(grad_centers, 3 * (size_t)num_surfels, st);
+        if (e == hipSuccess) e = fill_zero(grad_normals, 3 * (size_t)num_surfels, st);
+        if (e == hipSuccess) e = fill_zero(grad_weights, 2 * (size_t)num_surfels, st);
+        return e;
+    }
+    if (!positions || !centers || !normals || !weights || !upstream || !workspace) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_surfel_vjp_workspace(num_apertures, num_surfels)) return hipErrorInvalidValue;
+    return launch_vjp(c, positions, num_apertures, centers, normals, reinterpret_cast<const float2*>(weights),
+                      num_surfels, reinterpret_cast<const float2*>(upstream), static_cast<float*>(workspace),
+                      grad_centers, grad_normals, grad_weights, st);
+}
+
+}  // extern "C"

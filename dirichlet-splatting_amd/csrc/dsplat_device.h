@@ -1,0 +1,239 @@
+// Device math shared by the surfel kernels (gfx950, wave64).
+//
+// Model (reference: dsplat/kernels/scattering.slang, common.slang, surfel.slang).
+// For sensor p, surfel center c and unit normal n:
+//   r = |p - c| + 1e-12, u = (p - c) / r, cos = u . n
+//   amp = cos^2 F(cos) lambda / (8 pi r), 0 for cos <= 0
+//   rot = exp(2 pi i (fc tau + K tau (t0 - tau / 2))), tau = 2 r / c0
+//   column[k] = amp rot D(k - k0), k0 = r / spacing (mod N)
+// and with S ADC samples, N = S P FFT bins, d = k - k0:
+//   D(d)  = nu(d) / S (cot(pi d / N) + i),  nu(d) = sin(pi d / P) exp(-i pi d / P)
+//   D'(d) = [(pi / P) q(d) (cot + i) - (pi / N) nu(d) (1 + cot^2)] / S, q = exp(-2 i pi d / P)
+//
+// Precision plan (every result within 1e-5 relative L2 of float64):
+//  * Range, carrier phase and k0 come from float64 per pair: the round-trip phase
+//    is ~1.8e3 rad at 120 GHz, so float32 r alone costs 1.5e-4 rad.
+//  * k0 = lobe + delta with an integer lobe and |delta| <= 1/2 taken in float64, so
+//    every bin offset d = m - delta (m = k - lobe integer) is exact to float32.
+//  * cot(pi (m - delta) / N) = (T_m + t) / (1 - T_m t), T_m = cot(pi m / N) from an
+//    exactly rounded table and t = tan(pi delta / N): |T_m t| <= 1/2 for m != 0, so
+//    the per-bin cotangent is relatively accurate next to the lobe, with one rcp and
+//    no angle-addition drift.
+//  * nu and q have period P in d; per pair they are P constants from sin and cos of
+//    beta_r = pi (r - delta) / P, relatively accurate even for delta -> 0.
+//  * The lobe bin m = 0: the forward keeps the exact formula (delta is kept off 0 by
+//    1e-8 bins, so nu cot stays finite); derivatives (cancelling O(1/delta) terms)
+//    take the lobe bin from lobe_direct() and see a table entry T_0 = 0 in the loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dsplat {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr double kPiD = 3.14159265358979323846;
+
+// Kernel-side chirp constants (by value).
+struct Chirp {
+    int K, N, S, P;
+    int Kp;              // K rounded up to a multiple of 16
+    double k0pm, fc, slope, t0, c0;
+    float lam, fresnel_n;
+    float inv_S;         // 1 / S
+    float pi_P, pi_N;    // pi / P, pi / N
+    float h;             // pi (S - 1) / N
+    float k0pm_f;        // 1 / spacing as float
+    float rate0, rate1;  // carrier rate = rate0 - rate1 * r (rad / m)
+    float c2, c4;        // Taylor coefficients of the lobe ratio
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // conj(a) * b
+    return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float2 operator+(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 operator-(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 operator*(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+__device__ __forceinline__ float2& operator+=(float2& a, float2 b) { a.x += b.x; a.y += b.y; return a; }
+__device__ __forceinline__ float2& operator-=(float2& a, float2 b) { a.x -= b.x; a.y -= b.y; return a; }
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// sin(pi x), cos(pi x) for a small reduced argument: Taylor-Horner in y = pi x,
+// |y| <= pi / 4 (|x| <= 1/4), truncation < 4e-10.
+__device__ __forceinline__ void sincos_small(float y, float* s, float* c) {
+    const float y2 = y * y;
+    float ps = fmaf(y2, -2.50521084e-8f, 2.75573192e-6f);
+    ps = fmaf(y2, ps, -1.98412698e-4f);
+    ps = fmaf(y2, ps, 8.33333333e-3f);
+    ps = fmaf(y2, ps, -1.66666667e-1f);
+    *s = fmaf(y * y2, ps, y);
+    float pc = fmaf(y2, 2.08767570e-9f, -2.75573192e-7f);
+    pc = fmaf(y2, pc, 2.48015873e-5f);
+    pc = fmaf(y2, pc, -1.38888889e-3f);
+    pc = fmaf(y2, pc, 4.16666667e-2f);
+    pc = fmaf(y2, pc, -0.5f);
+    *c = fmaf(y2, pc, 1.0f);
+}
+
+// tan(y) for |y| <= 0.025 (y = pi delta / N with N >= 64): y + y^3/3 + 2 y^5/15 + 17 y^7/315.
+__device__ __forceinline__ float tan_small(float y) {
+    const float y2 = y * y;
+    float p = fmaf(y2, 5.39682540e-2f, 1.33333333e-1f);
+    p = fmaf(y2, p, 3.33333333e-1f);
+    return fmaf(y * y2, p, y);
+}
+
+// g(y) = cot(y) - 1/y for |y| <= 0.8 (Laurent series, truncation < 1e-9 relative).
+__device__ __forceinline__ float cot_minus_inv(float y) {
+    const float y2 = y * y;
+    float p = fmaf(y2, -2.19259479e-7f, -2.16440428e-6f);
+    p = fmaf(y2, p, -2.13777990e-5f);
+    p = fmaf(y2, p, -2.11640212e-4f);
+    p = fmaf(y2, p, -2.11640212e-3f);
+    p = fmaf(y2, p, -2.22222222e-2f);
+    p = fmaf(y2, p, -3.33333333e-1f);
+    return y * p;
+}
+
+// Geometry of one aperture-surfel pair (surfel.slang:63-85, scattering.slang).
+struct Pair {
+    float r, inv_r;
+    float ux, uy, uz;
+    float cosine, amp, amp_cos, rate;
+    float2 rot;
+    int lobe;     // nearest bin to k0, in [0, N)
+    float delta;  // k0 - lobe in [-1/2, 1/2], from float64
+};
+
+__device__ __forceinline__ Pair pair_geometry(const Chirp& c, float px, float py, float pz,
+                                              float cx, float cy, float cz,
+                                              float nx, float ny, float nz) {
+    Pair q;
+    const double dx = (double)px - (double)cx;
+    const double dy = (double)py - (double)cy;
+    const double dz = (double)pz - (double)cz;
+    const double r = sqrt(fma(dx, dx, fma(dy, dy, dz * dz))) + 1e-12;
+    q.r = (float)r;
+    const double inv = 1.0 / r;
+    q.ux = (float)(dx * inv);
+    q.uy = (float)(dy * inv);
+    q.uz = (float)(dz * inv);
+    q.inv_r = (float)inv;
+    q.cosine = q.ux * nx + q.uy * ny + q.uz * nz;
+    // Scattering amplitude and its cos-derivative (scattering.slang:16-38).
+    q.amp = 0.0f;
+    q.amp_cos = 0.0f;
+    if (q.cosine > 0.0f) {
+        float f = 1.0f, df = 0.0f;
+        const float n = c.fresnel_n;
+        if (n > 1.0f) {
+            const float root = sqrtf(n * n - 1.0f + q.cosine * q.cosine);
+            const float inv_sum = 1.0f / (root + q.cosine);
+            f = (root - q.cosine) * inv_sum;
+            df = -2.0f * f / root;
+        }
+        const float path = c.lam * q.inv_r * (1.0f / (8.0f * kPi));
+        q.amp = path * q.cosine * q.cosine * f;
+        q.amp_cos = path * (2.0f * q.cosine * f + q.cosine * q.cosine * df);
+    }
+    // Chirp phase in cycles, reduced in float64 (scattering.slang:40-45).
+    const double tau = 2.0 * r / c.c0;
+    double cyc = c.fc * tau + c.slope * tau * (c.t0 - 0.5 * tau);
+    cyc -= rint(cyc);
+    float sp, cp;
+    sincospif(2.0f * (float)cyc, &sp, &cp);
+    q.rot = make_float2(cp, sp);
+    q.rate = fmaf(-c.rate1, q.r, c.rate0);
+    // Peak bin k0 = r / spacing wrapped to [0, N), split as lobe + delta.
+    double k0 = r * c.k0pm;
+    const double nf = (double)c.N;
+    k0 -= nf * floor(k0 / nf);
+    const double lobe = rint(k0);
+    q.delta = (float)(k0 - lobe);
+    int L = (int)lobe;
+    q.lobe = L >= c.N ? L - c.N : L;
+    return q;
+}
+
+// sin and cos of beta_r = pi (r - delta) / P for r = 0..P-1, relative-accurate.
+template <int P>
+__device__ __forceinline__ void class_angles(float delta, float* sb, float* cb) {
+    float s0, c0;
+    sincos_small(kPi * delta / (float)P, &s0, &c0);  // |delta / P| <= 1/2 / P <= 1/4 for P >= 2
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        // exact constants sin, cos (pi r / P) for P in {1, 2, 4, 8}
+        float sa, ca;
+        if (r == 0) { sa = 0.0f; ca = 1.0f; }
+        else if (P == 2) { sa = 1.0f; ca = 0.0f; }
+        else if (P == 4) {
+            const float t[4][2] = {{0.f, 1.f}, {0.70710678f, 0.70710678f}, {1.f, 0.f}, {0.70710678f, -0.70710678f}};
+            sa = t[r][0]; ca = t[r][1];
+        } else {
+            const float t[8][2] = {{0.f, 1.f}, {0.38268343f, 0.92387953f}, {0.70710678f, 0.70710678f},
+                                   {0.92387953f, 0.38268343f}, {1.f, 0.f}, {0.92387953f, -0.38268343f},
+                                   {0.70710678f, -0.70710678f}, {0.38268343f, -0.92387953f}};
+            sa = t[r][0]; ca = t[r][1];
+        }
+        sb[r] = sa * c0 - ca * s0;
+        cb[r] = ca * c0 + sa * s0;
+    }
+}
+
+// P = 1: beta_0 = -pi delta with |pi delta| <= pi / 2; use the accurate library sincos.
+template <>
+__device__ __forceinline__ void class_angles<1>(float delta, float* sb, float* cb) {
+    float s0, c0;
+    sincospif(delta, &s0, &c0);
+    sb[0] = -s0;
+    cb[0] = c0;
+}
+
+// D(-delta) and D'(-delta) at the lobe bin, without cancellation:
+// D = R e^{-i h x}, D' = e^{-i h x} (R' - i h R), x = -delta,
+// R = sin(a x) / (S sin(b x)), R' / R = a g(a x) - b g(b x), g(y) = cot y - 1 / y.
+// (common.slang:10-33 evaluates R' by a quotient that cancels for small x.)
+__device__ __forceinline__ void lobe_direct(const Chirp& c, float delta, float2* D, float2* Dp) {
+    const float x = -delta;
+    const float a = c.pi_P, b = c.pi_N;
+    float R;
+    if (fabsf(x) < 1e-3f) {
+        const float x2 = x * x;
+        R = fmaf(x2, fmaf(x2, c.c4, c.c2), 1.0f);
+    } else {
+        R = sinf(a * x) / ((float)c.S * sinf(b * x));
+    }
+    float ga;
+    const float ax = a * x;
+    if (fabsf(ax) <= 0.8f) ga = cot_minus_inv(ax);
+    else ga = cosf(ax) / sinf(ax) - 1.0f / ax;
+    const float gb = cot_minus_inv(b * x);
+    const float Rp = R * (a * ga - b * gb);
+    float sh, ch;
+    sincosf(c.h * x, &sh, &ch);
+    const float2 ph = make_float2(ch, -sh);
+    *D = R * ph;
+    *Dp = cmul(ph, make_float2(Rp, -c.h * R));
+}
+
+// Exactly rounded cot(pi m / N) table entry for index i = m + N, i in [0, 2N).
+// m = 0 holds `poison`: 1e30 for the forward (then cot -> -1/t, the exact limit),
+// 0 for derivative kernels (cot -> t, removed again after the bin loop).
+__device__ __forceinline__ float cot_entry(int i, int N, float poison) {
+    const int m = i - N;
+    if (m == 0) return poison;
+    if (m == -N) return 0.0f;  // never indexed
+    const double ang = kPiD * (double)m / (double)N;
+    return (float)(cos(ang) / sin(ang));
+}
+
+// Per-bin cotangent of pi (m - delta) / N from table entry T_m and t = tan(pi delta / N).
+__device__ __forceinline__ float cot_bin(float Tm, float t) {
+    return (Tm + t) * rcp(fmaf(-Tm, t, 1.0f));
+}
+
+}  // namespace dsplat

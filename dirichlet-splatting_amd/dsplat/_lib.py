@@ -1,0 +1,110 @@
+"""ctypes binding of libdsplat_hip.so, the C ABI declared in include/dsplat_hip.h.
+
+torch is imported first on purpose: its wheel ships the HIP runtime
+(libamdhip64.so.7), and the library must bind to that same runtime so that torch's
+device pointers and streams are valid for our launches. There is no fallback: if the
+library is missing every kernel call raises.
+"""
+
+import ctypes
+import os
+from functools import cache
+from pathlib import Path
+
+import torch
+
+LIB_PATH = Path(os.environ.get("DSPLAT_LIB", Path(__file__).parent / "_lib" / "libdsplat_hip.so"))
+
+c_int, c_size, c_double, c_float = ctypes.c_int32, ctypes.c_size_t, ctypes.c_double, ctypes.c_float
+c_ptr = ctypes.c_void_p
+
+
+class Fmcw(ctypes.Structure):
+    """dsplat_fmcw of include/dsplat_hip.h."""
+
+    _fields_ = [
+        ("num_bins", c_int),
+        ("n_fft", c_int),
+        ("samples", c_int),
+        ("pad", c_int),
+        ("k0_per_m", c_double),
+        ("fc_hz", c_double),
+        ("slope_hz_s", c_double),
+        ("t_start_s", c_double),
+        ("lambda_m", c_double),
+        ("c0_m_s", c_double),
+        ("fresnel_n", c_double),
+    ]
+
+
+# name -> (restype, argtypes); one line per entry point of include/dsplat_hip.h
+SIGNATURES = {
+    "dsplat_version": (ctypes.c_char_p, []),
+    "dsplat_error_string": (ctypes.c_char_p, [c_int]),
+    "dsplat_surfel_render": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
+    "dsplat_surfel_vjp_workspace": (c_size, [c_int, c_int]),
+    "dsplat_surfel_vjp": (
+        c_int,
+        [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
+    ),
+}
+
+
+@cache
+def library():
+    """The loaded library with every declared signature; raises if it is not built."""
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is not built: run `python dirichlet-splatting_amd/csrc/build.py` "
+            "(or __graft_entry__.build()); dsplat has no CPU fallback."
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype, fn.argtypes = restype, argtypes
+    return lib
+
+
+def call(name, *args):
+    """Call an entry point and raise RuntimeError on a non-zero hipError_t."""
+    lib = library()
+    code = getattr(lib, name)(*args)
+    if code:
+        message = lib.dsplat_error_string(code).decode()
+        raise RuntimeError(f"{name} failed with hipError {code}: {message}")
+
+
+def ptr(tensor):
+    """Device address of a contiguous tensor (None -> NULL)."""
+    if tensor is None:
+        return None
+    if not tensor.is_cuda:
+        raise ValueError("dsplat kernels take device tensors")
+    if not tensor.is_contiguous():
+        raise ValueError("dsplat kernels take contiguous tensors")
+    return ctypes.c_void_p(tensor.data_ptr())
+
+
+def stream(device):
+    """The current HIP stream of a device as a raw hipStream_t."""
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def fmcw(config, fresnel_n=0.0):
+    """The ABI struct of an FMCWConfig (kernels/surfel.py:21-32 sensor_args, plus fresnel_n)."""
+    pad = int(config.pad_factor)
+    if config.n_fft != config.adc_samples * pad:
+        raise ValueError("n_fft must equal adc_samples * pad_factor")
+    return Fmcw(
+        num_bins=int(config.num_range_bins),
+        n_fft=int(config.n_fft),
+        samples=int(config.adc_samples),
+        pad=pad,
+        k0_per_m=1.0 / config.range_bin_spacing_m,
+        fc_hz=float(config.fc_hz),
+        slope_hz_s=float(config.slope_hz_s),
+        t_start_s=float(config.adc_start_time_s),
+        lambda_m=float(config.lambda_m),
+        c0_m_s=float(config.c0_m_s),
+        fresnel_n=float(fresnel_n),
+    )

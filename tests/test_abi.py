@@ -1,0 +1,75 @@
+"""The C ABI library: it loads, exports every declared symbol, and rejects bad input on the host.
+
+No kernel is launched here: rejected configurations return before any GPU call.
+"""
+
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "dsplat_hip.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"\b(dsplat_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    assert "dsplat_surfel_render" in names and "dsplat_surfel_vjp" in names
+
+
+def test_library_exports_every_declared_symbol():
+    from dsplat import _lib
+
+    lib = _lib.library()
+    missing = [name for name in declared() if not hasattr(lib, name)]
+    assert not missing, missing
+    # the Python binding declares a signature for every entry point of the header
+    assert sorted(_lib.SIGNATURES) == declared()
+    assert b"gfx950" in lib.dsplat_version()
+
+
+def test_rejected_configurations_return_invalid_value_without_launching():
+    from dsplat import FMCWConfig, _lib
+
+    lib = _lib.library()
+    bad = _lib.fmcw(FMCWConfig())
+    bad.pad = 3  # pad factors outside {1, 2, 4, 8} are rejected
+    code = lib.dsplat_surfel_render(ctypes.byref(bad), None, 1, None, None, None, 1, None, None)
+    assert code == 1
+    assert b"invalid" in lib.dsplat_error_string(code).lower()
+    bad = _lib.fmcw(FMCWConfig())
+    bad.num_bins = 100  # K must be n_fft / 2
+    args = [None, 1, None, None, None, 1, None, None, None, None, None, 0, None]
+    assert lib.dsplat_surfel_vjp(ctypes.byref(bad), *args) == 1
+
+
+def test_vjp_workspace_is_one_partial_per_surfel_and_aperture_block():
+    from dsplat import _lib
+
+    lib = _lib.library()
+    assert lib.dsplat_surfel_vjp_workspace(0, 10) == 0
+    assert lib.dsplat_surfel_vjp_workspace(16384, 2048) == 2048 * 512 * 8 * 4
+
+
+def test_struct_layout_matches_the_header():
+    from dsplat import _lib
+
+    # four int32 then seven doubles, naturally aligned: 16 + 56 bytes
+    assert ctypes.sizeof(_lib.Fmcw) == 72
+    assert _lib.Fmcw.k0_per_m.offset == 16
+
+
+def test_kernels_refuse_host_tensors():
+    import torch
+
+    from dsplat import FMCWConfig
+    from dsplat.kernels import surfel
+
+    with pytest.raises(ValueError, match="device"):
+        surfel._profiles(FMCWConfig(), torch.zeros(1, 3), torch.zeros(1, 3), torch.zeros(1, 3), torch.zeros(1, 2), 0.0)
