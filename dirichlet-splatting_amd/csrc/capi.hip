@@ -83,6 +83,22 @@ int dsplat_surfel_render(const dsplat_fmcw* cfg, const float* positions, int32_t
                          num_surfels, reinterpret_cast<float2*>(out), (hipStream_t)stream);
 }
 
+int dsplat_surfel_jvp(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* centers,
+                      const float* normals, const float* weights, const float* delta_centers,
+                      const float* delta_normals, const float* delta_weights, int32_t num_surfels, float* out,
+                      void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0) return hipErrorInvalidValue;
+    if (num_apertures == 0) return hipSuccess;
+    if (!positions || !out) return hipErrorInvalidValue;
+    if (num_surfels == 0) return fill_zero(out, 2 * (size_t)num_apertures * c.K, (hipStream_t)stream);
+    if (!centers || !normals || !weights || !delta_centers || !delta_normals || !delta_weights)
+        return hipErrorInvalidValue;
+    return launch_jvp(c, positions, num_apertures, centers, normals, reinterpret_cast<const float2*>(weights),
+                      delta_centers, delta_normals, reinterpret_cast<const float2*>(delta_weights), num_surfels,
+                      reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
 size_t dsplat_surfel_vjp_workspace(int32_t num_apertures, int32_t num_surfels) {
     if (num_apertures <= 0 || num_surfels <= 0) return 0;
     const VjpShape v = vjp_shape(num_apertures, num_surfels);
@@ -109,6 +125,86 @@ int dsplat_surfel_vjp(const dsplat_fmcw* cfg, const float* positions, int32_t nu
     return launch_vjp(c, positions, num_apertures, centers, normals, reinterpret_cast<const float2*>(weights),
                       num_surfels, reinterpret_cast<const float2*>(upstream), static_cast<float*>(workspace),
                       grad_centers, grad_normals, grad_weights, st);
+}
+
+int dsplat_surfel_design(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* centers,
+                         const float* normals, const float* areas, int32_t num_surfels, int32_t real_layout,
+                         float* out, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0) return hipErrorInvalidValue;
+    if (num_apertures == 0 || num_surfels == 0) return hipSuccess;
+    if (!positions || !centers || !normals || !areas || !out) return hipErrorInvalidValue;
+    return launch_design(c, positions, num_apertures, centers, normals, areas, num_surfels, real_layout ? 0 : 1, out,
+                         (hipStream_t)stream);
+}
+
+int dsplat_surfel_jacobian(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                           const float* centers, const float* normals, int32_t num_surfels, float* columns,
+                           float* derivatives, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0) return hipErrorInvalidValue;
+    if (num_apertures == 0 || num_surfels == 0) return hipSuccess;
+    if (!positions || !centers || !normals || !columns || !derivatives) return hipErrorInvalidValue;
+    return launch_jacobian(c, positions, num_apertures, centers, normals, num_surfels,
+                           reinterpret_cast<float2*>(columns), reinterpret_cast<float2*>(derivatives),
+                           (hipStream_t)stream);
+}
+
+size_t dsplat_surfel_correlate_workspace(int32_t num_apertures, int32_t num_surfels, int32_t width) {
+    if (num_apertures <= 0 || num_surfels <= 0 || width <= 0) return 0;
+    return sizeof(float) * (size_t)width * num_surfels * cor_shape(num_apertures, num_surfels).nblk;
+}
+
+static int correlate_common(const dsplat_fmcw* cfg, const float* positions, int32_t A, const float* centers,
+                            const float* normals, int32_t S, const float* residual, int W, float* out,
+                            void* workspace, size_t workspace_bytes, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || A < 0 || S < 0) return hipErrorInvalidValue;
+    if (S == 0) return hipSuccess;
+    if (!out) return hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    const int width = 3 + 3 * W;
+    if (A == 0) return fill_zero(out, (size_t)width * S, st);
+    if (!positions || !centers || !normals || !residual || !workspace) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_surfel_correlate_workspace(A, S, width)) return hipErrorInvalidValue;
+    return launch_correlate(c, W, positions, A, centers, normals, S, reinterpret_cast<const float2*>(residual),
+                            static_cast<float*>(workspace), out, st);
+}
+
+int dsplat_surfel_correlate(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                            const float* centers, const float* normals, int32_t num_surfels, const float* residual,
+                            float* out, void* workspace, size_t workspace_bytes, void* stream) {
+    return correlate_common(cfg, positions, num_apertures, centers, normals, num_surfels, residual, 0, out, workspace,
+                            workspace_bytes, stream);
+}
+
+int dsplat_surfel_correlate_gradient(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                                     const float* centers, const float* normals, int32_t num_surfels,
+                                     const float* residual, int32_t include_normals, float* out, void* workspace,
+                                     size_t workspace_bytes, void* stream) {
+    return correlate_common(cfg, positions, num_apertures, centers, normals, num_surfels, residual,
+                            include_normals ? 6 : 3, out, workspace, workspace_bytes, stream);
+}
+
+int dsplat_surfel_correlate_orientations(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                                         const float* centers, const float* normals, int32_t num_surfels,
+                                         const float* bank, int32_t num_normals, const float* residual,
+                                         const float* range_table, const float* norm_table, int32_t oversample,
+                                         float* out, void* workspace, size_t workspace_bytes, void* stream) {
+    Chirp c;
+    const int A = num_apertures, S = num_surfels, B = num_normals;
+    if (!make_chirp(cfg, &c) || A < 0 || S < 0 || B < 0 || B > 255) return hipErrorInvalidValue;
+    if (S == 0) return hipSuccess;
+    if (!out || (B > 0 && !bank)) return hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    const int width = 3 * (1 + B);
+    if (A == 0) return fill_zero(out, (size_t)width * S, st);
+    if (!positions || !centers || !normals || !workspace) return hipErrorInvalidValue;
+    if (range_table ? (!norm_table || oversample < 1) : !residual) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_surfel_correlate_workspace(A, S, width)) return hipErrorInvalidValue;
+    return launch_orientations(c, positions, A, centers, normals, bank, B, S, reinterpret_cast<const float2*>(residual),
+                               reinterpret_cast<const float2*>(range_table), norm_table, oversample,
+                               static_cast<float*>(workspace), out, st);
 }
 
 }  // extern "C"

@@ -159,38 +159,41 @@ __device__ __forceinline__ Pair pair_geometry(const Chirp& c, float px, float py
     return q;
 }
 
-// sin and cos of beta_r = pi (r - delta) / P for r = 0..P-1, relative-accurate.
+// sin and cos of beta = pi (rho - delta) / P for every bin class r = k mod P, where
+// rho = (r - lobe) mod P is the class of the offset m = k - lobe that nu and q depend
+// on (both have period P in d = m - delta). Relative-accurate: the small rotation by
+// pi delta / P is applied to exact constants sin, cos (pi rho / P), P in {1, 2, 4, 8}.
 template <int P>
-__device__ __forceinline__ void class_angles(float delta, float* sb, float* cb) {
+__device__ __forceinline__ void class_angles(float delta, int lobe, float* sb, float* cb) {
     float s0, c0;
-    sincos_small(kPi * delta / (float)P, &s0, &c0);  // |delta / P| <= 1/2 / P <= 1/4 for P >= 2
+    if (P == 1) {
+        sincospif(delta, &s0, &c0);  // |pi delta| <= pi / 2: the library routine
+    } else {
+        sincos_small(kPi * delta / (float)P, &s0, &c0);  // |pi delta / P| <= pi / 4
+    }
 #pragma unroll
     for (int r = 0; r < P; ++r) {
-        // exact constants sin, cos (pi r / P) for P in {1, 2, 4, 8}
-        float sa, ca;
-        if (r == 0) { sa = 0.0f; ca = 1.0f; }
-        else if (P == 2) { sa = 1.0f; ca = 0.0f; }
-        else if (P == 4) {
-            const float t[4][2] = {{0.f, 1.f}, {0.70710678f, 0.70710678f}, {1.f, 0.f}, {0.70710678f, -0.70710678f}};
-            sa = t[r][0]; ca = t[r][1];
-        } else {
-            const float t[8][2] = {{0.f, 1.f}, {0.38268343f, 0.92387953f}, {0.70710678f, 0.70710678f},
-                                   {0.92387953f, 0.38268343f}, {1.f, 0.f}, {0.92387953f, -0.38268343f},
-                                   {0.70710678f, -0.70710678f}, {0.38268343f, -0.92387953f}};
-            sa = t[r][0]; ca = t[r][1];
+        const int rho = (r - lobe) & (P - 1);
+        float sa = 0.0f, ca = 1.0f;
+        if (P == 2) {
+            sa = rho ? 1.0f : 0.0f;
+            ca = rho ? 0.0f : 1.0f;
+        } else if (P == 4) {
+            const float ts[4] = {0.f, 0.70710678f, 1.f, 0.70710678f};
+            const float tc[4] = {1.f, 0.70710678f, 0.f, -0.70710678f};
+#pragma unroll
+            for (int i = 1; i < 4; ++i)
+                if (rho == i) { sa = ts[i]; ca = tc[i]; }
+        } else if (P == 8) {
+            const float ts[8] = {0.f, 0.38268343f, 0.70710678f, 0.92387953f, 1.f, 0.92387953f, 0.70710678f, 0.38268343f};
+            const float tc[8] = {1.f, 0.92387953f, 0.70710678f, 0.38268343f, 0.f, -0.38268343f, -0.70710678f, -0.92387953f};
+#pragma unroll
+            for (int i = 1; i < 8; ++i)
+                if (rho == i) { sa = ts[i]; ca = tc[i]; }
         }
         sb[r] = sa * c0 - ca * s0;
         cb[r] = ca * c0 + sa * s0;
     }
-}
-
-// P = 1: beta_0 = -pi delta with |pi delta| <= pi / 2; use the accurate library sincos.
-template <>
-__device__ __forceinline__ void class_angles<1>(float delta, float* sb, float* cb) {
-    float s0, c0;
-    sincospif(delta, &s0, &c0);
-    sb[0] = -s0;
-    cb[0] = c0;
 }
 
 // D(-delta) and D'(-delta) at the lobe bin, without cancellation:
@@ -234,6 +237,37 @@ __device__ __forceinline__ float cot_entry(int i, int N, float poison) {
 // Per-bin cotangent of pi (m - delta) / N from table entry T_m and t = tan(pi delta / N).
 __device__ __forceinline__ float cot_bin(float Tm, float t) {
     return (Tm + t) * rcp(fmaf(-Tm, t, 1.0f));
+}
+
+// D(m - delta) and D'(m - delta) at one integer bin offset m (any sign), relatively
+// accurate: the lobe (m = 0 mod N) directly, every other bin in cot form with
+// T_m = cot(pi m / N) evaluated here rather than read from a table.
+__device__ __forceinline__ void kernel_at(const Chirp& c, int m, float delta, float2* D, float2* Dp) {
+    int mm = m % c.N;
+    if (mm < 0) mm += c.N;
+    if (mm == 0) {
+        lobe_direct(c, delta, D, Dp);
+        return;
+    }
+    if (mm >= c.N / 2 + (c.N & 1)) mm -= c.N;  // into [-N/2, N/2)
+    float sm, cm;
+    sincospif((float)mm / (float)c.N, &sm, &cm);
+    const float cot = cot_bin(cm / sm, tan_small(c.pi_N * delta));
+    int r = mm % c.P;
+    if (r < 0) r += c.P;
+    float sb, cb;
+    sincospif(((float)r - delta) / (float)c.P, &sb, &cb);
+    const float2 nu = make_float2(sb * cb, -sb * sb);
+    const float2 q = make_float2(fmaf(-2.0f * sb, sb, 1.0f), -2.0f * sb * cb);
+    *D = c.inv_S * cmul(nu, make_float2(cot, 1.0f));
+    *Dp = c.inv_S * (c.pi_P * cmul(q, make_float2(cot, 1.0f)) - (c.pi_N * fmaf(cot, cot, 1.0f)) * nu);
+}
+
+// d amp / d c_j = amp u_j / r + amp_cos (cos u_j - n_j) / r (correlate.slang:223).
+__device__ __forceinline__ void amp_gradient(const Pair& q, float nx, float ny, float nz, float* da) {
+    da[0] = (q.amp * q.ux + q.amp_cos * (q.cosine * q.ux - nx)) * q.inv_r;
+    da[1] = (q.amp * q.uy + q.amp_cos * (q.cosine * q.uy - ny)) * q.inv_r;
+    da[2] = (q.amp * q.uz + q.amp_cos * (q.cosine * q.uz - nz)) * q.inv_r;
 }
 
 }  // namespace dsplat

@@ -21,8 +21,28 @@ struct VjpShape {
 };
 VjpShape vjp_shape(int A, int S);
 
+struct CorShape {
+    int nblk;    // aperture blocks of 32
+    int groups;  // candidate chunks
+    int chunk;   // candidates per chunk
+};
+CorShape cor_shape(int A, int S);
+size_t cor_smem(const Chirp& c);
+
+hipError_t launch_design(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                         const float* areas, int S, int complex_layout, float* out, hipStream_t st);
+hipError_t launch_jacobian(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, int S,
+                           float2* col, float2* jac, hipStream_t st);
+hipError_t launch_correlate(const Chirp& c, int W, const float* pos, int A, const float* ctr, const float* nrm, int S,
+                            const float2* res, float* partial, float* out, hipStream_t st);
+hipError_t launch_orientations(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                               const float* bank, int B, int S, const float2* res, const float2* table,
+                               const float* norm_table, int oversample, float* partial, float* out, hipStream_t st);
+
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st);
+hipError_t launch_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, const float2* w,
+                      const float* dctr, const float* dnrm, const float2* dw, int S, float2* out, hipStream_t st);
 hipError_t launch_vjp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, const float2* w,
                       int S, const float2* up, float* partial, float* gc, float* gn, float* gw, hipStream_t st);
 

@@ -1,0 +1,387 @@
+// Candidate correlations against a residual (reference: dsplat/kernels/correlate.slang:75-235;
+// bindings dsplat/kernels/surfel.py:242-315).
+//
+// Layout (shared by every kernel here): a workgroup owns 32 apertures with their residual
+// tile [K][32] in LDS; each half-wave takes one candidate and lane = aperture, reducing
+// all bins of its pair in registers as per-class sums of the cot form
+//   F_r = sum cot_k res_k,  Q_r = sum cot_k^2,  (gradient) G_r = sum cot_k^2 res_k,
+//   C1_r = sum cot_k, C3_r = sum cot_k^3,  U_r = sum res_k, n_r = #bins of class r,
+// from which, with the lobe bin taken out and added directly,
+//   g0 = sum conj(D) res,  |D|^2 sum,  g1 = sum conj(D') res,  sum Re(conj(D') D).
+// Each half-wave sums its per-aperture results over the 32 apertures and writes one
+// partial per (candidate, aperture block); reduce_partials adds them in float64 in a
+// fixed order. The reference adds them with float atomics instead (nondeterministic).
+#include "dsplat_device.h"
+#include "dsplat_internal.h"
+
+namespace dsplat {
+
+constexpr int kCorApertures = 32;
+
+struct Sums {
+    float2 g0, g1;   // sum conj(D) res, sum conj(D') res
+    float nrm, rx;   // sum |D|^2, sum Re(conj(D') D)
+};
+
+// Residual tile, class sums of the residual, and the derivative cot table of one workgroup.
+struct CorTile {
+    float* T;        // [2N], T_0 = 0
+    float2* R;       // [Kp][32]
+    float2* Us;      // [P][32]
+};
+
+__device__ __forceinline__ CorTile cor_tile(const Chirp& c, float* smem, int P) {
+    CorTile t;
+    t.T = smem;
+    t.R = reinterpret_cast<float2*>(smem + 2 * c.N);
+    t.Us = t.R + c.Kp * kCorApertures;
+    return t;
+}
+
+__device__ __forceinline__ void load_tile(const Chirp& c, const CorTile& tile, int P, const float2* __restrict__ res,
+                                          int A, int a0) {
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 2 * c.N; i += blockDim.x) tile.T[i] = cot_entry(i, c.N, 0.0f);
+    for (int i = tid; i < kCorApertures * c.Kp; i += blockDim.x) {
+        const int ap = i / c.Kp, k = i - ap * c.Kp;
+        const int a = a0 + ap;
+        tile.R[k * kCorApertures + ap] = (a < A && k < c.K) ? res[(size_t)a * c.K + k] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    if (tid < P * kCorApertures) {
+        const int r = tid / kCorApertures, ap = tid - r * kCorApertures;
+        float2 s = make_float2(0.f, 0.f);
+        for (int k = r; k < c.Kp; k += P) s += tile.R[k * kCorApertures + ap];
+        tile.Us[r * kCorApertures + ap] = s;
+    }
+    __syncthreads();
+}
+
+// Bin sums of one pair (lane = aperture al of the tile). GRAD adds g1 and rx.
+template <int P, bool GRAD>
+__device__ __forceinline__ Sums pair_sums(const Chirp& c, const CorTile& tile, int al, const Pair& q) {
+    const float t = tan_small(c.pi_N * q.delta);
+    const float* Tb = tile.T + (c.N - q.lobe);
+    const float2* Rb = tile.R + al;
+    float2 F[P], G[P];
+    float Q[P], C1[P], C3[P];
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        F[r] = make_float2(0.f, 0.f);
+        G[r] = make_float2(0.f, 0.f);
+        Q[r] = 0.f;
+        C1[r] = 0.f;
+        C3[r] = 0.f;
+    }
+    for (int k0 = 0; k0 < c.K; k0 += 16) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            const int k = k0 + j;
+            if (k < c.K) {  // uniform
+                const float cot = cot_bin(Tb[k], t);
+                const float2 u = Rb[k * kCorApertures];
+                F[j % P].x = fmaf(cot, u.x, F[j % P].x);
+                F[j % P].y = fmaf(cot, u.y, F[j % P].y);
+                Q[j % P] = fmaf(cot, cot, Q[j % P]);
+                if (GRAD) {
+                    const float c2 = cot * cot;
+                    G[j % P].x = fmaf(c2, u.x, G[j % P].x);
+                    G[j % P].y = fmaf(c2, u.y, G[j % P].y);
+                    C1[j % P] += cot;
+                    C3[j % P] = fmaf(c2, cot, C3[j % P]);
+                }
+            }
+        }
+    }
+    const bool seen = q.lobe < c.K;
+    const int rl = q.lobe % P;
+    const float2 ul = seen ? tile.R[q.lobe * kCorApertures + al] : make_float2(0.f, 0.f);
+    float2 D, Dp;
+    lobe_direct(c, q.delta, &D, &Dp);
+    Sums out;
+    out.g0 = seen ? cmulc(D, ul) : make_float2(0.f, 0.f);
+    out.nrm = seen ? D.x * D.x + D.y * D.y : 0.f;
+    out.g1 = seen ? cmulc(Dp, ul) : make_float2(0.f, 0.f);
+    out.rx = seen ? cmulc(Dp, D).x : 0.f;
+    float sb[P], cb[P];
+    class_angles<P>(q.delta, q.lobe, sb, cb);
+    const float t2 = t * t, invS2 = c.inv_S * c.inv_S;
+#pragma unroll
+    for (int r = 0; r < P; ++r) {
+        float2 Ur = tile.Us[r * kCorApertures + al];
+        float2 Fr = F[r], Gr = G[r];
+        float Qr = Q[r], C1r = C1[r], C3r = C3[r];
+        float nr = (float)((c.K - r + P - 1) / P);
+        if (seen && r == rl) {  // the lobe bin saw cot = t: take it out
+            Ur -= ul;
+            Fr.x = fmaf(-t, ul.x, Fr.x);
+            Fr.y = fmaf(-t, ul.y, Fr.y);
+            Qr -= t2;
+            nr -= 1.0f;
+            if (GRAD) {
+                Gr.x = fmaf(-t2, ul.x, Gr.x);
+                Gr.y = fmaf(-t2, ul.y, Gr.y);
+                C1r -= t;
+                C3r = fmaf(-t2, t, C3r);
+            }
+        }
+        const float2 nu = make_float2(sb[r] * cb[r], -sb[r] * sb[r]);
+        const float nu2 = sb[r] * sb[r];
+        const float2 Fm = make_float2(Fr.x + Ur.y, Fr.y - Ur.x);  // F - i U
+        out.g0 += c.inv_S * cmulc(nu, Fm);
+        out.nrm += nu2 * invS2 * (nr + Qr);
+        if (GRAD) {
+            const float2 qr = make_float2(fmaf(-2.0f * sb[r], sb[r], 1.0f), -2.0f * sb[r] * cb[r]);
+            out.g1 += c.inv_S * (c.pi_P * cmulc(qr, Fm) - c.pi_N * cmulc(nu, Ur + Gr));
+            out.rx += invS2 * (c.pi_P * cmulc(qr, nu).x * (nr + Qr) - c.pi_N * nu2 * (C1r + C3r));
+        }
+    }
+    return out;
+}
+
+template <int NV>
+__device__ __forceinline__ void half_wave_sum(float* v) {
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] += __shfl_xor(v[j], o, 64);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// correlate (correlate.slang:78-108) and correlate_gradient (correlate.slang:174-235).
+// Outputs per candidate (without area), in the gradient binding's column order:
+//   [corr.re, corr.im, norm, dcorr.re[W], dcorr.im[W], dnorm[W]],  W = 0, 3 or 6.
+// ---------------------------------------------------------------------------------------
+template <int P, int W>
+__global__ __launch_bounds__(256) void correlate_kernel(const Chirp c, const float* __restrict__ pos, int A,
+                                                        const float* __restrict__ ctr,
+                                                        const float* __restrict__ nrm, int S,
+                                                        const float2* __restrict__ res, int chunk,
+                                                        float* __restrict__ partial) {
+    constexpr int NV = 3 + 3 * W;
+    extern __shared__ float smem[];
+    const CorTile tile = cor_tile(c, smem, P);
+    const int blk = blockIdx.x, nblk = gridDim.x, a0 = blk * kCorApertures;
+    load_tile(c, tile, P, res, A, a0);
+    const int tid = threadIdx.x, wave = tid >> 6, half = (tid >> 5) & 1, al = tid & 31;
+    const int a = min(a0 + al, A - 1);
+    const bool aperture_ok = a0 + al < A;
+    const float px = pos[3 * a], py = pos[3 * a + 1], pz = pos[3 * a + 2];
+    const int s_begin = blockIdx.y * chunk, s_end = min(S, s_begin + chunk);
+    for (int s = s_begin + 2 * wave + half; s < s_end; s += 8) {
+        const float nx = nrm[3 * s], ny = nrm[3 * s + 1], nz = nrm[3 * s + 2];
+        Pair q = pair_geometry(c, px, py, pz, ctr[3 * s], ctr[3 * s + 1], ctr[3 * s + 2], nx, ny, nz);
+        if (!aperture_ok) { q.amp = 0.0f; q.amp_cos = 0.0f; }
+        const Sums z = pair_sums<P, (W > 0)>(c, tile, al, q);
+        const float2 corr = cmulc(q.rot, z.g0);  // sum conj(D rot) res
+        float v[NV];
+        v[0] = q.amp * corr.x;
+        v[1] = q.amp * corr.y;
+        v[2] = q.amp * q.amp * z.nrm;
+        if (W > 0) {
+            const float2 corr_x = cmulc(q.rot, z.g1);
+            float da[3];
+            amp_gradient(q, nx, ny, nz, da);
+            const float u[3] = {q.ux, q.uy, q.uz};
+#pragma unroll
+            for (int j = 0; j < W; ++j) {
+                float2 dc;
+                float dn;
+                if (j < 3) {
+                    const float2 icorr = make_float2(-corr.y, corr.x);
+                    dc = da[j] * corr + (q.amp * u[j]) * (c.k0pm_f * corr_x + q.rate * icorr);
+                    dn = 2.0f * q.amp * (da[j] * z.nrm + q.amp * c.k0pm_f * u[j] * z.rx);
+                } else {
+                    const float d = q.amp_cos * u[j - 3];
+                    dc = d * corr;
+                    dn = 2.0f * q.amp * d * z.nrm;
+                }
+                v[3 + j] = dc.x;
+                v[3 + W + j] = dc.y;
+                v[3 + 2 * W + j] = dn;
+            }
+        }
+        half_wave_sum<NV>(v);
+        if (al == 0) {
+            float* dst = partial + ((size_t)s * nblk + blk) * NV;
+#pragma unroll
+            for (int j = 0; j < NV; ++j) dst[j] = v[j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// correlate_orientations (correlate.slang:110-169): distance, phase and the bin sums are
+// shared by the own normal (j = 0) and B bank normals; only the amplitude changes.
+// With a range table the bin sum is replaced by a 4-tap Catmull-Rom interpolation of
+// the FFT-oversampled correlation of each aperture (fmcw.py:337-355).
+// Outputs per candidate: [1 + B][corr.re, corr.im, norm].
+// ---------------------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(256) void orientations_kernel(
+    const Chirp c, const float* __restrict__ pos, int A, const float* __restrict__ ctr, const float* __restrict__ nrm,
+    const float* __restrict__ bank, int B, int S, const float2* __restrict__ res, const float2* __restrict__ table,
+    const float* __restrict__ norm_table, int oversample, int chunk, float* __restrict__ partial) {
+    extern __shared__ float smem[];
+    const CorTile tile = cor_tile(c, smem, P);
+    const int blk = blockIdx.x, nblk = gridDim.x, a0 = blk * kCorApertures;
+    if (!table) load_tile(c, tile, P, res, A, a0);
+    const int tid = threadIdx.x, wave = tid >> 6, half = (tid >> 5) & 1, al = tid & 31;
+    const int a = min(a0 + al, A - 1);
+    const bool aperture_ok = a0 + al < A;
+    const float px = pos[3 * a], py = pos[3 * a + 1], pz = pos[3 * a + 2];
+    const int s_begin = blockIdx.y * chunk, s_end = min(S, s_begin + chunk);
+    const int NV = 3 * (1 + B);
+    for (int s = s_begin + 2 * wave + half; s < s_end; s += 8) {
+        const float nx = nrm[3 * s], ny = nrm[3 * s + 1], nz = nrm[3 * s + 2];
+        const Pair q = pair_geometry(c, px, py, pz, ctr[3 * s], ctr[3 * s + 1], ctr[3 * s + 2], nx, ny, nz);
+        float2 corr;
+        float zn;
+        if (table) {
+            const int size = c.N * oversample;
+            const double x = (double)q.r * c.k0pm * oversample;
+            const double fl = floor(x);
+            const float tt = (float)(x - fl);
+            const int base = (int)fl;
+            const float t2 = tt * tt, t3 = t2 * tt;
+            const float wts[4] = {-0.5f * tt + t2 - 0.5f * t3, 1.0f - 2.5f * t2 + 1.5f * t3,
+                                  0.5f * tt + 2.0f * t2 - 1.5f * t3, -0.5f * t2 + 0.5f * t3};
+            float2 value = make_float2(0.f, 0.f);
+            zn = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int k = (base + j - 1) % size;
+                if (k < 0) k += size;
+                value += wts[j] * table[(size_t)a * size + k];
+                zn = fmaf(wts[j], norm_table[k], zn);
+            }
+            corr = cmul(value, make_float2(q.rot.x, -q.rot.y));
+            zn = fmaxf(zn, 0.0f);
+        } else {
+            const Sums z = pair_sums<P, false>(c, tile, al, q);
+            corr = cmulc(q.rot, z.g0);
+            zn = z.nrm;
+        }
+        float* dst = partial + ((size_t)s * nblk + blk) * NV;
+        for (int j = 0; j <= B; ++j) {
+            const float mx = j == 0 ? nx : bank[3 * (j - 1)];
+            const float my = j == 0 ? ny : bank[3 * (j - 1) + 1];
+            const float mz = j == 0 ? nz : bank[3 * (j - 1) + 2];
+            const float cosine = q.ux * mx + q.uy * my + q.uz * mz;
+            float amp = 0.0f;
+            if (cosine > 0.0f && aperture_ok) {
+                float f = 1.0f;
+                if (c.fresnel_n > 1.0f) {
+                    const float root = sqrtf(c.fresnel_n * c.fresnel_n - 1.0f + cosine * cosine);
+                    f = (root - cosine) / (root + cosine);
+                }
+                amp = c.lam * q.inv_r * (1.0f / (8.0f * kPi)) * cosine * cosine * f;
+            }
+            float v[3] = {amp * corr.x, amp * corr.y, amp * amp * zn};
+            half_wave_sum<3>(v);
+            if (al == 0) {
+                dst[3 * j] = v[0];
+                dst[3 * j + 1] = v[1];
+                dst[3 * j + 2] = v[2];
+            }
+        }
+    }
+}
+
+// out[s, f] = sum over blocks of partial[s, blk, f] in float64, fixed order (one wave per row).
+__global__ __launch_bounds__(256) void reduce_partials(const float* __restrict__ partial, int rows, int nblk,
+                                                       int width, float* __restrict__ out) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (row >= rows) return;
+    for (int f = 0; f < width; ++f) {
+        double acc = 0.0;
+        for (int b = lane; b < nblk; b += 64) acc += partial[((size_t)row * nblk + b) * width + f];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) out[(size_t)row * width + f] = (float)acc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+CorShape cor_shape(int A, int S) {
+    CorShape v;
+    v.nblk = (A + kCorApertures - 1) / kCorApertures;
+    int groups = (4096 + v.nblk - 1) / v.nblk;
+    const int max_groups = (S + 7) / 8;
+    groups = groups < 1 ? 1 : (groups > max_groups ? max_groups : groups);
+    const int chunk = (S + groups - 1) / groups;
+    v.chunk = (chunk + 7) / 8 * 8;
+    v.groups = (S + v.chunk - 1) / v.chunk;
+    return v;
+}
+
+size_t cor_smem(const Chirp& c) {
+    return sizeof(float) * 2 * c.N + sizeof(float2) * ((size_t)c.Kp * kCorApertures + 8 * kCorApertures);
+}
+
+template <int P, int W>
+static hipError_t launch_cor_pw(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, int S,
+                                const float2* res, float* partial, float* out, hipStream_t st) {
+    const CorShape v = cor_shape(A, S);
+    const size_t smem = cor_smem(c);
+    hipError_t e = allow_lds((const void*)correlate_kernel<P, W>, smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((correlate_kernel<P, W>), dim3(v.nblk, v.groups), dim3(256), smem, st, c, pos, A, ctr, nrm, S,
+                       res, v.chunk, partial);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_partials, dim3((S + 3) / 4), dim3(256), 0, st, partial, S, v.nblk, 3 + 3 * W, out);
+    return hipGetLastError();
+}
+
+template <int P>
+static hipError_t launch_cor_p(const Chirp& c, int W, const float* pos, int A, const float* ctr, const float* nrm,
+                               int S, const float2* res, float* partial, float* out, hipStream_t st) {
+    switch (W) {
+        case 0: return launch_cor_pw<P, 0>(c, pos, A, ctr, nrm, S, res, partial, out, st);
+        case 3: return launch_cor_pw<P, 3>(c, pos, A, ctr, nrm, S, res, partial, out, st);
+        case 6: return launch_cor_pw<P, 6>(c, pos, A, ctr, nrm, S, res, partial, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_correlate(const Chirp& c, int W, const float* pos, int A, const float* ctr, const float* nrm, int S,
+                            const float2* res, float* partial, float* out, hipStream_t st) {
+    switch (c.P) {
+        case 1: return launch_cor_p<1>(c, W, pos, A, ctr, nrm, S, res, partial, out, st);
+        case 2: return launch_cor_p<2>(c, W, pos, A, ctr, nrm, S, res, partial, out, st);
+        case 4: return launch_cor_p<4>(c, W, pos, A, ctr, nrm, S, res, partial, out, st);
+        case 8: return launch_cor_p<8>(c, W, pos, A, ctr, nrm, S, res, partial, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int P>
+static hipError_t launch_orient_p(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                  const float* bank, int B, int S, const float2* res, const float2* table,
+                                  const float* norm_table, int oversample, float* partial, float* out,
+                                  hipStream_t st) {
+    const CorShape v = cor_shape(A, S);
+    const size_t smem = table ? 0 : cor_smem(c);
+    hipError_t e = allow_lds((const void*)orientations_kernel<P>, smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((orientations_kernel<P>), dim3(v.nblk, v.groups), dim3(256), smem, st, c, pos, A, ctr, nrm, bank,
+                       B, S, res, table, norm_table, oversample, v.chunk, partial);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(reduce_partials, dim3((S + 3) / 4), dim3(256), 0, st, partial, S, v.nblk, 3 * (1 + B), out);
+    return hipGetLastError();
+}
+
+hipError_t launch_orientations(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                               const float* bank, int B, int S, const float2* res, const float2* table,
+                               const float* norm_table, int oversample, float* partial, float* out, hipStream_t st) {
+    switch (c.P) {
+        case 1: return launch_orient_p<1>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, partial, out, st);
+        case 2: return launch_orient_p<2>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, partial, out, st);
+        case 4: return launch_orient_p<4>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, partial, out, st);
+        case 8: return launch_orient_p<8>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, partial, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace dsplat

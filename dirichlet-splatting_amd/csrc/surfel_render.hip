@@ -64,7 +64,7 @@ __global__ __launch_bounds__(256) void render_kernel(const Chirp c, const float*
                     off = c.N - q.lobe;
                     t = tan_small(c.pi_N * dl);
                     float sb[P], cb[P];
-                    class_angles<P>(dl, sb, cb);
+                    class_angles<P>(dl, q.lobe, sb, cb);
                     const float2 cw = (q.amp * c.inv_S) * cmul(w[s], q.rot);
 #pragma unroll
                     for (int r = 0; r < P; ++r) gr[r] = cmul(cw, make_float2(sb[r] * cb[r], -sb[r] * sb[r]));
@@ -97,6 +97,136 @@ __global__ __launch_bounds__(256) void render_kernel(const Chirp c, const float*
         for (int j = 0; j < KB; ++j) {
             const int k = g + 16 * j;
             if (k < c.K) out[(size_t)a * c.K + k] = make_float2(acc[j].x - cc.y, acc[j].y + cc.x);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// JVP (surfel.slang:146-221): the forward's layout with per-class coefficients
+//   term = h1_r (cot + i) + h2_r (1 + cot^2),  h1_r = (e1 nu_r + (pi/P) e2 q_r) / S,
+//   h2_r = -(pi/N) e2 nu_r / S,   e1 = rot ((damp + i amp dphase) w + amp dw),
+//   e2 = -amp k0pm dr rot w.
+// The derivative table has T_0 = 0, so the lobe bin first receives
+// h1 (t + i) + h2 (1 + t^2); the thread owning that bin replaces it by the direct
+// e1 D(-delta) + e2 D'(-delta), in pair order in its own LDS slots: deterministic.
+// Per (a, k, s): one LDS table read, add, fma, rcp, mul, mul, 4 fma.
+// ---------------------------------------------------------------------------------------
+template <int P>
+struct JvpShape {
+    static constexpr int SC = 64 / P;
+};
+
+template <int P, int KB>
+__global__ __launch_bounds__(256) void jvp_kernel(const Chirp c, const float* __restrict__ pos, int A,
+                                                  const float* __restrict__ ctr,
+                                                  const float* __restrict__ nrm,
+                                                  const float2* __restrict__ w, const float* __restrict__ dctr,
+                                                  const float* __restrict__ dnrm, const float2* __restrict__ dw,
+                                                  int S, float2* __restrict__ out) {
+    constexpr int AT = kFwdApertures;
+    constexpr int SC = JvpShape<P>::SC;
+    extern __shared__ float smem[];
+    float* T = smem;                                                // [2N]
+    float2* corr = reinterpret_cast<float2*>(T + 2 * c.N);          // [AT][Kp] lobe corrections
+    float2* p_h = corr + AT * c.Kp;                                 // [AT * SC][2P]: h1_r, h2_r
+    float2* p_fix = p_h + AT * SC * 2 * P;                          // [AT * SC] lobe correction
+    int* p_off = reinterpret_cast<int*>(p_fix + AT * SC);           // [AT * SC]
+    int* p_lobe = p_off + AT * SC;                                  // [AT * SC] lobe, -1 if unobserved
+    float* p_t = reinterpret_cast<float*>(p_lobe + AT * SC);        // [AT * SC]
+
+    const int tid = threadIdx.x;
+    const int al = tid >> 4, g = tid & 15, cls = g % P;
+    const int a0 = blockIdx.x * AT;
+    for (int i = tid; i < 2 * c.N; i += 256) T[i] = cot_entry(i, c.N, 0.0f);
+    for (int i = tid; i < AT * c.Kp; i += 256) corr[i] = make_float2(0.f, 0.f);
+
+    float2 acc[KB];
+#pragma unroll
+    for (int j = 0; j < KB; ++j) acc[j] = make_float2(0.f, 0.f);
+    float2 cc = make_float2(0.f, 0.f);
+
+    for (int s0 = 0; s0 < S; s0 += SC) {
+        __syncthreads();
+        for (int qi = tid; qi < AT * SC; qi += 256) {
+            const int ap = qi / SC, sp = qi - ap * SC;
+            const int a = min(a0 + ap, A - 1), s = s0 + sp;
+            int off = c.N, lobe = -1;
+            float t = 0.5f;
+            float2 h1[P], h2[P], fix = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int r = 0; r < P; ++r) { h1[r] = make_float2(0.f, 0.f); h2[r] = make_float2(0.f, 0.f); }
+            if (s < S) {
+                const float nx = nrm[3 * s], ny = nrm[3 * s + 1], nz = nrm[3 * s + 2];
+                const Pair q = pair_geometry(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], ctr[3 * s],
+                                             ctr[3 * s + 1], ctr[3 * s + 2], nx, ny, nz);
+                if (q.amp != 0.0f) {
+                    const float dcx = dctr[3 * s], dcy = dctr[3 * s + 1], dcz = dctr[3 * s + 2];
+                    const float dr = -(q.ux * dcx + q.uy * dcy + q.uz * dcz);
+                    const float dcos = ((q.cosine * q.ux - nx) * dcx + (q.cosine * q.uy - ny) * dcy +
+                                        (q.cosine * q.uz - nz) * dcz) * q.inv_r +
+                                       q.ux * dnrm[3 * s] + q.uy * dnrm[3 * s + 1] + q.uz * dnrm[3 * s + 2];
+                    const float damp = -q.amp * dr * q.inv_r + q.amp_cos * dcos;
+                    const float2 ws = w[s];
+                    const float2 e1 = cmul(q.rot, cmul(make_float2(damp, q.amp * q.rate * dr), ws) + q.amp * dw[s]);
+                    const float2 e2 = (-q.amp * c.k0pm_f * dr) * cmul(q.rot, ws);
+                    off = c.N - q.lobe;
+                    t = tan_small(c.pi_N * q.delta);
+                    float sb[P], cb[P];
+                    class_angles<P>(q.delta, q.lobe, sb, cb);
+#pragma unroll
+                    for (int r = 0; r < P; ++r) {
+                        const float2 nu = make_float2(sb[r] * cb[r], -sb[r] * sb[r]);
+                        const float2 qr = make_float2(fmaf(-2.0f * sb[r], sb[r], 1.0f), -2.0f * sb[r] * cb[r]);
+                        h1[r] = c.inv_S * (cmul(e1, nu) + c.pi_P * cmul(e2, qr));
+                        h2[r] = (-c.pi_N * c.inv_S) * cmul(e2, nu);
+                    }
+                    if (q.lobe < c.K) {
+                        lobe = q.lobe;
+                        float2 D, Dp;
+                        lobe_direct(c, q.delta, &D, &Dp);
+                        const int rl = q.lobe % P;
+                        float2 h1l = h1[0], h2l = h2[0];
+#pragma unroll
+                        for (int r = 1; r < P; ++r)
+                            if (r == rl) { h1l = h1[r]; h2l = h2[r]; }
+                        const float2 seen = cmul(h1l, make_float2(t, 1.0f)) + fmaf(t, t, 1.0f) * h2l;
+                        fix = cmul(e1, D) + cmul(e2, Dp) - seen;
+                    }
+                }
+            }
+            p_off[qi] = off;
+            p_lobe[qi] = lobe;
+            p_t[qi] = t;
+            p_fix[qi] = fix;
+#pragma unroll
+            for (int r = 0; r < P; ++r) { p_h[(qi * P + r) * 2] = h1[r]; p_h[(qi * P + r) * 2 + 1] = h2[r]; }
+        }
+        __syncthreads();
+        const int ns = min(SC, S - s0);
+        const int row = al * SC;
+        for (int sp = 0; sp < ns; ++sp) {
+            const int qi = row + sp;
+            const int base = p_off[qi] + g;
+            const float t = p_t[qi];
+            const float2 h1 = p_h[(qi * P + cls) * 2], h2 = p_h[(qi * P + cls) * 2 + 1];
+            cc += make_float2(h2.x - h1.y, h2.y + h1.x);  // i h1 + h2
+            const int lobe = p_lobe[qi];
+            if (lobe >= 0 && (lobe & 15) == g) corr[al * c.Kp + lobe] += p_fix[qi];
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                const float cot = cot_bin(T[base + 16 * j], t);
+                const float c2 = cot * cot;
+                acc[j].x = fmaf(h1.x, cot, fmaf(h2.x, c2, acc[j].x));
+                acc[j].y = fmaf(h1.y, cot, fmaf(h2.y, c2, acc[j].y));
+            }
+        }
+    }
+    const int a = a0 + al;
+    if (a < A) {
+#pragma unroll
+        for (int j = 0; j < KB; ++j) {
+            const int k = g + 16 * j;
+            if (k < c.K) out[(size_t)a * c.K + k] = acc[j] + cc + corr[al * c.Kp + k];
         }
     }
 }
@@ -180,7 +310,7 @@ __global__ __launch_bounds__(256) void vjp_kernel(const Chirp c, const float* __
         lobe_direct(c, q.delta, &D, &Dp);
         float2 g0 = cmulc(D, ul), g1 = cmulc(Dp, ul);
         float sb[P], cb[P];
-        class_angles<P>(q.delta, sb, cb);
+        class_angles<P>(q.delta, q.lobe, sb, cb);
         const float t2 = t * t;
 #pragma unroll
         for (int r = 0; r < P; ++r) {
@@ -290,6 +420,47 @@ hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* c
         case 2: return launch_render_p<2>(c, pos, A, ctr, nrm, w, S, out, st);
         case 4: return launch_render_p<4>(c, pos, A, ctr, nrm, w, S, out, st);
         case 8: return launch_render_p<8>(c, pos, A, ctr, nrm, w, S, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int P>
+static hipError_t launch_jvp_p(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                               const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S,
+                               float2* out, hipStream_t st) {
+    constexpr int SC = JvpShape<P>::SC;
+    const int KB = c.Kp / 16;
+    const size_t smem = sizeof(float) * 2 * c.N + sizeof(float2) * (size_t)kFwdApertures * c.Kp +
+                        (size_t)kFwdApertures * SC * (sizeof(float2) * (2 * P + 1) + 2 * sizeof(int) + sizeof(float));
+    const dim3 grid((A + kFwdApertures - 1) / kFwdApertures);
+#define DSPLAT_JVP(KBV)                                                                                      \
+    case KBV: {                                                                                              \
+        hipError_t e = allow_lds((const void*)jvp_kernel<P, KBV>, smem);                                     \
+        if (e != hipSuccess) return e;                                                                       \
+        hipLaunchKernelGGL((jvp_kernel<P, KBV>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr, dnrm, dw, \
+                           S, out);                                                                          \
+        break;                                                                                               \
+    }
+    switch (KB) {
+        DSPLAT_JVP(1)
+        DSPLAT_JVP(2)
+        DSPLAT_JVP(4)
+        DSPLAT_JVP(8)
+        DSPLAT_JVP(16)
+        DSPLAT_JVP(32)
+        default: return hipErrorInvalidValue;
+    }
+#undef DSPLAT_JVP
+    return hipGetLastError();
+}
+
+hipError_t launch_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, const float2* w,
+                      const float* dctr, const float* dnrm, const float2* dw, int S, float2* out, hipStream_t st) {
+    switch (c.P) {
+        case 1: return launch_jvp_p<1>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
+        case 2: return launch_jvp_p<2>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
+        case 4: return launch_jvp_p<4>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
+        case 8: return launch_jvp_p<8>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
         default: return hipErrorInvalidValue;
     }
 }

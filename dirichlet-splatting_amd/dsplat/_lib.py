@@ -42,10 +42,27 @@ SIGNATURES = {
     "dsplat_version": (ctypes.c_char_p, []),
     "dsplat_error_string": (ctypes.c_char_p, [c_int]),
     "dsplat_surfel_render": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
+    "dsplat_surfel_jvp": (
+        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr]
+    ),
     "dsplat_surfel_vjp_workspace": (c_size, [c_int, c_int]),
     "dsplat_surfel_vjp": (
         c_int,
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
+    ),
+    "dsplat_surfel_design": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr]),
+    "dsplat_surfel_jacobian": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr]),
+    "dsplat_surfel_correlate_workspace": (c_size, [c_int, c_int, c_int]),
+    "dsplat_surfel_correlate": (
+        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
+    ),
+    "dsplat_surfel_correlate_gradient": (
+        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_size, c_ptr]
+    ),
+    "dsplat_surfel_correlate_orientations": (
+        c_int,
+        [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_size,
+         c_ptr],
     ),
 }
 

@@ -79,15 +79,55 @@ def vjp(config, positions, centers, normals, weights, upstream, fresnel_n):
     return gc, gn, torch.view_as_complex(gw)
 
 
+def jvp(config, positions, centers, normals, weights, delta_centers, delta_normals, delta_weights, fresnel_n):
+    """Directional derivative of the profiles [A, K] (kernels/surfel.py:140-164)."""
+    _check(positions, centers, normals)
+    positions, centers, normals = positions.contiguous(), centers.contiguous(), normals.contiguous()
+    out = centers.new_empty((len(positions), config.num_range_bins, 2))
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_surfel_jvp",
+        _lib.ctypes.byref(cfg),
+        _lib.ptr(positions),
+        len(positions),
+        _lib.ptr(centers),
+        _lib.ptr(normals),
+        _lib.ptr(interleaved(weights)),
+        _lib.ptr(delta_centers.to(torch.float32).contiguous()),
+        _lib.ptr(delta_normals.to(torch.float32).contiguous()),
+        _lib.ptr(interleaved(delta_weights)),
+        len(centers),
+        _lib.ptr(out),
+        _lib.stream(centers.device),
+    )
+    return torch.view_as_complex(out)
+
+
 class _Render(torch.autograd.Function):
     """Forward = render; backward = the VJP recomputed from the saved inputs (the forward
-    is never stored), as kernels/surfel.py:167-183."""
+    is never stored), as kernels/surfel.py:167-183. jvp gives forward-mode AD through
+    the JVP kernel (the reference defines no forward mode)."""
 
     @staticmethod
     def forward(ctx, centers, normals, weights, positions, config, fresnel_n):
         ctx.save_for_backward(centers, normals, weights, positions)
+        ctx.save_for_forward(centers, normals, weights, positions)
         ctx.config, ctx.fresnel_n = config, fresnel_n
         return _profiles(config, positions, centers, normals, weights, fresnel_n)
+
+    @staticmethod
+    def jvp(ctx, dcenters, dnormals, dweights, _dpositions, _config, _fresnel):
+        # sensor positions carry no tangent: the scan geometry is data, not a parameter
+        centers, normals, weights, positions = ctx.saved_tensors
+        zero = torch.zeros_like(centers)
+        dcenters = zero if dcenters is None else dcenters
+        dnormals = zero if dnormals is None else dnormals
+        dw = torch.zeros_like(weights) if dweights is None else dweights
+        out = jvp(
+            ctx.config, positions, centers, normals, torch.view_as_complex(weights), dcenters, dnormals,
+            torch.view_as_complex(dw.contiguous()), ctx.fresnel_n,
+        )  # fmt: skip
+        return torch.view_as_real(out)
 
     @staticmethod
     def backward(ctx, grad):
@@ -105,3 +145,147 @@ def render(config, positions, centers, normals, weights, fresnel_n):
         centers.contiguous(), normals.contiguous(), interleaved(weights), positions, config, fresnel_n
     )
     return torch.view_as_complex(output)
+
+
+def design(config, positions, centers, normals, areas, fresnel_n, real):
+    """Dictionary rows of whole apertures, including sqrt(area) (kernels/surfel.py:199-220).
+
+    Complex rows are [A * K, S]. real gives [2 * A * K, S] ordered bin by bin, row
+    (2 k + part) * A + a, so one product projects every aperture onto a row basis.
+    """
+    _check(positions, centers, normals)
+    count, rows = len(centers), len(positions) * config.num_range_bins
+    output = centers.new_empty((2 * rows, count) if real else (rows, 2 * count))
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_surfel_design",
+        _lib.ctypes.byref(cfg),
+        _lib.ptr(positions.contiguous()),
+        len(positions),
+        _lib.ptr(centers.contiguous()),
+        _lib.ptr(normals.contiguous()),
+        _lib.ptr(areas.to(torch.float32).contiguous()),
+        count,
+        1 if real else 0,
+        _lib.ptr(output),
+        _lib.stream(centers.device),
+    )
+    return output if real else torch.view_as_complex(output.view(rows, count, 2))
+
+
+def jacobian(config, positions, centers, normals, fresnel_n):
+    """Columns [A, K, S] and their center and normal derivatives [A, K, S, 3], without area
+    (kernels/surfel.py:223-239)."""
+    _check(positions, centers, normals)
+    out = centers.new_empty((len(positions), config.num_range_bins, len(centers), 2))
+    jac = centers.new_empty((*out.shape[:-1], 6, 2))
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_surfel_jacobian",
+        _lib.ctypes.byref(cfg),
+        _lib.ptr(positions.contiguous()),
+        len(positions),
+        _lib.ptr(centers.contiguous()),
+        _lib.ptr(normals.contiguous()),
+        len(centers),
+        _lib.ptr(out),
+        _lib.ptr(jac),
+        _lib.stream(centers.device),
+    )
+    jac = torch.view_as_complex(jac)
+    return torch.view_as_complex(out), jac[..., :3], jac[..., 3:]
+
+
+def _workspace(like, apertures, count, width):
+    size = _lib.library().dsplat_surfel_correlate_workspace(apertures, count, width)
+    return like.new_empty(max(1, size // 4)), size
+
+
+def correlate(config, positions, centers, normals, residual, fresnel_n):
+    """conj(column) . residual and |column|^2 per candidate, without area (kernels/surfel.py:242-260)."""
+    _check(positions, centers, normals)
+    count = len(centers)
+    output = centers.new_empty((count, 3))
+    workspace, size = _workspace(centers, len(positions), count, 3)
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_surfel_correlate",
+        _lib.ctypes.byref(cfg),
+        _lib.ptr(positions.contiguous()),
+        len(positions),
+        _lib.ptr(centers.contiguous()),
+        _lib.ptr(normals.contiguous()),
+        count,
+        _lib.ptr(interleaved(residual.reshape(len(positions), config.num_range_bins))),
+        _lib.ptr(output),
+        _lib.ptr(workspace),
+        size,
+        _lib.stream(centers.device),
+    )
+    return torch.complex(output[:, 0], output[:, 1]), output[:, 2]
+
+
+def correlate_orientations(config, positions, centers, normals, bank, residual, fresnel_n, table=None):
+    """Correlations and norms [S, 1 + bank] for the current normal and every bank normal.
+
+    table, an fmcw.RangeTable, replaces the per-bin sum by FFT-interpolated range
+    correlations; it only ranks proposals. Without area (kernels/surfel.py:263-290).
+    """
+    _check(positions, centers, normals)
+    count, extra = len(centers), len(bank)
+    output = centers.new_empty((count, extra + 1, 3))
+    workspace, size = _workspace(centers, len(positions), count, 3 * (extra + 1))
+    cfg = _lib.fmcw(config, fresnel_n)
+    res = None if table is not None else interleaved(residual.reshape(len(positions), config.num_range_bins))
+    _lib.call(
+        "dsplat_surfel_correlate_orientations",
+        _lib.ctypes.byref(cfg),
+        _lib.ptr(positions.contiguous()),
+        len(positions),
+        _lib.ptr(centers.contiguous()),
+        _lib.ptr(normals.contiguous()),
+        count,
+        _lib.ptr(bank.to(torch.float32).contiguous()) if extra else None,
+        extra,
+        _lib.ptr(res),
+        None if table is None else _lib.ptr(table.correlation.contiguous()),
+        None if table is None else _lib.ptr(table.norm.contiguous()),
+        1 if table is None else int(table.oversample),
+        _lib.ptr(output),
+        _lib.ptr(workspace),
+        size,
+        _lib.stream(centers.device),
+    )
+    return torch.complex(output[..., 0], output[..., 1]), output[..., 2]
+
+
+def correlate_gradient(config, positions, centers, normals, residual, fresnel_n, include_normals):
+    """Correlation, norm and their center (and normal) derivatives [S, 3 or 6], without area
+    (kernels/surfel.py:293-315)."""
+    _check(positions, centers, normals)
+    width = 6 if include_normals else 3
+    count = len(centers)
+    output = centers.new_empty((count, 3 + 3 * width))
+    workspace, size = _workspace(centers, len(positions), count, 3 + 3 * width)
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_surfel_correlate_gradient",
+        _lib.ctypes.byref(cfg),
+        _lib.ptr(positions.contiguous()),
+        len(positions),
+        _lib.ptr(centers.contiguous()),
+        _lib.ptr(normals.contiguous()),
+        count,
+        _lib.ptr(interleaved(residual.reshape(len(positions), config.num_range_bins))),
+        1 if include_normals else 0,
+        _lib.ptr(output),
+        _lib.ptr(workspace),
+        size,
+        _lib.stream(centers.device),
+    )
+    return (
+        torch.complex(output[:, 0], output[:, 1]),
+        output[:, 2],
+        torch.complex(output[:, 3 : 3 + width], output[:, 3 + width : 3 + 2 * width]),
+        output[:, 3 + 2 * width :],
+    )

@@ -22,6 +22,18 @@ def render(config, positions, params, coefficients, fresnel_n=0.0):
     return kernel.render(config, positions, params["centers"], params["normals"], weights, fresnel_n)
 
 
+def jvp(config, positions, params, coefficients, tangent, delta_coefficients, fresnel_n=0.0):
+    """Change of the profiles along tangent (centers, normals, areas) and delta_coefficients
+    (surfel.py:53-61)."""
+    scale = params["areas"].sqrt()
+    weights = coefficients * scale
+    delta_weights = delta_coefficients * scale + coefficients * tangent["areas"] / (2 * scale)
+    return kernel.jvp(
+        config, positions, params["centers"], params["normals"], weights,
+        tangent["centers"], tangent["normals"], delta_weights, fresnel_n,
+    )  # fmt: skip
+
+
 def vjp(config, positions, params, coefficients, upstream, fresnel_n=0.0):
     """Gradients of Re(conj(upstream) . profiles): per parameter, and for the coefficients
     (surfel.py:64-77)."""

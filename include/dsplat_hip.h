@@ -50,6 +50,13 @@ int dsplat_surfel_render(const dsplat_fmcw* cfg, const float* positions, int32_t
                          const float* centers, const float* normals, const float* weights,
                          int32_t num_surfels, float* out, void* stream);
 
+/* Directional derivative out[A, K] of the profiles along tangents of the centers [S,3],
+ * normals [S,3] and complex weights [S]. Replaces kernels/surfel.py:140-164 (jvp: jvp_bins). */
+int dsplat_surfel_jvp(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                      const float* centers, const float* normals, const float* weights,
+                      const float* delta_centers, const float* delta_normals, const float* delta_weights,
+                      int32_t num_surfels, float* out, void* stream);
+
 /* Workspace bytes dsplat_surfel_vjp needs for A apertures and S surfels. */
 size_t dsplat_surfel_vjp_workspace(int32_t num_apertures, int32_t num_surfels);
 
@@ -62,6 +69,54 @@ int dsplat_surfel_vjp(const dsplat_fmcw* cfg, const float* positions, int32_t nu
                       int32_t num_surfels, const float* upstream, float* grad_centers,
                       float* grad_normals, float* grad_weights, void* workspace,
                       size_t workspace_bytes, void* stream);
+
+/* Dictionary rows of whole apertures including sqrt(area): complex [A*K, S]
+ * (row a*K + k) when real_layout == 0, else real [2*K*A, S] with row (2k + part)*A + a.
+ * areas [S]. Replaces kernels/surfel.py:199-220 (design: design_bins). */
+int dsplat_surfel_design(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                         const float* centers, const float* normals, const float* areas,
+                         int32_t num_surfels, int32_t real_layout, float* out, void* stream);
+
+/* Explicit columns [A,K,S] complex and their derivatives by center xyz and normal xyz
+ * [A,K,S,6] complex, without area. Replaces kernels/surfel.py:223-239
+ * (jacobian: surfel_design_jacobian). */
+int dsplat_surfel_jacobian(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                           const float* centers, const float* normals, int32_t num_surfels,
+                           float* columns, float* derivatives, void* stream);
+
+/* Workspace bytes of the correlation entry points: width = 3 (correlate),
+ * 3 + 3w (gradient, w = 3 or 6) or 3 (1 + B) (orientations). */
+size_t dsplat_surfel_correlate_workspace(int32_t num_apertures, int32_t num_surfels, int32_t width);
+
+/* Per candidate: conj(column) . residual and |column|^2, without area; out [S, 3] =
+ * (corr.re, corr.im, norm). residual [A,K] complex. Deterministic (no atomics).
+ * Replaces kernels/surfel.py:242-260 (correlate: surfel_correlate). */
+int dsplat_surfel_correlate(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                            const float* centers, const float* normals, int32_t num_surfels,
+                            const float* residual, float* out, void* workspace,
+                            size_t workspace_bytes, void* stream);
+
+/* Correlation, norm and their center (and normal) derivatives: out [S, 3 + 3w] =
+ * (corr.re, corr.im, norm, dcorr.re[w], dcorr.im[w], dnorm[w]), w = 6 with
+ * include_normals else 3. Replaces kernels/surfel.py:293-315
+ * (correlate_gradient: surfel_correlate_gradient). */
+int dsplat_surfel_correlate_gradient(const dsplat_fmcw* cfg, const float* positions,
+                                     int32_t num_apertures, const float* centers,
+                                     const float* normals, int32_t num_surfels,
+                                     const float* residual, int32_t include_normals, float* out,
+                                     void* workspace, size_t workspace_bytes, void* stream);
+
+/* Correlations and norms for each surfel's own normal and B bank normals: out
+ * [S, 1 + B, 3]. With range_table [A, n_fft*oversample] complex and norm_table
+ * [n_fft*oversample], the bin sum is a 4-tap Catmull-Rom lookup (residual unused).
+ * Replaces kernels/surfel.py:263-290 (correlate_orientations). */
+int dsplat_surfel_correlate_orientations(const dsplat_fmcw* cfg, const float* positions,
+                                         int32_t num_apertures, const float* centers,
+                                         const float* normals, int32_t num_surfels,
+                                         const float* bank, int32_t num_normals,
+                                         const float* residual, const float* range_table,
+                                         const float* norm_table, int32_t oversample, float* out,
+                                         void* workspace, size_t workspace_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -136,7 +136,7 @@ def render(positions, centers, normals, weights, cfg, fresnel_n=0.0, chunk=64):
     return np.concatenate(out)
 
 
-def jvp(positions, centers, normals, weights, dcenters, dnormals, dweights, cfg, fresnel_n=0.0, chunk=32):
+def jvp(positions, centers, normals, weights, dcenters, dnormals, dweights, cfg, fresnel_n=0.0, chunk=8):
     """Directional derivative [A, K] (`surfel.slang:146-221`)."""
     w = np.asarray(weights, np.complex128)
     dw = np.asarray(dweights, np.complex128)

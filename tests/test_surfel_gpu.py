@@ -184,3 +184,119 @@ def test_full_size_render_and_vjp():
     errs = [rel_l2(g.cpu().numpy(), e) for g, e in zip(sub, want, strict=True)]
     say("full vjp subset rel L2 %s" % ["%.3e" % e for e in errs])
     assert max(errs) < TOL
+
+
+def run_jvp(cfg, positions, centers, normals, weights, dc, dn, dw, fresnel_n=0.0):
+    from dsplat.kernels import surfel
+
+    out = surfel.jvp(
+        cfg, dev(positions, torch.float32), dev(centers, torch.float32), dev(normals, torch.float32),
+        dev(weights, torch.complex64), dev(dc, torch.float32), dev(dn, torch.float32),
+        dev(dw, torch.complex64), fresnel_n,
+    )  # fmt: skip
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("pad,fresnel_n", [(2, 0.0), (2, 1.6), (4, 0.0), (1, 0.0), (8, 1.6)])
+def test_jvp_matches_the_oracle(pad, fresnel_n):
+    from dsplat import FMCWConfig
+
+    cfg = FMCWConfig(pad_factor=pad)
+    positions = aperture_grid(8)
+    p = surfels(10, seed=6)
+    rng = np.random.default_rng(10 + pad)
+    w = (rng.standard_normal(10) + 1j * rng.standard_normal(10)) * 1e-2
+    dc, dn = f32(rng.standard_normal((10, 3)) * 1e-4), f32(rng.standard_normal((10, 3)) * 1e-2)
+    dw = (rng.standard_normal(10) + 1j * rng.standard_normal(10)) * 1e-2
+    say(f"jvp pad {pad} n {fresnel_n}")
+    got = run_jvp(cfg, positions, p["centers"], p["normals"], w, dc, dn, dw, fresnel_n)
+    want = O.jvp(positions, p["centers"], p["normals"], w, dc, dn, dw, cfg, fresnel_n)
+    err = rel_l2(got, want)
+    say(f"jvp rel L2 {err:.3e}")
+    assert err < TOL
+
+
+def test_jvp_lobe_edge_cases_and_the_adjoint_identity():
+    from dsplat import FMCWConfig
+
+    cfg = FMCWConfig()
+    spacing = cfg.range_bin_spacing_m
+    positions = np.zeros((3, 3), np.float32)
+    positions[1, 0], positions[2, 1] = 0.003, -0.002
+    ranges = [60 * spacing, 60.5 * spacing, 200.25 * spacing, 255.9 * spacing, 0.3, 70.0001 * spacing]
+    centers = f32([[0, 0, r] for r in ranges])
+    normals = f32([[0, 0, -1]] * 4 + [[0, 0, 1], [0.1, 0, -1]])
+    normals = f32(normals / np.linalg.norm(normals, axis=1, keepdims=True))
+    rng = np.random.default_rng(7)
+    w = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+    dc, dn = f32(rng.standard_normal((6, 3)) * 1e-4), f32(rng.standard_normal((6, 3)) * 1e-2)
+    dw = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+    got = run_jvp(cfg, positions, centers, normals, w, dc, dn, dw)
+    want = O.jvp(positions, centers, normals, w, dc, dn, dw, cfg)
+    assert rel_l2(got, want) < TOL
+    up = rng.standard_normal((3, 128)) + 1j * rng.standard_normal((3, 128))
+    gc, gn, gw = run_vjp(cfg, positions, centers, normals, w, up)
+    left = (up.conj() * got).real.sum()
+    right = (gc * dc).sum() + (gn * dn).sum() + (gw.conj() * dw).real.sum()
+    assert abs(left - right) <= 1e-5 * abs(left)
+
+
+def test_forward_mode_autograd_uses_the_jvp_kernel():
+    import torch.autograd.forward_ad as fwAD
+
+    from dsplat import FMCWConfig
+    from dsplat.kernels import surfel
+
+    cfg = FMCWConfig()
+    positions = dev(aperture_grid(8), torch.float32)
+    p = surfels(7, seed=8)
+    c, n = dev(p["centers"], torch.float32), dev(p["normals"], torch.float32)
+    w = torch.full((7,), 0.01 + 0.005j, dtype=torch.complex64, device="cuda")
+    g = torch.Generator("cuda").manual_seed(3)
+    dc = torch.randn(7, 3, device="cuda", generator=g) * 1e-4
+    dn = torch.randn(7, 3, device="cuda", generator=g) * 1e-2
+    dw = torch.randn(7, dtype=torch.complex64, device="cuda", generator=g) * 1e-3
+    with fwAD.dual_level():
+        out = surfel.render(cfg, positions, fwAD.make_dual(c, dc), fwAD.make_dual(n, dn), fwAD.make_dual(w, dw), 0.0)
+        tangent = fwAD.unpack_dual(out).tangent
+    direct = surfel.jvp(cfg, positions, c, n, w, dc, dn, dw, 0.0)
+    torch.testing.assert_close(tangent, direct, rtol=0, atol=0)
+
+
+def test_full_size_jvp_against_finite_differences_of_the_render():
+    """The 128^2 x 2048 scene: JVP against central differences of the HIP render with
+    h = 1e-6 m on centers and 1e-4 on normals, and the aperture-subset oracle."""
+    from dsplat import workloads
+    from dsplat.kernels import surfel
+
+    w = workloads.ellipsoid_scene()
+    cfg = w["config"]
+    positions, centers, normals, weights = (w[k].cuda() for k in ("positions", "centers", "normals", "weights"))
+    g = torch.Generator().manual_seed(2)
+    dc = torch.randn(len(centers), 3, generator=g).cuda()
+    dn = torch.randn(len(centers), 3, generator=g).cuda()
+    zero_c, zero_w = torch.zeros_like(dc), torch.zeros_like(weights)
+    say("full jvp vs FD")
+    for delta_c, delta_n, h in ((dc, zero_c, 1e-6), (zero_c, dn, 1e-4)):
+        cp, cm = centers + h * delta_c, centers - h * delta_c
+        np_, nm = normals + h * delta_n, normals - h * delta_n
+        # float32 coordinates quantize the step (ulp(0.35 m) = 3e-8 m): differentiate
+        # along the step that was actually taken
+        eff_c = ((cp.double() - cm.double()) / (2 * h)).float()
+        eff_n = ((np_.double() - nm.double()) / (2 * h)).float()
+        jvp = surfel.jvp(cfg, positions, centers, normals, weights, eff_c, eff_n, zero_w, 0.0)
+        plus = surfel.render(cfg, positions, cp, np_, weights, 0.0)
+        minus = surfel.render(cfg, positions, cm, nm, weights, 0.0)
+        fd = (plus.to(torch.complex128) - minus.to(torch.complex128)) / (2 * h)
+        err = float((jvp.to(torch.complex128) - fd).norm() / fd.norm())
+        say(f"h {h}: jvp vs FD rel L2 {err:.3e}")
+        assert err < 2e-3  # float32 render differences at these steps
+    ids = np.arange(0, 16384, 512)
+    dcs, dns, dws = (w[k] for k in ("dcenters", "dnormals", "dweights"))
+    got = surfel.jvp(cfg, positions[ids], centers, normals, weights, dcs.cuda(), dns.cuda(), dws.cuda(), 0.0)
+    want = O.jvp(w["positions"].numpy()[ids], w["centers"].numpy(), w["normals"].numpy(), w["weights"].numpy(),
+                 dcs.numpy(), dns.numpy(), dws.numpy(), cfg)
+    err = rel_l2(got.cpu().numpy(), want)
+    say(f"full jvp subset rel L2 {err:.3e}")
+    assert err < TOL
