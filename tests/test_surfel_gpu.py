@@ -278,7 +278,9 @@ def test_full_size_jvp_against_finite_differences_of_the_render():
     dn = torch.randn(len(centers), 3, generator=g).cuda()
     zero_c, zero_w = torch.zeros_like(dc), torch.zeros_like(weights)
     say("full jvp vs FD")
-    for delta_c, delta_n, h in ((dc, zero_c, 1e-6), (zero_c, dn, 1e-4)):
+    # float32 render noise (~3e-7 |y|) over the FD signal: 4e-5 for centers (phase
+    # moves 5e-3 rad per step), 1e-3 for normals (amplitude moves 2e-4)
+    for delta_c, delta_n, h, tol in ((dc, zero_c, 1e-6, 1e-3), (zero_c, dn, 1e-4, 1e-2)):
         cp, cm = centers + h * delta_c, centers - h * delta_c
         np_, nm = normals + h * delta_n, normals - h * delta_n
         # float32 coordinates quantize the step (ulp(0.35 m) = 3e-8 m): differentiate
@@ -291,8 +293,21 @@ def test_full_size_jvp_against_finite_differences_of_the_render():
         fd = (plus.to(torch.complex128) - minus.to(torch.complex128)) / (2 * h)
         err = float((jvp.to(torch.complex128) - fd).norm() / fd.norm())
         say(f"h {h}: jvp vs FD rel L2 {err:.3e}")
-        assert err < 2e-3  # float32 render differences at these steps
+        assert err < tol
     ids = np.arange(0, 16384, 512)
+    # float64 central differences of the oracle at h = 1e-6 m (centers), 1e-4 (normals)
+    pos_np, c_np, n_np, w_np = (w[k].numpy() for k in ("positions", "centers", "normals", "weights"))
+    c64, n64 = c_np.astype(np.float64), n_np.astype(np.float64)
+    for which, h in (("c", 1e-6), ("n", 1e-4)):
+        d = dc.cpu().numpy().astype(np.float64) if which == "c" else dn.cpu().numpy().astype(np.float64)
+        dcc, dnn = (d, np.zeros_like(d)) if which == "c" else (np.zeros_like(d), d)
+        plus = O.render(pos_np[ids], c64 + h * dcc, n64 + h * dnn, w_np, cfg)
+        minus = O.render(pos_np[ids], c64 - h * dcc, n64 - h * dnn, w_np, cfg)
+        got = surfel.jvp(cfg, positions[ids], centers, normals, weights, torch.as_tensor(dcc, dtype=torch.float32).cuda(),
+                         torch.as_tensor(dnn, dtype=torch.float32).cuda(), zero_w, 0.0)
+        err = rel_l2(got.cpu().numpy(), (plus - minus) / (2 * h))
+        say(f"h {h}: jvp vs float64 FD of the oracle rel L2 {err:.3e}")
+        assert err < 1e-4
     dcs, dns, dws = (w[k] for k in ("dcenters", "dnormals", "dweights"))
     got = surfel.jvp(cfg, positions[ids], centers, normals, weights, dcs.cuda(), dns.cuda(), dws.cuda(), 0.0)
     want = O.jvp(w["positions"].numpy()[ids], w["centers"].numpy(), w["normals"].numpy(), w["weights"].numpy(),
