@@ -207,4 +207,153 @@ int dsplat_surfel_correlate_orientations(const dsplat_fmcw* cfg, const float* po
                                static_cast<float*>(workspace), out, st);
 }
 
+// ------------------------------------------------------------------------------ spectral
+static bool make_grid(int dim, const int32_t* samples, const int32_t* nfft, int window, float radius, int taps_stride,
+                      SpecGrid* g) {
+    if (dim < 1 || dim > 3 || window < 0 || window > 3 || !samples || !nfft || !(radius >= 0.0f)) return false;
+    g->dim = dim;
+    g->window = window;
+    g->radius = radius;
+    g->taps_stride = taps_stride;
+    for (int d = 0; d < 3; ++d) {
+        g->samples[d] = d < dim ? samples[d] : 1;
+        g->nfft[d] = d < dim ? nfft[d] : 1;
+        if (d < dim && (samples[d] < 1 || nfft[d] < samples[d])) return false;
+        if (d < dim && window == 3 && taps_stride < samples[d]) return false;
+    }
+    return true;
+}
+
+int dsplat_spectral_forward(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window, float radius,
+                            const float* taps, int32_t taps_stride, const float* centers, const float* coefficients,
+                            int32_t num_atoms, const float* bins, int32_t num_bins, float* out, void* stream) {
+    SpecGrid g;
+    if (!make_grid(dim, samples, nfft, window, radius, taps_stride, &g) || num_atoms < 0 || num_bins < 0)
+        return hipErrorInvalidValue;
+    if (num_bins == 0) return hipSuccess;
+    if (!out || !bins || (num_atoms > 0 && (!centers || !coefficients)) || (window == 3 && !taps))
+        return hipErrorInvalidValue;
+    return launch_spectral_forward(g, taps, centers, reinterpret_cast<const float2*>(coefficients), nullptr, nullptr,
+                                   bins, num_atoms, num_bins, reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
+int dsplat_spectral_jvp(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window, float radius,
+                        const float* taps, int32_t taps_stride, const float* centers, const float* coefficients,
+                        const float* delta_centers, const float* delta_coefficients, int32_t num_atoms,
+                        const float* bins, int32_t num_bins, float* out, void* stream) {
+    SpecGrid g;
+    if (!make_grid(dim, samples, nfft, window, radius, taps_stride, &g) || num_atoms < 0 || num_bins < 0)
+        return hipErrorInvalidValue;
+    if (num_bins == 0) return hipSuccess;
+    if (!out || !bins || (window == 3 && !taps)) return hipErrorInvalidValue;
+    if (num_atoms > 0 && (!centers || !coefficients || !delta_centers || !delta_coefficients))
+        return hipErrorInvalidValue;
+    if (num_atoms == 0) return fill_zero(out, 2 * (size_t)num_bins, (hipStream_t)stream);
+    return launch_spectral_forward(g, taps, centers, reinterpret_cast<const float2*>(coefficients), delta_centers,
+                                   reinterpret_cast<const float2*>(delta_coefficients), bins, num_atoms, num_bins,
+                                   reinterpret_cast<float2*>(out), (hipStream_t)stream);
+}
+
+int dsplat_spectral_design(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window, float radius,
+                           const float* taps, int32_t taps_stride, const float* centers, int32_t num_atoms,
+                           const float* bins, int32_t num_bins, float* out, float* derivatives, void* stream) {
+    SpecGrid g;
+    if (!make_grid(dim, samples, nfft, window, radius, taps_stride, &g) || num_atoms < 0 || num_bins < 0)
+        return hipErrorInvalidValue;
+    if (num_atoms == 0 || num_bins == 0) return hipSuccess;
+    if (!out || !bins || !centers || (window == 3 && !taps)) return hipErrorInvalidValue;
+    return launch_spectral_design(g, taps, centers, bins, num_atoms, num_bins, reinterpret_cast<float2*>(out),
+                                  reinterpret_cast<float2*>(derivatives), (hipStream_t)stream);
+}
+
+size_t dsplat_spectral_backward_workspace(int32_t dim, int32_t num_atoms, int32_t num_bins) {
+    if (num_atoms <= 0 || num_bins <= 0 || dim < 1 || dim > 3) return 0;
+    return sizeof(float) * (size_t)(dim + 2) * num_atoms * spectral_spans(num_bins);
+}
+
+int dsplat_spectral_backward(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window, float radius,
+                             const float* taps, int32_t taps_stride, const float* centers, const float* coefficients,
+                             int32_t num_atoms, const float* bins, int32_t num_bins, const float* upstream,
+                             float* out, void* workspace, size_t workspace_bytes, void* stream) {
+    SpecGrid g;
+    if (!make_grid(dim, samples, nfft, window, radius, taps_stride, &g) || num_atoms < 0 || num_bins < 0)
+        return hipErrorInvalidValue;
+    if (num_atoms == 0) return hipSuccess;
+    if (!out) return hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    if (num_bins == 0) return fill_zero(out, (size_t)(dim + 2) * num_atoms, st);
+    if (!centers || !coefficients || !bins || !upstream || !workspace || (window == 3 && !taps))
+        return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_spectral_backward_workspace(dim, num_atoms, num_bins)) return hipErrorInvalidValue;
+    return launch_spectral_backward(g, taps, centers, reinterpret_cast<const float2*>(coefficients), bins,
+                                    reinterpret_cast<const float2*>(upstream), num_atoms, num_bins,
+                                    static_cast<float*>(workspace), out, st);
+}
+
+// -------------------------------------------------------------------------------- search
+int dsplat_search_range_gate(const float* points, int32_t count, const float* positions, int32_t num_apertures,
+                             const int32_t* intervals, int32_t num_bins, float spacing, int32_t required,
+                             int32_t* out, void* stream) {
+    if (count < 0 || num_apertures < 0 || num_bins < 0 || !(spacing > 0.0f)) return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    if (!points || !out || (num_apertures > 0 && (!positions || !intervals))) return hipErrorInvalidValue;
+    return launch_range_gate(points, positions, intervals, count, num_apertures, num_bins, spacing, required, out,
+                             (hipStream_t)stream);
+}
+
+int dsplat_search_peaks(float* scores_scratch, const float* centers, int32_t count, int32_t dim, int32_t peaks,
+                        float separation, int32_t* out, void* stream) {
+    if (count < 0 || peaks < 0 || dim < 1) return hipErrorInvalidValue;
+    if (peaks == 0) return hipSuccess;
+    if (!out || (count > 0 && (!scores_scratch || !centers))) return hipErrorInvalidValue;
+    return launch_peaks(scores_scratch, centers, count, peaks, dim, separation, out, (hipStream_t)stream);
+}
+
+int dsplat_search_direction(const float* inverse, const float* gradient, const float* normals, const float* mask,
+                            int32_t count, int32_t width, float radius, float* out, void* stream) {
+    if (count < 0 || !(width == 3 || width == 6)) return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    if (!inverse || !gradient || !mask || !out || (width == 6 && !normals)) return hipErrorInvalidValue;
+    return launch_direction(inverse, gradient, normals, mask, count, width, radius, out, (hipStream_t)stream);
+}
+
+int dsplat_search_trial(const float* centers, const float* normals, const float* direction, const float* controls,
+                        int32_t count, int32_t width, float fraction, float* out_centers, float* out_normals,
+                        void* stream) {
+    if (count < 0 || !(width == 3 || width == 6)) return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    if (!centers || !normals || !direction || !controls || !out_centers || !out_normals) return hipErrorInvalidValue;
+    return launch_trial(centers, normals, direction, controls, count, width, fraction, out_centers, out_normals,
+                        (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------------- reduce, response
+size_t dsplat_energy_workspace(size_t count) { return sizeof(double) * (size_t)energy_blocks_for(count); }
+
+int dsplat_energy(const float* value, const float* reference, size_t count, float* out, void* workspace,
+                  size_t workspace_bytes, void* stream) {
+    if (!out || !workspace) return hipErrorInvalidValue;
+    if (count > 0 && (!value || !reference)) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_energy_workspace(count)) return hipErrorInvalidValue;
+    return launch_energy(reinterpret_cast<const float2*>(value), reinterpret_cast<const float2*>(reference), count,
+                         static_cast<double*>(workspace), out, (hipStream_t)stream);
+}
+
+int dsplat_range_response(double* out, double low, double high, int32_t count, int32_t num_bins, int32_t samples,
+                          int32_t n_fft, void* stream) {
+    if (count < 0 || num_bins < 0 || samples < 1 || n_fft < samples) return hipErrorInvalidValue;
+    if (count == 0 || num_bins == 0) return hipSuccess;
+    if (!out) return hipErrorInvalidValue;
+    return launch_range_response(reinterpret_cast<double2*>(out), low, high, count, num_bins, samples, n_fft,
+                                 (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------------------------ blas
+int dsplat_syrk(const float* rows, int32_t m, int32_t n, float* out, void* stream) {
+    if (m < 0 || n < 0) return hipErrorInvalidValue;
+    if (m == 0 || n == 0) return hipSuccess;
+    if (!rows || !out) return hipErrorInvalidValue;
+    return launch_syrk(rows, m, n, out, (hipStream_t)stream);
+}
+
 }  // extern "C"

@@ -39,6 +39,43 @@ hipError_t launch_orientations(const Chirp& c, const float* pos, int A, const fl
                                const float* bank, int B, int S, const float2* res, const float2* table,
                                const float* norm_table, int oversample, float* partial, float* out, hipStream_t st);
 
+// Spectral grid of up to three separable axes (bin units).
+struct SpecGrid {
+    int dim;          // 1..3
+    int window;       // 0 rect, 1 hann, 2 hamming, 3 kaiser (taps)
+    int samples[3];   // observed samples per axis
+    int nfft[3];      // FFT length per axis
+    float radius;     // support radius in bins, 0 = exact
+    int taps_stride;  // row stride of taps [dim, taps_stride]
+};
+constexpr int kSpectralSpan = 256;  // bins per backward partial
+constexpr int kEnergyBlocks = 1024;
+int spectral_spans(int M);
+int energy_blocks_for(size_t n);
+
+hipError_t launch_reduce_partials(const float* partial, int rows, int nblk, int width, float* out, hipStream_t st);
+hipError_t launch_spectral_forward(const SpecGrid& g, const float* taps, const float* ctr, const float2* coeff,
+                                   const float* dctr, const float2* dcoeff, const float* bins, int S, int M,
+                                   float2* out, hipStream_t st);
+hipError_t launch_spectral_design(const SpecGrid& g, const float* taps, const float* ctr, const float* bins, int S,
+                                  int M, float2* out, float2* jac, hipStream_t st);
+hipError_t launch_spectral_backward(const SpecGrid& g, const float* taps, const float* ctr, const float2* coeff,
+                                    const float* bins, const float2* up, int S, int M, float* partial, float* out,
+                                    hipStream_t st);
+hipError_t launch_range_gate(const float* pts, const float* pos, const int* intervals, int count, int apertures,
+                             int bins, float spacing, int required, int* out, hipStream_t st);
+hipError_t launch_peaks(float* scores, const float* ctr, int count, int peaks, int dim, float separation, int* out,
+                        hipStream_t st);
+hipError_t launch_direction(const float* inv, const float* grad, const float* nrm, const float* mask, int count,
+                            int width, float radius, float* out, hipStream_t st);
+hipError_t launch_trial(const float* ctr, const float* nrm, const float* dir, const float* ctl, int count, int width,
+                        float fraction, float* oc, float* on, hipStream_t st);
+hipError_t launch_energy(const float2* x, const float2* y, size_t n, double* part, float* out, hipStream_t st);
+hipError_t launch_range_response(double2* out, double low, double high, int count, int bins, int samples, int nfft,
+                                 hipStream_t st);
+
+hipError_t launch_syrk(const float* rows, int m, int n, float* out, hipStream_t st);
+
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st);
 hipError_t launch_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, const float2* w,

@@ -303,6 +303,12 @@ __global__ __launch_bounds__(256) void reduce_partials(const float* __restrict__
     }
 }
 
+hipError_t launch_reduce_partials(const float* partial, int rows, int nblk, int width, float* out, hipStream_t st) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(reduce_partials, dim3((rows + 3) / 4), dim3(256), 0, st, partial, rows, nblk, width, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------------------
 CorShape cor_shape(int A, int S) {
     CorShape v;

@@ -64,6 +64,33 @@ SIGNATURES = {
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_size,
          c_ptr],
     ),
+    "dsplat_spectral_forward": (
+        c_int, [c_int, c_ptr, c_ptr, c_int, c_float, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr]
+    ),
+    "dsplat_spectral_jvp": (
+        c_int,
+        [c_int, c_ptr, c_ptr, c_int, c_float, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr,
+         c_ptr],
+    ),
+    "dsplat_spectral_design": (
+        c_int, [c_int, c_ptr, c_ptr, c_int, c_float, c_ptr, c_int, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_ptr]
+    ),
+    "dsplat_spectral_backward_workspace": (c_size, [c_int, c_int, c_int]),
+    "dsplat_spectral_backward": (
+        c_int,
+        [c_int, c_ptr, c_ptr, c_int, c_float, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_ptr,
+         c_size, c_ptr],
+    ),
+    "dsplat_search_range_gate": (
+        c_int, [c_ptr, c_int, c_ptr, c_int, c_ptr, c_int, c_float, c_int, c_ptr, c_ptr]
+    ),
+    "dsplat_search_peaks": (c_int, [c_ptr, c_ptr, c_int, c_int, c_int, c_float, c_ptr, c_ptr]),
+    "dsplat_search_direction": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_float, c_ptr, c_ptr]),
+    "dsplat_search_trial": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_float, c_ptr, c_ptr, c_ptr]),
+    "dsplat_energy_workspace": (c_size, [c_size]),
+    "dsplat_energy": (c_int, [c_ptr, c_ptr, c_size, c_ptr, c_ptr, c_size, c_ptr]),
+    "dsplat_range_response": (c_int, [c_ptr, c_double, c_double, c_int, c_int, c_int, c_int, c_ptr]),
+    "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
 }
 
 

@@ -1,0 +1,64 @@
+"""Bindings of the search kernels: range gating, separated peaks and quasi-Newton steps
+(mirrors dsplat/kernels/search.py of the reference)."""
+
+import math
+
+import torch
+
+from .. import _lib
+
+
+def range_gate(points, positions, intervals, spacing, min_fraction):
+    """Points whose spherical range bin lies in a marked interval at min_fraction of apertures
+    (kernels/search.py:10-24)."""
+    output = torch.empty(len(points), device=points.device, dtype=torch.int32)
+    required = math.ceil(min_fraction * len(positions) - 1e-7)
+    _lib.call(
+        "dsplat_search_range_gate", _lib.ptr(points.float().contiguous()), len(points),
+        _lib.ptr(positions.float().contiguous()), len(positions), _lib.ptr(intervals.to(torch.int32).contiguous()),
+        intervals.shape[1], float(spacing), required, _lib.ptr(output), _lib.stream(points.device),
+    )  # fmt: skip
+    return output.bool()
+
+
+def peaks(scores, centers, count, separation):
+    """Greedy best-first indices at least separation apart, in one workgroup; scores are
+    not modified (kernels/search.py:27-39)."""
+    output = torch.full((min(count, len(scores)),), -1, device=centers.device, dtype=torch.int32)
+    if len(output) == 0:
+        return output.long()
+    scratch = scores.float().clone()
+    _lib.call(
+        "dsplat_search_peaks", _lib.ptr(scratch), _lib.ptr(centers.float().contiguous()), len(scores),
+        centers.shape[1], len(output), float(separation), _lib.ptr(output), _lib.stream(centers.device),
+    )  # fmt: skip
+    return output[output >= 0].long()
+
+
+def direction(inverse, gradient, normals, mask, radius):
+    """Quasi-Newton descent per candidate: tangent to unit normals, inside the trust radius
+    (kernels/search.py:42-56)."""
+    count, width = gradient.shape
+    output = torch.empty_like(gradient)
+    _lib.call(
+        "dsplat_search_direction", _lib.ptr(inverse.contiguous()), _lib.ptr(gradient.contiguous()),
+        _lib.ptr(normals.contiguous()), _lib.ptr(mask.float().contiguous()), count, width, float(radius),
+        _lib.ptr(output), _lib.stream(gradient.device),
+    )  # fmt: skip
+    return output
+
+
+def trial(centers, normals, direction, controls, fraction):
+    """Step fraction along direction, clamped to the box and renormalized; frozen axes kept.
+
+    controls rows: per-coordinate scale, trainable mask, lower and upper bounds
+    (kernels/search.py:59-77).
+    """
+    count, width = direction.shape
+    out_centers, out_normals = torch.empty_like(centers), torch.empty_like(normals)
+    _lib.call(
+        "dsplat_search_trial", _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()),
+        _lib.ptr(direction.contiguous()), _lib.ptr(controls.float().contiguous()), count, width, float(fraction),
+        _lib.ptr(out_centers), _lib.ptr(out_normals), _lib.stream(centers.device),
+    )  # fmt: skip
+    return out_centers, out_normals

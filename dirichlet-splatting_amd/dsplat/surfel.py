@@ -43,3 +43,47 @@ def vjp(config, positions, params, coefficients, upstream, fresnel_n=0.0):
     )
     areas = (weights.conj() * coefficients).real / (2 * scale)
     return dict(centers=centers, normals=normals, areas=areas), weights * scale
+
+
+def design(config, positions, params, fresnel_n=0.0, real=False):
+    """Dictionary columns of every surfel for whole apertures (surfel.py:80-84)."""
+    return kernel.design(config, positions, params["centers"], params["normals"], params["areas"], fresnel_n, real)
+
+
+def jacobian(config, positions, params, fresnel_n=0.0):
+    """Derivatives of the columns [A, K, S] by centers and normals [..., 3] and areas [..., 1]
+    (surfel.py:87-97)."""
+    columns, dc, dn = kernel.jacobian(config, positions, params["centers"], params["normals"], fresnel_n)
+    scale = params["areas"].sqrt()
+    return {
+        "centers": dc * scale[:, None],
+        "normals": dn * scale[:, None],
+        "areas": (columns / (2 * scale))[..., None],
+    }
+
+
+def correlate(config, positions, params, residual, fresnel_n=0.0):
+    """a^H r and |a|^2 of every surfel's column against the residual profiles [A, K]
+    (surfel.py:100-105)."""
+    corr, norm = kernel.correlate(config, positions, params["centers"], params["normals"], residual, fresnel_n)
+    return corr * params["areas"].sqrt(), norm * params["areas"]
+
+
+def correlate_gradient(config, positions, params, residual, fresnel_n=0.0, include_normals=False):
+    """correlate and its derivatives by centers (and normals): corr, norm, dcorr, dnorm
+    (surfel.py:108-120)."""
+    corr, norm, dcorr, dnorm = kernel.correlate_gradient(
+        config, positions, params["centers"], params["normals"], residual, fresnel_n, include_normals
+    )
+    scale, area = params["areas"].sqrt(), params["areas"]
+    return corr * scale, norm * area, dcorr * scale[:, None], dnorm * area[:, None]
+
+
+def correlate_orientations(config, positions, params, bank, residual, fresnel_n=0.0, table=None):
+    """correlate for each surfel's own normal and every bank normal, [S, 1 + bank]
+    (surfel.py:123-129)."""
+    corr, norm = kernel.correlate_orientations(
+        config, positions, params["centers"], params["normals"], bank, residual, fresnel_n, table
+    )
+    scale = params["areas"].sqrt()[:, None]
+    return corr * scale, norm * scale.square()

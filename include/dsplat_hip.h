@@ -118,6 +118,93 @@ int dsplat_surfel_correlate_orientations(const dsplat_fmcw* cfg, const float* po
                                          const float* norm_table, int32_t oversample, float* out,
                                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- Spectral atoms: separable windowed Dirichlet kernels on a 1-3-D grid, bin units.
+ * dim axes with samples[dim], nfft[dim]; window 0 rect, 1 hann, 2 hamming, 3 kaiser
+ * (then taps [dim, taps_stride] are the normalized window taps); radius > 0 truncates.
+ * centers [S, dim], coefficients [S] complex, bins [M, dim]. Gradient convention dc = -dk. */
+
+/* out[M] complex = sum_s coeff_s a(s, m). Replaces kernels/spectral.py:168-177 (spectral_forward). */
+int dsplat_spectral_forward(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window,
+                            float radius, const float* taps, int32_t taps_stride, const float* centers,
+                            const float* coefficients, int32_t num_atoms, const float* bins,
+                            int32_t num_bins, float* out, void* stream);
+
+/* out[M] complex: directional derivative along delta_centers [S, dim] and
+ * delta_coefficients [S] complex. Replaces kernels/spectral.py:192-201 (spectral_jvp). */
+int dsplat_spectral_jvp(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window,
+                        float radius, const float* taps, int32_t taps_stride, const float* centers,
+                        const float* coefficients, const float* delta_centers,
+                        const float* delta_coefficients, int32_t num_atoms, const float* bins,
+                        int32_t num_bins, float* out, void* stream);
+
+/* out [M, S] complex and, when derivatives is not NULL, d/dcenters [M, S, dim] complex.
+ * Replaces kernels/spectral.py:146-154 (spectral_design). */
+int dsplat_spectral_design(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window,
+                           float radius, const float* taps, int32_t taps_stride, const float* centers,
+                           int32_t num_atoms, const float* bins, int32_t num_bins, float* out,
+                           float* derivatives, void* stream);
+
+size_t dsplat_spectral_backward_workspace(int32_t dim, int32_t num_atoms, int32_t num_bins);
+
+/* Gradients of Re(conj(upstream) . render): out [S, dim + 2] = (grad centers[dim],
+ * grad coeff re, im). upstream [M] complex. Deterministic (no atomics).
+ * Replaces kernels/spectral.py:157-165 (spectral_backward). */
+int dsplat_spectral_backward(int32_t dim, const int32_t* samples, const int32_t* nfft, int32_t window,
+                             float radius, const float* taps, int32_t taps_stride,
+                             const float* centers, const float* coefficients, int32_t num_atoms,
+                             const float* bins, int32_t num_bins, const float* upstream, float* out,
+                             void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- Spatial search. */
+
+/* out[count] int32 = 1 where a point's round-half-even range bin lies in a marked
+ * interval (intervals [A, bins] int32) at >= required apertures.
+ * Replaces kernels/search.py:10-24 (candidate_range_gate). */
+int dsplat_search_range_gate(const float* points, int32_t count, const float* positions,
+                             int32_t num_apertures, const int32_t* intervals, int32_t num_bins,
+                             float spacing, int32_t required, int32_t* out, void* stream);
+
+/* Greedy separated peaks: out[peaks] int32 indices best first, -1 once exhausted.
+ * scores_scratch [count] is overwritten (pass a copy). centers [count, dim].
+ * Replaces kernels/search.py:27-39 (candidate_peaks). */
+int dsplat_search_peaks(float* scores_scratch, const float* centers, int32_t count, int32_t dim,
+                        int32_t peaks, float separation, int32_t* out, void* stream);
+
+/* Per candidate: -H^-1 g (or -g), masked, tangent to unit normals, in the trust radius.
+ * inverse [C, w, w], gradient [C, w], normals [C, 3], mask [w], w = 3 or 6.
+ * Replaces kernels/search.py:42-56 (candidate_direction). */
+int dsplat_search_direction(const float* inverse, const float* gradient, const float* normals,
+                            const float* mask, int32_t count, int32_t width, float radius, float* out,
+                            void* stream);
+
+/* Trial step: centers clamped to the box where trainable, normals renormalized.
+ * controls [4, 6]: scale, trainable mask, lower, upper per coordinate.
+ * Replaces kernels/search.py:59-77 (candidate_trial). */
+int dsplat_search_trial(const float* centers, const float* normals, const float* direction,
+                        const float* controls, int32_t count, int32_t width, float fraction,
+                        float* out_centers, float* out_normals, void* stream);
+
+/* ---- Reductions and responses. */
+
+size_t dsplat_energy_workspace(size_t count);
+
+/* out[0] = sum |value - reference|^2 over count complex entries, block sums added in
+ * float64 in a fixed order. Replaces kernels/reduce.py:8-18 (complex_energy + sum). */
+int dsplat_energy(const float* value, const float* reference, size_t count, float* out,
+                  void* workspace, size_t workspace_bytes, void* stream);
+
+/* out [bins, count] complex128: float64 D(bin - center) for count centers spread over
+ * [low, high]. Replaces kernels/response.py:8-20 (range_response). */
+int dsplat_range_response(double* out, double low, double high, int32_t count, int32_t num_bins,
+                          int32_t samples, int32_t n_fft, void* stream);
+
+/* ---- Dense products. */
+
+/* Upper triangle of out [n, n] (row-major) += rows^T rows, rows [m, n] row-major, FP32
+ * on MFMA; the lower triangle is untouched. Replaces kernels/cublas.py:31-44 (syrk:
+ * cublasSsyrk_v2 through ctypes). */
+int dsplat_syrk(const float* rows, int32_t m, int32_t n, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

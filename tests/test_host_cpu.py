@@ -1,0 +1,124 @@
+"""Host algebra of the solver on the CPU: no kernel runs here.
+
+Known answers and brute force, following the reference's solver tests
+(tests/test_solver.py:46-136, 351-406).
+"""
+
+import pytest
+import torch
+
+from dsplat import ParameterConfig
+from dsplat.replacement import backward_elimination, regularized_inverse, utility
+from dsplat.varpro import pcg, solve
+
+
+def test_solve_of_a_singular_system_predicts_like_the_pseudoinverse():
+    g = torch.Generator().manual_seed(0)
+    basis = torch.randn(6, 3, dtype=torch.float64, generator=g)
+    h = basis @ basis.T
+    q = basis @ torch.randn(3, dtype=torch.float64, generator=g)
+    x = solve(h, q)
+    torch.testing.assert_close(basis.T @ x, basis.T @ (torch.linalg.pinv(h, hermitian=True) @ q), atol=1e-8, rtol=0)
+    torch.testing.assert_close(h @ x, q, atol=1e-8, rtol=0)
+    assert torch.count_nonzero(solve(torch.zeros(3, 3, dtype=torch.float64), torch.zeros(3, dtype=torch.float64))) == 0
+
+
+def test_backward_elimination_follows_brute_force_refits():
+    g = torch.Generator().manual_seed(4)
+    x = torch.randn(20, 8, generator=g, dtype=torch.float64)
+    h, q = x.T @ x, x.T @ torch.randn(20, generator=g, dtype=torch.float64)
+
+    def objective(keep):
+        keep = list(keep)
+        return float(-q[keep] @ torch.linalg.solve(h[keep][:, keep], q[keep]))
+
+    alive, expected = set(range(8)), []
+    for _ in range(3):
+        best = min(alive, key=lambda j: objective(alive - {j}))
+        expected.append(best)
+        alive.remove(best)
+    assert backward_elimination(torch.linalg.inv(h), q, 3).tolist() == expected
+
+
+def test_backward_elimination_respects_the_limit_on_marked_unknowns():
+    g = torch.Generator().manual_seed(5)
+    x = torch.randn(30, 6, generator=g, dtype=torch.float64)
+    h, q = x.T @ x, x.T @ torch.randn(30, generator=g, dtype=torch.float64)
+    limited = torch.tensor([True, True, True, False, False, False])
+    removed = backward_elimination(torch.linalg.inv(h), q, 4, limited, 1)
+    assert int(limited[removed].sum()) <= 1 and len(set(removed.tolist())) == 4
+
+
+def test_regularized_inverse_shifts_by_a_fraction_of_the_largest_eigenvalue():
+    h = torch.diag(torch.tensor([0.0, 1.0, 2.0, 3.0, 4.0, 10.0], dtype=torch.float64))
+    inverse = regularized_inverse(h, 1e-3)
+    torch.testing.assert_close(inverse, torch.linalg.inv(h + 0.01 * torch.eye(6, dtype=torch.float64)), atol=1e-8, rtol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [1, 3, 4, 28])
+def test_regularized_inverse_has_a_positive_scalar_shift_and_scales_with_the_gram(seed):
+    g = torch.Generator().manual_seed(seed)
+    x = torch.randn(30, 6, dtype=torch.float64, generator=g)
+    h = x.T @ x
+    inverse = regularized_inverse(h, 1e-3)
+    difference = torch.linalg.inv(inverse) - h
+    shift = difference.diag().mean()
+    assert 0 < shift <= 1e-3 * torch.linalg.eigvalsh(h)[-1] * (1 + 1e-10)
+    torch.testing.assert_close(difference, shift * torch.eye(6, dtype=h.dtype), atol=1e-12, rtol=1e-9)
+    torch.testing.assert_close(regularized_inverse(7 * h, 1e-3) * 7, inverse)
+
+
+def test_utility_marks_a_duplicated_column_redundant():
+    x = torch.randn(40, 4, dtype=torch.float64, generator=torch.Generator().manual_seed(2))
+    x = torch.cat((x, x[:, :1]), 1)
+    efficiency = 2 * utility(x.T @ x, torch.zeros(5, dtype=torch.float64))
+    assert efficiency[0] < 1e-6 and efficiency[4] < 1e-6 and (efficiency[1:4] > 0.1).all()
+
+
+def test_pcg_solves_a_spd_system():
+    g = torch.Generator().manual_seed(7)
+    x = torch.randn(30, 10, dtype=torch.float64, generator=g)
+    h = x.T @ x + 0.1 * torch.eye(10, dtype=torch.float64)
+    b = torch.randn(10, dtype=torch.float64, generator=g)
+    solution = pcg(lambda v: h @ v, b, h.diag(), iterations=200, tolerance=1e-12)
+    torch.testing.assert_close(solution, torch.linalg.solve(h, b), atol=1e-9, rtol=1e-9)
+
+
+def test_unit_vectors_with_a_frozen_component_stay_unit():
+    spec = ParameterConfig(trainable=(True, True, False), normalize=True)
+    reference = torch.nn.functional.normalize(torch.tensor([[0.3, 0.4, -0.6]]), dim=1)
+    value = spec.project(torch.tensor([[2.0, -1.0, 5.0]]), reference)
+    assert value[0, 2] == reference[0, 2]
+    torch.testing.assert_close(value.norm(dim=1), torch.ones(1))
+
+
+def test_unit_vector_tangent_matches_the_projection_with_a_frozen_component():
+    spec = ParameterConfig(trainable=(True, False, True), normalize=True)
+    value = torch.nn.functional.normalize(torch.randn(4, 3, generator=torch.Generator().manual_seed(3), dtype=torch.float64), dim=1)
+    for axis in (0, 2):
+        step = torch.zeros_like(value)
+        step[:, axis] = 1e-6
+        numeric = (spec.project(value + step, value) - spec.project(value - step, value)) / 2e-6
+        torch.testing.assert_close(spec.tangent(step / 1e-6, value), numeric, atol=1e-7, rtol=0)
+
+
+def test_a_free_part_of_zero_length_takes_the_reference_direction():
+    spec = ParameterConfig(trainable=(True, False, True), normalize=True)
+    reference = torch.tensor([[0.6, 0.8, 0.0]])
+    torch.testing.assert_close(spec.project(torch.tensor([[0.0, 0.8, 0.0]]), reference), reference)
+
+
+@pytest.mark.parametrize("trainable", [True, (True, False, True)])
+def test_normalizing_without_any_reference_direction_raises(trainable):
+    spec = ParameterConfig(trainable=trainable, normalize=True)
+    with pytest.raises(ValueError, match="reference direction"):
+        spec.project(torch.zeros(1, 3), torch.zeros(1, 3))
+
+
+def test_bounds_period_and_frozen_components():
+    spec = ParameterConfig(trainable=(True, False), lower=(-1.0, -5.0), upper=(1.0, 5.0))
+    value = spec.project(torch.tensor([[3.0, 9.0], [-2.0, 0.0]]), torch.tensor([[0.0, 0.5], [0.0, -0.5]]))
+    assert value.tolist() == [[1.0, 0.5], [-1.0, -0.5]]
+    periodic = ParameterConfig(period=(10.0,))
+    assert periodic.project(torch.tensor([12.5, -1.0]), torch.zeros(2)).tolist() == [2.5, 9.0]
+

@@ -171,3 +171,48 @@ def test_correlations_equal_dictionary_products():
     assert real.shape == (2 * 128 * 16, 5)
     # row (2 k + part) A + a of the real layout
     np.testing.assert_array_equal(real[(2 * 7 + 1) * 16 + 3], a[3 * 128 + 7].imag)
+
+
+def test_spectral_atoms_equal_a_brute_force_windowed_dft():
+    """A 2-D atom is the product of per-axis tapped DFT sums (pins oracle.spectral)."""
+    from oracle import spectral as Sp
+
+    shape, samples = (24, 20), (16, 20)
+    centers = np.array([[3.3, 4.1], [12.0, 15.0], [18.4, 7.9]])
+    bins = Sp.grid_bins(shape)
+    for window in ("rect", "hann", "hamming", "kaiser"):
+        a, da = Sp.design(shape, samples, window, 6.0, 0.0, centers, bins)
+        expected = np.ones_like(a)
+        for d in range(2):
+            delta = bins[:, None, d] - centers[None, :, d]
+            expected = expected * K.dirichlet_bruteforce(delta, samples[d], shape[d], K.window_taps(window, samples[d], 6.0))[0]
+        assert np.abs(a - expected).max() < 1e-12
+        h = 1e-6
+        shifted = centers.copy()
+        shifted[:, 0] += h
+        fd = (Sp.design(shape, samples, window, 6.0, 0.0, shifted, bins)[0] - a) / h
+        assert np.abs(da[..., 0] - fd).max() < 1e-4 * np.abs(fd).max()
+
+
+def test_search_oracles_on_known_answers():
+    from oracle import search as Se
+
+    centers = np.array([[0.0, 0, 0], [0.05, 0, 0], [1.0, 0, 0], [1.02, 0, 0]])
+    scores = np.array([0.5, 0.9, 0.9, 0.1])
+    # index 1 wins the tie with 2 by index; it suppresses 0; then 2; 3 is suppressed by 2
+    assert Se.peaks(scores, centers, 4, 0.1) == [1, 2]
+    assert Se.peaks(scores, centers, 4, 0.0) == [1, 2, 0, 3]
+    pos = np.zeros((1, 3), np.float32)
+    pts = np.array([[0, 0, 2.5], [0, 0, 3.5], [0, 0, 4.2]], np.float32)  # bins 2.5 -> 2, 3.5 -> 4, 4.2 -> 4
+    intervals = np.zeros((1, 8), np.int64)
+    intervals[0, 4] = 1
+    assert Se.range_gate(pts, pos, intervals, 1.0, 1).tolist() == [False, True, True]
+
+
+def test_range_response_and_energy():
+    from oracle import linalg as L
+
+    r = L.range_response(60.0, 70.0, 11, 128, 128, 256)
+    assert r.shape == (128, 11)
+    assert abs(r[60, 0] - 1) < 1e-15 and abs(r[70, 10] - 1) < 1e-15
+    assert L.energy([1 + 1j, 2], [1, 0]) == 5.0
