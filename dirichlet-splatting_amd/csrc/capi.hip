@@ -356,4 +356,27 @@ int dsplat_syrk(const float* rows, int32_t m, int32_t n, float* out, void* strea
     return launch_syrk(rows, m, n, out, (hipStream_t)stream);
 }
 
+size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels) {
+    if (num_apertures <= 0 || num_surfels <= 0) return 0;
+    const GramShape g = gram_shape(num_apertures, num_surfels);
+    return sizeof(float) * 2 * (size_t)g.splits * num_surfels * num_surfels;
+}
+
+int dsplat_gram(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* centers,
+                const float* normals, const float* areas, int32_t num_surfels, int32_t mode, int32_t real_only,
+                float inv_rows, const float* column_exponents, float* out, void* workspace, size_t workspace_bytes,
+                void* stream) {
+    Chirp c;
+    const int A = num_apertures, S = num_surfels;
+    if (!make_chirp(cfg, &c) || A < 0 || S < 0 || mode < 0 || mode > 2) return hipErrorInvalidValue;
+    if (S == 0) return hipSuccess;
+    if (!out) return hipErrorInvalidValue;
+    hipStream_t st = (hipStream_t)stream;
+    if (A == 0) return fill_zero(out, (size_t)S * S * (real_only ? 1 : 2), st);
+    if (!positions || !centers || !normals || !areas || !workspace) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_gram_workspace(A, S)) return hipErrorInvalidValue;
+    return launch_gram(c, mode, positions, A, centers, normals, areas, mode == 2 ? column_exponents : nullptr, S,
+                       real_only, inv_rows, static_cast<float*>(workspace), out, st);
+}
+
 }  // extern "C"

@@ -74,6 +74,15 @@ hipError_t launch_energy(const float2* x, const float2* y, size_t n, double* par
 hipError_t launch_range_response(double2* out, double low, double high, int count, int bins, int samples, int nfft,
                                  hipStream_t st);
 
+struct GramShape {
+    int tiles;      // 128 x 128 tiles of the upper triangle
+    int splits;     // aperture ranges (split-K)
+    int per_split;  // apertures per range
+};
+GramShape gram_shape(int A, int S);
+hipError_t launch_gram(const Chirp& c, int mode, const float* pos, int A, const float* ctr, const float* nrm,
+                       const float* areas, const float* col_exp, int S, int real_only, float inv_rows,
+                       float* partial, float* out, hipStream_t st);
 hipError_t launch_syrk(const float* rows, int m, int n, float* out, hipStream_t st);
 
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,

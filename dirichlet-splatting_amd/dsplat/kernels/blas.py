@@ -16,3 +16,39 @@ def syrk(rows, out):
     rows = rows.contiguous()
     _lib.call("dsplat_syrk", _lib.ptr(rows), rows.shape[0], rows.shape[1], _lib.ptr(out), _lib.stream(rows.device))
     return out
+
+
+GRAM_MODES = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
+FP16_TARGET = 2.0**14  # largest scaled atom magnitude in FP16 mode (max finite 65504)
+
+
+def column_exponents(config, positions, centers, areas):
+    """Power-of-two column scales for the FP16 split: 2^e_s |atom| <= 2^14 for every atom.
+
+    |D| <= 1, cos^2 F <= 1, so |atom| <= sqrt(area) lambda / (8 pi r_min) with r_min the
+    column's nearest aperture.
+    """
+    r_min = torch.cdist(centers.double(), positions.double()).min(1).values.clamp(min=1e-9)
+    bound = areas.double().sqrt() * config.lambda_m / (8 * torch.pi * r_min)
+    return torch.floor(torch.log2(FP16_TARGET / bound.clamp(min=1e-300))).clamp(-120, 120).float()
+
+
+def gram(config, positions, centers, normals, areas, mode="bf16x3", real_only=False, scale=1.0, fresnel_n=0.0):
+    """scale * A^H A of the surfel dictionary (atoms include sqrt(area)), atoms generated
+    on chip and contracted on MFMA; complex [S, S] Hermitian, or real [S, S] = Re when
+    real_only. mode: fp32 | bf16x3 | fp16x3 (hi+lo split, FP32 accumulation)."""
+    count = len(centers)
+    code = GRAM_MODES[mode]
+    positions, centers, normals = (t.float().contiguous() for t in (positions, centers, normals))
+    areas = areas.float().contiguous()
+    out = centers.new_empty((count, count) if real_only else (count, count, 2))
+    size = _lib.library().dsplat_gram_workspace(len(positions), count)
+    workspace = centers.new_empty(max(1, size // 4))
+    exponents = column_exponents(config, positions, centers, areas) if code == 2 else None
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_gram", _lib.ctypes.byref(cfg), _lib.ptr(positions), len(positions), _lib.ptr(centers),
+        _lib.ptr(normals), _lib.ptr(areas), count, code, 1 if real_only else 0, float(scale),
+        _lib.ptr(exponents), _lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(centers.device),
+    )  # fmt: skip
+    return out if real_only else torch.view_as_complex(out)

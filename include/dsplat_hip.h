@@ -205,6 +205,20 @@ int dsplat_range_response(double* out, double low, double high, int32_t count, i
  * cublasSsyrk_v2 through ctypes). */
 int dsplat_syrk(const float* rows, int32_t m, int32_t n, float* out, void* stream);
 
+size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels);
+
+/* Gram of the surfel dictionary, atoms (with sqrt(area)) generated on chip and contracted
+ * on MFMA: out = inv_rows * A^H A, complex [S, S] Hermitian, or its real part [S, S]
+ * float when real_only. Rows are every aperture's K bins. mode 0 FP32 MFMA; 1 BF16
+ * hi+lo split (3 products, FP32 accumulation); 2 FP16 hi+lo split with per-column
+ * power-of-two scales 2^column_exponents[s] (float, integral values; NULL = 1).
+ * Split-K partials are summed in float64 in a fixed order. Replaces the streamed
+ * normal equations of varpro.py:91-120 (design blocks + cublasSsyrk) for the Gram. */
+int dsplat_gram(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                const float* centers, const float* normals, const float* areas, int32_t num_surfels,
+                int32_t mode, int32_t real_only, float inv_rows, const float* column_exponents,
+                float* out, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
