@@ -26,7 +26,8 @@ sys.path.insert(0, str(ROOT / "dirichlet-splatting_amd"))
 # Algorithmic FP32 work per (a, k, s) evaluation: the cot-form kernel.
 #   forward 13 FLOP: cot by angle addition (3 + 3), cot (1), complex accumulate (4 + 2)
 #   VJP     19 FLOP: the forward's 13 plus the (1 + cot^2) u class sum (6)
-FWD_FLOP, VJP_FLOP = 13, 19
+#   scan    11 FLOP: cot by angle addition (5), residual accumulate (4), |cot|^2 (2)
+FWD_FLOP, VJP_FLOP, SCAN_FLOP = 13, 19, 11
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
 PEAK_HBM_GBS = 8000.0
 
@@ -38,6 +39,11 @@ def parse():
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
+    parser.add_argument(
+        "--workload", choices=("splat", "gram", "scan"), default="splat",
+        help="splat: the fwd+VJP metric (default); gram: the split-precision Gram of the same scene; "
+        "scan: the batched certificate scan of 64^3 candidates against 64^2, 96^2 and 128^2 scans",
+    )  # fmt: skip
     return parser.parse_args()
 
 
@@ -75,6 +81,205 @@ def cpu_baseline(work, target_seconds):
     }
 
 
+def timed(fn, repeats, stream):
+    """Mean milliseconds of fn() between HIP events on the launch stream, after one warm-up."""
+    import torch
+
+    fn()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(repeats):
+        out = fn()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / repeats, out
+
+
+def gram_workload(args):
+    """A^H A and A^H r of the full-size dictionary (16384 x 128 rows, 2048 surfel columns)
+    in FP32, BF16 hi+lo and FP16 hi+lo MFMA, each against float64 products of the same
+    float32 atoms, then an FP64 Cholesky solve and the coefficient error."""
+    import torch
+
+    from dsplat import workloads
+    from dsplat.kernels import blas, surfel
+    from dsplat.varpro import solve
+
+    work = workloads.ellipsoid_scene()
+    cfg = work["config"]
+    dev = torch.device("cuda", 0)
+    pos, c, n, areas = (work[k].to(dev) for k in ("positions", "centers", "normals", "areas"))
+    A, K, S = len(pos), cfg.num_range_bins, len(c)
+    M = A * K
+    residual = workloads.complex_normal((M,), torch.Generator().manual_seed(3)).to(dev)
+    stream = torch.cuda.current_stream(dev)
+    # float64 reference: products of the float32 atoms accumulated in complex128
+    g64 = torch.zeros((S, S), dtype=torch.complex128, device=dev)
+    q64 = torch.zeros(S, dtype=torch.complex128, device=dev)
+    chunk = 128
+    for start in range(0, A, chunk):
+        a = surfel.design(cfg, pos[start : start + chunk], c, n, areas, 0.0, False).to(torch.complex128)
+        g64 += a.H @ a
+        q64 += a.H @ residual[start * K : (start + chunk) * K].to(torch.complex128)
+        del a
+    g64, q64 = g64 / M, q64 / M
+    eye = torch.eye(S, dtype=torch.complex128, device=dev)
+    scale = float(g64.diagonal().real.mean())
+    ridges = {"1e-4": 1e-4 * scale, "1e-2": 1e-2 * scale}  # relative to the mean diagonal
+    b64 = {key: solve(g64 + ridge * eye, q64) for key, ridge in ridges.items()}
+
+    def coefficient_errors(g, q):
+        out = {}
+        for key, ridge in ridges.items():
+            b = solve(g + ridge * eye, q)
+            d = b - b64[key]
+            out[f"coef_rel_err_ridge{key}"] = float(d.norm() / b64[key].norm())
+            # prediction error ||A (b - b64)|| / ||A b64||, well conditioned
+            out[f"pred_rel_err_ridge{key}"] = float(
+                ((d.conj() @ g64 @ d).real / (b64[key].conj() @ g64 @ b64[key]).real).sqrt()
+            )
+        return out
+    q_ms, (corr, _) = timed(lambda: surfel.correlate(cfg, pos, c, n, residual.reshape(A, K), 0.0), 3, stream)
+    q = (corr * areas.sqrt()).to(torch.complex128) / M
+    useful = 4.0 * M * S * (S + 1)  # Hermitian half, complex MAC = 8 FLOP
+    modes = {}
+    for mode in ("fp32", "bf16x3", "fp16x3"):
+        ms, g = timed(lambda mode=mode: blas.gram(cfg, pos, c, n, areas, mode, scale=1.0 / M), 3, stream)
+        g = g.to(torch.complex128)
+        executed = (1 if mode == "fp32" else 3) * 8.0 * M * (S * S / 2 + 64 * S)  # full upper tiles
+        modes[mode] = {
+            "ms": ms,
+            "useful_tflops": useful / (ms * 1e-3) / 1e12,
+            "mfma_tflops": executed / (ms * 1e-3) / 1e12,
+            "gram_rel_err": float((g - g64).norm() / g64.norm()),
+            **coefficient_errors(g, q),
+        }
+    peak = {"fp32": 157.3, "bf16x3": 2516.6, "fp16x3": 2516.6}
+    best = "bf16x3"
+    return {
+        "metric": "VarPro Gram A^H A ms (16384x128 rows x 2048 surfels, complex)",
+        "value": modes[best]["ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "bf16 hi+lo split, fp32 accumulation",
+        "data": "synthetic",
+        "config": {"workload": "Gram of the 128x128 x 128-bin x 2048-surfel ellipsoid dictionary", "rows": M,
+                   "columns": S, "ridges_relative_to_mean_diagonal": list(ridges)},
+        "reference_fp64_coefficients_from_fp64_rhs": True,
+        "rhs": {"ms": q_ms, "rel_err": float((q - q64).norm() / q64.norm()), "path": "correlate kernel, fp32 VALU"},
+        "modes": modes,
+        "speedup_bf16x3_vs_fp32": modes["fp32"]["ms"] / modes["bf16x3"]["ms"],
+        "roofline": {"kernel": "gram_kernel<2,1>", "bound": "mfma", "achieved": modes[best]["mfma_tflops"],
+                     "peak": peak[best], "unit": "TFLOP/s", "frac": modes[best]["mfma_tflops"] / peak[best],
+                     "traffic": None},
+    }
+
+
+def scan_workload(args):
+    """Certificates |a_c^H r|^2 / (|a_c|^2 |r|^2) of 262144 candidates (64^3 grid over the
+    70 mm box, seeded -z-facing normals) against three scans of uneven size (64^2, 96^2
+    and 128^2 apertures at 2.5 mm, residual = the ellipsoid scene plus seeded noise at
+    30 dB), all scans in one launch with the autotuned candidate chunk, and the top 256
+    per scan. Also times the same correlations as one launch per scan."""
+    import numpy as np
+    import torch
+    from threadpoolctl import threadpool_limits
+
+    from dsplat import FMCWModel, Scan, workloads
+    from dsplat.certificate import certificate_scan
+    from dsplat.groundtruth import planar_array
+    from dsplat.kernels import surfel
+
+    scene = workloads.ellipsoid_scene()
+    cfg = scene["config"]
+    dev = torch.device("cuda", 0)
+    sc, sn, sw = (scene[k].to(dev) for k in ("centers", "normals", "weights"))
+    centers, normals = (x.to(dev) for x in workloads.certificate_candidates())
+    C, K = len(centers), cfg.num_range_bins
+    candidates = dict(centers=centers, normals=normals, areas=torch.full((C,), float(scene["areas"][0]), device=dev))
+    models, residuals = [], []
+    for i, side in enumerate((64, 96, 128)):
+        pos = planar_array(side, workloads.PITCH).to(dev)
+        data = surfel.render(cfg, pos, sc, sn, sw, 0.0)
+        sigma = float(data.abs().square().mean().sqrt()) * 10 ** (-30 / 20)
+        noise = workloads.complex_normal(tuple(data.shape), torch.Generator().manual_seed(40 + i)).to(dev)
+        models.append(FMCWModel(Scan(data + sigma * noise, pos, cfg)))
+        residuals.append(models[-1].scan.data)
+    A = sum(m.apertures for m in models)
+    evals = A * K * C
+    stream = torch.cuda.current_stream(dev)
+    scans = [(m.scan.scan_positions_m, r) for m, r in zip(models, residuals)]
+
+    def step():
+        return certificate_scan(models, residuals, candidates, "complex", top=256)
+
+    for _ in range(max(1, args.warmup)):  # the first call autotunes the chunk
+        step()
+    torch.cuda.synchronize()
+    from dsplat import autotune
+
+    tuning = autotune.record("scan_batched")
+    steps = max(1, min(args.steps, 10))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        scores, ids = step()
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / steps
+    kernel_ms, _ = timed(lambda: surfel.scan_batched(cfg, scans, centers, normals, 0.0), 3, stream)
+    single_ms, _ = timed(lambda: [surfel.correlate(cfg, p, centers, normals, r, 0.0) for p, r in scans], 3, stream)
+    achieved = SCAN_FLOP * evals / (kernel_ms * 1e-3) / 1e12
+    # the top candidates lie near the surface: distance of the best per scan to the nearest surfel
+    best = centers[ids[:, :16].flatten()]
+    near = torch.cdist(best, sc).min(1).values.reshape(3, 16).median(1).values
+    result = {
+        "metric": "batched certificate scan evals/s (aperture·bin·candidate)",
+        "value": evals / (ms_per_step * 1e-3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(1, args.warmup),
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 per-pair phase, f64 block reduction)",
+        "data": "synthetic",
+        "config": {"workload": "certificate scan: 262144 candidates (64^3 grid, 70 mm box) x scans of 64^2, 96^2 "
+                   "and 128^2 apertures @ 2.5 mm, 128 range bins, top 256 per scan", "apertures": A,
+                   "bins": K, "candidates": C, "evals_per_step": evals, "parallelism": "single GPU"},
+        "kernels_ms": {"scan_batched": kernel_ms, "one_launch_per_scan": single_ms},
+        "autotune": tuning,
+        "median_distance_top16_to_surface_m": [float(x) for x in near],
+        "roofline": {"kernel": "scan_kernel<2>", "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "flop_per_eval": SCAN_FLOP},
+    }
+    if not args.no_cpu:
+        from oracle import surfel as O
+
+        pos = models[0].scan.scan_positions_m.cpu().numpy()
+        res = residuals[0].cpu().numpy()
+        cc, cn = centers.cpu().numpy(), normals.cpu().numpy()
+        with threadpool_limits(limits=1):
+            t = time.perf_counter()
+            O.correlate(pos, cc[:8], cn[:8], res, cfg, 0.0)
+            probe = time.perf_counter() - t
+            count = int(min(1024, max(8, 8 * args.cpu_seconds / max(probe, 1e-3))))
+            ids = np.linspace(0, C - 1, count).astype(np.int64)
+            t = time.perf_counter()
+            O.correlate(pos, cc[ids], cn[ids], res, cfg, 0.0)
+            seconds = time.perf_counter() - t
+        sample = len(pos) * K * count
+        result["cpu_baseline"] = {
+            "value": sample / seconds, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"float64 numpy oracle correlate of {count} of {C} candidates against the 64^2 scan "
+            f"({sample:.3e} evals, {seconds:.1f} s, 1 thread)",
+        }  # fmt: skip
+    return result
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -84,6 +289,12 @@ def main():
             file=sys.stderr,
         )
         sys.exit(2)
+    if args.workload == "gram":
+        print(json.dumps(gram_workload(args)), flush=True)
+        return
+    if args.workload == "scan":
+        print(json.dumps(scan_workload(args)), flush=True)
+        return
     import torch
 
     from dsplat import workloads

@@ -379,4 +379,47 @@ int dsplat_gram(const dsplat_fmcw* cfg, const float* positions, int32_t num_aper
                        real_only, inv_rows, static_cast<float*>(workspace), out, st);
 }
 
+static bool make_scanset(int num_scans, const float* const* positions, const float* const* residuals,
+                         const int32_t* apertures, ScanSet* set) {
+    if (num_scans < 1 || num_scans > kMaxScans || !positions || !residuals || !apertures) return false;
+    set->count = num_scans;
+    set->first_block[0] = 0;
+    for (int i = 0; i < num_scans; ++i) {
+        if (apertures[i] < 1 || !positions[i] || !residuals[i]) return false;
+        set->positions[i] = positions[i];
+        set->residuals[i] = reinterpret_cast<const float2*>(residuals[i]);
+        set->apertures[i] = apertures[i];
+        set->first_block[i + 1] = set->first_block[i] + (apertures[i] + 31) / 32;
+    }
+    return true;
+}
+
+size_t dsplat_scan_workspace(int32_t num_scans, const int32_t* apertures, int32_t num_candidates) {
+    if (num_scans < 1 || num_scans > 8 || !apertures || num_candidates <= 0) return 0;
+    size_t blocks = 0;
+    for (int i = 0; i < num_scans; ++i) blocks += (size_t)(apertures[i] + 31) / 32;
+    return sizeof(float) * 3 * blocks * (size_t)num_candidates;
+}
+
+int dsplat_scan_batched(const dsplat_fmcw* cfg, int32_t num_scans, const float* const* positions,
+                        const float* const* residuals, const int32_t* apertures, const float* centers,
+                        const float* normals, int32_t num_candidates, int32_t chunk, float* out, void* workspace,
+                        size_t workspace_bytes, void* stream) {
+    Chirp c;
+    ScanSet set;
+    if (!make_chirp(cfg, &c) || !make_scanset(num_scans, positions, residuals, apertures, &set) || num_candidates < 0)
+        return hipErrorInvalidValue;
+    if (num_candidates == 0) return hipSuccess;
+    if (!centers || !normals || !out || !workspace) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_scan_workspace(num_scans, apertures, num_candidates)) return hipErrorInvalidValue;
+    if (chunk <= 0) {
+        const int groups = (4096 + set.first_block[num_scans] - 1) / set.first_block[num_scans];
+        chunk = (num_candidates + groups - 1) / groups;
+    }
+    if ((num_candidates + chunk - 1) / chunk > 65535) chunk = (num_candidates + 65534) / 65535;  // grid y limit
+    chunk = (chunk + 7) / 8 * 8;
+    return launch_scan(c, set, centers, normals, num_candidates, chunk, static_cast<float*>(workspace), out,
+                       (hipStream_t)stream);
+}
+
 }  // extern "C"

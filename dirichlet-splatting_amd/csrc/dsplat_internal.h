@@ -27,6 +27,18 @@ struct CorShape {
     int chunk;   // candidates per chunk
 };
 CorShape cor_shape(int A, int S);
+
+constexpr int kMaxScans = 8;
+// A set of scans of one chirp with uneven aperture counts, launched together.
+struct ScanSet {
+    int count;
+    const float* positions[kMaxScans];   // [A_i, 3]
+    const float2* residuals[kMaxScans];  // [A_i, K]
+    int apertures[kMaxScans];
+    int first_block[kMaxScans + 1];      // prefix sums of 32-aperture blocks
+};
+hipError_t launch_scan(const Chirp& c, const ScanSet& set, const float* ctr, const float* nrm, int S, int chunk,
+                       float* partial, float* out, hipStream_t st);
 size_t cor_smem(const Chirp& c);
 
 hipError_t launch_design(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,

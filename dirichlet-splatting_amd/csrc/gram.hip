@@ -24,6 +24,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kGramTile = 128;  // complex columns per tile side
 constexpr int kGramStep = 8;    // bins per K-step (16 real K values)
+constexpr int kFoldApertures = 128;  // apertures per float32 partial (float64 fold beyond)
 
 template <int MODE>
 struct GramTypes;
@@ -300,10 +301,15 @@ __global__ __launch_bounds__(256) void gram_reduce(const float* __restrict__ par
 }
 
 GramShape gram_shape(int A, int S) {
+    // Split-K over apertures: at most kFoldApertures apertures (128 x K rows) accumulate in
+    // one float32 partial before the float64 reduction, and enough ranges to give ~4
+    // workgroups per CU.
     GramShape g;
     const int nb = (S + kGramTile - 1) / kGramTile;
     g.tiles = nb * (nb + 1) / 2;
-    int splits = (1024 + g.tiles - 1) / g.tiles;  // ~4 workgroups per CU
+    int splits = (1024 + g.tiles - 1) / g.tiles;
+    const int fold = (A + kFoldApertures - 1) / kFoldApertures;
+    if (splits < fold) splits = fold;
     if (splits > A) splits = A;
     if (splits < 1) splits = 1;
     g.per_split = (A + splits - 1) / splits;

@@ -289,6 +289,89 @@ __global__ __launch_bounds__(256) void orientations_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Batched certificate scan: every scan of a set (uneven aperture counts) in one launch.
+// blockIdx.x runs over the aperture blocks of all scans back to back; a workgroup finds
+// its scan in the set's prefix table, loads that scan's residual tile and correlates a
+// chunk of candidates (blockIdx.y) exactly as correlate_kernel<P, 0>.
+// partial [C][total blocks][3]; scan_reduce sums each scan's block range in float64.
+// ---------------------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(256) void scan_kernel(const Chirp c, const ScanSet set, const float* __restrict__ ctr,
+                                                   const float* __restrict__ nrm, int S, int chunk,
+                                                   float* __restrict__ partial) {
+    extern __shared__ float smem[];
+    const CorTile tile = cor_tile(c, smem, P);
+    const int gblk = blockIdx.x, nblk = gridDim.x;
+    int sc = 0;
+    while (sc + 1 < set.count && gblk >= set.first_block[sc + 1]) ++sc;
+    const int A = set.apertures[sc];
+    const float* __restrict__ pos = set.positions[sc];
+    const int a0 = (gblk - set.first_block[sc]) * kCorApertures;
+    load_tile(c, tile, P, set.residuals[sc], A, a0);
+    const int tid = threadIdx.x, wave = tid >> 6, half = (tid >> 5) & 1, al = tid & 31;
+    const int a = min(a0 + al, A - 1);
+    const bool aperture_ok = a0 + al < A;
+    const float px = pos[3 * a], py = pos[3 * a + 1], pz = pos[3 * a + 2];
+    const int s_begin = blockIdx.y * chunk, s_end = min(S, s_begin + chunk);
+    for (int s = s_begin + 2 * wave + half; s < s_end; s += 8) {
+        Pair q = pair_geometry(c, px, py, pz, ctr[3 * s], ctr[3 * s + 1], ctr[3 * s + 2], nrm[3 * s], nrm[3 * s + 1],
+                               nrm[3 * s + 2]);
+        if (!aperture_ok) q.amp = 0.0f;
+        const Sums z = pair_sums<P, false>(c, tile, al, q);
+        const float2 corr = cmulc(q.rot, z.g0);
+        float v[3] = {q.amp * corr.x, q.amp * corr.y, q.amp * q.amp * z.nrm};
+        half_wave_sum<3>(v);
+        if (al == 0) {
+            float* dst = partial + ((size_t)s * nblk + gblk) * 3;
+            dst[0] = v[0];
+            dst[1] = v[1];
+            dst[2] = v[2];
+        }
+    }
+}
+
+// out[scan][s][3] = sum of partial[s][b][3] over the scan's blocks, float64, fixed order.
+__global__ __launch_bounds__(256) void scan_reduce(const float* __restrict__ partial, int S, int nblk,
+                                                   const ScanSet set, float* __restrict__ out) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int sc = blockIdx.y;
+    if (row >= S) return;
+    const int b0 = set.first_block[sc], b1 = set.first_block[sc + 1];
+    for (int f = 0; f < 3; ++f) {
+        double acc = 0.0;
+        for (int b = b0 + lane; b < b1; b += 64) acc += partial[((size_t)row * nblk + b) * 3 + f];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) out[((size_t)sc * S + row) * 3 + f] = (float)acc;
+    }
+}
+
+template <int P>
+static hipError_t launch_scan_p(const Chirp& c, const ScanSet& set, const float* ctr, const float* nrm, int S,
+                                int chunk, float* partial, float* out, hipStream_t st) {
+    const int nblk = set.first_block[set.count];
+    const size_t smem = cor_smem(c);
+    hipError_t e = allow_lds((const void*)scan_kernel<P>, smem);
+    if (e != hipSuccess) return e;
+    const int groups = (S + chunk - 1) / chunk;
+    hipLaunchKernelGGL((scan_kernel<P>), dim3(nblk, groups), dim3(256), smem, st, c, set, ctr, nrm, S, chunk, partial);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_reduce, dim3((S + 3) / 4, set.count), dim3(256), 0, st, partial, S, nblk, set, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(const Chirp& c, const ScanSet& set, const float* ctr, const float* nrm, int S, int chunk,
+                       float* partial, float* out, hipStream_t st) {
+    switch (c.P) {
+        case 1: return launch_scan_p<1>(c, set, ctr, nrm, S, chunk, partial, out, st);
+        case 2: return launch_scan_p<2>(c, set, ctr, nrm, S, chunk, partial, out, st);
+        case 4: return launch_scan_p<4>(c, set, ctr, nrm, S, chunk, partial, out, st);
+        case 8: return launch_scan_p<8>(c, set, ctr, nrm, S, chunk, partial, out, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 // out[s, f] = sum over blocks of partial[s, blk, f] in float64, fixed order (one wave per row).
 __global__ __launch_bounds__(256) void reduce_partials(const float* __restrict__ partial, int rows, int nblk,
                                                        int width, float* __restrict__ out) {

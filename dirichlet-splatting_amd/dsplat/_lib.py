@@ -91,6 +91,10 @@ SIGNATURES = {
     "dsplat_energy": (c_int, [c_ptr, c_ptr, c_size, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_range_response": (c_int, [c_ptr, c_double, c_double, c_int, c_int, c_int, c_int, c_ptr]),
     "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
+    "dsplat_scan_workspace": (c_size, [c_int, c_ptr, c_int]),
+    "dsplat_scan_batched": (
+        c_int, [c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]
+    ),
     "dsplat_gram_workspace": (c_size, [c_int, c_int]),
     "dsplat_gram": (
         c_int,

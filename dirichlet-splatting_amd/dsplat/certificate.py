@@ -189,3 +189,26 @@ def search_certificate(
         scores = certificate(full_model, combined, full_residual, mode)
     order = scores.argsort(descending=True, stable=True)
     return {key: v[order] for key, v in combined.items()}, scores[order]
+
+
+@torch.no_grad()
+def certificate_scan(models, residuals, candidates, mode="complex", top=256, chunk=None):
+    """Certificates of every candidate against several scans of one chirp in one batched
+    launch, and the `top` best per scan (stable: ties keep the lower index).
+
+    models are FMCWModels sharing the chirp and material; residuals [A_i, K]. Returns
+    (scores [n, C] float32, top ids [n, top] int64).
+    """
+    from .kernels.surfel import scan_batched
+
+    config = models[0].scan.fmcw_config
+    scans = [(m.scan.scan_positions_m, r) for m, r in zip(models, residuals, strict=True)]
+    corr, norm = scan_batched(config, scans, candidates["centers"], candidates["normals"], models[0].fresnel_n, chunk)
+    area = candidates["areas"].double()
+    c = corr.to(torch.complex128) * area.sqrt()
+    n = norm.double() * area
+    power = torch.stack([r.abs().square().sum().double() for r in residuals]).clamp(min=1e-30)[:, None]
+    s = c.real if mode == "real" else c
+    scores = s.abs().square() / n.clamp(min=1e-60) / power
+    ids = torch.stack([torch.argsort(row, descending=True, stable=True)[:top] for row in scores])
+    return scores.float(), ids

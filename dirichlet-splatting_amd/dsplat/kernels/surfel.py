@@ -289,3 +289,41 @@ def correlate_gradient(config, positions, centers, normals, residual, fresnel_n,
         torch.complex(output[:, 3 : 3 + width], output[:, 3 + width : 3 + 2 * width]),
         output[:, 3 + 2 * width :],
     )
+
+
+SCAN_CHUNKS = (512, 2048, 8192, 32768)  # candidates per workgroup tried by the autotuner
+
+
+def scan_batched(config, scans, centers, normals, fresnel_n, chunk=None):
+    """corr [n, C] complex and norm [n, C] of every candidate against each of n scans
+    (positions [A_i, 3], residual [A_i, K]) in one launch over uneven aperture counts,
+    without area. chunk None autotunes the candidates per workgroup."""
+    import ctypes
+
+    from .. import autotune
+
+    _check(scans[0][0], centers, normals)
+    count, n = len(centers), len(scans)
+    positions = [p.float().contiguous() for p, _ in scans]
+    residuals = [interleaved(r.reshape(len(p), config.num_range_bins)) for p, r in scans]
+    pos_ptrs = (ctypes.c_void_p * n)(*[p.data_ptr() for p in positions])
+    res_ptrs = (ctypes.c_void_p * n)(*[r.data_ptr() for r in residuals])
+    apertures = (ctypes.c_int32 * n)(*[len(p) for p in positions])
+    size = _lib.library().dsplat_scan_workspace(n, apertures, count)
+    workspace = centers.new_empty(max(1, size // 4))
+    out = centers.new_empty((n, count, 3))
+    cfg = _lib.fmcw(config, fresnel_n)
+    centers, normals = centers.contiguous(), normals.contiguous()
+
+    def run(value):
+        _lib.call(
+            "dsplat_scan_batched", _lib.ctypes.byref(cfg), n, pos_ptrs, res_ptrs, apertures, _lib.ptr(centers),
+            _lib.ptr(normals), count, int(value), _lib.ptr(out), _lib.ptr(workspace), size,
+            _lib.stream(centers.device),
+        )  # fmt: skip
+
+    if chunk is None:
+        key = f"C{count}|A{'-'.join(str(len(p)) for p in positions)}|K{config.num_range_bins}|P{config.pad_factor}"
+        chunk = autotune.tune("scan_batched", key, list(SCAN_CHUNKS), run, centers.device)
+    run(chunk)
+    return torch.complex(out[..., 0], out[..., 1]), out[..., 2]

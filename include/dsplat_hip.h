@@ -118,6 +118,21 @@ int dsplat_surfel_correlate_orientations(const dsplat_fmcw* cfg, const float* po
                                          const float* norm_table, int32_t oversample, float* out,
                                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* Workspace bytes of dsplat_scan_batched. */
+size_t dsplat_scan_workspace(int32_t num_scans, const int32_t* apertures, int32_t num_candidates);
+
+/* Certificate scan of candidates against up to 8 scans of one chirp with uneven aperture
+ * counts, in one launch: out [num_scans, C, 3] = (corr.re, corr.im, norm) of every
+ * candidate against each scan's residual, without area. positions[i] [A_i, 3] and
+ * residuals[i] [A_i, K] complex are device pointers held in host arrays; apertures[i] =
+ * A_i (host). chunk = candidates per workgroup (0: a fill heuristic; the Python side
+ * autotunes it). No counterpart launch in the reference, which scores one scan at a
+ * time through kernels/surfel.py:242-260 (correlate) in certificate.py:11-31. */
+int dsplat_scan_batched(const dsplat_fmcw* cfg, int32_t num_scans, const float* const* positions,
+                        const float* const* residuals, const int32_t* apertures, const float* centers,
+                        const float* normals, int32_t num_candidates, int32_t chunk, float* out,
+                        void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- Spectral atoms: separable windowed Dirichlet kernels on a 1-3-D grid, bin units.
  * dim axes with samples[dim], nfft[dim]; window 0 rect, 1 hann, 2 hamming, 3 kaiser
  * (then taps [dim, taps_stride] are the normalized window taps); radius > 0 truncates.
