@@ -301,12 +301,19 @@ int dsplat_search_range_gate(const float* points, int32_t count, const float* po
                              (hipStream_t)stream);
 }
 
-int dsplat_search_peaks(float* scores_scratch, const float* centers, int32_t count, int32_t dim, int32_t peaks,
-                        float separation, int32_t* out, void* stream) {
-    if (count < 0 || peaks < 0 || dim < 1) return hipErrorInvalidValue;
+size_t dsplat_search_peaks_workspace(int32_t count, int32_t peaks, int32_t dim) {
+    if (count < 0 || peaks < 0 || dim < 1 || dim > 8) return 0;
+    return peaks_workspace(count, peaks, dim);
+}
+
+int dsplat_search_peaks(const float* scores, const float* centers, int32_t count, int32_t dim, int32_t peaks,
+                        float separation, int32_t* out, void* workspace, size_t workspace_bytes, void* stream) {
+    if (count < 0 || peaks < 0 || dim < 1 || dim > 8) return hipErrorInvalidValue;
     if (peaks == 0) return hipSuccess;
-    if (!out || (count > 0 && (!scores_scratch || !centers))) return hipErrorInvalidValue;
-    return launch_peaks(scores_scratch, centers, count, peaks, dim, separation, out, (hipStream_t)stream);
+    if (!out || !workspace || (count > 0 && (!scores || !centers))) return hipErrorInvalidValue;
+    if (workspace_bytes < peaks_workspace(count, peaks, dim)) return hipErrorInvalidValue;
+    return launch_peaks(scores, centers, count, peaks, dim, separation, out, workspace, workspace_bytes,
+                        (hipStream_t)stream);
 }
 
 int dsplat_search_direction(const float* inverse, const float* gradient, const float* normals, const float* mask,
@@ -349,11 +356,17 @@ int dsplat_range_response(double* out, double low, double high, int32_t count, i
 }
 
 // ------------------------------------------------------------------------------------ blas
-int dsplat_syrk(const float* rows, int32_t m, int32_t n, float* out, void* stream) {
-    if (m < 0 || n < 0) return hipErrorInvalidValue;
+size_t dsplat_syrk_workspace(int32_t m, int32_t n, int32_t mode) {
+    if (m < 0 || n < 0 || mode < 0 || mode > 2) return 0;
+    return syrk_workspace(m, n, mode);
+}
+
+int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* out, void* workspace,
+                size_t workspace_bytes, void* stream) {
+    if (m < 0 || n < 0 || mode < 0 || mode > 2) return hipErrorInvalidValue;
     if (m == 0 || n == 0) return hipSuccess;
-    if (!rows || !out) return hipErrorInvalidValue;
-    return launch_syrk(rows, m, n, out, (hipStream_t)stream);
+    if (!rows || !out || !workspace || workspace_bytes < syrk_workspace(m, n, mode)) return hipErrorInvalidValue;
+    return launch_syrk(rows, m, n, mode, out, workspace, (hipStream_t)stream);
 }
 
 size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels) {

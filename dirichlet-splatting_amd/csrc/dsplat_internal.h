@@ -76,8 +76,9 @@ hipError_t launch_spectral_backward(const SpecGrid& g, const float* taps, const 
                                     hipStream_t st);
 hipError_t launch_range_gate(const float* pts, const float* pos, const int* intervals, int count, int apertures,
                              int bins, float spacing, int required, int* out, hipStream_t st);
-hipError_t launch_peaks(float* scores, const float* ctr, int count, int peaks, int dim, float separation, int* out,
-                        hipStream_t st);
+size_t peaks_workspace(int count, int peaks, int dim);
+hipError_t launch_peaks(const float* scores, const float* ctr, int count, int peaks, int dim, float separation,
+                        int* out, void* workspace, size_t bytes, hipStream_t st);
 hipError_t launch_direction(const float* inv, const float* grad, const float* nrm, const float* mask, int count,
                             int width, float radius, float* out, hipStream_t st);
 hipError_t launch_trial(const float* ctr, const float* nrm, const float* dir, const float* ctl, int count, int width,
@@ -95,7 +96,8 @@ GramShape gram_shape(int A, int S);
 hipError_t launch_gram(const Chirp& c, int mode, const float* pos, int A, const float* ctr, const float* nrm,
                        const float* areas, const float* col_exp, int S, int real_only, float inv_rows,
                        float* partial, float* out, hipStream_t st);
-hipError_t launch_syrk(const float* rows, int m, int n, float* out, hipStream_t st);
+size_t syrk_workspace(int m, int n, int mode);
+hipError_t launch_syrk(const float* rows, int m, int n, int mode, float* out, void* workspace, hipStream_t st);
 
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st);

@@ -2,6 +2,9 @@
 // batched quasi-Newton direction and trial step (reference: dsplat/kernels/search.slang:1-109,
 // bindings dsplat/kernels/search.py:1-77).
 #include <float.h>
+#include <stdint.h>
+
+#include <hipcub/hipcub.hpp>
 
 #include "dsplat_internal.h"
 
@@ -29,52 +32,106 @@ __global__ __launch_bounds__(256) void range_gate_kernel(const float* __restrict
     out[i] = matches >= required ? 1 : 0;
 }
 
-// Greedy best-first separated peaks in one workgroup; scores is a scratch copy the
-// kernel overwrites (search.slang:27-64). Ties go to the lower index.
-__global__ __launch_bounds__(256) void peaks_kernel(float* __restrict__ scores, const float* __restrict__ ctr,
-                                                    int count, int peaks, int dim, float separation,
-                                                    int* __restrict__ out) {
-    __shared__ float best_v[256];
-    __shared__ int best_i[256];
-    const int lane = threadIdx.x;
-    const float sep2 = separation * separation;
-    for (int p = 0; p < peaks; ++p) {
-        float best = -INFINITY;
-        int index = -1;
-        for (int i = lane; i < count; i += 256) {
-            const float v = scores[i];
-            if (isfinite(v) && (v > best || index < 0)) {  // ascending i: first max wins
-                best = v;
-                index = i;
-            }
-        }
-        best_v[lane] = best;
-        best_i[lane] = index;
-        __syncthreads();
-        for (int stride = 128; stride > 0; stride >>= 1) {
-            if (lane < stride) {
-                const float v = best_v[lane + stride];
-                const int o = best_i[lane + stride];
-                if (o >= 0 && (best_i[lane] < 0 || v > best_v[lane] || (v == best_v[lane] && o < best_i[lane]))) {
-                    best_v[lane] = v;
-                    best_i[lane] = o;
-                }
-            }
-            __syncthreads();
-        }
-        index = best_i[0];
-        if (lane == 0) out[p] = index;
-        if (index < 0) return;  // uniform: every lane read the same best_i[0]
-        for (int i = lane; i < count; i += 256) {
-            float d2 = 0.0f;
-            for (int j = 0; j < dim; ++j) {
-                const float d = ctr[(size_t)i * dim + j] - ctr[(size_t)index * dim + j];
-                d2 = fmaf(d, d, d2);
-            }
-            if (i == index || d2 < sep2) scores[i] = -INFINITY;
-        }
-        __syncthreads();
+// Greedy best-first separated peaks (search.slang:27-64): repeatedly take the best finite
+// score (ties to the lower index) and drop every candidate closer than the separation.
+// The same selection is made here from the candidates sorted by (score desc, index asc):
+// a candidate is taken iff no earlier taken one lies within the separation. Blocks of 256
+// sorted candidates are resolved at a time: each thread tests its candidate against all
+// peaks taken so far, the block's pairwise conflicts go into bitmasks, and one thread
+// walks the block in order. Distances are formed exactly as in the serial definition,
+// d = c_candidate - c_taken accumulated by fmaf over the coordinates.
+
+constexpr int kPeakBlock = 256;
+constexpr int kPeakLdsFloats = 8192;  // taken centers cached in LDS (32 KB); the rest in the spill
+
+__global__ __launch_bounds__(256) void peak_keys(const float* __restrict__ scores, int count,
+                                                 float* __restrict__ keys, int* __restrict__ ids) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const float v = scores[i];
+    keys[i] = isfinite(v) ? v + 0.0f : -INFINITY;  // +0.0 folds -0.0 into 0.0: equal scores tie
+    ids[i] = i;
+}
+
+__device__ __forceinline__ float peak_d2(const float* a, const float* b, int dim) {
+    float d2 = 0.0f;
+    for (int j = 0; j < dim; ++j) {
+        const float d = a[j] - b[j];
+        d2 = fmaf(d, d, d2);
     }
+    return d2;
+}
+
+__global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restrict__ keys,
+                                                           const int* __restrict__ order,
+                                                           const float* __restrict__ ctr, int count, int peaks,
+                                                           int dim, float sep2, float* __restrict__ spill,
+                                                           int* __restrict__ out) {
+    __shared__ float taken_lds[kPeakLdsFloats];
+    __shared__ float block_c[kPeakBlock * 8];
+    __shared__ unsigned long long conflict[kPeakBlock][kPeakBlock / 64];
+    __shared__ int alive[kPeakBlock];
+    __shared__ int n_taken, finished;
+    const int tid = threadIdx.x;
+    const int lds_cap = kPeakLdsFloats / dim;
+    if (tid == 0) {
+        n_taken = 0;
+        finished = 0;
+    }
+    __syncthreads();
+    for (int base = 0; base < count; base += kPeakBlock) {
+        const int p = base + tid;
+        const bool valid = p < count && keys[p] != -INFINITY;
+        const int id = valid ? order[p] : -1;
+        float c[8];
+        for (int j = 0; j < dim; ++j) c[j] = valid ? ctr[(size_t)id * dim + j] : 0.0f;
+        bool ok = valid;
+        const int n = n_taken;
+        for (int q = 0; q < n && ok; ++q) {
+            const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
+            if (peak_d2(c, t, dim) < sep2) ok = false;
+        }
+        alive[tid] = ok;
+        for (int j = 0; j < dim; ++j) block_c[tid * dim + j] = c[j];
+        __syncthreads();
+        unsigned long long row[kPeakBlock / 64] = {0ull, 0ull, 0ull, 0ull};
+        if (ok)
+            for (int q = 0; q < tid; ++q)
+                if (alive[q] && peak_d2(c, block_c + q * dim, dim) < sep2) row[q >> 6] |= 1ull << (q & 63);
+        for (int w = 0; w < kPeakBlock / 64; ++w) conflict[tid][w] = row[w];
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long mine[kPeakBlock / 64] = {0ull, 0ull, 0ull, 0ull};
+            int k = n_taken;
+            for (int t = 0; t < kPeakBlock && k < peaks; ++t) {
+                if (!alive[t]) continue;
+                bool clash = false;
+                for (int w = 0; w < kPeakBlock / 64; ++w) clash |= (conflict[t][w] & mine[w]) != 0ull;
+                if (clash) continue;
+                mine[t >> 6] |= 1ull << (t & 63);
+                out[k] = order[base + t];
+                float* dst = k < lds_cap ? taken_lds + k * dim : spill + (size_t)(k - lds_cap) * dim;
+                for (int j = 0; j < dim; ++j) dst[j] = block_c[t * dim + j];
+                ++k;
+            }
+            n_taken = k;
+            // sorted keys: a non-finite key ends the finite candidates
+            const int last = min(count, base + kPeakBlock) - 1;
+            finished = k >= peaks || keys[last] == -INFINITY;
+        }
+        __syncthreads();
+        if (finished) break;
+    }
+    for (int k = n_taken + tid; k < peaks; k += kPeakBlock) out[k] = -1;
+}
+
+size_t peaks_workspace(int count, int peaks, int dim) {
+    size_t sort_bytes = 0;
+    const hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(
+        nullptr, sort_bytes, (const float*)nullptr, (float*)nullptr, (const int*)nullptr, (int*)nullptr, count);
+    if (e != hipSuccess) return 0;
+    const size_t arrays = 4 * (size_t)count * sizeof(float);
+    return 256 + arrays + sizeof(float) * (size_t)peaks * dim + sort_bytes;
 }
 
 // -H^{-1} g (or -g if not a descent direction), masked, tangent to unit normals and
@@ -132,9 +189,30 @@ hipError_t launch_range_gate(const float* pts, const float* pos, const int* inte
     return hipGetLastError();
 }
 
-hipError_t launch_peaks(float* scores, const float* ctr, int count, int peaks, int dim, float separation, int* out,
-                        hipStream_t st) {
-    hipLaunchKernelGGL(peaks_kernel, dim3(1), dim3(256), 0, st, scores, ctr, count, peaks, dim, separation, out);
+hipError_t launch_peaks(const float* scores, const float* ctr, int count, int peaks, int dim, float separation,
+                        int* out, void* workspace, size_t bytes, hipStream_t st) {
+    char* w = static_cast<char*>(workspace);
+    float* keys_in = reinterpret_cast<float*>(w);
+    float* keys = keys_in + count;
+    int* ids_in = reinterpret_cast<int*>(keys + count);
+    int* order = ids_in + count;
+    float* spill = reinterpret_cast<float*>(order + count);
+    char* temp = reinterpret_cast<char*>(spill + (size_t)peaks * dim);
+    temp += (256 - reinterpret_cast<uintptr_t>(temp) % 256) % 256;
+    size_t temp_bytes = 0;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, keys_in, keys, ids_in, order,
+                                                                count, 0, 32, st);
+    if (e != hipSuccess) return e;
+    if (temp + temp_bytes > w + bytes) return hipErrorInvalidValue;
+    if (count > 0) {
+        hipLaunchKernelGGL(peak_keys, dim3((count + 255) / 256), dim3(256), 0, st, scores, count, keys_in, ids_in);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        e = hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys, ids_in, order, count, 0,
+                                                         32, st);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(peaks_kernel, dim3(1), dim3(kPeakBlock), 0, st, keys, order, ctr, count, peaks, dim,
+                       separation * separation, spill, out);
     return hipGetLastError();
 }
 

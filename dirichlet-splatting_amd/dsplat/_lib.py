@@ -84,13 +84,15 @@ SIGNATURES = {
     "dsplat_search_range_gate": (
         c_int, [c_ptr, c_int, c_ptr, c_int, c_ptr, c_int, c_float, c_int, c_ptr, c_ptr]
     ),
-    "dsplat_search_peaks": (c_int, [c_ptr, c_ptr, c_int, c_int, c_int, c_float, c_ptr, c_ptr]),
+    "dsplat_search_peaks_workspace": (c_size, [c_int, c_int, c_int]),
+    "dsplat_search_peaks": (c_int, [c_ptr, c_ptr, c_int, c_int, c_int, c_float, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_search_direction": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_float, c_ptr, c_ptr]),
     "dsplat_search_trial": (c_int, [c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_float, c_ptr, c_ptr, c_ptr]),
     "dsplat_energy_workspace": (c_size, [c_size]),
     "dsplat_energy": (c_int, [c_ptr, c_ptr, c_size, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_range_response": (c_int, [c_ptr, c_double, c_double, c_int, c_int, c_int, c_int, c_ptr]),
-    "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
+    "dsplat_syrk_workspace": (c_size, [c_int, c_int, c_int]),
+    "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_scan_workspace": (c_size, [c_int, c_ptr, c_int]),
     "dsplat_scan_batched": (
         c_int, [c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]

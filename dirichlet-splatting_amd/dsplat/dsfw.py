@@ -3,8 +3,8 @@
 Same interface as dsplat/dsfw.py of the reference (dsfw.py:23-174). Every function
 takes a state and returns a new one. Defaults are the fast 3-D configuration:
 signed-real coefficients, merged replacement, joint Gauss-Newton on 1024 apertures.
-DSFWConfig.gram selects the Gram backend of varpro ("plain" FP32 or the fused
-split-precision "bf16x3" / "fp16x3" / "fp32" kernels).
+DSFWConfig.gram selects the SYRK of varpro's normal equations ("plain" FP32 MFMA, or the
+split-precision "bf16x3" / "fp16x3" MFMA products).
 """
 
 from dataclasses import dataclass, replace
@@ -42,7 +42,7 @@ class DSFWConfig:
     row_chunk: int = 65536
     dictionary_elements: int = 32_000_000
     seed: int = 0
-    gram: str = "plain"  # normal-equation backend: plain | fp32 | bf16x3 | fp16x3
+    gram: str = "plain"  # normal-equation SYRK: plain (FP32) | bf16x3 | fp16x3
 
     @property
     def real(self):

@@ -6,15 +6,25 @@ import torch
 from .. import _lib
 
 
-def syrk(rows, out):
+SYRK_MODES = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
+
+
+def syrk(rows, out, mode="fp32"):
     """Add rows^T rows to the upper triangle of out [n, n]; the lower triangle is untouched
-    (kernels/cublas.py:31-44). rows [m, n] float32, out float32 contiguous."""
+    (kernels/cublas.py:31-44). rows [m, n] float32, out float32 contiguous. mode: fp32
+    (FP32 MFMA), bf16x3 or fp16x3 (hi + lo split, three MFMA products, FP32 accumulation)."""
     if rows.dtype != torch.float32 or out.dtype != torch.float32:
         raise TypeError("syrk takes float32 tensors")
     if not out.is_contiguous() or out.shape != (rows.shape[1], rows.shape[1]):
         raise ValueError("out must be a contiguous [n, n] tensor")
+    code = SYRK_MODES[mode]
     rows = rows.contiguous()
-    _lib.call("dsplat_syrk", _lib.ptr(rows), rows.shape[0], rows.shape[1], _lib.ptr(out), _lib.stream(rows.device))
+    m, n = rows.shape
+    size = _lib.library().dsplat_syrk_workspace(m, n, code)
+    workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=rows.device)
+    _lib.call(
+        "dsplat_syrk", _lib.ptr(rows), m, n, code, _lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(rows.device)
+    )
     return out
 
 

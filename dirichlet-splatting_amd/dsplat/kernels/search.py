@@ -22,15 +22,19 @@ def range_gate(points, positions, intervals, spacing, min_fraction):
 
 
 def peaks(scores, centers, count, separation):
-    """Greedy best-first indices at least separation apart, in one workgroup; scores are
-    not modified (kernels/search.py:27-39)."""
+    """Greedy best-first indices at least separation apart; scores are not modified
+    (kernels/search.py:27-39). The kernel sorts once and resolves blocks of sorted
+    candidates instead of one arg-max sweep per peak."""
     output = torch.full((min(count, len(scores)),), -1, device=centers.device, dtype=torch.int32)
     if len(output) == 0:
         return output.long()
-    scratch = scores.float().clone()
+    centers = centers.float().contiguous()
+    dim = centers.shape[1]
+    size = _lib.library().dsplat_search_peaks_workspace(len(scores), len(output), dim)
+    workspace = torch.empty(max(1, size), dtype=torch.uint8, device=centers.device)
     _lib.call(
-        "dsplat_search_peaks", _lib.ptr(scratch), _lib.ptr(centers.float().contiguous()), len(scores),
-        centers.shape[1], len(output), float(separation), _lib.ptr(output), _lib.stream(centers.device),
+        "dsplat_search_peaks", _lib.ptr(scores.float().contiguous()), _lib.ptr(centers), len(scores), dim,
+        len(output), float(separation), _lib.ptr(output), _lib.ptr(workspace), size, _lib.stream(centers.device),
     )  # fmt: skip
     return output[output >= 0].long()
 

@@ -155,16 +155,13 @@ def pooled_statistics(state, pool, config):
     gram = state.gram.new_empty((total, total))
     rhs = state.rhs.new_empty(total)
     gram[:count, :count], rhs[:count] = state.gram, state.rhs
-    if hasattr(model, "fused_cross_statistics") and getattr(config, "gram", "plain") != "plain":
-        cross, right = model.fused_cross_statistics(pool, count, state.target, config)
-    else:
-        cross = state.gram.new_zeros((total, added))
-        right = state.rhs.new_zeros(added)
-        for a, y in model.blocks(pool, state.target, config.real, row_chunk(model, total, config)):
-            new = a[:, count:]
-            cross.addmm_(a.H, new)
-            right.addmv_(new.H, y)
-        cross, right = cross / model.size, right / model.size
+    cross = state.gram.new_zeros((total, added))
+    right = state.rhs.new_zeros(added)
+    for a, y in model.blocks(pool, state.target, config.real, row_chunk(model, total, config)):
+        new = a[:, count:]
+        cross.addmm_(a.H, new)
+        right.addmv_(new.H, y)
+    cross, right = cross / model.size, right / model.size
     gram[:, count:], gram[count:, :] = cross, cross.H
     gram[count:, count:] = (cross[count:] + cross[count:].H) * 0.5
     rhs[count:] = right

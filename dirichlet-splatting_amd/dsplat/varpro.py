@@ -4,11 +4,12 @@ Objective mean(|A b - y|^2) + ridge |b|^2 (varpro.py:1-6 of the reference). Comp
 coefficients use the full Gram G = A^H A / M, signed-real ones Re(G) and Re(q). A new
 geometry is accepted only if its refitted objective is lower.
 
-Gram backends (DSFWConfig.gram):
-  "plain"   the reference's stream: row-projected design blocks, FP32 SYRK on MFMA,
-            folded into an FP64 total every FOLD blocks (varpro.py:82-120)
-  "fused"   one launch per row chunk that generates the atoms on chip and contracts
-            them on split-precision MFMA (kernels.blas.gram); see blas.gram
+Gram backends (DSFWConfig.gram) of the real-coefficient normal equations: the row
+stream of the reference (row-projected design blocks, a SYRK per block, folded into an
+FP64 total every FOLD blocks; varpro.py:82-120) with the SYRK on
+  "plain"   FP32 MFMA operands (the reference's cuBLAS SSYRK)
+  "bf16x3"  BF16 hi + lo split operands, three MFMA products, FP32 accumulation
+  "fp16x3"  FP16 hi + lo split operands with per-column scaling, likewise
 """
 
 from dataclasses import replace
@@ -81,7 +82,11 @@ def row_chunk(model, columns, config):
     return max(model.row_alignment, chunk - chunk % model.row_alignment)
 
 
-def _plain_normal_equations(model, params, target, config):
+SYRK_MODE = {"plain": "fp32", "fp32": "fp32", "bf16x3": "bf16x3", "fp16x3": "fp16x3"}
+
+
+def normal_equations(model, params, target, config):
+    """G = A^H A / M and q = A^H y / M (Re of both for real coefficients)."""
     count = len(params["centers"])
     stream = model.blocks(params, target, config.real, row_chunk(model, count, config))
     if not config.real:
@@ -91,12 +96,13 @@ def _plain_normal_equations(model, params, target, config):
             gram.addmm_(a.H, a)
             rhs.addmv_(a.H, y)
         return (gram + gram.H) / (2 * model.size), rhs / model.size
+    mode = SYRK_MODE[getattr(config, "gram", "plain")]
     device = target.device
     part = torch.zeros((count, count), dtype=torch.float32, device=device)
     total = torch.zeros((count, count), dtype=torch.float64, device=device)
     rhs = torch.zeros(count, dtype=torch.float64, device=device)
     for index, (a, y) in enumerate(stream):
-        syrk(a.contiguous(), part)
+        syrk(a.contiguous(), part, mode)
         rhs += a.T @ y
         if (index + 1) % FOLD == 0:
             total += part
@@ -105,14 +111,6 @@ def _plain_normal_equations(model, params, target, config):
     upper = torch.triu(total)
     gram = (upper + torch.triu(total, 1).T) / model.size
     return gram.float(), (rhs / model.size).float()
-
-
-def normal_equations(model, params, target, config):
-    """G = A^H A / M and q = A^H y / M (Re of both for real coefficients)."""
-    backend = getattr(config, "gram", "plain")
-    if backend == "plain" or not hasattr(model, "fused_normal_equations"):
-        return _plain_normal_equations(model, params, target, config)
-    return model.fused_normal_equations(params, target, config)
 
 
 def fit_coefficients(gram, rhs, config):

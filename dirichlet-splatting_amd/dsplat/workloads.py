@@ -8,7 +8,8 @@ same inputs; tensors are returned on the CPU. Sizes:
                   30/30/25 mm ellipsoid at (0,0,0.35), area = ellipsoid area / 4096
   certificate     262144 candidates on a 64^3 grid over the 70 mm box, scans of
                   64^2, 96^2 and 128^2 apertures
-  dsfw_truth      4000 ellipsoid samples (front faces kept), 256 ADC samples
+  dsfw_problem    4000 ellipsoid draws (front faces kept) or 2048 front faces, 128x128
+                  array, float64 synthesis + 30 dB noise, random initial surfels
 """
 
 import math
@@ -127,3 +128,53 @@ def certificate_candidates(per_axis=64, seed=4):
     grid = torch.stack(torch.meshgrid(axis, axis, axis, indexing="ij"), -1).reshape(-1, 3)
     centers = grid + torch.tensor(CENTER)
     return centers, hemisphere_normals(len(centers), _gen(seed))
+
+
+def dsfw_problem(adc_samples=256, truth_draws=4000, front_count=None, snr_db=30.0, device="cuda"):
+    """A DSFW fit: a float64 scan of ellipsoid surfels on the 128x128 array plus complex
+    Gaussian noise (seed 5, sigma^2 = mean|y|^2 / 10^(snr/10)), and an initial guess of as
+    many surfels uniform in the truth's box with 5 mm margin and random normals (seed 1).
+
+    truth_draws area-uniform draws keep the front faces (about half; area = ellipsoid
+    area / draws), or, with front_count, exactly that many front faces (area = ellipsoid
+    area / (2 front_count)). Returns dict(model, target, initial, parameters, truth).
+    """
+    from .fmcw import FMCWModel, Scan
+    from .groundtruth import synthesize
+    from .parameters import ParameterConfig
+
+    config = FMCWConfig(adc_samples=adc_samples)
+    if front_count is None:
+        centers, normals = ellipsoid_samples(truth_draws, 0, keep_all_draws=True)
+        area = ellipsoid_area(*SEMI_AXES) / truth_draws
+    else:
+        centers, normals = ellipsoid_samples(front_count, 0)
+        area = ellipsoid_area(*SEMI_AXES) / (2 * front_count)
+    centers, normals = centers.to(device), normals.to(device)
+    positions = planar_array(128, PITCH).to(device)
+    amplitudes = torch.full((len(centers),), area**0.5, device=device)
+    clean = synthesize(positions, centers, normals, amplitudes, config).data
+    sigma = float(clean.abs().square().mean().sqrt()) * 10 ** (-snr_db / 20)
+    target = clean + sigma * complex_normal(tuple(clean.shape), _gen(5)).to(device)
+    g = _gen(1)
+    low = centers.min(0).values.cpu() - 0.005
+    high = centers.max(0).values.cpu() + 0.005
+    count = len(centers)
+    initial = dict(
+        centers=(low + (high - low) * torch.rand(count, 3, generator=g)).to(device),
+        normals=torch.nn.functional.normalize(torch.randn(count, 3, generator=g), dim=1).to(device),
+        areas=torch.full((count,), area, device=device),
+    )
+    parameters = dict(
+        centers=ParameterConfig(scale=5e-4, lower=tuple(low.tolist()), upper=tuple(high.tolist())),
+        normals=ParameterConfig(scale=0.1, normalize=True),
+        areas=ParameterConfig(trainable=False),
+    )
+    return dict(
+        model=FMCWModel(Scan(target, positions, config)),
+        target=target,
+        initial=initial,
+        parameters=parameters,
+        truth=dict(centers=centers, normals=normals, area=area),
+        noise_sigma=sigma,
+    )

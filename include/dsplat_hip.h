@@ -179,11 +179,15 @@ int dsplat_search_range_gate(const float* points, int32_t count, const float* po
                              int32_t num_apertures, const int32_t* intervals, int32_t num_bins,
                              float spacing, int32_t required, int32_t* out, void* stream);
 
-/* Greedy separated peaks: out[peaks] int32 indices best first, -1 once exhausted.
- * scores_scratch [count] is overwritten (pass a copy). centers [count, dim].
+/* Greedy separated peaks: out[peaks] int32 indices best first, -1 once exhausted. Same
+ * selection as taking the best finite score (ties to the lower index) and dropping every
+ * candidate closer than separation, repeatedly. scores [count] are not modified;
+ * centers [count, dim], dim <= 8. Workspace from dsplat_search_peaks_workspace.
  * Replaces kernels/search.py:27-39 (candidate_peaks). */
-int dsplat_search_peaks(float* scores_scratch, const float* centers, int32_t count, int32_t dim,
-                        int32_t peaks, float separation, int32_t* out, void* stream);
+size_t dsplat_search_peaks_workspace(int32_t count, int32_t peaks, int32_t dim);
+int dsplat_search_peaks(const float* scores, const float* centers, int32_t count, int32_t dim,
+                        int32_t peaks, float separation, int32_t* out, void* workspace,
+                        size_t workspace_bytes, void* stream);
 
 /* Per candidate: -H^-1 g (or -g), masked, tangent to unit normals, in the trust radius.
  * inverse [C, w, w], gradient [C, w], normals [C, 3], mask [w], w = 3 or 6.
@@ -215,10 +219,15 @@ int dsplat_range_response(double* out, double low, double high, int32_t count, i
 
 /* ---- Dense products. */
 
-/* Upper triangle of out [n, n] (row-major) += rows^T rows, rows [m, n] row-major, FP32
- * on MFMA; the lower triangle is untouched. Replaces kernels/cublas.py:31-44 (syrk:
- * cublasSsyrk_v2 through ctypes). */
-int dsplat_syrk(const float* rows, int32_t m, int32_t n, float* out, void* stream);
+/* Upper triangle of out [n, n] (row-major) += rows^T rows, rows [m, n] row-major float32,
+ * on MFMA; the lower triangle is untouched. mode 0: FP32 operands; 1: BF16 hi + lo split
+ * (3 products); 2: FP16 hi + lo split with per-column power-of-two scaling (3 products);
+ * FP32 accumulation, float64 split-K reduction. Deterministic. Workspace from
+ * dsplat_syrk_workspace. Replaces kernels/cublas.py:31-44 (syrk: cublasSsyrk_v2 through
+ * ctypes). */
+size_t dsplat_syrk_workspace(int32_t m, int32_t n, int32_t mode);
+int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* out, void* workspace,
+                size_t workspace_bytes, void* stream);
 
 size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels);
 

@@ -90,6 +90,37 @@ def test_peaks_match_brute_force_greedy_suppression():
     assert select_peaks(scores.cuda(), centers.cuda(), 3, 0.0).tolist()[:2] == [17, 4000]
 
 
+@pytest.mark.parametrize("dim,count,separation", [(3, 500, 0.05), (2, 64, 0.2), (1, 40, 0.01), (3, 3000, 0.0)])
+def test_peaks_kernel_matches_serial_greedy_selection(dim, count, separation):
+    """Many sorted blocks, taken centers beyond the LDS cache (3000 x 3 floats), 1-D/2-D
+    points, non-finite scores that are never taken and a -0.0 / 0.0 tie."""
+    from dsplat.kernels import search
+
+    g = torch.Generator().manual_seed(dim + count)
+    n = 20000
+    centers = torch.rand(n, dim, generator=g)
+    scores = torch.rand(n, generator=g)
+    scores[::97] = float("nan")
+    scores[5::101] = float("inf")
+    scores[7::103] = -float("inf")
+    scores[11], scores[12] = 0.0, -0.0
+    scores[13] = scores[14] = 0.5  # an exact tie: the lower index wins
+    got = search.peaks(scores.cuda(), centers.cuda(), count, separation).cpu().tolist()
+    want = Se.peaks(scores.numpy(), centers.numpy(), count, separation)
+    assert got == want
+    assert torch.isnan(scores[0])  # the scores were not modified
+
+
+def test_peaks_kernel_stops_when_every_candidate_is_suppressed():
+    from dsplat.kernels import search
+
+    centers = torch.zeros(700, 3)
+    centers[:, 0] = torch.arange(700) * 1e-3
+    scores = torch.linspace(1, 2, 700)
+    got = search.peaks(scores.cuda(), centers.cuda(), 50, 10.0).cpu().tolist()
+    assert got == [699]
+
+
 def test_range_gate_matches_the_oracle_and_the_reference_known_answer():
     from dsplat.kernels import search
 
@@ -152,17 +183,51 @@ def test_range_response_matches_the_oracle():
     assert np.abs(got - want).max() < 1e-13
 
 
-@pytest.mark.parametrize("m,n", [(5, 7), (256, 64), (1000, 130), (4096, 513)])
-def test_syrk_updates_the_upper_triangle_only(m, n):
+SYRK_BARS = {"fp32": 2e-6, "bf16x3": 6e-5, "fp16x3": 2e-6}
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "fp16x3"])
+@pytest.mark.parametrize("m,n", [(5, 7), (256, 64), (1000, 130), (4096, 513), (20000, 300)])
+def test_syrk_updates_the_upper_triangle_only(m, n, mode):
+    """Every mode against the float64 product: the lower triangle is untouched, the upper
+    one gets rows^T rows within the mode's bar (relative to the largest entry)."""
     from dsplat.kernels.blas import syrk
 
     g = torch.Generator().manual_seed(m + n)
     rows = torch.randn(m, n, generator=g)
     out = torch.randn(n, n, generator=g)
     before = out.clone()
-    got = syrk(rows.cuda(), out.cuda()).cpu()
+    got = syrk(rows.cuda(), out.cuda(), mode).cpu()
     full = before.double() + rows.double().T @ rows.double()
     upper = torch.triu(torch.ones(n, n, dtype=torch.bool))
     assert torch.equal(got[~upper], before[~upper])
     err = (got.double() - full)[upper].abs().max() / full[upper].abs().max()
-    assert err < 1e-5
+    assert err < SYRK_BARS[mode]
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "fp16x3"])
+def test_split_syrk_keeps_columns_of_very_different_scale(mode):
+    """Columns scaled from 1e-12 to 1e12, and an all-zero column: each Gram entry is
+    within the bar of its own scale sqrt(G_ii G_jj) (FP16 needs the per-column scaling)."""
+    from dsplat.kernels.blas import syrk
+
+    g = torch.Generator().manual_seed(7)
+    n = 160
+    rows = torch.randn(3000, n, generator=g) * torch.logspace(-12, 12, n)
+    rows[:, 17] = 0
+    got = syrk(rows.cuda(), torch.zeros(n, n, device="cuda"), mode).cpu().double()
+    want = rows.double().T @ rows.double()
+    scale = want.diagonal().sqrt()
+    bound = scale[:, None] * scale[None, :]
+    upper = torch.triu(torch.ones(n, n, dtype=torch.bool)) & (bound > 0)
+    assert ((got - want).abs()[upper] / bound[upper]).max() < SYRK_BARS[mode]
+    assert torch.all(got[17] == 0) and torch.all(got[:, 17] == 0)
+
+
+def test_syrk_is_deterministic():
+    from dsplat.kernels.blas import syrk
+
+    rows = torch.randn(9000, 700, generator=torch.Generator().manual_seed(3)).cuda()
+    first = syrk(rows, torch.zeros(700, 700, device="cuda"), "fp16x3")
+    for _ in range(2):
+        assert torch.equal(syrk(rows, torch.zeros(700, 700, device="cuda"), "fp16x3"), first)
