@@ -207,6 +207,24 @@ int dsplat_surfel_correlate_orientations(const dsplat_fmcw* cfg, const float* po
                                static_cast<float*>(workspace), out, st);
 }
 
+int dsplat_surfel_orient_select(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                                const float* centers, const float* normals, const float* areas, int32_t num_surfels,
+                                const float* bank, int32_t num_normals, const float* residual,
+                                const float* range_table, const float* norm_table, int32_t oversample,
+                                int32_t real_part, double* best_value, int32_t* best_index, void* stream) {
+    Chirp c;
+    const int A = num_apertures, S = num_surfels, B = num_normals;
+    if (!make_chirp(cfg, &c) || A < 1 || S < 0 || B < 0 || B > 24) return hipErrorInvalidValue;
+    if (S == 0) return hipSuccess;
+    if (!best_value || !best_index || !areas || (B > 0 && !bank)) return hipErrorInvalidValue;
+    if (!positions || !centers || !normals) return hipErrorInvalidValue;
+    if (range_table ? (!norm_table || oversample < 1) : !residual) return hipErrorInvalidValue;
+    const OrientPick pick = {areas, real_part != 0, best_value, reinterpret_cast<int*>(best_index)};
+    return launch_orient_best(c, positions, A, centers, normals, bank, B, S, reinterpret_cast<const float2*>(residual),
+                              reinterpret_cast<const float2*>(range_table), norm_table, oversample, pick,
+                              (hipStream_t)stream);
+}
+
 // ------------------------------------------------------------------------------ spectral
 static bool make_grid(int dim, const int32_t* samples, const int32_t* nfft, int window, float radius, int taps_stride,
                       SpecGrid* g) {
@@ -356,6 +374,41 @@ int dsplat_range_response(double* out, double low, double high, int32_t count, i
 }
 
 // ------------------------------------------------------------------------------------ blas
+size_t dsplat_qn_workspace(const dsplat_fmcw* cfg, int32_t num_apertures) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0) return 0;
+    return qn_workspace(num_apertures, c.P) + 16;
+}
+
+int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* residual,
+                     const float* centers, const float* normals, const float* areas, int32_t count,
+                     const dsplat_qn_spec* spec, float* centers_out, float* normals_out, float* values_out,
+                     void* workspace, size_t workspace_bytes, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 1 || count < 0 || !spec) return hipErrorInvalidValue;
+    if (!(spec->width == 3 || spec->width == 6) || spec->iterations < 0 || spec->backtracks < 0)
+        return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    if (!positions || !residual || !centers || !normals || !areas || !centers_out || !normals_out || !values_out ||
+        !workspace || workspace_bytes < qn_workspace(num_apertures, c.P))
+        return hipErrorInvalidValue;
+    return launch_qn_search(c, positions, num_apertures, reinterpret_cast<const float2*>(residual), centers, normals,
+                            areas, count, *spec, centers_out, normals_out, values_out, workspace, (hipStream_t)stream);
+}
+
+size_t dsplat_elimination_workspace(int32_t n, int32_t count) { return elimination_workspace(n, count); }
+
+int dsplat_backward_elimination(const double* inverse, const double* b, int32_t n, int32_t count,
+                                const int32_t* limited, int32_t limit, int64_t* removed, void* workspace,
+                                size_t workspace_bytes, void* stream) {
+    if (n < 0 || count < 0) return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    const size_t need = elimination_workspace(n, count);
+    if (need == 0 || !inverse || !b || !removed || !workspace || workspace_bytes < need) return hipErrorInvalidValue;
+    return launch_elimination(inverse, b, n, count, limited, limit, reinterpret_cast<long long*>(removed), workspace,
+                              (hipStream_t)stream);
+}
+
 size_t dsplat_syrk_workspace(int32_t m, int32_t n, int32_t mode) {
     if (m < 0 || n < 0 || mode < 0 || mode > 2) return 0;
     return syrk_workspace(m, n, mode);

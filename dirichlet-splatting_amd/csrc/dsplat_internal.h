@@ -47,9 +47,19 @@ hipError_t launch_jacobian(const Chirp& c, const float* pos, int A, const float*
                            float2* col, float2* jac, hipStream_t st);
 hipError_t launch_correlate(const Chirp& c, int W, const float* pos, int A, const float* ctr, const float* nrm, int S,
                             const float2* res, float* partial, float* out, hipStream_t st);
+// Fused choice of the best orientation per candidate (instead of the [S, 1 + B, 3] sums).
+struct OrientPick {
+    const float* areas;
+    int real_part;
+    double* value;  // [S] |s|^2 / max(norm, 1e-60) of the best orientation
+    int* index;     // [S] 0 = own normal, 1 + b = bank normal b
+};
 hipError_t launch_orientations(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                const float* bank, int B, int S, const float2* res, const float2* table,
                                const float* norm_table, int oversample, float* partial, float* out, hipStream_t st);
+hipError_t launch_orient_best(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                              const float* bank, int B, int S, const float2* res, const float2* table,
+                              const float* norm_table, int oversample, const OrientPick& pick, hipStream_t st);
 
 // Spectral grid of up to three separable axes (bin units).
 struct SpecGrid {
@@ -96,6 +106,14 @@ GramShape gram_shape(int A, int S);
 hipError_t launch_gram(const Chirp& c, int mode, const float* pos, int A, const float* ctr, const float* nrm,
                        const float* areas, const float* col_exp, int S, int real_only, float inv_rows,
                        float* partial, float* out, hipStream_t st);
+typedef dsplat_qn_spec QnSpec;
+size_t qn_workspace(int A, int P);
+hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,
+                            const float* nrm, const float* areas, int count, const QnSpec& sp, float* oc, float* on,
+                            float* ov, void* workspace, hipStream_t st);
+size_t elimination_workspace(int n, int count);
+hipError_t launch_elimination(const double* W, const double* b, int n, int count, const int* limited, int limit,
+                              long long* removed, void* workspace, hipStream_t st);
 size_t syrk_workspace(int m, int n, int mode);
 hipError_t launch_syrk(const float* rows, int m, int n, int mode, float* out, void* workspace, hipStream_t st);
 

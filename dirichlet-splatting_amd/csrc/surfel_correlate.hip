@@ -13,15 +13,11 @@
 // fixed order. The reference adds them with float atomics instead (nondeterministic).
 #include "dsplat_device.h"
 #include "dsplat_internal.h"
+#include "dsplat_sums.h"
 
 namespace dsplat {
 
 constexpr int kCorApertures = 32;
-
-struct Sums {
-    float2 g0, g1;   // sum conj(D) res, sum conj(D') res
-    float nrm, rx;   // sum |D|^2, sum Re(conj(D') D)
-};
 
 // Residual tile, class sums of the residual, and the derivative cot table of one workgroup.
 struct CorTile {
@@ -60,83 +56,7 @@ __device__ __forceinline__ void load_tile(const Chirp& c, const CorTile& tile, i
 // Bin sums of one pair (lane = aperture al of the tile). GRAD adds g1 and rx.
 template <int P, bool GRAD>
 __device__ __forceinline__ Sums pair_sums(const Chirp& c, const CorTile& tile, int al, const Pair& q) {
-    const float t = tan_small(c.pi_N * q.delta);
-    const float* Tb = tile.T + (c.N - q.lobe);
-    const float2* Rb = tile.R + al;
-    float2 F[P], G[P];
-    float Q[P], C1[P], C3[P];
-#pragma unroll
-    for (int r = 0; r < P; ++r) {
-        F[r] = make_float2(0.f, 0.f);
-        G[r] = make_float2(0.f, 0.f);
-        Q[r] = 0.f;
-        C1[r] = 0.f;
-        C3[r] = 0.f;
-    }
-    for (int k0 = 0; k0 < c.K; k0 += 16) {
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int k = k0 + j;
-            if (k < c.K) {  // uniform
-                const float cot = cot_bin(Tb[k], t);
-                const float2 u = Rb[k * kCorApertures];
-                F[j % P].x = fmaf(cot, u.x, F[j % P].x);
-                F[j % P].y = fmaf(cot, u.y, F[j % P].y);
-                Q[j % P] = fmaf(cot, cot, Q[j % P]);
-                if (GRAD) {
-                    const float c2 = cot * cot;
-                    G[j % P].x = fmaf(c2, u.x, G[j % P].x);
-                    G[j % P].y = fmaf(c2, u.y, G[j % P].y);
-                    C1[j % P] += cot;
-                    C3[j % P] = fmaf(c2, cot, C3[j % P]);
-                }
-            }
-        }
-    }
-    const bool seen = q.lobe < c.K;
-    const int rl = q.lobe % P;
-    const float2 ul = seen ? tile.R[q.lobe * kCorApertures + al] : make_float2(0.f, 0.f);
-    float2 D, Dp;
-    lobe_direct(c, q.delta, &D, &Dp);
-    Sums out;
-    out.g0 = seen ? cmulc(D, ul) : make_float2(0.f, 0.f);
-    out.nrm = seen ? D.x * D.x + D.y * D.y : 0.f;
-    out.g1 = seen ? cmulc(Dp, ul) : make_float2(0.f, 0.f);
-    out.rx = seen ? cmulc(Dp, D).x : 0.f;
-    float sb[P], cb[P];
-    class_angles<P>(q.delta, q.lobe, sb, cb);
-    const float t2 = t * t, invS2 = c.inv_S * c.inv_S;
-#pragma unroll
-    for (int r = 0; r < P; ++r) {
-        float2 Ur = tile.Us[r * kCorApertures + al];
-        float2 Fr = F[r], Gr = G[r];
-        float Qr = Q[r], C1r = C1[r], C3r = C3[r];
-        float nr = (float)((c.K - r + P - 1) / P);
-        if (seen && r == rl) {  // the lobe bin saw cot = t: take it out
-            Ur -= ul;
-            Fr.x = fmaf(-t, ul.x, Fr.x);
-            Fr.y = fmaf(-t, ul.y, Fr.y);
-            Qr -= t2;
-            nr -= 1.0f;
-            if (GRAD) {
-                Gr.x = fmaf(-t2, ul.x, Gr.x);
-                Gr.y = fmaf(-t2, ul.y, Gr.y);
-                C1r -= t;
-                C3r = fmaf(-t2, t, C3r);
-            }
-        }
-        const float2 nu = make_float2(sb[r] * cb[r], -sb[r] * sb[r]);
-        const float nu2 = sb[r] * sb[r];
-        const float2 Fm = make_float2(Fr.x + Ur.y, Fr.y - Ur.x);  // F - i U
-        out.g0 += c.inv_S * cmulc(nu, Fm);
-        out.nrm += nu2 * invS2 * (nr + Qr);
-        if (GRAD) {
-            const float2 qr = make_float2(fmaf(-2.0f * sb[r], sb[r], 1.0f), -2.0f * sb[r] * cb[r]);
-            out.g1 += c.inv_S * (c.pi_P * cmulc(qr, Fm) - c.pi_N * cmulc(nu, Ur + Gr));
-            out.rx += invS2 * (c.pi_P * cmulc(qr, nu).x * (nr + Qr) - c.pi_N * nu2 * (C1r + C3r));
-        }
-    }
-    return out;
+    return bin_sums<P, GRAD, kCorApertures>(c, tile.T, tile.R + al, tile.Us + al, q);
 }
 
 template <int NV>
@@ -445,6 +365,142 @@ hipError_t launch_correlate(const Chirp& c, int W, const float* pos, int A, cons
     }
 }
 
+// Best orientation per candidate in one pass (lane = candidate): every aperture's pair
+// geometry and bin sum (or 4-tap range-table interpolation) is shared by the own normal
+// and the B bank normals, whose sums stay in registers; after the last aperture the
+// certificates |s|^2 / max(area |a|^2, 1e-60) (s = sqrt(area) a^H r, or its real part)
+// are formed in float64 and the first maximum is kept (NaN counts as largest). No
+// per-block partials. The aperture sums accumulate in float32 in aperture order.
+constexpr int kOrientMax = 25;  // own normal + up to 24 bank normals
+
+template <int P, bool TABLE>
+__global__ __launch_bounds__(256) void orient_best_kernel(
+    const Chirp c, const float* __restrict__ pos, int A, const float* __restrict__ ctr, const float* __restrict__ nrm,
+    const float* __restrict__ areas, const float* __restrict__ bank, int B, int S, const float2* __restrict__ res,
+    const float2* __restrict__ table, const float* __restrict__ norm_table, int oversample, int real_part,
+    double* __restrict__ best_value, int* __restrict__ best_index) {
+    extern __shared__ float smem[];
+    __shared__ float dirs[3 * kOrientMax];
+    const int tid = threadIdx.x;
+    const int s = blockIdx.x * 256 + tid;
+    const int sc = s < S ? s : S - 1;
+    const float cx = ctr[3 * sc], cy = ctr[3 * sc + 1], cz = ctr[3 * sc + 2];
+    const float nx = nrm[3 * sc], ny = nrm[3 * sc + 1], nz = nrm[3 * sc + 2];
+    for (int i = tid; i < 3 * B; i += 256) dirs[3 + i] = bank[i];
+    float acc[kOrientMax][3];
+#pragma unroll
+    for (int j = 0; j < kOrientMax; ++j) acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
+    const float fres = c.fresnel_n;
+    auto accumulate = [&](const Pair& q, float2 corr, float zn) {
+#pragma unroll
+        for (int j = 0; j < kOrientMax; ++j) {
+            if (j > B) break;  // uniform
+            const float mx = j == 0 ? nx : dirs[3 * j], my = j == 0 ? ny : dirs[3 * j + 1];
+            const float mz = j == 0 ? nz : dirs[3 * j + 2];
+            const float cosine = q.ux * mx + q.uy * my + q.uz * mz;
+            float amp = 0.0f;
+            if (cosine > 0.0f) {
+                float f = 1.0f;
+                if (fres > 1.0f) {
+                    const float root = sqrtf(fres * fres - 1.0f + cosine * cosine);
+                    f = (root - cosine) / (root + cosine);
+                }
+                amp = c.lam * q.inv_r * (1.0f / (8.0f * kPi)) * cosine * cosine * f;
+            }
+            acc[j][0] = fmaf(amp, corr.x, acc[j][0]);
+            acc[j][1] = fmaf(amp, corr.y, acc[j][1]);
+            acc[j][2] = fmaf(amp * amp, zn, acc[j][2]);
+        }
+    };
+    __syncthreads();
+    if constexpr (TABLE) {
+        const int size = c.N * oversample;
+        for (int a = 0; a < A; ++a) {
+            const Pair q = pair_geometry(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
+            const double x = (double)q.r * c.k0pm * oversample;
+            const double fl = floor(x);
+            const float tt = (float)(x - fl);
+            const int base = (int)fl;
+            const float t2 = tt * tt, t3 = t2 * tt;
+            const float wts[4] = {-0.5f * tt + t2 - 0.5f * t3, 1.0f - 2.5f * t2 + 1.5f * t3,
+                                  0.5f * tt + 2.0f * t2 - 1.5f * t3, -0.5f * t2 + 0.5f * t3};
+            float2 value = make_float2(0.f, 0.f);
+            float zn = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                int k = (base + j - 1) % size;
+                if (k < 0) k += size;
+                value += wts[j] * table[(size_t)a * size + k];
+                zn = fmaf(wts[j], norm_table[k], zn);
+            }
+            accumulate(q, cmul(value, make_float2(q.rot.x, -q.rot.y)), fmaxf(zn, 0.0f));
+        }
+    } else {
+        const CorTile tile = cor_tile(c, smem, P);
+        for (int a0 = 0; a0 < A; a0 += kCorApertures) {
+            __syncthreads();  // the previous tile is no longer read
+            load_tile(c, tile, P, res, A, a0);
+            const int na = min(kCorApertures, A - a0);
+            for (int al = 0; al < na; ++al) {
+                const int a = a0 + al;
+                const Pair q = pair_geometry(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
+                const Sums z = pair_sums<P, false>(c, tile, al, q);
+                accumulate(q, cmulc(q.rot, z.g0), z.nrm);
+            }
+        }
+    }
+    if (s >= S) return;
+    const float scale = sqrtf(areas[s]);
+    const float scale2 = scale * scale;
+    double best = 0.0;
+    int index = 0;
+#pragma unroll
+    for (int j = 0; j < kOrientMax; ++j) {
+        if (j > B) break;
+        const double re = (double)(acc[j][0] * scale), im = (double)(acc[j][1] * scale);
+        const double mag = real_part ? fabs(re) : hypot(re, im);
+        const double value = mag * mag / fmax((double)(acc[j][2] * scale2), 1e-60);
+        if (j == 0 || (isnan(value) && !isnan(best)) || (!isnan(best) && value > best)) {
+            best = value;
+            index = j;
+        }
+    }
+    best_value[s] = best;
+    best_index[s] = index;
+}
+
+template <int P>
+static hipError_t launch_orient_best_p(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                       const float* bank, int B, int S, const float2* res, const float2* table,
+                                       const float* norm_table, int oversample, const OrientPick& pick,
+                                       hipStream_t st) {
+    const dim3 grid((S + 255) / 256);
+    hipError_t e;
+    if (table) {
+        hipLaunchKernelGGL((orient_best_kernel<P, true>), grid, dim3(256), 0, st, c, pos, A, ctr, nrm, pick.areas,
+                           bank, B, S, res, table, norm_table, oversample, pick.real_part, pick.value, pick.index);
+    } else {
+        const size_t smem = cor_smem(c);
+        if ((e = allow_lds((const void*)orient_best_kernel<P, false>, smem)) != hipSuccess) return e;
+        hipLaunchKernelGGL((orient_best_kernel<P, false>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, pick.areas,
+                           bank, B, S, res, table, norm_table, oversample, pick.real_part, pick.value, pick.index);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_orient_best(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                              const float* bank, int B, int S, const float2* res, const float2* table,
+                              const float* norm_table, int oversample, const OrientPick& pick, hipStream_t st) {
+    if (B + 1 > kOrientMax) return hipErrorInvalidValue;
+    switch (c.P) {
+        case 1: return launch_orient_best_p<1>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, pick, st);
+        case 2: return launch_orient_best_p<2>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, pick, st);
+        case 4: return launch_orient_best_p<4>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, pick, st);
+        case 8: return launch_orient_best_p<8>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, pick, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <int P>
 static hipError_t launch_orient_p(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                   const float* bank, int B, int S, const float2* res, const float2* table,
@@ -463,7 +519,8 @@ static hipError_t launch_orient_p(const Chirp& c, const float* pos, int A, const
 
 hipError_t launch_orientations(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                const float* bank, int B, int S, const float2* res, const float2* table,
-                               const float* norm_table, int oversample, float* partial, float* out, hipStream_t st) {
+                               const float* norm_table, int oversample, float* partial, float* out,
+                               hipStream_t st) {
     switch (c.P) {
         case 1: return launch_orient_p<1>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, partial, out, st);
         case 2: return launch_orient_p<2>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, partial, out, st);

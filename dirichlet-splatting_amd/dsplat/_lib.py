@@ -38,6 +38,28 @@ class Fmcw(ctypes.Structure):
 
 
 # name -> (restype, argtypes); one line per entry point of include/dsplat_hip.h
+class QnSpec(ctypes.Structure):
+    """dsplat_qn_spec of dsplat_hip.h."""
+
+    _fields_ = [
+        ("center_scale", ctypes.c_float),
+        ("normal_scale", ctypes.c_float),
+        ("center_mask", ctypes.c_float * 3),
+        ("normal_mask", ctypes.c_float * 3),
+        ("direction_mask", ctypes.c_float * 6),
+        ("step_scale", ctypes.c_float * 6),
+        ("lower", ctypes.c_float * 3),
+        ("upper", ctypes.c_float * 3),
+        ("width", ctypes.c_int32),
+        ("normalize", ctypes.c_int32),
+        ("real_part", ctypes.c_int32),
+        ("iterations", ctypes.c_int32),
+        ("backtracks", ctypes.c_int32),
+        ("radius", ctypes.c_float),
+        ("power", ctypes.c_double),
+    ]
+
+
 SIGNATURES = {
     "dsplat_version": (ctypes.c_char_p, []),
     "dsplat_error_string": (ctypes.c_char_p, [c_int]),
@@ -91,6 +113,17 @@ SIGNATURES = {
     "dsplat_energy_workspace": (c_size, [c_size]),
     "dsplat_energy": (c_int, [c_ptr, c_ptr, c_size, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_range_response": (c_int, [c_ptr, c_double, c_double, c_int, c_int, c_int, c_int, c_ptr]),
+    "dsplat_surfel_orient_select": (
+        c_int,
+        [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr,
+         c_ptr],
+    ),
+    "dsplat_qn_workspace": (c_size, [c_ptr, c_int]),
+    "dsplat_qn_search": (
+        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
+    ),
+    "dsplat_elimination_workspace": (c_size, [c_int, c_int]),
+    "dsplat_backward_elimination": (c_int, [c_ptr, c_ptr, c_int, c_int, c_ptr, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_syrk_workspace": (c_size, [c_int, c_int, c_int]),
     "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_scan_workspace": (c_size, [c_int, c_ptr, c_int]),

@@ -84,12 +84,18 @@ def fibonacci_sphere(count, device="cuda"):
 
 
 @torch.no_grad()
-def orient_candidates(model, params, residual, count, mode, table=None, chunk=8192):
+def orient_candidates(model, params, residual, count, mode, table=None, chunk=8192, *, batched=False):
     """Give every position the best of its own normal and a count-normal sphere bank;
-    returns the candidates with the chosen normals and their certificates."""
+    returns the candidates with the chosen normals and their certificates. batched scores
+    every candidate and picks its normal in one launch (no [C, 1 + count] sums)."""
     centers = params["centers"]
     bank = fibonacci_sphere(count, centers.device)
     power = residual.abs().square().sum().double().clamp(min=1e-30)
+    if batched and hasattr(model, "orient_select"):
+        value, ids = model.orient_select(params, residual, bank, table, mode == "real")
+        picked = bank[(ids - 1).clamp(min=0)]
+        normals = torch.where((ids > 0)[:, None], picked, params["normals"])
+        return {**params, "normals": normals}, (value / power).float()
     scores, normals = [], []
     for start in range(0, len(centers), chunk):
         block = {key: value[start : start + chunk] for key, value in params.items()}
@@ -120,11 +126,12 @@ def _controls(reference, center_spec, normal_spec, masks):
 @torch.no_grad()
 def search_certificate(
     model, residual, candidates, center_spec, config, mode, *, normal_spec=None,
-    initial_scores=None, full_model=None, full_residual=None,
+    initial_scores=None, full_model=None, full_residual=None, batched=False,
 ):  # fmt: skip
     """Refine separated peaks in xyz (and unit normals) by batched quasi-Newton ascent
     with damped BFGS and per-candidate backtracking; coarse peaks stay beside the refined
-    ones and all are reranked on full_model when given (certificate.py:106-201)."""
+    ones and all are reranked on full_model when given (certificate.py:106-201). batched
+    runs the whole ascent of every candidate in one launch instead of a host loop."""
     reference = candidates["centers"]
     feasible = dict(candidates, centers=center_spec.project(reference, reference))
     specs = {"centers": center_spec}
@@ -147,6 +154,11 @@ def search_certificate(
     coarse_scores = score(feasible) if initial_scores is None else initial_scores
     ids = select_peaks(coarse_scores, feasible["centers"], config.peaks, config.separation)
     coarse = {key: value[ids] for key, value in feasible.items()}
+    if batched and hasattr(model, "qn_search") and not center_spec.normalize:
+        spec = _qn_spec(controls, center_spec, normal_spec, masks, config, mode, residual)
+        centers, normals, value = model.qn_search(coarse, residual, spec)
+        return _rerank(coarse, dict(coarse, centers=centers, normals=normals), coarse_scores[ids], value, full_model,
+                       full_residual, mode)  # fmt: skip
     params = {key: value.clone() for key, value in coarse.items()}
     value, grad = score(params, True)
     grad = ascent(params, grad)
@@ -182,13 +194,42 @@ def search_certificate(
         updated = left @ inverse @ left.transpose(-1, -2) + rho[:, None, None] * step[:, :, None] * step[:, None, :]
         inverse = torch.where(valid[:, None, None], updated, inverse)
         value, grad = new_value, new_grad
+    return _rerank(coarse, params, coarse_scores[ids], value, full_model, full_residual, mode)
+
+
+def _rerank(coarse, params, coarse_scores, value, full_model, full_residual, mode):
+    """Coarse and refined candidates together, best first (on full_model when given)."""
     combined = {key: torch.cat((coarse[key], params[key])) for key in params}
     if full_model is None:
-        scores = torch.cat((coarse_scores[ids], value))
+        scores = torch.cat((coarse_scores, value))
     else:
         scores = certificate(full_model, combined, full_residual, mode)
     order = scores.argsort(descending=True, stable=True)
     return {key: v[order] for key, v in combined.items()}, scores[order]
+
+
+def _qn_spec(controls, center_spec, normal_spec, masks, config, mode, residual):
+    """The constants of the one-launch ascent (dsplat_qn_spec)."""
+    from ._lib import QnSpec
+
+    ctl = controls.cpu()
+    spec = QnSpec()
+    spec.center_scale = float(center_spec.scale)
+    spec.normal_scale = float(normal_spec.scale) if normal_spec is not None else 1.0
+    spec.center_mask[:] = masks["centers"].float().cpu().tolist()
+    spec.normal_mask[:] = masks["normals"].float().cpu().tolist() if normal_spec is not None else [1.0, 1.0, 1.0]
+    spec.direction_mask[:] = ctl[1].tolist()
+    spec.step_scale[:] = ctl[0].tolist()
+    spec.lower[:] = ctl[2, :3].tolist()
+    spec.upper[:] = ctl[3, :3].tolist()
+    spec.width = 6 if normal_spec is not None else 3
+    spec.normalize = int(bool(normal_spec.normalize)) if normal_spec is not None else 0
+    spec.real_part = int(mode == "real")
+    spec.iterations = int(config.iterations)
+    spec.backtracks = int(config.backtracks)
+    spec.radius = float(config.trust_radius)
+    spec.power = float(residual.abs().square().sum().double().clamp(min=1e-30))
+    return spec
 
 
 @torch.no_grad()

@@ -43,6 +43,7 @@ class DSFWConfig:
     dictionary_elements: int = 32_000_000
     seed: int = 0
     gram: str = "plain"  # normal-equation SYRK: plain (FP32) | bf16x3 | fp16x3
+    batched: bool = False  # one-launch candidate scoring, backtracking and elimination
 
     @property
     def real(self):

@@ -279,6 +279,19 @@ class FMCWModel:
             *self.sensor, params, bank, residual.reshape(self.shape), self.fresnel_n, table
         )
 
+    def qn_search(self, params, residual, spec):
+        """Quasi-Newton certificate ascent of every candidate in one launch."""
+        config, positions = self.sensor
+        return search.qn_search(
+            config, positions, residual.reshape(self.shape), params["centers"], params["normals"], params["areas"], spec,
+            self.fresnel_n,
+        )  # fmt: skip
+
+    def orient_select(self, params, residual, bank, table=None, real_part=False):
+        return surfel.orient_select(
+            *self.sensor, params, bank, residual.reshape(self.shape), self.fresnel_n, table, real_part
+        )
+
     @torch.no_grad()
     def range_gate(self, points, measurement, relative_power, margin_cells, min_fraction):
         """Points whose spherical range falls in a measured range interval at enough

@@ -6,6 +6,7 @@ import math
 import torch
 
 from .. import _lib
+from . import interleaved
 
 
 def range_gate(points, positions, intervals, spacing, min_fraction):
@@ -66,3 +67,24 @@ def trial(centers, normals, direction, controls, fraction):
         _lib.ptr(out_centers), _lib.ptr(out_normals), _lib.stream(centers.device),
     )  # fmt: skip
     return out_centers, out_normals
+
+
+def qn_search(config, positions, residual, centers, normals, areas, spec, fresnel_n):
+    """Batched quasi-Newton certificate ascent of every candidate in one launch: final
+    (centers, normals, values) (certificate.py:106-201). spec is a filled _lib.QnSpec."""
+    count = len(centers)
+    out_c, out_n = torch.empty_like(centers), torch.empty_like(normals)
+    values = torch.empty(count, dtype=torch.float32, device=centers.device)
+    if count == 0:
+        return out_c, out_n, values
+    cfg = _lib.fmcw(config, fresnel_n)
+    size = _lib.library().dsplat_qn_workspace(_lib.ctypes.byref(cfg), len(positions))
+    workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=centers.device)
+    res = interleaved(residual.reshape(len(positions), config.num_range_bins))
+    _lib.call(
+        "dsplat_qn_search", _lib.ctypes.byref(cfg), _lib.ptr(positions.float().contiguous()), len(positions),
+        _lib.ptr(res), _lib.ptr(centers.float().contiguous()), _lib.ptr(normals.float().contiguous()),
+        _lib.ptr(areas.float().contiguous()), count, _lib.ctypes.byref(spec), _lib.ptr(out_c), _lib.ptr(out_n),
+        _lib.ptr(values), _lib.ptr(workspace), size, _lib.stream(centers.device),
+    )  # fmt: skip
+    return out_c, out_n, values

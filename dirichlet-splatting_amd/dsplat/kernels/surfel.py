@@ -259,6 +259,30 @@ def correlate_orientations(config, positions, centers, normals, bank, residual, 
     return torch.complex(output[..., 0], output[..., 1]), output[..., 2]
 
 
+def orient_select(config, positions, centers, normals, areas, bank, residual, fresnel_n, table=None, real_part=False):
+    """Best orientation per candidate in one launch: (value [S] float64, index [S] int64),
+    value = |s|^2 / max(area |a|^2, 1e-60) with s = sqrt(area) a^H r (or its real part),
+    index 0 for the own normal and 1 + b for bank normal b (certificate.py:73-103)."""
+    _check(positions, centers, normals)
+    count, extra = len(centers), len(bank)
+    value = torch.empty(count, dtype=torch.float64, device=centers.device)
+    index = torch.empty(count, dtype=torch.int32, device=centers.device)
+    if count == 0:
+        return value, index.long()
+    cfg = _lib.fmcw(config, fresnel_n)
+    res = None if table is not None else interleaved(residual.reshape(len(positions), config.num_range_bins))
+    _lib.call(
+        "dsplat_surfel_orient_select", _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
+        _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()), count,
+        _lib.ptr(bank.to(torch.float32).contiguous()) if extra else None, extra, _lib.ptr(res),
+        None if table is None else _lib.ptr(table.correlation.contiguous()),
+        None if table is None else _lib.ptr(table.norm.contiguous()),
+        1 if table is None else int(table.oversample), int(bool(real_part)), _lib.ptr(value), _lib.ptr(index),
+        _lib.stream(centers.device),
+    )  # fmt: skip
+    return value, index.long()
+
+
 def correlate_gradient(config, positions, centers, normals, residual, fresnel_n, include_normals):
     """Correlation, norm and their center (and normal) derivatives [S, 3 or 6], without area
     (kernels/surfel.py:293-315)."""

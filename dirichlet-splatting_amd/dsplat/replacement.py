@@ -11,6 +11,7 @@ from dataclasses import replace
 import torch
 
 from .certificate import certificate
+from .kernels import solver as elimination
 from .parameters import periodic_delta
 from .varpro import Fit, accept, fit_coefficients, rendered_loss, row_chunk, screen_bar, shifted_cholesky, varpro
 
@@ -81,16 +82,19 @@ def regularized_inverse(h, floor):
     return None if factor is None else torch.cholesky_inverse(factor)
 
 
-def backward_elimination(inverse, rhs, count, limited=None, limit=0):
+def backward_elimination(inverse, rhs, count, limited=None, limit=0, *, kernel=False):
     """Remove `count` unknowns one at a time from the least squares with inverse W.
 
     Deleting unknown j raises the objective by |b_j|^2 / W_jj with b = W q; each step
     deletes the cheapest and keeps the exact inverse of the rest as rank-one
     corrections W - w w^H / W_jj. At most `limit` of the unknowns marked in `limited`
-    leave. Returns the removed indices in order (replacement.py:104-139).
+    leave. Returns the removed indices in order (replacement.py:104-139). With kernel
+    (real float64 W) the whole loop runs as one HIP launch instead of a host loop.
     """
     size = len(inverse)
     b = inverse @ rhs.to(inverse.dtype)
+    if kernel and not inverse.is_complex():
+        return elimination.backward_elimination(inverse, b, count, limited, limit)
     diagonal = inverse.diagonal().real.clone()
     alive = torch.ones(size, device=inverse.device, dtype=torch.bool)
     updates = inverse.new_zeros((size, count))
@@ -195,7 +199,9 @@ def merge_support(state, config, pool, scores=None):
     old = torch.arange(len(q), device=q.device) < count
     limit = None
     while True:
-        removed = backward_elimination(inverse, q, len(chosen), None if limit is None else old, limit or 0)
+        removed = backward_elimination(
+            inverse, q, len(chosen), None if limit is None else old, limit or 0, kernel=getattr(config, "batched", False)
+        )
         gone = torch.zeros(len(q), device=q.device, dtype=torch.bool)
         gone[removed] = True
         owners = gone[:count].nonzero().flatten()

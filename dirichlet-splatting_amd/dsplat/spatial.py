@@ -65,7 +65,9 @@ def propose(state, config, search):
 
     def score(candidates):
         if banked:
-            oriented, scores = orient_candidates(model, candidates, r, search.normal_candidates, mode, table)
+            oriented, scores = orient_candidates(
+                model, candidates, r, search.normal_candidates, mode, table, batched=getattr(config, "batched", False)
+            )
             candidates["normals"] = oriented["normals"]
             return scores
         return certificate(model, candidates, r, mode)
@@ -135,13 +137,14 @@ def propose(state, config, search):
     params, scores = search_certificate(
         model, r, candidates, spec, replace(search.continuous, peaks=peaks, separation=0), mode,
         normal_spec=refine_normals, initial_scores=initial, full_model=state.model, full_residual=residual,
+        batched=getattr(config, "batched", False),
     )  # fmt: skip
     if search.full_iterations:
         full = select_peaks(scores, params["centers"], search.full_peaks, min(separation, spec.scale / 2))
         refined, refined_scores = search_certificate(
             state.model, residual, {key: value[full] for key, value in params.items()}, spec,
             replace(search.continuous, peaks=search.full_peaks, iterations=search.full_iterations, separation=0),
-            mode, normal_spec=refine_normals, initial_scores=scores[full],
+            mode, normal_spec=refine_normals, initial_scores=scores[full], batched=getattr(config, "batched", False),
         )  # fmt: skip
         scores = torch.cat((refined_scores, scores))
         order = scores.argsort(descending=True, stable=True)

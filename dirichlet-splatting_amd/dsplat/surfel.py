@@ -87,3 +87,12 @@ def correlate_orientations(config, positions, params, bank, residual, fresnel_n=
     )
     scale = params["areas"].sqrt()[:, None]
     return corr * scale, norm * scale.square()
+
+
+def orient_select(config, positions, params, bank, residual, fresnel_n=0.0, table=None, real_part=False):
+    """Best of each surfel's own normal and the bank by |s|^2 / |a|^2 (area included), in
+    one launch: (value [S] float64, index [S] int64, 0 = own normal, 1 + b = bank b)."""
+    return kernel.orient_select(
+        config, positions, params["centers"], params["normals"], params["areas"], bank, residual, fresnel_n, table,
+        real_part,
+    )  # fmt: skip

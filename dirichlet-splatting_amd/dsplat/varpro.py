@@ -97,19 +97,25 @@ def normal_equations(model, params, target, config):
             rhs.addmv_(a.H, y)
         return (gram + gram.H) / (2 * model.size), rhs / model.size
     mode = SYRK_MODE[getattr(config, "gram", "plain")]
+    batched = getattr(config, "batched", False) and hasattr(model, "correlate")
     device = target.device
     part = torch.zeros((count, count), dtype=torch.float32, device=device)
     total = torch.zeros((count, count), dtype=torch.float64, device=device)
     rhs = torch.zeros(count, dtype=torch.float64, device=device)
     for index, (a, y) in enumerate(stream):
         syrk(a.contiguous(), part, mode)
-        rhs += a.T @ y
+        if not batched:
+            rhs += a.T @ y
         if (index + 1) % FOLD == 0:
             total += part
             part.zero_()
     total += part
     upper = torch.triu(total)
     gram = (upper + torch.triu(total, 1).T) / model.size
+    if batched:
+        # A^H y in one correlation launch over every aperture and bin: the row basis holds
+        # every atom, so projecting y first changes nothing
+        rhs = model.correlate(params, target)[0].real.double()
     return gram.float(), (rhs / model.size).float()
 
 

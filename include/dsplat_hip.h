@@ -118,6 +118,21 @@ int dsplat_surfel_correlate_orientations(const dsplat_fmcw* cfg, const float* po
                                          const float* norm_table, int32_t oversample, float* out,
                                          void* workspace, size_t workspace_bytes, void* stream);
 
+/* The orientation choice of a candidate search in one launch: for each candidate the
+ * certificate |s|^2 / max(norm, 1e-60) (float64; s = sqrt(area) a^H r, or its real part
+ * with real_part; norm = area |a|^2) of its own normal and of every bank normal
+ * (num_normals <= 24), and the first best: best_value [S] float64, best_index [S] int32
+ * (0 = own, 1 + b = bank b). Inputs as dsplat_surfel_correlate_orientations plus areas [S];
+ * one thread per candidate sums over all apertures in float32, so no workspace and no
+ * [S, 1 + B] sums. Replaces the chunked orientation loop of certificate.py:73-103
+ * (orient_candidates) over kernels/surfel.py:263-290. */
+int dsplat_surfel_orient_select(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                                const float* centers, const float* normals, const float* areas,
+                                int32_t num_surfels, const float* bank, int32_t num_normals,
+                                const float* residual, const float* range_table, const float* norm_table,
+                                int32_t oversample, int32_t real_part, double* best_value,
+                                int32_t* best_index, void* stream);
+
 /* Workspace bytes of dsplat_scan_batched. */
 size_t dsplat_scan_workspace(int32_t num_scans, const int32_t* apertures, int32_t num_candidates);
 
@@ -217,6 +232,36 @@ int dsplat_energy(const float* value, const float* reference, size_t count, floa
 int dsplat_range_response(double* out, double low, double high, int32_t count, int32_t num_bins,
                           int32_t samples, int32_t n_fft, void* stream);
 
+/* ---- Batched quasi-Newton certificate search. */
+
+/* Per-search constants of dsplat_qn_search, from the parameter specs (certificate.py:75-86
+ * _controls and parameters.py:57-70 tangent). Lower/upper are +-inf when unbounded. */
+typedef struct dsplat_qn_spec {
+    float center_scale, normal_scale;
+    float center_mask[3], normal_mask[3]; /* 1 = trainable */
+    float direction_mask[6];              /* controls[1] */
+    float step_scale[6];                  /* controls[0] */
+    float lower[3], upper[3];             /* controls[2], controls[3] */
+    int32_t width;                        /* 3: centers; 6: centers and unit normals */
+    int32_t normalize;                    /* the normal field is a unit vector */
+    int32_t real_part;                    /* certificate of Re(a^H r) */
+    int32_t iterations, backtracks;
+    float radius;                         /* trust radius of a direction */
+    double power;                         /* |r|^2 (clamped) */
+} dsplat_qn_spec;
+
+/* Quasi-Newton ascent of the residual certificate for `count` candidates in one launch:
+ * from centers/normals [count, 3], each candidate takes damped BFGS directions with halving
+ * backtracks until its certificate stops rising or `iterations` steps are done. Outputs
+ * the final centers, normals [count, 3] and certificate values [count] (float32). residual
+ * [A, K] complex64 interleaved; areas [count]. Workspace from dsplat_qn_workspace.
+ * Replaces the host loop of certificate.py:106-201 (search_certificate). */
+size_t dsplat_qn_workspace(const dsplat_fmcw* cfg, int32_t num_apertures);
+int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                     const float* residual, const float* centers, const float* normals, const float* areas,
+                     int32_t count, const dsplat_qn_spec* spec, float* centers_out, float* normals_out,
+                     float* values_out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- Dense products. */
 
 /* Upper triangle of out [n, n] (row-major) += rows^T rows, rows [m, n] row-major float32,
@@ -225,6 +270,18 @@ int dsplat_range_response(double* out, double low, double high, int32_t count, i
  * FP32 accumulation, float64 split-K reduction. Deterministic. Workspace from
  * dsplat_syrk_workspace. Replaces kernels/cublas.py:31-44 (syrk: cublasSsyrk_v2 through
  * ctypes). */
+/* Backward elimination of `count` unknowns from a least-squares solution, in one launch:
+ * inverse [n, n] float64 (of the regularized real Gram), b [n] = inverse q. Each step
+ * removes the unknown with the smallest b_j^2 / W_jj (ties to the lower index) among those
+ * still allowed, keeping the exact inverse of the rest by rank-one corrections; at most
+ * `limit` unknowns with limited[j] != 0 leave (limited may be NULL). removed [count] int64
+ * in order. n <= 8192, count <= min(n, 4096). Replaces the host loop of
+ * replacement.py:104-139 (backward_elimination). */
+size_t dsplat_elimination_workspace(int32_t n, int32_t count);
+int dsplat_backward_elimination(const double* inverse, const double* b, int32_t n, int32_t count,
+                                const int32_t* limited, int32_t limit, int64_t* removed, void* workspace,
+                                size_t workspace_bytes, void* stream);
+
 size_t dsplat_syrk_workspace(int32_t m, int32_t n, int32_t mode);
 int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* out, void* workspace,
                 size_t workspace_bytes, void* stream);

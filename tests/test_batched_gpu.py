@@ -1,0 +1,116 @@
+"""The batched DSFW pieces against the plain host paths they replace: the one-launch
+orientation choice, the one-launch backward elimination and the split-precision SYRK in
+the streamed normal equations."""
+
+import pytest
+import torch
+from test_solver_gpu import aperture_grid, close, fmcw, fmcw_specs, surfels  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("mode,use_table", [("complex", False), ("real", False), ("complex", True)])
+def test_batched_orientation_choice_equals_the_chunked_loop(fmcw, mode, use_table):  # noqa: F811
+    from dsplat.certificate import fibonacci_sphere, orient_candidates
+
+    model, params, _ = fmcw
+    g = torch.Generator().manual_seed(11)
+    count = 3000
+    candidates = dict(
+        centers=(torch.tensor([0, 0, 0.3]) + 0.02 * (2 * torch.rand(count, 3, generator=g) - 1)).cuda(),
+        normals=torch.nn.functional.normalize(torch.randn(count, 3, generator=g), dim=1).cuda(),
+        areas=torch.full((count,), 1e-4).cuda() * torch.linspace(0.5, 2, count).cuda(),
+    )
+    residual = model.scan.data
+    table = model.range_table(residual, 8) if use_table else None
+    plain, plain_scores = orient_candidates(model, candidates, residual, 24, mode, table, chunk=700)
+    fused, fused_scores = orient_candidates(model, candidates, residual, 24, mode, table, batched=True)
+    # the fused pass sums apertures in float32 in order, the chunked one per 32-aperture
+    # block in float64: scores agree to float32 accuracy, and the fused choice scores within
+    # that of the best under the chunked sums
+    torch.testing.assert_close(fused_scores, plain_scores, rtol=2e-5, atol=1e-30)
+    bank = fibonacci_sphere(24, "cuda")
+    corr, norm = model.correlate_orientations(candidates, residual, bank, table)
+    s = corr.real.double() if mode == "real" else corr.to(torch.complex128)
+    values = s.abs().square() / norm.double().clamp(min=1e-60)
+    normals = torch.cat((candidates["normals"][:, None], bank[None].expand(count, -1, -1)), 1)
+    chosen = (normals == fused["normals"][:, None]).all(-1).float().argmax(1)
+    picked = values.gather(1, chosen[:, None])[:, 0]
+    assert torch.all(picked >= values.max(1).values * (1 - 2e-5))
+    assert (chosen == values.argmax(1)).float().mean() > 0.99
+
+
+@pytest.mark.parametrize("n,count,limit", [(300, 40, None), (300, 40, 5), (2304, 256, None), (37, 37, None)])
+def test_elimination_kernel_equals_the_host_loop(n, count, limit):
+    from dsplat.replacement import backward_elimination
+
+    g = torch.Generator().manual_seed(n + count)
+    a = torch.randn(n + 50, n, generator=g, dtype=torch.float64)
+    h = a.T @ a / n + 1e-3 * torch.eye(n, dtype=torch.float64)
+    inverse = torch.linalg.inv(h).cuda()
+    rhs = torch.randn(n, generator=g, dtype=torch.float64).cuda()
+    limited = None if limit is None else torch.arange(n, device="cuda") < n // 2
+    host = backward_elimination(inverse, rhs, count, limited, limit or 0)
+    kernel = backward_elimination(inverse, rhs, count, limited, limit or 0, kernel=True)
+    assert torch.equal(host, kernel)
+    assert len(set(kernel.tolist())) == count
+    if limit is not None:
+        assert int((kernel < n // 2).sum()) <= limit
+
+
+@pytest.mark.parametrize("gram", ["bf16x3", "fp16x3"])
+def test_split_syrk_normal_equations_match_the_fp32_stream(fmcw, gram):  # noqa: F811
+    from dsplat import DSFWConfig
+    from dsplat.varpro import normal_equations
+
+    model, params, _ = fmcw
+    model = model.prepare(fmcw_specs())
+    config = DSFWConfig(row_chunk=2 * model.shape[1])
+    plain = normal_equations(model, params, model.scan.data, config)
+    split = normal_equations(model, params, model.scan.data, DSFWConfig(row_chunk=2 * model.shape[1], gram=gram))
+    close(split[0], plain[0], 1e-4 if gram == "bf16x3" else 1e-5)
+    assert torch.equal(split[1], plain[1])
+
+
+def test_batched_normal_equations_take_the_rhs_from_one_correlation(fmcw):  # noqa: F811
+    from dsplat import DSFWConfig
+    from dsplat.varpro import normal_equations
+
+    model, params, _ = fmcw
+    model = model.prepare(fmcw_specs())
+    plain = normal_equations(model, params, model.scan.data, DSFWConfig(row_chunk=2 * model.shape[1]))
+    fused = normal_equations(
+        model, params, model.scan.data, DSFWConfig(row_chunk=2 * model.shape[1], gram="fp16x3", batched=True)
+    )
+    close(fused[0], plain[0], 1e-5)
+    close(fused[1], plain[1], 1e-5)
+
+
+@pytest.mark.parametrize("mode,normals", [("real", True), ("complex", True), ("complex", False)])
+def test_qn_kernel_follows_the_host_ascent(fmcw, mode, normals):  # noqa: F811
+    """Same candidates, same settings: the one-launch ascent and the host loop end at the
+    same certificates (their sums are rounded in different orders, so trajectories agree
+    to float32 accuracy rather than bit for bit), and both reach the true reflector."""
+    from dsplat.certificate import CertificateSearchConfig, search_certificate
+
+    model, params, coefficients = fmcw
+    one = {k: v[:1] for k, v in params.items()}
+    residual = model.render(one, coefficients[:1])
+    start = {k: v.repeat(24, *([1] * (v.ndim - 1))) for k, v in one.items()}
+    g = torch.Generator("cuda").manual_seed(1)
+    start["centers"] = start["centers"] + 6e-4 * torch.randn(start["centers"].shape, device="cuda", generator=g)
+    start["normals"] = torch.nn.functional.normalize(
+        start["normals"] + 0.2 * torch.randn(start["normals"].shape, device="cuda", generator=g), dim=1
+    )
+    specs = fmcw_specs()
+    config = CertificateSearchConfig(peaks=24, separation=0, iterations=25)
+    kwargs = dict(normal_spec=specs["normals"] if normals else None)
+    host, host_scores = search_certificate(model, residual, start, specs["centers"], config, mode, **kwargs)
+    fused, fused_scores = search_certificate(
+        model, residual, start, specs["centers"], config, mode, batched=True, **kwargs
+    )
+    torch.testing.assert_close(fused_scores, host_scores, rtol=2e-3, atol=1e-6)
+    assert float((fused["centers"][0] - one["centers"][0]).norm()) < 5e-5
+    assert float((host["centers"][0] - one["centers"][0]).norm()) < 5e-5
+    if normals:
+        torch.testing.assert_close(fused["normals"].norm(dim=1), torch.ones(len(fused["normals"]), device="cuda"))
