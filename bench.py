@@ -40,10 +40,14 @@ def parse():
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     parser.add_argument(
-        "--workload", choices=("splat", "gram", "scan"), default="splat",
+        "--workload", choices=("splat", "gram", "scan", "dsfw"), default="splat",
         help="splat: the fwd+VJP metric (default); gram: the split-precision Gram of the same scene; "
-        "scan: the batched certificate scan of 64^3 candidates against 64^2, 96^2 and 128^2 scans",
+        "scan: the batched certificate scan of 64^3 candidates against 64^2, 96^2 and 128^2 scans; "
+        "dsfw: DSFW steps, plain FP32 form against the mixed-precision batched form",
     )  # fmt: skip
+    parser.add_argument("--adc-samples", type=int, default=128, help="dsfw: ADC samples (bins = samples)")
+    parser.add_argument("--truth-draws", type=int, default=0,
+                        help="dsfw: ellipsoid draws with front faces kept (0: exactly 2048 front faces)")
     return parser.parse_args()
 
 
@@ -280,6 +284,75 @@ def scan_workload(args):
     return result
 
 
+def dsfw_workload(args):
+    """DSFW steps (spatial search, merged replacement of 256 atoms, joint Gauss-Newton) on a
+    float64-synthesized 128x128 scan of the ellipsoid with 30 dB noise, from random initial
+    surfels, in two forms: plain (FP32 SYRK normal equations, candidates scored in chunks,
+    host-loop search, backtracking and elimination) and mixed-precision batched (FP16 hi+lo
+    split SYRK, one-launch orientation choice, quasi-Newton search and elimination,
+    correlation rhs, 2^29-element design blocks). Both start from the same state; per-step
+    wall time between synchronizations and the NMSE after each step."""
+    import statistics
+
+    import torch
+
+    from dsplat import DSFWConfig, workloads
+    from dsplat.dsfw import initialize, step
+    from dsplat.spatial import SpatialSearchConfig
+
+    draws = args.truth_draws or 4000
+    problem = workloads.dsfw_problem(
+        adc_samples=args.adc_samples, truth_draws=draws, front_count=None if args.truth_draws else 2048
+    )
+    S = len(problem["initial"]["centers"])
+    forms = {
+        "plain_fp32": DSFWConfig(batch=256),
+        "mixed_batched": DSFWConfig(batch=256, gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22),
+    }
+    search = SpatialSearchConfig()
+    steps, warmup = max(1, args.steps), max(1, args.warmup)
+    out = {}
+    for name, config in forms.items():
+        state = initialize(problem["model"], problem["target"], problem["initial"], config,
+                           parameters=problem["parameters"])  # fmt: skip
+        for _ in range(warmup):
+            state = step(state, config, search)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            state = step(state, config, search)
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        error = torch.cdist(state.params["centers"], problem["truth"]["centers"]).min(1).values
+        out[name] = {
+            "ms_per_step_mean": statistics.fmean(times),
+            "ms_per_step_median": statistics.median(times),
+            "ms": times,
+            "nmse": [h["nmse"] for h in state.history],
+            "median_center_error_m": float(error.median()),
+        }
+    plain, mixed = out["plain_fp32"], out["mixed_batched"]
+    deviation = max(abs(a - b) / abs(a) for a, b in zip(plain["nmse"], mixed["nmse"], strict=True))
+    return {
+        "metric": "DSFW step ms (2048-surfel fit, 128x128 scan)" if not args.truth_draws else "DSFW step ms",
+        "value": mixed["ms_per_step_mean"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "higher_is_better": False,
+        "dtype": "f32 solver, fp16 hi+lo split SYRK (mixed form)",
+        "data": "synthetic",
+        "config": {"workload": f"DSFW steps: {S} surfels, 128x128 array @ 2.5 mm, {args.adc_samples} ADC samples "
+                   f"({args.adc_samples} bins), batch 256, spatial search", "surfels": S,
+                   "bins": args.adc_samples, "parallelism": "single GPU"},
+        "speedup_mixed_batched_vs_plain": plain["ms_per_step_mean"] / mixed["ms_per_step_mean"],
+        "nmse_max_relative_deviation": deviation,
+        "forms": out,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -294,6 +367,9 @@ def main():
         return
     if args.workload == "scan":
         print(json.dumps(scan_workload(args)), flush=True)
+        return
+    if args.workload == "dsfw":
+        print(json.dumps(dsfw_workload(args)), flush=True)
         return
     import torch
 

@@ -374,6 +374,18 @@ int dsplat_range_response(double* out, double low, double high, int32_t count, i
 }
 
 // ------------------------------------------------------------------------------------ blas
+int dsplat_project_design(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                          const float* centers, const float* normals, const float* areas, int32_t num_surfels,
+                          const float* basis, int32_t rows, float* out, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0 || !project_supported(c, rows))
+        return hipErrorInvalidValue;
+    if (num_apertures == 0 || num_surfels == 0) return hipSuccess;
+    if (!positions || !centers || !normals || !areas || !basis || !out) return hipErrorInvalidValue;
+    return launch_project(c, positions, num_apertures, centers, normals, areas, num_surfels, basis, rows, out,
+                          (hipStream_t)stream);
+}
+
 size_t dsplat_qn_workspace(const dsplat_fmcw* cfg, int32_t num_apertures) {
     Chirp c;
     if (!make_chirp(cfg, &c) || num_apertures < 0) return 0;
@@ -411,15 +423,29 @@ int dsplat_backward_elimination(const double* inverse, const double* b, int32_t 
 
 size_t dsplat_syrk_workspace(int32_t m, int32_t n, int32_t mode) {
     if (m < 0 || n < 0 || mode < 0 || mode > 2) return 0;
-    return syrk_workspace(m, n, mode);
+    return syrk_workspace(m, n, -1, mode);
 }
 
 int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* out, void* workspace,
                 size_t workspace_bytes, void* stream) {
     if (m < 0 || n < 0 || mode < 0 || mode > 2) return hipErrorInvalidValue;
     if (m == 0 || n == 0) return hipSuccess;
-    if (!rows || !out || !workspace || workspace_bytes < syrk_workspace(m, n, mode)) return hipErrorInvalidValue;
-    return launch_syrk(rows, m, n, mode, out, workspace, (hipStream_t)stream);
+    if (!rows || !out || !workspace || workspace_bytes < syrk_workspace(m, n, -1, mode)) return hipErrorInvalidValue;
+    return launch_syrk(rows, m, n, -1, mode, out, workspace, (hipStream_t)stream);
+}
+
+size_t dsplat_gemm_tn_workspace(int32_t m, int32_t n, int32_t col0, int32_t mode) {
+    if (m < 0 || n < 0 || col0 < 0 || col0 >= n || mode < 0 || mode > 2) return 0;
+    return syrk_workspace(m, n, col0, mode);
+}
+
+int dsplat_gemm_tn(const float* rows, int32_t m, int32_t n, int32_t col0, int32_t mode, float* out, void* workspace,
+                   size_t workspace_bytes, void* stream) {
+    if (m < 0 || n < 0 || col0 < 0 || col0 > n || mode < 0 || mode > 2) return hipErrorInvalidValue;
+    if (m == 0 || col0 == n) return hipSuccess;
+    if (!rows || !out || !workspace || workspace_bytes < syrk_workspace(m, n, col0, mode))
+        return hipErrorInvalidValue;
+    return launch_syrk(rows, m, n, col0, mode, out, workspace, (hipStream_t)stream);
 }
 
 size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels) {

@@ -106,6 +106,9 @@ GramShape gram_shape(int A, int S);
 hipError_t launch_gram(const Chirp& c, int mode, const float* pos, int A, const float* ctr, const float* nrm,
                        const float* areas, const float* col_exp, int S, int real_only, float inv_rows,
                        float* partial, float* out, hipStream_t st);
+bool project_supported(const Chirp& c, int rows);
+hipError_t launch_project(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                          const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st);
 typedef dsplat_qn_spec QnSpec;
 size_t qn_workspace(int A, int P);
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,
@@ -114,8 +117,9 @@ hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float
 size_t elimination_workspace(int n, int count);
 hipError_t launch_elimination(const double* W, const double* b, int n, int count, const int* limited, int limit,
                               long long* removed, void* workspace, hipStream_t st);
-size_t syrk_workspace(int m, int n, int mode);
-hipError_t launch_syrk(const float* rows, int m, int n, int mode, float* out, void* workspace, hipStream_t st);
+size_t syrk_workspace(int m, int n, int col0, int mode);
+hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, float* out, void* workspace,
+                       hipStream_t st);
 
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st);

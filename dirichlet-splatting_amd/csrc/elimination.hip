@@ -16,6 +16,7 @@ namespace dsplat {
 constexpr int kElimThreads = 1024;
 constexpr int kElimRows = 8;  // n <= 8192
 constexpr int kElimMaxCount = 4096;
+constexpr int kElimUnroll = 8;
 
 struct Best {
     double cost;
@@ -89,7 +90,20 @@ __global__ __launch_bounds__(kElimThreads) void elimination_kernel(const double*
             const int j = tid + kElimThreads * r;
             w[r] = j < n ? W[(size_t)j * n + i] : 0.0;
         }
-        for (int s = 0; s < step; ++s) {
+        int s = 0;
+        for (; s + kElimUnroll <= step; s += kElimUnroll) {  // kElimUnroll rows of U in flight
+#pragma unroll
+            for (int r = 0; r < kElimRows; ++r) {
+                const int j = tid + kElimThreads * r;
+                if (j >= n) continue;
+                double x[kElimUnroll];
+#pragma unroll
+                for (int q = 0; q < kElimUnroll; ++q) x[q] = U[(size_t)(s + q) * n + j];
+#pragma unroll
+                for (int q = 0; q < kElimUnroll; ++q) w[r] -= x[q] * u_at[s + q];
+            }
+        }
+        for (; s < step; ++s) {
             const double us = u_at[s];
             const double* __restrict__ row = U + (size_t)s * n;
 #pragma unroll

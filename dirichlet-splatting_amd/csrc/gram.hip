@@ -13,6 +13,7 @@
 // MODE 2: FP16 hi + lo split, same with _f16, columns scaled by 2^e_s into FP16 range.
 // Partials [split][S][S] complex (upper tiles only) are summed in float64 in a fixed order,
 // unscaled, and mirrored to the full Hermitian matrix by gram_reduce.
+#include "dsplat_atoms.h"
 #include "dsplat_internal.h"
 
 namespace dsplat {
@@ -53,56 +54,6 @@ __device__ __forceinline__ V negate_if(V v, bool neg) {
     const unsigned mask = neg ? 0x80008000u : 0u;
     bits ^= mask;
     return __builtin_bit_cast(V, bits);
-}
-
-// Per-column pair state of one aperture (register resident across its K-steps).
-template <int P>
-struct Column {
-    float2 g[P];        // scale sqrt(area) amp rot nu_r / S per bin class r
-    float2 lobe_value;  // scale sqrt(area) amp rot D(-delta)
-    float t;            // tan(pi delta / N)
-    int lobe;           // nearest bin, -1 if the column is padding
-};
-
-template <int P>
-__device__ __forceinline__ Column<P> column_state(const Chirp& c, const float* __restrict__ pos, int a,
-                                                  const float* __restrict__ ctr, const float* __restrict__ nrm,
-                                                  const float* __restrict__ areas, float scale, int s, int S) {
-    Column<P> col;
-    col.lobe = -1;
-    col.t = 0.5f;
-    col.lobe_value = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int r = 0; r < P; ++r) col.g[r] = make_float2(0.f, 0.f);
-    if (s >= S) return col;
-    const Pair q = pair_geometry(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], ctr[3 * s], ctr[3 * s + 1],
-                                 ctr[3 * s + 2], nrm[3 * s], nrm[3 * s + 1], nrm[3 * s + 2]);
-    const float2 cw = (q.amp * sqrtf(areas[s]) * scale) * q.rot;
-    col.t = tan_small(c.pi_N * q.delta);
-    col.lobe = q.lobe;
-    float sb[P], cb[P];
-    class_angles<P>(q.delta, q.lobe, sb, cb);
-#pragma unroll
-    for (int r = 0; r < P; ++r) col.g[r] = c.inv_S * cmul(cw, make_float2(sb[r] * cb[r], -sb[r] * sb[r]));
-    float2 D, Dp;
-    lobe_direct(c, q.delta, &D, &Dp);
-    col.lobe_value = cmul(cw, D);
-    return col;
-}
-
-// Atom of a column at bin k (0 beyond K), cot form with the derivative table (T_0 = 0);
-// cls = k mod P is the bin class (class_angles already rotated the classes by the lobe).
-template <int P>
-__device__ __forceinline__ float2 column_atom(const Chirp& c, const float* __restrict__ T, const Column<P>& col,
-                                              int k, int cls) {
-    if (k >= c.K) return make_float2(0.f, 0.f);
-    const float cot = cot_bin(T[c.N - col.lobe + k], col.t);
-    float2 gr = col.g[0];
-#pragma unroll
-    for (int r = 1; r < P; ++r)
-        if (cls == r) gr = col.g[r];
-    float2 v = make_float2(gr.x * cot - gr.y, gr.y * cot + gr.x);
-    return k == col.lobe ? col.lobe_value : v;
 }
 
 template <int P, int MODE>

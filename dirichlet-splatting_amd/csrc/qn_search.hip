@@ -96,8 +96,7 @@ __device__ __forceinline__ void qn_sums(const Chirp& c, const float* __restrict_
 
 struct QnState {
     float c[3], n[3];
-    float g[6];     // ascent gradient (negated, masked, scaled, tangent)
-    float H[6][6];  // inverse-Hessian estimate
+    float g[6];  // ascent gradient (negated, masked, scaled, tangent)
     float value;
 };
 
@@ -162,6 +161,7 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
                                                                float* __restrict__ value_out) {
     extern __shared__ float T[];  // [2N] derivative cot table
     __shared__ double red[(kQnThreads / 64) * kQnMaxValues];
+    __shared__ float H[36], LH[36];  // inverse-Hessian estimate [6][6] and a product buffer
     const int s = blockIdx.x, tid = threadIdx.x;
     for (int i = tid; i < 2 * c.N; i += kQnThreads) T[i] = cot_entry(i, c.N, 0.0f);
     const float area = areas[s];
@@ -172,11 +172,8 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
         st.n[j] = nrm0[3 * s + j];
     }
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        st.g[i] = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) st.H[i][j] = i == j ? 1.0f : 0.0f;
-    }
+    for (int i = 0; i < 6; ++i) st.g[i] = 0.0f;
+    if (tid < 36) H[tid] = tid / 6 == tid % 6 ? 1.0f : 0.0f;
     __syncthreads();
     float t[kQnMaxValues];
     qn_sums<P, W>(c, T, pos, A, res, us, st.c, st.n, red, t);
@@ -190,7 +187,7 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
         for (int j = 0; j < W; ++j) {
             float v = 0.0f;
 #pragma unroll
-            for (int k = 0; k < W; ++k) v -= st.H[j][k] * st.g[k];
+            for (int k = 0; k < W; ++k) v -= H[j * 6 + k] * st.g[k];
             d[j] = v;
             descent += v * st.g[j];
         }
@@ -263,31 +260,21 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
             yy += change[j] * change[j];
         }
         const bool valid = sy > 1e-8f * sqrtf(ss) * sqrtf(yy) && sy > 1e-20f;
-        if (valid) {
+        if (valid) {  // uniform; H <- L H L^T + rho s s^T, L = I - rho s y^T, one element per thread
             const float rho = 1.0f / fmaxf(sy, 1e-30f);
-            float L[6][6], LH[6][6];
-#pragma unroll
-            for (int i = 0; i < W; ++i)
-#pragma unroll
-                for (int j = 0; j < W; ++j) L[i][j] = (i == j ? 1.0f : 0.0f) - rho * step[i] * change[j];
-#pragma unroll
-            for (int i = 0; i < W; ++i)
-#pragma unroll
-                for (int j = 0; j < W; ++j) {
-                    float v = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < W; ++k) v += L[i][k] * st.H[k][j];
-                    LH[i][j] = v;
-                }
-#pragma unroll
-            for (int i = 0; i < W; ++i)
-#pragma unroll
-                for (int j = 0; j < W; ++j) {
-                    float v = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < W; ++k) v += LH[i][k] * L[j][k];
-                    st.H[i][j] = v + rho * step[i] * step[j];
-                }
+            const int i = tid / W, j = tid - (tid / W) * W;
+            if (tid < W * W) {
+                float v = 0.0f;
+                for (int k = 0; k < W; ++k) v += ((i == k ? 1.0f : 0.0f) - rho * step[i] * change[k]) * H[k * 6 + j];
+                LH[i * 6 + j] = v;
+            }
+            __syncthreads();
+            if (tid < W * W) {
+                float v = 0.0f;
+                for (int k = 0; k < W; ++k) v += LH[i * 6 + k] * ((j == k ? 1.0f : 0.0f) - rho * step[j] * change[k]);
+                H[i * 6 + j] = v + rho * step[i] * step[j];
+            }
+            __syncthreads();
         }
         st.value = v_new;
 #pragma unroll

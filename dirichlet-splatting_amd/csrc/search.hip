@@ -62,6 +62,12 @@ __device__ __forceinline__ float peak_d2(const float* a, const float* b, int dim
     return d2;
 }
 
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int lane) {
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, lane);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restrict__ keys,
                                                            const int* __restrict__ order,
                                                            const float* __restrict__ ctr, int count, int peaks,
@@ -70,21 +76,28 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
     __shared__ float taken_lds[kPeakLdsFloats];
     __shared__ float block_c[kPeakBlock * 8];
     __shared__ unsigned long long conflict[kPeakBlock][kPeakBlock / 64];
+    __shared__ unsigned long long chosen[kPeakBlock / 64];
     __shared__ int alive[kPeakBlock];
     __shared__ int n_taken, finished;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lds_cap = kPeakLdsFloats / dim;
     if (tid == 0) {
         n_taken = 0;
         finished = 0;
     }
+    // the block's candidate of this thread, loaded one block ahead
+    auto fetch = [&](int base, bool* valid, int* id, float* c) {
+        const int p = base + tid;
+        *valid = p < count && keys[p] != -INFINITY;
+        *id = *valid ? order[p] : -1;
+        for (int j = 0; j < dim; ++j) c[j] = *valid ? ctr[(size_t)*id * dim + j] : 0.0f;
+    };
+    bool valid;
+    int id;
+    float c[8];
+    fetch(0, &valid, &id, c);
     __syncthreads();
     for (int base = 0; base < count; base += kPeakBlock) {
-        const int p = base + tid;
-        const bool valid = p < count && keys[p] != -INFINITY;
-        const int id = valid ? order[p] : -1;
-        float c[8];
-        for (int j = 0; j < dim; ++j) c[j] = valid ? ctr[(size_t)id * dim + j] : 0.0f;
         bool ok = valid;
         const int n = n_taken;
         for (int q = 0; q < n && ok; ++q) {
@@ -100,27 +113,61 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
                 if (alive[q] && peak_d2(c, block_c + q * dim, dim) < sep2) row[q >> 6] |= 1ull << (q & 63);
         for (int w = 0; w < kPeakBlock / 64; ++w) conflict[tid][w] = row[w];
         __syncthreads();
-        if (tid == 0) {
-            unsigned long long mine[kPeakBlock / 64] = {0ull, 0ull, 0ull, 0ull};
-            int k = n_taken;
-            for (int t = 0; t < kPeakBlock && k < peaks; ++t) {
-                if (!alive[t]) continue;
-                bool clash = false;
-                for (int w = 0; w < kPeakBlock / 64; ++w) clash |= (conflict[t][w] & mine[w]) != 0ull;
-                if (clash) continue;
-                mine[t >> 6] |= 1ull << (t & 63);
-                out[k] = order[base + t];
-                float* dst = k < lds_cap ? taken_lds + k * dim : spill + (size_t)(k - lds_cap) * dim;
-                for (int j = 0; j < dim; ++j) dst[j] = block_c[t * dim + j];
-                ++k;
+        bool next_valid;
+        int next_id;
+        float next_c[8];
+        fetch(base + kPeakBlock, &next_valid, &next_id, next_c);
+        // wave 0 walks the block in order on scalar bit masks: a candidate is taken iff it
+        // is alive and clashes with none taken before it in this block
+        if (wave == 0) {
+            unsigned long long rows[kPeakBlock / 64][kPeakBlock / 64];
+            unsigned long long live[kPeakBlock / 64];
+#pragma unroll
+            for (int slot = 0; slot < kPeakBlock / 64; ++slot) {
+#pragma unroll
+                for (int w = 0; w < kPeakBlock / 64; ++w) rows[slot][w] = conflict[slot * 64 + lane][w];
+                live[slot] = __ballot(alive[slot * 64 + lane] != 0);
             }
-            n_taken = k;
-            // sorted keys: a non-finite key ends the finite candidates
-            const int last = min(count, base + kPeakBlock) - 1;
-            finished = k >= peaks || keys[last] == -INFINITY;
+            unsigned long long mine[kPeakBlock / 64] = {0ull, 0ull, 0ull, 0ull};
+            int k = n;
+#pragma unroll
+            for (int slot = 0; slot < kPeakBlock / 64; ++slot) {
+                unsigned long long m = live[slot];
+                while (m != 0ull && k < peaks) {
+                    const int l = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    unsigned long long clash = 0ull;
+#pragma unroll
+                    for (int w = 0; w < kPeakBlock / 64; ++w) clash |= readlane64(rows[slot][w], l) & mine[w];
+                    if (clash != 0ull) continue;
+                    mine[slot] |= 1ull << l;
+                    ++k;
+                }
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int w = 0; w < kPeakBlock / 64; ++w) chosen[w] = mine[w];
+                n_taken = k;
+                // sorted keys: a non-finite key ends the finite candidates
+                const int last = min(count, base + kPeakBlock) - 1;
+                finished = k >= peaks || keys[last] == -INFINITY;
+            }
+        }
+        __syncthreads();
+        // every taken candidate writes its own slot: its rank among this block's choices
+        const unsigned long long bits = chosen[tid >> 6];
+        if ((bits >> lane) & 1ull) {
+            int k = n + __popcll(bits & ((1ull << lane) - 1ull));
+            for (int w = 0; w < (tid >> 6); ++w) k += __popcll(chosen[w]);
+            out[k] = id;
+            float* dst = k < lds_cap ? taken_lds + k * dim : spill + (size_t)(k - lds_cap) * dim;
+            for (int j = 0; j < dim; ++j) dst[j] = c[j];
         }
         __syncthreads();
         if (finished) break;
+        valid = next_valid;
+        id = next_id;
+        for (int j = 0; j < dim; ++j) c[j] = next_c[j];
     }
     for (int k = n_taken + tid; k < peaks; k += kPeakBlock) out[k] = -1;
 }

@@ -2,7 +2,9 @@
 // call of dsplat/kernels/cublas.py:31-44; the lower triangle is left untouched).
 //
 // rows [m, n] row-major, out [n, n] row-major. A workgroup (4 waves) owns a 128 x 128
-// tile of the upper triangle and a range of rows (split-K); each wave accumulates a
+// tile of the upper triangle and a range of rows (split-K); with col0 >= 0 the tiles cover
+// instead the rectangle out [n, n - col0] += rows^T rows[:, col0:] (the cross products of new
+// columns against all of them, the pooled statistics' GEMM); each wave accumulates a
 // 64 x 64 block as 2 x 2 MFMA blocks of 32 x 32. Rows are staged through LDS in chunks
 // of 32, column-major within a panel (k contiguous) so each lane reads its K values with
 // 128-bit loads, double-buffered with the next chunk's global loads in flight.
@@ -62,6 +64,22 @@ __device__ __forceinline__ void syrk_tile_of(int t, int nb, int* bi, int* bj) {
     *bj = row + t;
 }
 
+// First output row and column of tile t: the upper triangle of rows^T rows (col0 < 0), or
+// the full rectangle rows^T rows[:, col0:] (columns col0..n-1) row-major over tiles.
+__device__ __forceinline__ void tile_origin(int t, int n, int col0, int* ci, int* cj) {
+    const int nb = (n + kTile - 1) / kTile;
+    if (col0 < 0) {
+        int bi, bj;
+        syrk_tile_of(t, nb, &bi, &bj);
+        *ci = bi * kTile;
+        *cj = bj * kTile;
+    } else {
+        const int nbj = (n - col0 + kTile - 1) / kTile;
+        *ci = (t / nbj) * kTile;
+        *cj = col0 + (t % nbj) * kTile;
+    }
+}
+
 // Exponent that scales a column whose largest magnitude is `top` into [2^13, 2^14).
 __device__ __forceinline__ int column_exponent(float top) {
     if (!(top > 0.0f) || !isfinite(top)) return 0;
@@ -99,15 +117,13 @@ __device__ __forceinline__ void stage_load(Staged& s, const float* __restrict__ 
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ rows, int m, int n, int per_split,
-                                                   const unsigned* __restrict__ colmax,
+__global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ rows, int m, int n, int col0,
+                                                   int per_split, const unsigned* __restrict__ colmax,
                                                    float* __restrict__ partial) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
-    const int nb = (n + kTile - 1) / kTile;
-    int bi, bj;
-    syrk_tile_of(blockIdx.x, nb, &bi, &bj);
-    const bool two = bi != bj;
-    const int ci = bi * kTile, cj = bj * kTile;
+    int ci, cj;
+    tile_origin(blockIdx.x, n, col0, &ci, &cj);
+    const bool two = ci != cj;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wi = wave >> 1, wj = wave & 1, h = lane >> 5, l32 = lane & 31;
     const int k_begin = blockIdx.y * per_split, k_end = min(m, k_begin + per_split);
@@ -258,23 +274,25 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
         }
 }
 
-// out[gi][gj] += sum over splits (fixed order, float64) for gi <= gj, unscaled by
-// 2^-(e_i + e_j) in MODE 2.
-__global__ __launch_bounds__(256) void syrk_reduce(const float* __restrict__ partial, int n, int tiles, int splits,
-                                                   const unsigned* __restrict__ colmax, float* __restrict__ out) {
-    const int nb = (n + kTile - 1) / kTile;
-    int bi, bj;
-    syrk_tile_of(blockIdx.x, nb, &bi, &bj);
+// out[gi][gj] += sum over splits (fixed order, float64), unscaled by 2^-(e_i + e_j) in
+// MODE 2: gi <= gj of out [n, n] for the triangle, every entry of out [n, n - col0]
+// (column gj - col0) for the rectangle.
+__global__ __launch_bounds__(256) void syrk_reduce(const float* __restrict__ partial, int n, int col0, int tiles,
+                                                   int splits, const unsigned* __restrict__ colmax,
+                                                   float* __restrict__ out) {
+    int ci, cj;
+    tile_origin(blockIdx.x, n, col0, &ci, &cj);
+    const int width = col0 < 0 ? n : n - col0, shift = col0 < 0 ? 0 : col0;
     for (int e = blockIdx.y * 256 + threadIdx.x; e < kTile * kTile; e += gridDim.y * 256) {
         const int row = e / kTile, col = e % kTile;
-        const int gi = bi * kTile + row, gj = bj * kTile + col;
-        if (gi >= n || gj >= n || gi > gj) continue;
+        const int gi = ci + row, gj = cj + col;
+        if (gi >= n || gj >= n || (col0 < 0 && gi > gj)) continue;
         double sum = 0.0;
         for (int s = 0; s < splits; ++s) sum += partial[((size_t)s * tiles + blockIdx.x) * kTile * kTile + e];
         if (colmax)
             sum = ldexp(sum, -column_exponent(__uint_as_float(colmax[gi])) -
                                  column_exponent(__uint_as_float(colmax[gj])));
-        out[(size_t)gi * n + gj] += (float)sum;
+        out[(size_t)gi * width + (gj - shift)] += (float)sum;
     }
 }
 
@@ -282,10 +300,10 @@ struct SyrkShape {
     int tiles, splits, per_split;
 };
 
-static SyrkShape syrk_shape(int m, int n) {
+static SyrkShape syrk_shape(int m, int n, int col0) {
     SyrkShape s;
     const int nb = (n + kTile - 1) / kTile;
-    s.tiles = nb * (nb + 1) / 2;
+    s.tiles = col0 < 0 ? nb * (nb + 1) / 2 : nb * ((n - col0 + kTile - 1) / kTile);
     int splits = (1024 + s.tiles - 1) / s.tiles;
     const int most = (m + kMinSplitRows - 1) / kMinSplitRows;
     if (splits > most) splits = most;
@@ -297,9 +315,9 @@ static SyrkShape syrk_shape(int m, int n) {
     return s;
 }
 
-size_t syrk_workspace(int m, int n, int mode) {
-    if (m <= 0 || n <= 0) return 0;
-    const SyrkShape s = syrk_shape(m, n);
+size_t syrk_workspace(int m, int n, int col0, int mode) {
+    if (m <= 0 || n <= 0 || col0 >= n) return 0;
+    const SyrkShape s = syrk_shape(m, n, col0);
     return sizeof(float) * (size_t)s.splits * s.tiles * kTile * kTile + (mode == 2 ? 256 + sizeof(unsigned) * n : 0);
 }
 
@@ -308,19 +326,20 @@ static size_t syrk_lds(int mode) {
 }
 
 template <int MODE>
-static hipError_t launch_syrk_mode(const float* rows, int m, int n, const SyrkShape& s, const unsigned* colmax,
-                                   float* partial, hipStream_t st) {
+static hipError_t launch_syrk_mode(const float* rows, int m, int n, int col0, const SyrkShape& s,
+                                   const unsigned* colmax, float* partial, hipStream_t st) {
     const size_t smem = syrk_lds(MODE);
     hipError_t e = allow_lds((const void*)syrk_kernel<MODE>, smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((syrk_kernel<MODE>), dim3(s.tiles, s.splits), dim3(256), smem, st, rows, m, n, s.per_split,
-                       colmax, partial);
+    hipLaunchKernelGGL((syrk_kernel<MODE>), dim3(s.tiles, s.splits), dim3(256), smem, st, rows, m, n, col0,
+                       s.per_split, colmax, partial);
     return hipGetLastError();
 }
 
-hipError_t launch_syrk(const float* rows, int m, int n, int mode, float* out, void* workspace, hipStream_t st) {
-    if (m <= 0 || n <= 0) return hipSuccess;
-    const SyrkShape s = syrk_shape(m, n);
+hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, float* out, void* workspace,
+                       hipStream_t st) {
+    if (m <= 0 || n <= 0 || col0 >= n) return hipSuccess;
+    const SyrkShape s = syrk_shape(m, n, col0);
     float* partial = static_cast<float*>(workspace);
     unsigned* colmax = nullptr;
     hipError_t e;
@@ -333,13 +352,14 @@ hipError_t launch_syrk(const float* rows, int m, int n, int mode, float* out, vo
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     switch (mode) {
-        case 0: e = launch_syrk_mode<0>(rows, m, n, s, colmax, partial, st); break;
-        case 1: e = launch_syrk_mode<1>(rows, m, n, s, colmax, partial, st); break;
-        case 2: e = launch_syrk_mode<2>(rows, m, n, s, colmax, partial, st); break;
+        case 0: e = launch_syrk_mode<0>(rows, m, n, col0, s, colmax, partial, st); break;
+        case 1: e = launch_syrk_mode<1>(rows, m, n, col0, s, colmax, partial, st); break;
+        case 2: e = launch_syrk_mode<2>(rows, m, n, col0, s, colmax, partial, st); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(syrk_reduce, dim3(s.tiles, 8), dim3(256), 0, st, partial, n, s.tiles, s.splits, colmax, out);
+    hipLaunchKernelGGL(syrk_reduce, dim3(s.tiles, 8), dim3(256), 0, st, partial, n, col0, s.tiles, s.splits, colmax,
+                       out);
     return hipGetLastError();
 }
 

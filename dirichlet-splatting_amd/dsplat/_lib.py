@@ -118,6 +118,7 @@ SIGNATURES = {
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr,
          c_ptr],
     ),
+    "dsplat_project_design": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr]),
     "dsplat_qn_workspace": (c_size, [c_ptr, c_int]),
     "dsplat_qn_search": (
         c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
@@ -126,6 +127,8 @@ SIGNATURES = {
     "dsplat_backward_elimination": (c_int, [c_ptr, c_ptr, c_int, c_int, c_ptr, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_syrk_workspace": (c_size, [c_int, c_int, c_int]),
     "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
+    "dsplat_gemm_tn_workspace": (c_size, [c_int, c_int, c_int, c_int]),
+    "dsplat_gemm_tn": (c_int, [c_ptr, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_scan_workspace": (c_size, [c_int, c_ptr, c_int]),
     "dsplat_scan_batched": (
         c_int, [c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]

@@ -236,6 +236,24 @@ class FMCWModel:
                 a, y = self.design(params, rows), flat[rows]
                 yield (a, y) if basis is None else (basis.reduce(a), basis.reduce(y))
 
+    def projected_blocks(self, params, chunk):
+        """Real row-projected design blocks of whole apertures on the shared row basis, atoms
+        generated on chip and never written (the blocks of `blocks` without y), or None
+        when there is no basis to project on."""
+        if self.basis is None:
+            return None
+        config, positions = self.sensor
+        bins = self.shape[1]
+        apertures = max(1, chunk // bins)
+        basis = self.basis.interleaved
+        return (
+            surfel.kernel.project_design(
+                config, positions[a0 : a0 + apertures], params["centers"], params["normals"], params["areas"], basis,
+                self.fresnel_n,
+            )  # fmt: skip
+            for a0 in range(0, len(positions), apertures)
+        )
+
     def _apertures(self, rows):
         apertures, inverse = torch.unique(rows // self.shape[1], return_inverse=True)
         return self.scan.scan_positions_m[apertures], inverse, rows % self.shape[1]

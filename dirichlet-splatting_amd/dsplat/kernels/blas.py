@@ -28,6 +28,24 @@ def syrk(rows, out, mode="fp32"):
     return out
 
 
+def gemm_tn(rows, col0, out, mode="fp32"):
+    """out [n, n - col0] += rows^T rows[:, col0:] on the SYRK tiles (rows [m, n] float32)."""
+    if rows.dtype != torch.float32 or out.dtype != torch.float32:
+        raise TypeError("gemm_tn takes float32 tensors")
+    m, n = rows.shape
+    if not out.is_contiguous() or out.shape != (n, n - col0):
+        raise ValueError("out must be a contiguous [n, n - col0] tensor")
+    code = SYRK_MODES[mode]
+    rows = rows.contiguous()
+    size = _lib.library().dsplat_gemm_tn_workspace(m, n, col0, code)
+    workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=rows.device)
+    _lib.call(
+        "dsplat_gemm_tn", _lib.ptr(rows), m, n, int(col0), code, _lib.ptr(out), _lib.ptr(workspace), size,
+        _lib.stream(rows.device),
+    )  # fmt: skip
+    return out
+
+
 GRAM_MODES = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
 FP16_TARGET = 2.0**14  # largest scaled atom magnitude in FP16 mode (max finite 65504)
 

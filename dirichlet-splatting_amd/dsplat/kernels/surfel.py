@@ -259,6 +259,21 @@ def correlate_orientations(config, positions, centers, normals, bank, residual, 
     return torch.complex(output[..., 0], output[..., 1]), output[..., 2]
 
 
+def project_design(config, positions, centers, normals, areas, basis, fresnel_n):
+    """Row-projected real design [rows * A, S] (row j * A + a) of a basis [rows, 2K] in its
+    interleaved real form, atoms generated on chip (fmcw.py:243-261 blocks + reduce_major)."""
+    _check(positions, centers, normals)
+    rows = basis.shape[0]
+    out = centers.new_empty((rows * len(positions), len(centers)))
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_project_design", _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
+        _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()),
+        len(centers), _lib.ptr(basis.float().contiguous()), rows, _lib.ptr(out), _lib.stream(centers.device),
+    )  # fmt: skip
+    return out
+
+
 def orient_select(config, positions, centers, normals, areas, bank, residual, fresnel_n, table=None, real_part=False):
     """Best orientation per candidate in one launch: (value [S] float64, index [S] int64),
     value = |s|^2 / max(area |a|^2, 1e-60) with s = sqrt(area) a^H r (or its real part),

@@ -13,7 +13,10 @@ import torch
 from .certificate import certificate
 from .kernels import solver as elimination
 from .parameters import periodic_delta
-from .varpro import Fit, accept, fit_coefficients, rendered_loss, row_chunk, screen_bar, shifted_cholesky, varpro
+from .kernels.blas import gemm_tn
+from .varpro import (
+    FOLD, SYRK_MODE, Fit, accept, fit_coefficients, rendered_loss, row_chunk, screen_bar, shifted_cholesky, varpro,
+)
 
 
 def _feasible(state, pool, owner):
@@ -142,13 +145,38 @@ def _distinct(scores, points, occupied, period, separation, width):
             free &= (near >= separation) & (near > 0)
         inside = distance(block, block)
         clash = ((inside < separation) | (inside == 0)).triu(1)
-        if bool(clash[free][:, free].any()):
+        if separation <= 0:
+            # collisions are exact coincidences, an equivalence: the in-order greedy keeps
+            # the first free member of each class, so one masked reduction decides it
+            free &= ~(clash & free[:, None]).any(0)
+        elif bool(clash[free][:, free].any()):
             for i in free.nonzero().flatten().tolist():
                 if bool((clash[:i, i] & free[:i]).any()):
                     free[i] = False
         kept.append(ids[free])
         found += int(free.sum())
     return torch.cat(kept)[:width] if kept else order[:0]
+
+
+def _pooled_cross(model, pool, count, target, config):
+    """Real cross products of the new pooled columns with all of them on the split SYRK
+    tiles (folded into float64 like the Gram), and their rhs from one correlation."""
+    total = len(pool["centers"])
+    mode = SYRK_MODE[config.gram]
+    part = torch.zeros((total, total - count), dtype=torch.float32, device=target.device)
+    folded = torch.zeros_like(part, dtype=torch.float64)
+    chunk = row_chunk(model, total, config)
+    fused = model.projected_blocks(pool, chunk) if hasattr(model, "projected_blocks") else None
+    stream = fused if fused is not None else (a for a, _ in model.blocks(pool, target, True, chunk))
+    for index, a in enumerate(stream):
+        gemm_tn(a.contiguous(), count, part, mode)
+        if (index + 1) % FOLD == 0:
+            folded += part
+            part.zero_()
+    folded += part
+    fresh = {key: value[count:] for key, value in pool.items()}
+    right = model.correlate(fresh, target)[0].real.double()
+    return (folded / model.size).float(), (right / model.size).float()
 
 
 def pooled_statistics(state, pool, config):
@@ -159,13 +187,16 @@ def pooled_statistics(state, pool, config):
     gram = state.gram.new_empty((total, total))
     rhs = state.rhs.new_empty(total)
     gram[:count, :count], rhs[:count] = state.gram, state.rhs
-    cross = state.gram.new_zeros((total, added))
-    right = state.rhs.new_zeros(added)
-    for a, y in model.blocks(pool, state.target, config.real, row_chunk(model, total, config)):
-        new = a[:, count:]
-        cross.addmm_(a.H, new)
-        right.addmv_(new.H, y)
-    cross, right = cross / model.size, right / model.size
+    if config.real and getattr(config, "batched", False) and hasattr(model, "correlate"):
+        cross, right = _pooled_cross(model, pool, count, state.target, config)
+    else:
+        cross = state.gram.new_zeros((total, added))
+        right = state.rhs.new_zeros(added)
+        for a, y in model.blocks(pool, state.target, config.real, row_chunk(model, total, config)):
+            new = a[:, count:]
+            cross.addmm_(a.H, new)
+            right.addmv_(new.H, y)
+        cross, right = cross / model.size, right / model.size
     gram[:, count:], gram[count:, :] = cross, cross.H
     gram[count:, count:] = (cross[count:] + cross[count:].H) * 0.5
     rhs[count:] = right

@@ -102,6 +102,11 @@ def normal_equations(model, params, target, config):
     part = torch.zeros((count, count), dtype=torch.float32, device=device)
     total = torch.zeros((count, count), dtype=torch.float64, device=device)
     rhs = torch.zeros(count, dtype=torch.float64, device=device)
+    if batched and hasattr(model, "projected_blocks"):
+        # projected blocks straight from the atoms on chip; y enters through the correlation
+        fused = model.projected_blocks(params, row_chunk(model, count, config))
+        if fused is not None:
+            stream = ((a, None) for a in fused)
     for index, (a, y) in enumerate(stream):
         syrk(a.contiguous(), part, mode)
         if not batched:

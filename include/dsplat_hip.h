@@ -232,6 +232,17 @@ int dsplat_energy(const float* value, const float* reference, size_t count, floa
 int dsplat_range_response(double* out, double low, double high, int32_t count, int32_t num_bins,
                           int32_t samples, int32_t n_fft, void* stream);
 
+/* Row-projected real dictionary block in one pass: out [rows * A, S] row-major, row
+ * j * A + a = sum over (k, part) of basis [rows, 2K][j][2k + part] times the real (part 0)
+ * or imaginary (part 1) part of atom (a, k, s) = sqrt(area_s) amp rot D(k - k0). basis is
+ * the interleaved real form of a row basis U^H (rows even); atoms are never written.
+ * Replaces the design + reduce_major product of the streamed blocks (fmcw.py:243-261,
+ * surfel.slang:313-378). */
+int dsplat_project_design(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                          const float* centers, const float* normals, const float* areas,
+                          int32_t num_surfels, const float* basis, int32_t rows, float* out,
+                          void* stream);
+
 /* ---- Batched quasi-Newton certificate search. */
 
 /* Per-search constants of dsplat_qn_search, from the parameter specs (certificate.py:75-86
@@ -285,6 +296,13 @@ int dsplat_backward_elimination(const double* inverse, const double* b, int32_t 
 size_t dsplat_syrk_workspace(int32_t m, int32_t n, int32_t mode);
 int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* out, void* workspace,
                 size_t workspace_bytes, void* stream);
+
+/* out [n, n - col0] (row-major) += rows^T rows[:, col0:], rows [m, n] float32: the cross
+ * products of the trailing columns with all columns, on the SYRK kernel's tiles and modes.
+ * Replaces the streamed addmm_ of the pooled statistics (replacement.py:178-199). */
+size_t dsplat_gemm_tn_workspace(int32_t m, int32_t n, int32_t col0, int32_t mode);
+int dsplat_gemm_tn(const float* rows, int32_t m, int32_t n, int32_t col0, int32_t mode, float* out,
+                   void* workspace, size_t workspace_bytes, void* stream);
 
 size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels);
 

@@ -114,3 +114,70 @@ def test_qn_kernel_follows_the_host_ascent(fmcw, mode, normals):  # noqa: F811
     assert float((host["centers"][0] - one["centers"][0]).norm()) < 5e-5
     if normals:
         torch.testing.assert_close(fused["normals"].norm(dim=1), torch.ones(len(fused["normals"]), device="cuda"))
+
+
+@pytest.mark.parametrize("gram", ["plain", "fp16x3", "bf16x3"])
+def test_batched_pooled_statistics_match_a_full_stream(fmcw, gram):  # noqa: F811
+    from dsplat import DSFWConfig, initialize
+    from dsplat.replacement import pooled_statistics
+    from dsplat.varpro import normal_equations
+
+    model, params, _ = fmcw
+    config = DSFWConfig(gram=gram, batched=True)
+    state = initialize(model, model.scan.data, surfels(5, seed=2), config, parameters=fmcw_specs())
+    pool = {k: torch.cat((v, params[k])) for k, v in state.params.items()}
+    gram_, rhs = pooled_statistics(state, pool, config)
+    expected_gram, expected_rhs = normal_equations(state.model, pool, model.scan.data, DSFWConfig())
+    close(gram_, expected_gram, 1e-4)
+    close(rhs, expected_rhs, 1e-4)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp16x3"])
+@pytest.mark.parametrize("m,n,col0", [(3000, 300, 256), (700, 130, 0), (5000, 513, 500), (64, 200, 199)])
+def test_gemm_tn_is_the_trailing_cross_product(m, n, col0, mode):
+    from dsplat.kernels.blas import gemm_tn
+
+    g = torch.Generator().manual_seed(m + n + col0)
+    rows = torch.randn(m, n, generator=g)
+    out = torch.randn(n, n - col0, generator=g)
+    got = gemm_tn(rows.cuda(), col0, out.clone().cuda(), mode).cpu().double()
+    want = out.double() + rows.double().T @ rows.double()[:, col0:]
+    assert ((got - want).abs().max() / want.abs().max()) < 2e-6
+
+
+@pytest.mark.parametrize("pad,fresnel_n", [(2, 0.0), (2, 1.6), (1, 0.0), (4, 0.0)])
+def test_projected_design_equals_the_projected_dense_design(pad, fresnel_n):
+    """The fused kernel's rows equal the interleaved basis times the real bin-major design
+    (design kernel, then reduce_major), to FP16 hi + lo accuracy."""
+    from dsplat import FMCWConfig
+    from dsplat.fmcw import response_basis
+    from dsplat.kernels import surfel
+
+    cfg = FMCWConfig(pad_factor=pad)
+    positions = aperture_grid(6)
+    p = surfels(300, seed=pad, spread=0.03)
+    p["areas"] = p["areas"] * torch.linspace(0.5, 2.0, 300, device="cuda")
+    k0 = (torch.cdist(p["centers"], positions) / cfg.range_bin_spacing_m)
+    basis = response_basis(cfg, float(k0.min()) - 1, float(k0.max()) + 1, "cuda")
+    assert basis is not None
+    got = surfel.project_design(cfg, positions, p["centers"], p["normals"], p["areas"], basis.interleaved, fresnel_n)
+    dense = surfel.design(cfg, positions, p["centers"], p["normals"], p["areas"], fresnel_n, True)
+    want = basis.reduce_major(dense)
+    assert got.shape == want.shape
+    err = (got.double() - want.double()).norm() / want.double().norm()
+    assert err < 2e-6, err
+
+
+def test_fused_normal_equations_match_the_plain_stream(fmcw):  # noqa: F811
+    from dsplat import DSFWConfig
+    from dsplat.varpro import normal_equations
+
+    model, params, _ = fmcw
+    model = model.prepare(fmcw_specs())
+    plain = normal_equations(model, params, model.scan.data, DSFWConfig(row_chunk=2 * model.shape[1]))
+    for gram in ("plain", "fp16x3"):
+        fused = normal_equations(
+            model, params, model.scan.data, DSFWConfig(row_chunk=3 * model.shape[1], gram=gram, batched=True)
+        )
+        close(fused[0], plain[0], 1e-5)
+        close(fused[1], plain[1], 1e-5)

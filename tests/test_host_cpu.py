@@ -122,3 +122,32 @@ def test_bounds_period_and_frozen_components():
     periodic = ParameterConfig(period=(10.0,))
     assert periodic.project(torch.tensor([12.5, -1.0]), torch.zeros(2)).tolist() == [2.5, 9.0]
 
+
+
+@pytest.mark.parametrize("separation", [0.0, 0.3])
+def test_distinct_matches_the_sequential_greedy(separation):
+    """Coincident candidates (as coarse peaks beside unmoved refinements) keep only their
+    best copy; the vectorized coincidence rule equals the one-by-one greedy."""
+    from dsplat.replacement import _distinct
+
+    g = torch.Generator().manual_seed(5)
+    points = torch.rand(600, 3, generator=g)
+    points[300:450] = points[:150]  # exact duplicates
+    scores = torch.rand(600, generator=g)
+    scores[400] = scores[100]  # a tie between copies
+    occupied = torch.rand(50, 3, generator=g)
+    occupied[0] = points[7]
+    got = _distinct(scores, points, occupied, None, separation, 200)
+
+    order = sorted(range(600), key=lambda i: (-float(scores[i]), i))
+    kept = []
+    for i in order:
+        d_occ = (occupied - points[i]).norm(dim=1)
+        if bool(((d_occ < separation) | (d_occ == 0)).any()):
+            continue
+        if any(float((points[j] - points[i]).norm()) < separation or torch.equal(points[j], points[i]) for j in kept):
+            continue
+        kept.append(i)
+        if len(kept) == 200:
+            break
+    assert got.tolist() == kept
