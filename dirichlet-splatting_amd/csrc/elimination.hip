@@ -7,16 +7,16 @@
 // rest through rank-one corrections stored as columns u_s = w_s / sqrt(pivot_s):
 //   w = W[:, i] - sum_{s < step} u_s u_s[i],  b -= w b_i / w_i,  d -= u^2,
 // where d is the running diagonal. At most `limit` unknowns flagged in `limited` leave.
-// 1024 threads; thread t owns rows t + 1024 r (r < kElimRows). U [count][n] lives in the
+// 1024 threads; thread t owns rows t + 1024 r (r < kElimRows, so n <= 4096). U [count][n] lives in the
 // workspace (coalesced per step); the state b, d, flags stays in registers.
 #include "dsplat_internal.h"
 
 namespace dsplat {
 
 constexpr int kElimThreads = 1024;
-constexpr int kElimRows = 8;  // n <= 8192
+constexpr int kElimRows = 4;  // n <= 4096
 constexpr int kElimMaxCount = 4096;
-constexpr int kElimUnroll = 8;
+constexpr int kElimUnroll = 4;
 
 struct Best {
     double cost;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kElimThreads) void elimination_kernel(const double*
 #pragma unroll
         for (int r = 0; r < kElimRows; ++r) {
             const int j = tid + kElimThreads * r;
-            if (j >= n) continue;
+            if (j >= n) break;  // rows of a thread ascend
             const bool blocked = !alive[r] || (limited && lim[r] && spent >= limit);
             const Best c = {blocked ? INFINITY : b[r] * b[r] / fmax(d[r], tiny), j};
             if (better(c, best)) best = c;
@@ -95,12 +95,13 @@ __global__ __launch_bounds__(kElimThreads) void elimination_kernel(const double*
 #pragma unroll
             for (int r = 0; r < kElimRows; ++r) {
                 const int j = tid + kElimThreads * r;
-                if (j >= n) continue;
-                double x[kElimUnroll];
+                if (j < n) {
+                    double x[kElimUnroll];
 #pragma unroll
-                for (int q = 0; q < kElimUnroll; ++q) x[q] = U[(size_t)(s + q) * n + j];
+                    for (int q = 0; q < kElimUnroll; ++q) x[q] = U[(size_t)(s + q) * n + j];
 #pragma unroll
-                for (int q = 0; q < kElimUnroll; ++q) w[r] -= x[q] * u_at[s + q];
+                    for (int q = 0; q < kElimUnroll; ++q) w[r] -= x[q] * u_at[s + q];
+                }
             }
         }
         for (; s < step; ++s) {
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(kElimThreads) void elimination_kernel(const double*
 #pragma unroll
         for (int r = 0; r < kElimRows; ++r) {
             const int j = tid + kElimThreads * r;
-            if (j >= n) continue;
+            if (j >= n) break;
             b[r] -= w[r] * ratio;
             const double u = w[r] / root;
             U[(size_t)step * n + j] = u;

@@ -68,12 +68,24 @@ __device__ __forceinline__ unsigned long long readlane64(unsigned long long v, i
     return ((unsigned long long)hi << 32) | lo;
 }
 
+// Uniform grid over the first (up to 3) coordinates for the taken peaks: cells of edge
+// 1.01 separation, so points closer than the separation lie in the same or adjacent cells
+// even after rounding; buckets hold chains of taken indices (hash collisions are weeded out
+// by the distance test itself).
+constexpr int kPeakBuckets = 4096;
+
+__device__ __forceinline__ int peak_bucket(int ix, int iy, int iz) {
+    return (int)(((unsigned)ix * 73856093u ^ (unsigned)iy * 19349663u ^ (unsigned)iz * 83492791u) &
+                 (kPeakBuckets - 1));
+}
+
 __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restrict__ keys,
                                                            const int* __restrict__ order,
                                                            const float* __restrict__ ctr, int count, int peaks,
                                                            int dim, float sep2, float* __restrict__ spill,
-                                                           int* __restrict__ out) {
+                                                           int* __restrict__ chain, int* __restrict__ out) {
     __shared__ float taken_lds[kPeakLdsFloats];
+    __shared__ int head[kPeakBuckets];
     __shared__ float block_c[kPeakBlock * 8];
     __shared__ unsigned long long conflict[kPeakBlock][kPeakBlock / 64];
     __shared__ unsigned long long chosen[kPeakBlock / 64];
@@ -81,10 +93,16 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
     __shared__ int n_taken, finished;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lds_cap = kPeakLdsFloats / dim;
+    const bool grid = dim <= 3;  // taken lookups through the cell grid, else a linear scan
+    const float inv_h = 1.0f / (1.01f * sqrtf(sep2));
     if (tid == 0) {
         n_taken = 0;
         finished = 0;
     }
+    for (int i = tid; i < kPeakBuckets; i += kPeakBlock) head[i] = -1;
+    auto cell = [&](const float* p, int j) {  // clamped far from int overflow
+        return j < dim ? (int)fminf(fmaxf(floorf(p[j] * inv_h), -1e9f), 1e9f) : 0;
+    };
     // the block's candidate of this thread, loaded one block ahead
     auto fetch = [&](int base, bool* valid, int* id, float* c) {
         const int p = base + tid;
@@ -100,9 +118,20 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
     for (int base = 0; base < count; base += kPeakBlock) {
         bool ok = valid;
         const int n = n_taken;
-        for (int q = 0; q < n && ok; ++q) {
-            const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
-            if (peak_d2(c, t, dim) < sep2) ok = false;
+        if (sep2 > 0.0f && grid) {
+            const int ix = cell(c, 0), iy = cell(c, 1), iz = cell(c, 2);
+            for (int dz = dim > 2 ? -1 : 0; dz <= (dim > 2 ? 1 : 0) && ok; ++dz)
+                for (int dy = dim > 1 ? -1 : 0; dy <= (dim > 1 ? 1 : 0) && ok; ++dy)
+                    for (int dx = -1; dx <= 1 && ok; ++dx)
+                        for (int q = head[peak_bucket(ix + dx, iy + dy, iz + dz)]; q >= 0 && ok; q = chain[q]) {
+                            const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
+                            if (peak_d2(c, t, dim) < sep2) ok = false;
+                        }
+        } else if (sep2 > 0.0f) {
+            for (int q = 0; q < n && ok; ++q) {
+                const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
+                if (peak_d2(c, t, dim) < sep2) ok = false;
+            }
         }
         alive[tid] = ok;
         for (int j = 0; j < dim; ++j) block_c[tid * dim + j] = c[j];
@@ -162,6 +191,7 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
             out[k] = id;
             float* dst = k < lds_cap ? taken_lds + k * dim : spill + (size_t)(k - lds_cap) * dim;
             for (int j = 0; j < dim; ++j) dst[j] = c[j];
+            if (sep2 > 0.0f && grid) chain[k] = atomicExch(&head[peak_bucket(cell(c, 0), cell(c, 1), cell(c, 2))], k);
         }
         __syncthreads();
         if (finished) break;
@@ -178,7 +208,7 @@ size_t peaks_workspace(int count, int peaks, int dim) {
         nullptr, sort_bytes, (const float*)nullptr, (float*)nullptr, (const int*)nullptr, (int*)nullptr, count);
     if (e != hipSuccess) return 0;
     const size_t arrays = 4 * (size_t)count * sizeof(float);
-    return 256 + arrays + sizeof(float) * (size_t)peaks * dim + sort_bytes;
+    return 512 + arrays + sizeof(float) * (size_t)peaks * (dim + 1) + sort_bytes;
 }
 
 // -H^{-1} g (or -g if not a descent direction), masked, tangent to unit normals and
@@ -244,7 +274,8 @@ hipError_t launch_peaks(const float* scores, const float* ctr, int count, int pe
     int* ids_in = reinterpret_cast<int*>(keys + count);
     int* order = ids_in + count;
     float* spill = reinterpret_cast<float*>(order + count);
-    char* temp = reinterpret_cast<char*>(spill + (size_t)peaks * dim);
+    int* chain = reinterpret_cast<int*>(spill + (size_t)peaks * dim);
+    char* temp = reinterpret_cast<char*>(chain + peaks);
     temp += (256 - reinterpret_cast<uintptr_t>(temp) % 256) % 256;
     size_t temp_bytes = 0;
     hipError_t e = hipcub::DeviceRadixSort::SortPairsDescending(nullptr, temp_bytes, keys_in, keys, ids_in, order,
@@ -259,7 +290,7 @@ hipError_t launch_peaks(const float* scores, const float* ctr, int count, int pe
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(peaks_kernel, dim3(1), dim3(kPeakBlock), 0, st, keys, order, ctr, count, peaks, dim,
-                       separation * separation, spill, out);
+                       separation * separation, spill, chain, out);
     return hipGetLastError();
 }
 

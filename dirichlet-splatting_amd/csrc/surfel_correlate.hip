@@ -390,11 +390,12 @@ __global__ __launch_bounds__(256) void orient_best_kernel(
     float acc[kOrientMax][3];
 #pragma unroll
     for (int j = 0; j < kOrientMax; ++j) acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
+    for (int i = 3 * (B + 1) + tid; i < 3 * kOrientMax; i += 256) dirs[i] = 0.0f;  // unused slots: zero normals
     const float fres = c.fresnel_n;
     auto accumulate = [&](const Pair& q, float2 corr, float zn) {
+        // every slot is evaluated (a zero normal adds nothing), so the sums stay in registers
 #pragma unroll
         for (int j = 0; j < kOrientMax; ++j) {
-            if (j > B) break;  // uniform
             const float mx = j == 0 ? nx : dirs[3 * j], my = j == 0 ? ny : dirs[3 * j + 1];
             const float mz = j == 0 ? nz : dirs[3 * j + 2];
             const float cosine = q.ux * mx + q.uy * my + q.uz * mz;
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(256) void orient_best_kernel(
     int index = 0;
 #pragma unroll
     for (int j = 0; j < kOrientMax; ++j) {
-        if (j > B) break;
+        if (j > B) continue;
         const double re = (double)(acc[j][0] * scale), im = (double)(acc[j][1] * scale);
         const double mag = real_part ? fabs(re) : hypot(re, im);
         const double value = mag * mag / fmax((double)(acc[j][2] * scale2), 1e-60);

@@ -14,7 +14,7 @@ def backward_elimination(inverse, b, count, limited=None, limit=0):
         return removed
     size = _lib.library().dsplat_elimination_workspace(n, count)
     if size == 0:
-        raise ValueError(f"backward elimination kernel supports n <= 8192, count <= min(n, 4096); got {n}, {count}")
+        raise ValueError(f"backward elimination kernel supports n <= 4096, count <= n; got {n}, {count}")
     workspace = torch.empty(size // 8 + 1, dtype=torch.float64, device=inverse.device)
     flags = None if limited is None else limited.to(torch.int32).contiguous()
     _lib.call(

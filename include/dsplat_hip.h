@@ -286,7 +286,7 @@ int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num
  * removes the unknown with the smallest b_j^2 / W_jj (ties to the lower index) among those
  * still allowed, keeping the exact inverse of the rest by rank-one corrections; at most
  * `limit` unknowns with limited[j] != 0 leave (limited may be NULL). removed [count] int64
- * in order. n <= 8192, count <= min(n, 4096). Replaces the host loop of
+ * in order. n <= 4096, count <= n. Replaces the host loop of
  * replacement.py:104-139 (backward_elimination). */
 size_t dsplat_elimination_workspace(int32_t n, int32_t count);
 int dsplat_backward_elimination(const double* inverse, const double* b, int32_t n, int32_t count,
