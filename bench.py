@@ -40,10 +40,11 @@ def parse():
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     parser.add_argument(
-        "--workload", choices=("splat", "gram", "scan", "dsfw"), default="splat",
+        "--workload", choices=("splat", "gram", "scan", "dsfw", "copy"), default="splat",
         help="splat: the fwd+VJP metric (default); gram: the split-precision Gram of the same scene; "
         "scan: the batched certificate scan of 64^3 candidates against 64^2, 96^2 and 128^2 scans; "
-        "dsfw: DSFW steps, plain FP32 form against the mixed-precision batched form",
+        "dsfw: DSFW steps, plain FP32 form against the mixed-precision batched form; "
+        "copy: a plain 1 GiB device copy, the reference size for the HBM byte counters",
     )  # fmt: skip
     parser.add_argument("--adc-samples", type=int, default=128, help="dsfw: ADC samples (bins = samples)")
     parser.add_argument("--truth-draws", type=int, default=0,
@@ -353,6 +354,34 @@ def dsfw_workload(args):
     }
 
 
+def copy_workload(args):
+    """A plain device-to-device copy of 2^28 float32 (1 GiB read, 1 GiB written) per step:
+    the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    dev = torch.device("cuda", 0)
+    src = torch.ones(1 << 28, device=dev)
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream(dev)
+    for _ in range(max(1, args.warmup)):
+        dst.copy_(src)
+    ms, _ = timed(lambda: dst.copy_(src), max(1, args.steps), stream)
+    nbytes = 2 * src.numel() * src.element_size()
+    return {
+        "metric": "device copy GB/s (read + write)",
+        "value": nbytes / (ms * 1e-3) / 1e9,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": max(1, args.steps),
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "copy of 2^28 float32 (1 GiB each way)", "bytes_read": nbytes // 2,
+                   "bytes_written": nbytes // 2},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -370,6 +399,9 @@ def main():
         return
     if args.workload == "dsfw":
         print(json.dumps(dsfw_workload(args)), flush=True)
+        return
+    if args.workload == "copy":
+        print(json.dumps(copy_workload(args)), flush=True)
         return
     import torch
 

@@ -5,6 +5,9 @@
 
 namespace dsplat {
 
+static int g_splat_form = 0;
+int splat_form() { return g_splat_form; }
+
 // Zero fill without hipMemsetAsync, so a caller may capture these calls in a HIP graph.
 __global__ void zero_fill(float* __restrict__ p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -65,7 +68,15 @@ using namespace dsplat;
 
 extern "C" {
 
-const char* dsplat_version(void) { return "dsplat-hip 0.1.0 gfx950"; }
+const char* dsplat_version(void) { return "dsplat-hip 0.2.0 gfx950"; }
+
+int dsplat_set_splat_form(int32_t form) {
+    if (form != 0 && form != 1) return hipErrorInvalidValue;
+    g_splat_form = form;
+    return hipSuccess;
+}
+
+int32_t dsplat_splat_form(void) { return g_splat_form; }
 
 const char* dsplat_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 
@@ -99,8 +110,10 @@ int dsplat_surfel_jvp(const dsplat_fmcw* cfg, const float* positions, int32_t nu
                       reinterpret_cast<float2*>(out), (hipStream_t)stream);
 }
 
-size_t dsplat_surfel_vjp_workspace(int32_t num_apertures, int32_t num_surfels) {
-    if (num_apertures <= 0 || num_surfels <= 0) return 0;
+size_t dsplat_surfel_vjp_workspace(const dsplat_fmcw* cfg, int32_t num_apertures, int32_t num_surfels) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures <= 0 || num_surfels <= 0) return 0;
+    if (use_td(c)) return td_vjp_workspace(c, num_apertures, num_surfels);
     const VjpShape v = vjp_shape(num_apertures, num_surfels);
     return sizeof(float) * 8 * (size_t)num_surfels * v.nblk;
 }
@@ -121,7 +134,7 @@ int dsplat_surfel_vjp(const dsplat_fmcw* cfg, const float* positions, int32_t nu
         return e;
     }
     if (!positions || !centers || !normals || !weights || !upstream || !workspace) return hipErrorInvalidValue;
-    if (workspace_bytes < dsplat_surfel_vjp_workspace(num_apertures, num_surfels)) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_surfel_vjp_workspace(cfg, num_apertures, num_surfels)) return hipErrorInvalidValue;
     return launch_vjp(c, positions, num_apertures, centers, normals, reinterpret_cast<const float2*>(weights),
                       num_surfels, reinterpret_cast<const float2*>(upstream), static_cast<float*>(workspace),
                       grad_centers, grad_normals, grad_weights, st);

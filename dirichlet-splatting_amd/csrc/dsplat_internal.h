@@ -121,6 +121,29 @@ size_t syrk_workspace(int m, int n, int col0, int mode);
 hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, float* out, void* workspace,
                        hipStream_t st);
 
+// Splat form: 0 = sample-domain MFMA kernels where the chirp allows them (S in {128, 256}),
+// 1 = bin-domain cotangent kernels everywhere. Process-wide, set through the C ABI.
+int splat_form();
+bool td_supported(const Chirp& c);
+bool use_td(const Chirp& c);
+struct TdVjpShape {
+    int tiles;      // 128-surfel tiles
+    int ranges;     // aperture ranges (partials per surfel)
+    int per_range;  // apertures per range
+};
+TdVjpShape td_vjp_shape(int A, int S);
+size_t td_vjp_workspace(const Chirp& c, int A, int S);
+hipError_t launch_td_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                            const float2* w, int S, float2* out, hipStream_t st);
+hipError_t launch_td_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                         const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S, float2* out,
+                         hipStream_t st);
+hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                         const float2* w, int S, const float2* up, void* ws, float* gc, float* gn, float* gw,
+                         hipStream_t st);
+hipError_t launch_vjp_reduce(const float* partial, int S, int nblk, float* gc, float* gn, float* gw,
+                             hipStream_t st);
+
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st);
 hipError_t launch_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, const float2* w,

@@ -415,6 +415,7 @@ static hipError_t launch_render_p(const Chirp& c, const float* pos, int A, const
 
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st) {
+    if (use_td(c)) return launch_td_render(c, pos, A, ctr, nrm, w, S, out, st);
     switch (c.P) {
         case 1: return launch_render_p<1>(c, pos, A, ctr, nrm, w, S, out, st);
         case 2: return launch_render_p<2>(c, pos, A, ctr, nrm, w, S, out, st);
@@ -456,6 +457,7 @@ static hipError_t launch_jvp_p(const Chirp& c, const float* pos, int A, const fl
 
 hipError_t launch_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, const float2* w,
                       const float* dctr, const float* dnrm, const float2* dw, int S, float2* out, hipStream_t st) {
+    if (use_td(c)) return launch_td_jvp(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
     switch (c.P) {
         case 1: return launch_jvp_p<1>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
         case 2: return launch_jvp_p<2>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
@@ -490,12 +492,18 @@ static hipError_t launch_vjp_p(const Chirp& c, const float* pos, int A, const fl
                        v.chunk, partial);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(vjp_reduce, dim3((S + 3) / 4), dim3(256), 0, st, partial, S, v.nblk, gc, gn, gw);
+    return launch_vjp_reduce(partial, S, v.nblk, gc, gn, gw, st);
+}
+
+hipError_t launch_vjp_reduce(const float* partial, int S, int nblk, float* gc, float* gn, float* gw,
+                             hipStream_t st) {
+    hipLaunchKernelGGL(vjp_reduce, dim3((S + 3) / 4), dim3(256), 0, st, partial, S, nblk, gc, gn, gw);
     return hipGetLastError();
 }
 
 hipError_t launch_vjp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, const float2* w,
                       int S, const float2* up, float* partial, float* gc, float* gn, float* gw, hipStream_t st) {
+    if (use_td(c)) return launch_td_vjp(c, pos, A, ctr, nrm, w, S, up, partial, gc, gn, gw, st);
     switch (c.P) {
         case 1: return launch_vjp_p<1>(c, pos, A, ctr, nrm, w, S, up, partial, gc, gn, gw, st);
         case 2: return launch_vjp_p<2>(c, pos, A, ctr, nrm, w, S, up, partial, gc, gn, gw, st);

@@ -1,0 +1,562 @@
+// Surfel splat forward and VJP in the sample domain, on MFMA (reference semantics:
+// dsplat/kernels/surfel.slang:87-311, bindings dsplat/kernels/surfel.py:58-137).
+//
+// The Dirichlet kernel is the finite sum D(d) = (1/S) sum_{n<S} e^{-2 pi i n d / N}, so with
+// v = e^{2 pi i k0 / N} per pair and the pair coefficient c = w amp rot:
+//   out[a, k]  = (1/S) sum_n e^{-2 pi i n k / N} X[a, n],     X[a, n] = sum_s c_{a,s} v_{a,s}^n
+//   g0[a, s]   = sum_k conj(D(k - k0)) u[a, k]  = (1/S) sum_n w^n U[a, n]
+//   g1[a, s]   = sum_k conj(D'(k - k0)) u[a, k] = (2 pi i / (N S)) sum_n n w^n U[a, n]
+// with w = conj(v) and U[a, n] = sum_{k<K} e^{2 pi i n k / N} u[a, k]. Splitting n = 16 j + i,
+// v^n = Z_j W_i (W_i = v^i, Z_j = v^{16 j}), the sums over surfels (forward) and over i (VJP)
+// are small complex GEMMs per aperture whose operands are generated per pair: they run on
+// v_mfma_f32_16x16x4f32 / v_mfma_f32_32x32x2f32 at full FP32 precision. The sums are exact
+// (no lobe special case, no reciprocal per bin); the only approximations are the float32
+// roundings of the generated powers (binary-tree products from an exactly rounded e^{2 pi i m/N}
+// table times a small rotation, error growing like log2(n) ulps).
+//
+// Forward (td_render_kernel<J>, S = 16 J samples): a workgroup owns 16 apertures. Surfels
+// stream in chunks of 16; every thread generates one pair (Z_0..Z_{J-1}, c W_0..c W_15)
+// into LDS, then each wave accumulates X of its 4 apertures as [2J x 16] real tiles:
+//   rows (j, part), cols i, K = (surfel, part): Re X += Zr (cW)r - Zi (cW)i, Im X += Zi (cW)r + Zr (cW)i.
+// The DFT to bins (a [16 x S] x [S x K] complex GEMM with twiddles from the LDS table) runs
+// in the same workgroup, so X never leaves the chip.
+//
+// VJP (td_vjp_kernel<J>): a workgroup owns 128 surfels (32 per wave, lane % 32 = surfel) and a
+// range of apertures; U (the DFT of the cotangent, from the dft_rows_kernel pre-pass) is
+// staged per aperture in the MFMA A-operand layout. The two lane halves generate two pairs
+// (apertures a and a + 1, same surfel) and swap half of their W operands, so every pair is
+// generated once. Per aperture M0[s, j] = sum_i W_i U[16 j + i] and M1 = the same with
+// n U[n] form one 32-row tile per 8 j (v_mfma_f32_32x32x2f32); P0 = sum_j Z_j M0, P1 likewise.
+// The 8 chained gradients accumulate in registers over the aperture range (no cross-lane
+// reduction per pair) and are written as one partial per (surfel, range), summed in float64
+// by vjp_reduce.
+#include "dsplat_device.h"
+#include "dsplat_internal.h"
+
+namespace dsplat {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kTdApertures = 16;  // forward: apertures per workgroup
+constexpr int kTdChunk = 16;      // forward: surfels per LDS chunk
+constexpr int kTdVjpSurfels = 128;
+constexpr int kTdVjpGroup = 8;    // VJP: apertures staged per LDS fill
+
+__device__ __forceinline__ float2 csq(float2 a) { return make_float2(a.x * a.x - a.y * a.y, 2.0f * a.x * a.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// e^{2 pi i m / N} for m in [0, N), exactly rounded from float64.
+__device__ __forceinline__ void fill_twiddles(float2* E, int N, int tid, int nthreads) {
+    for (int m = tid; m < N; m += nthreads) {
+        double s, c;
+        sincospi(2.0 * (double)m / (double)N, &s, &c);
+        E[m] = make_float2((float)c, (float)s);
+    }
+}
+
+// e^{2 pi i (q lobe + q delta) / N} = E[q lobe mod N] * e^{2 pi i q delta / N}, |q delta| <= q / 2.
+// The small rotation needs |2 pi q delta / N| <= pi / 4: q <= N / 4.
+__device__ __forceinline__ float2 rotation(const float2* E, int N, int lobe, float delta, int q) {
+    float s, c;
+    sincos_small(2.0f * kPi * (float)q * delta / (float)N, &s, &c);
+    return cmul(E[(q * lobe) & (N - 1)], make_float2(c, s));
+}
+
+// out[r][q] = scale sum_{p < Lin} e^{sign 2 pi i p q / N} X[r][p] for the 16 rows of X (LDS,
+// row stride ldx float2) and q < Lout (multiple of 8), one 16 x 8-bin tile per wave at a time
+// on v_mfma_f32_16x16x4f32. Lane layouts (gfx950): A[i][k] -> lane i + 16 k, B[k][j] -> lane
+// j + 16 k, C[4 (lane / 16) + r][lane % 16] in register r. K index k = 2 dp + pin: sample
+// p = 2 t + dp, input part pin; column x = 2 (bin offset) + output part.
+__device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const float2* E, int N, float sign,
+                                      float scale, int Lout, float2* out, int ldo, int rows, int wave,
+                                      int lane) {
+    const int kk = lane >> 4, x = lane & 15;
+    const int pin = kk & 1, dp = kk >> 1, pout = x & 1;
+    const float* Xf = reinterpret_cast<const float*>(X);
+    for (int ct = wave; ct * 8 < Lout; ct += 4) {
+        const int q = ct * 8 + (x >> 1);
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        int m = (dp * q) & (N - 1);
+        const int step = (2 * q) & (N - 1);
+        const float* xrow = Xf + 2 * (x * ldx + dp) + pin;
+        for (int t = 0; t < Lin / 2; ++t) {
+            const float a = xrow[4 * t];
+            const float2 e = E[m];
+            const float fr = e.x * scale, fi = sign * e.y * scale;
+            const float b = pin ? (pout ? fr : -fi) : (pout ? fi : fr);
+            d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
+            m = (m + step) & (N - 1);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 4 * kk + r;
+            if (row < rows) reinterpret_cast<float*>(out + (size_t)row * ldo + q)[pout] = d[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Forward
+// ---------------------------------------------------------------------------------------
+template <int J, bool JVP>
+struct TdRec {
+    // floats per pair record: Z re[J], Z im[J], (e W) re[16], (e W) im[16] per coefficient set
+    static constexpr int sets = JVP ? 2 : 1;
+    static constexpr int size = 2 * J + 32 * sets;
+};
+
+// Forward (JVP = false): X = sum_s c v^n with c = w amp rot.
+// JVP (JVP = true, surfel.slang:146-221): dy = sum_s e1 D + e2 D', so X = X1 - (2 pi i n / N) X2
+// with X1 = sum_s e1 v^n, X2 = sum_s e2 v^n, e1 = rot ((damp + i amp dphase) w + amp dw),
+// e2 = -amp k0pm dr rot w; both sets share the Z operand.
+template <int J, bool JVP>
+__global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const float* __restrict__ pos, int A,
+                                                        const float* __restrict__ ctr,
+                                                        const float* __restrict__ nrm,
+                                                        const float2* __restrict__ w,
+                                                        const float* __restrict__ dctr,
+                                                        const float* __restrict__ dnrm,
+                                                        const float2* __restrict__ dw, int S,
+                                                        float2* __restrict__ out) {
+    constexpr int AT = kTdApertures, SC = kTdChunk, REC = TdRec<J, JVP>::size, SETS = TdRec<J, JVP>::sets;
+    constexpr int T = (2 * J + 15) / 16;
+    extern __shared__ float smem[];
+    float2* E = reinterpret_cast<float2*>(smem);  // [N]
+    float* rec = smem + 2 * c.N;                  // [AT * SC][REC]; later X [AT][S + 1] float2
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int a0 = blockIdx.x * AT;
+    fill_twiddles(E, c.N, tid, 256);
+
+    // producer: one pair per thread per chunk
+    const int pal = tid >> 4, psp = tid & 15;
+    const int pa = min(a0 + pal, A - 1);
+    const bool p_ok = a0 + pal < A;
+    const float px = pos[3 * pa], py = pos[3 * pa + 1], pz = pos[3 * pa + 2];
+    float* mine = rec + tid * REC;
+
+    // consumer lane roles (16x16x4): kk = surfel-in-step (kk >> 1) and part (kk & 1)
+    const int kk = lane >> 4, x = lane & 15;
+    const int part = kk & 1, sl = kk >> 1;
+    const int jrow = x >> 1, prow = x & 1;
+    const unsigned flip = (part && !prow) ? 0x80000000u : 0u;
+    const int a_off = (prow ^ part) * J + jrow;  // Z component for row (jrow, prow)
+    const int b_off = 2 * J + part * 16 + x;     // (e W) component for column x
+
+    f32x4 acc[SETS][4][T];
+#pragma unroll
+    for (int e = 0; e < SETS; ++e)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int tt = 0; tt < T; ++tt) acc[e][b][tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int s0 = 0; s0 < S; s0 += SC) {
+        __syncthreads();  // previous chunk consumed (and the twiddle table written)
+        {
+            const int s = s0 + psp;
+            float2 coef[SETS];
+#pragma unroll
+            for (int e = 0; e < SETS; ++e) coef[e] = make_float2(0.f, 0.f);
+            float2 v = make_float2(1.f, 0.f), z1 = make_float2(1.f, 0.f);
+            if (p_ok && s < S) {
+                const float nx = nrm[3 * s], ny = nrm[3 * s + 1], nz = nrm[3 * s + 2];
+                const Pair q = pair_geometry(c, px, py, pz, ctr[3 * s], ctr[3 * s + 1], ctr[3 * s + 2], nx, ny, nz);
+                if (q.amp != 0.0f) {
+                    if constexpr (JVP) {
+                        const float dcx = dctr[3 * s], dcy = dctr[3 * s + 1], dcz = dctr[3 * s + 2];
+                        const float dr = -(q.ux * dcx + q.uy * dcy + q.uz * dcz);
+                        const float dcos = ((q.cosine * q.ux - nx) * dcx + (q.cosine * q.uy - ny) * dcy +
+                                            (q.cosine * q.uz - nz) * dcz) * q.inv_r +
+                                           q.ux * dnrm[3 * s] + q.uy * dnrm[3 * s + 1] + q.uz * dnrm[3 * s + 2];
+                        const float damp = -q.amp * dr * q.inv_r + q.amp_cos * dcos;
+                        const float2 ws = w[s];
+                        coef[0] = cmul(q.rot, cmul(make_float2(damp, q.amp * q.rate * dr), ws) + q.amp * dw[s]);
+                        coef[SETS - 1] = (-q.amp * c.k0pm_f * dr) * cmul(q.rot, ws);
+                    } else {
+                        coef[0] = q.amp * cmul(w[s], q.rot);
+                    }
+                    v = rotation(E, c.N, q.lobe, q.delta, 1);
+                    z1 = rotation(E, c.N, q.lobe, q.delta, 16);
+                }
+            }
+            // Z_j = z1^j by a binary product tree
+            float2 Z[J];
+            Z[0] = make_float2(1.f, 0.f);
+            Z[1] = z1;
+#pragma unroll
+            for (int j = 2; j < J; ++j) Z[j] = (j & 1) ? cmul(Z[j - 1], z1) : csq(Z[j >> 1]);
+#pragma unroll
+            for (int j = 0; j < J; j += 4) {
+                *reinterpret_cast<float4*>(mine + j) = make_float4(Z[j].x, Z[j + 1].x, Z[j + 2].x, Z[j + 3].x);
+                *reinterpret_cast<float4*>(mine + J + j) = make_float4(Z[j].y, Z[j + 1].y, Z[j + 2].y, Z[j + 3].y);
+            }
+            // e W_i = e v^i, two interleaved chains by v^2
+            const float2 v2 = csq(v);
+#pragma unroll
+            for (int e = 0; e < SETS; ++e) {
+                float2 W[16];
+                W[0] = coef[e];
+                W[1] = cmul(coef[e], v);
+#pragma unroll
+                for (int i = 2; i < 16; ++i) W[i] = cmul(W[i - 2], v2);
+                float* dst = mine + 2 * J + 32 * e;
+#pragma unroll
+                for (int i = 0; i < 16; i += 4) {
+                    *reinterpret_cast<float4*>(dst + i) = make_float4(W[i].x, W[i + 1].x, W[i + 2].x, W[i + 3].x);
+                    *reinterpret_cast<float4*>(dst + 16 + i) = make_float4(W[i].y, W[i + 1].y, W[i + 2].y, W[i + 3].y);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float* base = rec + ((4 * wave + b) * SC + sl) * REC;
+#pragma unroll
+            for (int t = 0; t < SC / 2; ++t) {
+                const float* r = base + 2 * t * REC;
+#pragma unroll
+                for (int tt = 0; tt < T; ++tt) {
+                    const float av = __uint_as_float(__float_as_uint(r[a_off + 8 * tt]) ^ flip);
+#pragma unroll
+                    for (int e = 0; e < SETS; ++e)
+                        acc[e][b][tt] =
+                            __builtin_amdgcn_mfma_f32_16x16x4f32(av, r[b_off + 32 * e], acc[e][b][tt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // X tiles -> LDS [AT][S + 1] float2: register r of lane (g, i) is row 4 g + r = (j, part),
+    // so registers (0, 1) and (2, 3) are the (re, im) of samples 16 j + i.
+    const int ldx = c.S + 1;
+    float2* X = reinterpret_cast<float2*>(rec);
+    {
+        const int g = lane >> 4, i = lane & 15;
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int tt = 0; tt < T; ++tt)
+#pragma unroll
+                for (int r = 0; r < 4; r += 2) {
+                    const int j = 8 * tt + 2 * g + (r >> 1);
+                    if (j >= J) continue;
+                    const int n = 16 * j + i;
+                    float2 val = make_float2(acc[0][b][tt][r], acc[0][b][tt][r + 1]);
+                    if constexpr (JVP) {
+                        // - (2 pi i n / N) X2
+                        const float k = 2.0f * c.pi_N * (float)n;
+                        val.x += k * acc[SETS - 1][b][tt][r + 1];
+                        val.y -= k * acc[SETS - 1][b][tt][r];
+                    }
+                    X[(4 * wave + b) * ldx + n] = val;
+                }
+    }
+    __syncthreads();
+    dft16(X, ldx, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(AT, A - a0), wave, lane);
+}
+
+// ---------------------------------------------------------------------------------------
+// DFT of rows: out[a][q] = scale sum_{p<Lin} e^{sign 2 pi i p q / N} in[a][p], 16 rows per
+// workgroup (the VJP pre-pass U = DFT of the cotangent, sign +1).
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void dft_rows_kernel(const float2* __restrict__ in, int rows, int Lin, int N,
+                                                       float sign, float scale, int Lout,
+                                                       float2* __restrict__ out) {
+    extern __shared__ float smem[];
+    float2* E = reinterpret_cast<float2*>(smem);
+    float2* X = E + N;
+    const int tid = threadIdx.x;
+    const int r0 = blockIdx.x * 16;
+    const int ldx = Lin + 1;
+    fill_twiddles(E, N, tid, 256);
+    for (int i = tid; i < 16 * Lin; i += 256) {
+        const int r = i / Lin, p = i - r * Lin;
+        X[r * ldx + p] = r0 + r < rows ? in[(size_t)(r0 + r) * Lin + p] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    dft16(X, ldx, Lin, E, N, sign, scale, Lout, out + (size_t)r0 * Lout, Lout, min(16, rows - r0), tid >> 6,
+          tid & 63);
+}
+
+// ---------------------------------------------------------------------------------------
+// VJP
+// ---------------------------------------------------------------------------------------
+// Tile-local row of the 32 x 32 accumulator held in register rr of lane half h:
+// row = 8 (rr / 4) + 4 h + rr % 4 (gfx950 layout of v_mfma_f32_32x32x2f32). Rows hold
+// (mat, j, part): for J = 8 one tile, mat = q / 2, j = 4 (q % 2) + 2 h + (r / 2); for J = 16
+// tile T = mat, j = 4 q + 2 h + (r / 2); part = r % 2 (q = rr / 4, r = rr % 4).
+template <int J>
+__device__ __forceinline__ void td_row(int tile, int q, int h, int r, int* mat, int* j) {
+    if (J == 8) {
+        *mat = q >> 1;
+        *j = 4 * (q & 1) + 2 * h + (r >> 1);
+    } else {
+        *mat = tile;
+        *j = 4 * q + 2 * h + (r >> 1);
+    }
+}
+
+template <int J>
+__global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float* __restrict__ pos, int A,
+                                                     const float* __restrict__ ctr,
+                                                     const float* __restrict__ nrm,
+                                                     const float2* __restrict__ w, int S,
+                                                     const float2* __restrict__ U, int per_range,
+                                                     float* __restrict__ partial) {
+    constexpr int T = J / 8;  // 32-row tiles per aperture
+    constexpr int G = kTdVjpGroup;
+    extern __shared__ float smem[];
+    float2* E = reinterpret_cast<float2*>(smem);  // [N]
+    float4* Aop = reinterpret_cast<float4*>(E + c.N);  // [G][T][4 t-groups][64 lanes] float4
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
+    const int s = blockIdx.x * kTdVjpSurfels + 32 * wave + m;
+    const bool s_ok = s < S;
+    const int sc = min(s, S - 1);
+    const int range = blockIdx.y;
+    const int a_begin = range * per_range, a_end = min(A, a_begin + per_range);
+    fill_twiddles(E, c.N, tid, 256);
+
+    const float cx = ctr[3 * sc], cy = ctr[3 * sc + 1], cz = ctr[3 * sc + 2];
+    const float nx = nrm[3 * sc], ny = nrm[3 * sc + 1], nz = nrm[3 * sc + 2];
+    const float2 ws = w[sc];
+    float gacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+
+    for (int g0 = a_begin; g0 < a_end; g0 += G) {
+        __syncthreads();
+        // Stage A operands: value(t, lane L) for L = row + 32 k, row -> (mat, j, part).
+        for (int e = tid; e < G * T * 4 * 64; e += 256) {
+            const int L = e & 63, tg = (e >> 6) & 3, tile = (e >> 8) % T, ga = e / (256 * T);
+            const int a = g0 + ga;
+            const int row = L & 31, k = L >> 5;
+            const int q = row >> 3, hh = (row >> 2) & 1, r = row & 3;
+            int mat, j;
+            td_row<J>(tile, q, hh, r, &mat, &j);
+            const int part = r & 1;
+            float vals[4] = {0.f, 0.f, 0.f, 0.f};
+            if (a < a_end) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int t = 4 * tg + u;
+                    const int n = 16 * j + t;
+                    float2 z = U[(size_t)a * c.S + n];
+                    if (mat) z = (float)n * z;
+                    // k = 0 (B = Re W): (re) Re U, (im) Im U;  k = 1 (B = Im W): (re) -Im U, (im) Re U
+                    vals[u] = k == 0 ? (part ? z.y : z.x) : (part ? z.x : -z.y);
+                }
+            }
+            Aop[e] = make_float4(vals[0], vals[1], vals[2], vals[3]);
+        }
+        __syncthreads();
+        for (int ga = 0; ga < G; ga += 2) {
+            const int a = g0 + ga + h;  // this lane's pair: (a, s)
+            const bool a_ok = a < a_end;
+            const int ac = min(a, A - 1);
+            const Pair q = pair_geometry(c, pos[3 * ac], pos[3 * ac + 1], pos[3 * ac + 2], cx, cy, cz, nx, ny, nz);
+            const float amp = (a_ok && s_ok) ? q.amp : 0.0f;
+            // w = conj(v): W_t = w^t, Z_j = w^{16 j}
+            float2 W[16], Z[J];
+            {
+                const float2 w1 = cconj(rotation(E, c.N, q.lobe, q.delta, 1));
+                const float2 z1 = cconj(rotation(E, c.N, q.lobe, q.delta, 16));
+                W[0] = make_float2(1.f, 0.f);
+                W[1] = w1;
+#pragma unroll
+                for (int t = 2; t < 16; ++t) W[t] = (t & 1) ? cmul(W[t - 1], w1) : csq(W[t >> 1]);
+                Z[0] = make_float2(1.f, 0.f);
+                Z[1] = z1;
+#pragma unroll
+                for (int j = 2; j < J; ++j) Z[j] = (j & 1) ? cmul(Z[j - 1], z1) : csq(Z[j >> 1]);
+            }
+            // B operands: half 0 keeps Re W of its pair (aperture a) and receives Re W of
+            // aperture a + 1; half 1 keeps Im W (aperture a + 1) and receives Im W of aperture a.
+            float B0[16], B1[16];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const float send = h ? W[t].x : W[t].y;
+                const float recv = __shfl_xor(send, 32, 64);
+                const float own = h ? W[t].y : W[t].x;
+                B0[t] = h ? recv : own;  // aperture g0 + ga
+                B1[t] = h ? own : recv;  // aperture g0 + ga + 1
+            }
+            f32x16 acc0[T], acc1[T];
+#pragma unroll
+            for (int tile = 0; tile < T; ++tile)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    acc0[tile][r] = 0.f;
+                    acc1[tile][r] = 0.f;
+                }
+            const float4* A0 = Aop + ((ga * T) * 4) * 64 + lane;
+            const float4* A1 = Aop + (((ga + 1) * T) * 4) * 64 + lane;
+#pragma unroll
+            for (int tile = 0; tile < T; ++tile)
+#pragma unroll
+                for (int tg = 0; tg < 4; ++tg) {
+                    const float4 x0 = A0[(tile * 4 + tg) * 64];
+                    const float4 x1 = A1[(tile * 4 + tg) * 64];
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.x, B0[4 * tg], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.x, B1[4 * tg], acc1[tile], 0, 0, 0);
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.y, B0[4 * tg + 1], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.y, B1[4 * tg + 1], acc1[tile], 0, 0, 0);
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.z, B0[4 * tg + 2], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.z, B1[4 * tg + 2], acc1[tile], 0, 0, 0);
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.w, B0[4 * tg + 3], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.w, B1[4 * tg + 3], acc1[tile], 0, 0, 0);
+                }
+            // Contraction P_mat = sum_j Z_j M_mat[j]. Lane half h holds j of its row set for
+            // both apertures; Z_j of aperture (g0 + ga + h) is local, the other one comes from
+            // the partner lane.
+            float2 P[2][2];  // [aperture 0/1][mat]
+#pragma unroll
+            for (int x = 0; x < 2; ++x) P[x][0] = P[x][1] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int tile = 0; tile < T; ++tile)
+#pragma unroll
+                for (int rr = 0; rr < 16; rr += 2) {
+                    int mat, j0, j1;
+                    td_row<J>(tile, rr >> 2, 0, rr & 3, &mat, &j0);
+                    td_row<J>(tile, rr >> 2, 1, rr & 3, &mat, &j1);
+                    // own Z for the own aperture; the partner needs Z of its own row set
+                    const float2 zown = h ? Z[j1] : Z[j0];
+                    const float2 zsend = h ? Z[j0] : Z[j1];
+                    const float2 zrecv = make_float2(__shfl_xor(zsend.x, 32, 64), __shfl_xor(zsend.y, 32, 64));
+                    const float2 za0 = h ? zrecv : zown;  // Z_j of aperture g0 + ga
+                    const float2 za1 = h ? zown : zrecv;  // Z_j of aperture g0 + ga + 1
+                    P[0][mat] += cmul(za0, make_float2(acc0[tile][rr], acc0[tile][rr + 1]));
+                    P[1][mat] += cmul(za1, make_float2(acc1[tile][rr], acc1[tile][rr + 1]));
+                }
+            // Complete the own aperture's sums with the partner's half.
+            float2 P0, P1;
+            {
+                const float2 s0 = h ? P[0][0] : P[1][0];
+                const float2 s1 = h ? P[0][1] : P[1][1];
+                const float2 r0 = make_float2(__shfl_xor(s0.x, 32, 64), __shfl_xor(s0.y, 32, 64));
+                const float2 r1 = make_float2(__shfl_xor(s1.x, 32, 64), __shfl_xor(s1.y, 32, 64));
+                P0 = (h ? P[1][0] : P[0][0]) + r0;
+                P1 = (h ? P[1][1] : P[0][1]) + r1;
+            }
+            // g0 = P0 / S, g1 = (2 pi i / (N S)) P1; chain as the cot-form VJP.
+            const float2 g0 = c.inv_S * P0;
+            const float2 g1 = (2.0f * c.pi_N * c.inv_S) * make_float2(-P1.y, P1.x);
+            const float2 z0 = cmulc(q.rot, g0), zz1 = cmulc(q.rot, g1);
+            const float2 cw = cmulc(ws, z0);
+            const float along = cmulc(ws, zz1).x * c.k0pm_f;
+            const float gr = -cw.x * amp * q.inv_r + amp * (cw.y * q.rate - along);
+            const float gcos = cw.x * ((a_ok && s_ok) ? q.amp_cos : 0.0f);
+            gacc[0] += -gr * q.ux + gcos * (q.cosine * q.ux - nx) * q.inv_r;
+            gacc[1] += -gr * q.uy + gcos * (q.cosine * q.uy - ny) * q.inv_r;
+            gacc[2] += -gr * q.uz + gcos * (q.cosine * q.uz - nz) * q.inv_r;
+            gacc[3] += gcos * q.ux;
+            gacc[4] += gcos * q.uy;
+            gacc[5] += gcos * q.uz;
+            gacc[6] += amp * z0.x;
+            gacc[7] += amp * z0.y;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) gacc[j] += __shfl_xor(gacc[j], 32, 64);
+    if (h == 0 && s_ok) {
+        float4* dst = reinterpret_cast<float4*>(partial + ((size_t)s * gridDim.y + range) * 8);
+        dst[0] = make_float4(gacc[0], gacc[1], gacc[2], gacc[3]);
+        dst[1] = make_float4(gacc[4], gacc[5], gacc[6], gacc[7]);
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------------
+bool td_supported(const Chirp& c) {
+    // S = 16 J samples with J in {8, 16}; the Z_1 rotation needs 16 <= N / 4.
+    return (c.S == 128 || c.S == 256) && c.N >= 64 && (c.N & (c.N - 1)) == 0 && c.K % 8 == 0;
+}
+
+bool use_td(const Chirp& c) { return splat_form() == 0 && td_supported(c); }
+
+template <int J, bool JVP>
+static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                     const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S,
+                                     float2* out, hipStream_t st) {
+    const size_t rec = sizeof(float) * (size_t)kTdApertures * kTdChunk * TdRec<J, JVP>::size;
+    const size_t xbuf = sizeof(float2) * (size_t)kTdApertures * (c.S + 1);
+    const size_t smem = sizeof(float2) * c.N + (rec > xbuf ? rec : xbuf);
+    hipError_t e = allow_lds((const void*)td_render_kernel<J, JVP>, smem);
+    if (e != hipSuccess) return e;
+    const dim3 grid((A + kTdApertures - 1) / kTdApertures);
+    hipLaunchKernelGGL((td_render_kernel<J, JVP>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr, dnrm, dw,
+                       S, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_td_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                            const float2* w, int S, float2* out, hipStream_t st) {
+    if (c.S == 128) return launch_td_render_j<8, false>(c, pos, A, ctr, nrm, w, nullptr, nullptr, nullptr, S, out, st);
+    if (c.S == 256) return launch_td_render_j<16, false>(c, pos, A, ctr, nrm, w, nullptr, nullptr, nullptr, S, out, st);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_td_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                         const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S, float2* out,
+                         hipStream_t st) {
+    if (c.S == 128) return launch_td_render_j<8, true>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
+    if (c.S == 256) return launch_td_render_j<16, true>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
+    return hipErrorInvalidValue;
+}
+
+TdVjpShape td_vjp_shape(int A, int S) {
+    TdVjpShape v;
+    v.tiles = (S + kTdVjpSurfels - 1) / kTdVjpSurfels;
+    // enough workgroups to fill 256 CUs several times; ranges are multiples of the staging group
+    const int max_ranges = (A + kTdVjpGroup - 1) / kTdVjpGroup;
+    int ranges = (2048 + v.tiles - 1) / v.tiles;
+    ranges = ranges < 1 ? 1 : (ranges > max_ranges ? max_ranges : ranges);
+    int per = (A + ranges - 1) / ranges;
+    v.per_range = (per + kTdVjpGroup - 1) / kTdVjpGroup * kTdVjpGroup;
+    v.ranges = (A + v.per_range - 1) / v.per_range;
+    return v;
+}
+
+size_t td_vjp_workspace(const Chirp& c, int A, int S) {
+    const TdVjpShape v = td_vjp_shape(A, S);
+    return sizeof(float) * 8 * (size_t)S * v.ranges + sizeof(float2) * (size_t)A * c.S + 256;
+}
+
+template <int J>
+static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                  const float2* w, int S, const float2* up, void* ws, float* gc, float* gn,
+                                  float* gw, hipStream_t st) {
+    const TdVjpShape v = td_vjp_shape(A, S);
+    float* partial = static_cast<float*>(ws);
+    const size_t part_bytes = (sizeof(float) * 8 * (size_t)S * v.ranges + 255) / 256 * 256;
+    float2* U = reinterpret_cast<float2*>(static_cast<char*>(ws) + part_bytes);
+    // U = DFT of the cotangent rows: K bins -> S samples, sign +1
+    {
+        const size_t smem = sizeof(float2) * (c.N + 16 * (size_t)(c.K + 1));
+        hipError_t e = allow_lds((const void*)dft_rows_kernel, smem);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(dft_rows_kernel, dim3((A + 15) / 16), dim3(256), smem, st, up, A, c.K, c.N, 1.0f, 1.0f,
+                           c.S, U);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    const size_t smem = sizeof(float2) * c.N + sizeof(float4) * (size_t)kTdVjpGroup * (J / 8) * 4 * 64;
+    hipError_t e = allow_lds((const void*)td_vjp_kernel<J>, smem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(td_vjp_kernel<J>, dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A, ctr, nrm, w, S, U,
+                       v.per_range, partial);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_vjp_reduce(partial, S, v.ranges, gc, gn, gw, st);
+}
+
+hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                         const float2* w, int S, const float2* up, void* ws, float* gc, float* gn, float* gw,
+                         hipStream_t st) {
+    if (c.S == 128) return launch_td_vjp_j<8>(c, pos, A, ctr, nrm, w, S, up, ws, gc, gn, gw, st);
+    if (c.S == 256) return launch_td_vjp_j<16>(c, pos, A, ctr, nrm, w, S, up, ws, gc, gn, gw, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace dsplat

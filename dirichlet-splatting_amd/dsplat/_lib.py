@@ -67,7 +67,9 @@ SIGNATURES = {
     "dsplat_surfel_jvp": (
         c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr]
     ),
-    "dsplat_surfel_vjp_workspace": (c_size, [c_int, c_int]),
+    "dsplat_surfel_vjp_workspace": (c_size, [c_ptr, c_int, c_int]),
+    "dsplat_set_splat_form": (c_int, [c_int]),
+    "dsplat_splat_form": (c_int, []),
     "dsplat_surfel_vjp": (
         c_int,
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],

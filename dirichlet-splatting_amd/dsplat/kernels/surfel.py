@@ -56,9 +56,9 @@ def vjp(config, positions, centers, normals, weights, upstream, fresnel_n):
     gn = centers.new_empty((count, 3))
     gw = centers.new_empty((count, 2))
     lib = _lib.library()
-    size = lib.dsplat_surfel_vjp_workspace(len(positions), count)
-    workspace = centers.new_empty(max(1, size // 4))
     cfg = _lib.fmcw(config, fresnel_n)
+    size = lib.dsplat_surfel_vjp_workspace(_lib.ctypes.byref(cfg), len(positions), count)
+    workspace = centers.new_empty(max(1, size // 4))
     _lib.call(
         "dsplat_surfel_vjp",
         _lib.ctypes.byref(cfg),

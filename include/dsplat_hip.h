@@ -43,6 +43,13 @@ typedef struct dsplat_fmcw {
 const char* dsplat_version(void);
 const char* dsplat_error_string(int code);
 
+/* Splat form used by render / VJP (process-wide; no counterpart in the reference, whose
+ * Slang kernels have one form): 0 (default) evaluates the Dirichlet kernel as its sample sum
+ * on MFMA where the chirp has 128 or 256 ADC samples, 1 uses the bin-domain cotangent
+ * kernels for every chirp. Both match the float64 closed form to ~1e-7; 1 exists to compare. */
+int dsplat_set_splat_form(int32_t form);
+int32_t dsplat_splat_form(void);
+
 /* Range profiles out[A, K] = sum_s w_s column_s(p_a).
  * positions [A,3], centers [S,3], normals [S,3], weights [S] complex, out [A,K] complex.
  * Replaces kernels/surfel.py:58-108 (_profiles: pair_geometry + render_bins). */
@@ -57,8 +64,9 @@ int dsplat_surfel_jvp(const dsplat_fmcw* cfg, const float* positions, int32_t nu
                       const float* delta_centers, const float* delta_normals, const float* delta_weights,
                       int32_t num_surfels, float* out, void* stream);
 
-/* Workspace bytes dsplat_surfel_vjp needs for A apertures and S surfels. */
-size_t dsplat_surfel_vjp_workspace(int32_t num_apertures, int32_t num_surfels);
+/* Workspace bytes dsplat_surfel_vjp needs for A apertures and S surfels of this chirp
+ * (0 for an unsupported configuration). */
+size_t dsplat_surfel_vjp_workspace(const dsplat_fmcw* cfg, int32_t num_apertures, int32_t num_surfels);
 
 /* Gradients of Re(conj(upstream) . render) for centers [S,3], normals [S,3] and the
  * complex weights [S] (PyTorch convention: grad_weights = A^H upstream).

@@ -49,12 +49,25 @@ def test_rejected_configurations_return_invalid_value_without_launching():
     assert lib.dsplat_surfel_vjp(ctypes.byref(bad), *args) == 1
 
 
-def test_vjp_workspace_is_one_partial_per_surfel_and_aperture_block():
-    from dsplat import _lib
+def test_vjp_workspace_per_splat_form():
+    from dsplat import FMCWConfig, _lib
 
     lib = _lib.library()
-    assert lib.dsplat_surfel_vjp_workspace(0, 10) == 0
-    assert lib.dsplat_surfel_vjp_workspace(16384, 2048) == 2048 * 512 * 8 * 4
+    cfg = ctypes.byref(_lib.fmcw(FMCWConfig()))
+    assert lib.dsplat_splat_form() == 0
+    assert lib.dsplat_surfel_vjp_workspace(cfg, 0, 10) == 0
+    # sample-domain form: partials of 8 floats per (surfel, 128-aperture range) plus the
+    # cotangent's sample-domain rows U [A, 128] complex, 256 bytes of alignment slack
+    assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == 2048 * 128 * 8 * 4 + 16384 * 128 * 8 + 256
+    try:
+        assert lib.dsplat_set_splat_form(1) == 0
+        # bin-domain form: one partial per surfel and 32-aperture block
+        assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == 2048 * 512 * 8 * 4
+    finally:
+        lib.dsplat_set_splat_form(0)
+    assert lib.dsplat_set_splat_form(2) == 1
+    odd = _lib.fmcw(FMCWConfig(adc_samples=96))  # no sample-domain kernel: bin-domain sizes
+    assert lib.dsplat_surfel_vjp_workspace(ctypes.byref(odd), 1024, 200) == 200 * 32 * 8 * 4
 
 
 def test_struct_layout_matches_the_header():
