@@ -45,6 +45,9 @@ bool make_chirp(const dsplat_fmcw* cfg, Chirp* out) {
     c.slope = cfg->slope_hz_s;
     c.t0 = cfg->t_start_s;
     c.c0 = cfg->c0_m_s;
+    c.two_over_c0 = 2.0 / cfg->c0_m_s;
+    c.inv_N = 1.0 / N;
+    c.two_pi_N = (float)(2.0 * M_PI / N);
     c.lam = (float)cfg->lambda_m;
     c.fresnel_n = (float)cfg->fresnel_n;
     c.inv_S = (float)(1.0 / S);

@@ -39,9 +39,12 @@ struct Chirp {
     int K, N, S, P;
     int Kp;              // K rounded up to a multiple of 16
     double k0pm, fc, slope, t0, c0;
+    double two_over_c0;  // 2 / c0: round-trip delay per metre
+    double inv_N;        // 1 / N
     float lam, fresnel_n;
     float inv_S;         // 1 / S
     float pi_P, pi_N;    // pi / P, pi / N
+    float two_pi_N;      // 2 pi / N
     float h;             // pi (S - 1) / N
     float k0pm_f;        // 1 / spacing as float
     float rate0, rate1;  // carrier rate = rate0 - rate1 * r (rad / m)
@@ -109,54 +112,83 @@ struct Pair {
     float delta;  // k0 - lobe in [-1/2, 1/2], from float64
 };
 
-__device__ __forceinline__ Pair pair_geometry(const Chirp& c, float px, float py, float pz,
-                                              float cx, float cy, float cz,
-                                              float nx, float ny, float nz) {
+// sin(pi x), cos(pi x) for |x| <= 1: quarter-turn reduction y = x - q / 2, |y| <= 1/4,
+// then the Taylor pair of sincos_small and an exact rotation by q quarter turns.
+__device__ __forceinline__ void sincospi_unit(float x, float* s, float* c) {
+    const float qf = rintf(2.0f * x);
+    const float y = fmaf(-0.5f, qf, x);
+    float sy, cy;
+    sincos_small(kPi * y, &sy, &cy);
+    const int q = (int)qf;
+    const bool swap = q & 1;
+    float ss = swap ? cy : sy, cc = swap ? sy : cy;
+    // quarter turns q = 0, 1, 2, 3 (mod 4): (c, s) -> (c, s), (-s, c), (-c, -s), (s, -c)
+    if ((q + 1) & 2) cc = -cc;
+    if (q & 2) ss = -ss;
+    *s = ss;
+    *c = cc;
+}
+
+// Branch-free pair geometry: FRESNEL selects the Fresnel factor at compile time, the
+// cos <= 0 cut is a select, so callers can interleave it with matrix instructions.
+template <bool FRESNEL>
+__device__ __forceinline__ Pair pair_geometry_t(const Chirp& c, float px, float py, float pz, float cx, float cy,
+                                                float cz, float nx, float ny, float nz) {
     Pair q;
     const double dx = (double)px - (double)cx;
     const double dy = (double)py - (double)cy;
     const double dz = (double)pz - (double)cz;
-    const double r = sqrt(fma(dx, dx, fma(dy, dy, dz * dz))) + 1e-12;
+    // float64 range from a float32 square root and one float64 Newton correction
+    const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    const float r0 = sqrtf((float)r2);
+    const double r = fma(fma(-(double)r0, (double)r0, r2), (double)(0.5f * rcp(r0)), (double)r0) + 1e-12;
     q.r = (float)r;
-    const double inv = 1.0 / r;
-    q.ux = (float)(dx * inv);
-    q.uy = (float)(dy * inv);
-    q.uz = (float)(dz * inv);
-    q.inv_r = (float)inv;
+    float inv = rcp(q.r);
+    inv = fmaf(inv, fmaf(-q.r, inv, 1.0f), inv);
+    q.ux = (float)dx * inv;
+    q.uy = (float)dy * inv;
+    q.uz = (float)dz * inv;
+    q.inv_r = inv;
     q.cosine = q.ux * nx + q.uy * ny + q.uz * nz;
-    // Scattering amplitude and its cos-derivative (scattering.slang:16-38).
-    q.amp = 0.0f;
-    q.amp_cos = 0.0f;
-    if (q.cosine > 0.0f) {
-        float f = 1.0f, df = 0.0f;
+    // Scattering amplitude and its cos-derivative (scattering.slang:16-38); 0 for cos <= 0.
+    float f = 1.0f, df = 0.0f;
+    if constexpr (FRESNEL) {
         const float n = c.fresnel_n;
-        if (n > 1.0f) {
-            const float root = sqrtf(n * n - 1.0f + q.cosine * q.cosine);
-            const float inv_sum = 1.0f / (root + q.cosine);
-            f = (root - q.cosine) * inv_sum;
-            df = -2.0f * f / root;
-        }
-        const float path = c.lam * q.inv_r * (1.0f / (8.0f * kPi));
-        q.amp = path * q.cosine * q.cosine * f;
-        q.amp_cos = path * (2.0f * q.cosine * f + q.cosine * q.cosine * df);
+        const float root = sqrtf(n * n - 1.0f + q.cosine * q.cosine);
+        const float inv_sum = rcp(root + q.cosine);
+        f = (root - q.cosine) * inv_sum;
+        df = -2.0f * f * rcp(root);
     }
+    const float path = c.lam * q.inv_r * (1.0f / (8.0f * kPi));
+    const bool lit = q.cosine > 0.0f;
+    q.amp = lit ? path * q.cosine * q.cosine * f : 0.0f;
+    q.amp_cos = lit ? path * (2.0f * q.cosine * f + q.cosine * q.cosine * df) : 0.0f;
     // Chirp phase in cycles, reduced in float64 (scattering.slang:40-45).
-    const double tau = 2.0 * r / c.c0;
+    const double tau = r * c.two_over_c0;
     double cyc = c.fc * tau + c.slope * tau * (c.t0 - 0.5 * tau);
     cyc -= rint(cyc);
     float sp, cp;
-    sincospif(2.0f * (float)cyc, &sp, &cp);
+    sincospi_unit(2.0f * (float)cyc, &sp, &cp);
     q.rot = make_float2(cp, sp);
     q.rate = fmaf(-c.rate1, q.r, c.rate0);
-    // Peak bin k0 = r / spacing wrapped to [0, N), split as lobe + delta.
+    // Peak bin k0 = r / spacing wrapped to [0, N), split as lobe + delta. A product by 1/N
+    // that lands one ulp off an integer leaves k0 just below 0 or at N: rint and the
+    // wrap below keep lobe in [0, N) and |delta| <= 1/2 either way.
     double k0 = r * c.k0pm;
     const double nf = (double)c.N;
-    k0 -= nf * floor(k0 / nf);
+    k0 -= nf * floor(k0 * c.inv_N);
     const double lobe = rint(k0);
     q.delta = (float)(k0 - lobe);
     int L = (int)lobe;
-    q.lobe = L >= c.N ? L - c.N : L;
+    q.lobe = L >= c.N ? L - c.N : (L < 0 ? L + c.N : L);
     return q;
+}
+
+__device__ __forceinline__ Pair pair_geometry(const Chirp& c, float px, float py, float pz,
+                                              float cx, float cy, float cz,
+                                              float nx, float ny, float nz) {
+    return c.fresnel_n > 1.0f ? pair_geometry_t<true>(c, px, py, pz, cx, cy, cz, nx, ny, nz)
+                              : pair_geometry_t<false>(c, px, py, pz, cx, cy, cz, nx, ny, nz);
 }
 
 // sin and cos of beta = pi (rho - delta) / P for every bin class r = k mod P, where

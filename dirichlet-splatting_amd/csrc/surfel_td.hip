@@ -41,10 +41,19 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kTdApertures = 16;  // forward: apertures per workgroup
 constexpr int kTdChunk = 16;      // forward: surfels per LDS chunk
 constexpr int kTdVjpSurfels = 128;
-constexpr int kTdVjpGroup = 8;    // VJP: apertures staged per LDS fill
+constexpr int kTdVjpGroup = 16;   // VJP: apertures staged per LDS fill
 
 __device__ __forceinline__ float2 csq(float2 a) { return make_float2(a.x * a.x - a.y * a.y, 2.0f * a.x * a.y); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// v_permlane32_swap: lanes 32-63 of a trade places with lanes 0-31 of b. With a = x and
+// b = y held by both lane halves, afterwards a = (x of half 0, y of half 0) and
+// b = (x of half 1, y of half 1) across the two halves.
+__device__ __forceinline__ void swap_halves(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
 
 // e^{2 pi i m / N} for m in [0, N), exactly rounded from float64.
 __device__ __forceinline__ void fill_twiddles(float2* E, int N, int tid, int nthreads) {
@@ -57,10 +66,10 @@ __device__ __forceinline__ void fill_twiddles(float2* E, int N, int tid, int nth
 
 // e^{2 pi i (q lobe + q delta) / N} = E[q lobe mod N] * e^{2 pi i q delta / N}, |q delta| <= q / 2.
 // The small rotation needs |2 pi q delta / N| <= pi / 4: q <= N / 4.
-__device__ __forceinline__ float2 rotation(const float2* E, int N, int lobe, float delta, int q) {
+__device__ __forceinline__ float2 rotation(const Chirp& ch, const float2* E, int lobe, float delta, int q) {
     float s, c;
-    sincos_small(2.0f * kPi * (float)q * delta / (float)N, &s, &c);
-    return cmul(E[(q * lobe) & (N - 1)], make_float2(c, s));
+    sincos_small(ch.two_pi_N * (float)q * delta, &s, &c);
+    return cmul(E[(q * lobe) & (ch.N - 1)], make_float2(c, s));
 }
 
 // out[r][q] = scale sum_{p < Lin} e^{sign 2 pi i p q / N} X[r][p] for the 16 rows of X (LDS,
@@ -101,16 +110,171 @@ __device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const f
 // ---------------------------------------------------------------------------------------
 template <int J, bool JVP>
 struct TdRec {
-    // floats per pair record: Z re[J], Z im[J], (e W) re[16], (e W) im[16] per coefficient set
+    // floats per pair record: Z re[J], Z im[J], (e W) re[16], (e W) im[16] per coefficient set,
+    // plus 2 floats so the record stride is 2 mod 4 dwords: the 8-byte writes of 16
+    // consecutive producer lanes then fall on distinct LDS banks.
     static constexpr int sets = JVP ? 2 : 1;
-    static constexpr int size = 2 * J + 32 * sets;
+    static constexpr int used = 2 * J + 32 * sets;
+    static constexpr int size = used + 2;
 };
 
-// Forward (JVP = false): X = sum_s c v^n with c = w amp rot.
-// JVP (JVP = true, surfel.slang:146-221): dy = sum_s e1 D + e2 D', so X = X1 - (2 pi i n / N) X2
-// with X1 = sum_s e1 v^n, X2 = sum_s e2 v^n, e1 = rot ((damp + i amp dphase) w + amp dw),
-// e2 = -amp k0pm dr rot w; both sets share the Z operand.
-template <int J, bool JVP>
+// Per-surfel inputs of a producer, loaded one chunk ahead.
+template <bool JVP>
+struct SurfIn {
+    float cx, cy, cz, nx, ny, nz;
+    float2 w;
+    float dc[JVP ? 3 : 1], dn[JVP ? 3 : 1];
+    float2 dw;
+};
+
+template <bool JVP>
+__device__ __forceinline__ SurfIn<JVP> load_surfel(int s, const float* __restrict__ ctr,
+                                                   const float* __restrict__ nrm, const float2* __restrict__ w,
+                                                   const float* __restrict__ dctr, const float* __restrict__ dnrm,
+                                                   const float2* __restrict__ dw) {
+    SurfIn<JVP> in;
+    in.cx = ctr[3 * s];
+    in.cy = ctr[3 * s + 1];
+    in.cz = ctr[3 * s + 2];
+    in.nx = nrm[3 * s];
+    in.ny = nrm[3 * s + 1];
+    in.nz = nrm[3 * s + 2];
+    in.w = w[s];
+    if constexpr (JVP) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            in.dc[d] = dctr[3 * s + d];
+            in.dn[d] = dnrm[3 * s + d];
+        }
+        in.dw = dw[s];
+    }
+    return in;
+}
+
+// Producer: the record of pair (aperture p, surfel s) into `mine` (LDS). Forward
+// (JVP = false): e = c = w amp rot. JVP (surfel.slang:146-221): dy = sum_s e1 D + e2 D', so
+// X = X1 - (2 pi i n / N) X2 with X1 = sum_s e1 v^n, X2 = sum_s e2 v^n,
+// e1 = rot ((damp + i amp dphase) w + amp dw), e2 = -amp k0pm dr rot w; both share Z.
+template <int J, bool JVP, bool FRESNEL>
+__device__ __forceinline__ void td_produce(const Chirp& c, const float2* E, bool ok, float px, float py, float pz,
+                                           const SurfIn<JVP>& in, float* mine) {
+    constexpr int SETS = TdRec<J, JVP>::sets;
+    float2 coef[SETS];
+    const Pair q = pair_geometry_t<FRESNEL>(c, px, py, pz, in.cx, in.cy, in.cz, in.nx, in.ny, in.nz);
+    const float amp = ok ? q.amp : 0.0f;
+    if constexpr (JVP) {
+        const float dr = -(q.ux * in.dc[0] + q.uy * in.dc[1] + q.uz * in.dc[2]);
+        const float dcos = ((q.cosine * q.ux - in.nx) * in.dc[0] + (q.cosine * q.uy - in.ny) * in.dc[1] +
+                            (q.cosine * q.uz - in.nz) * in.dc[2]) * q.inv_r +
+                           q.ux * in.dn[0] + q.uy * in.dn[1] + q.uz * in.dn[2];
+        const float damp = -amp * dr * q.inv_r + (ok ? q.amp_cos : 0.0f) * dcos;
+        coef[0] = cmul(q.rot, cmul(make_float2(damp, amp * q.rate * dr), in.w) + amp * in.dw);
+        coef[SETS - 1] = (-amp * c.k0pm_f * dr) * cmul(q.rot, in.w);
+    } else {
+        coef[0] = amp * cmul(in.w, q.rot);
+    }
+    const float2 v = rotation(c, E, q.lobe, q.delta, 1);
+    const float2 z1 = rotation(c, E, q.lobe, q.delta, 16);
+    // Z_j = z1^j by a binary product tree
+    float2 Z[J];
+    Z[0] = make_float2(1.f, 0.f);
+    Z[1] = z1;
+#pragma unroll
+    for (int j = 2; j < J; ++j) Z[j] = (j & 1) ? cmul(Z[j - 1], z1) : csq(Z[j >> 1]);
+#pragma unroll
+    for (int j = 0; j < J; j += 2) {
+        *reinterpret_cast<float2*>(mine + j) = make_float2(Z[j].x, Z[j + 1].x);
+        *reinterpret_cast<float2*>(mine + J + j) = make_float2(Z[j].y, Z[j + 1].y);
+    }
+    // e W_i = e v^i, two interleaved chains by v^2
+    const float2 v2 = csq(v);
+#pragma unroll
+    for (int e = 0; e < SETS; ++e) {
+        float2 W[16];
+        W[0] = coef[e];
+        W[1] = cmul(coef[e], v);
+#pragma unroll
+        for (int i = 2; i < 16; ++i) W[i] = cmul(W[i - 2], v2);
+        float* dst = mine + 2 * J + 32 * e;
+#pragma unroll
+        for (int i = 0; i < 16; i += 2) {
+            *reinterpret_cast<float2*>(dst + i) = make_float2(W[i].x, W[i + 1].x);
+            *reinterpret_cast<float2*>(dst + 16 + i) = make_float2(W[i].y, W[i + 1].y);
+        }
+    }
+}
+
+// Consumer: wave cw accumulates its 4 apertures over one chunk of SC surfels on
+// v_mfma_f32_16x16x4f32 (A[i][k] -> lane i + 16 k, B[k][j] -> lane j + 16 k,
+// C[4 (lane / 16) + r][lane % 16] in register r). The four real products of the complex one
+// are kept apart, so no operand needs a sign: K index kk = surfel 4 t + kk; row x =
+// (j = x / 2, Z part x % 2); column tile (set e, W part p), column x = i. Then
+//   Re X = C(re)[(j, re)] - C(im)[(j, im)],  Im X = C(im)[(j, re)] + C(re)[(j, im)].
+template <int J, int SETS, int T, int SC>
+__device__ __forceinline__ void td_consume(const float* rec, int cw, int lane, f32x4 (&acc)[SETS][2][4][T]) {
+    constexpr int REC = TdRec<J, SETS == 2>::size;
+    const int kk = lane >> 4, x = lane & 15;
+    const int a_off = (x & 1) * J + (x >> 1);  // Z part x % 2 of j = x / 2
+    const int b_off = 2 * J + x;               // (e W) re of i = x; im at + 16
+    const float* base = rec + ((4 * cw) * SC + kk) * REC;
+#pragma unroll
+    for (int t = 0; t < SC / 4; ++t) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const float* r = base + (b * SC + 4 * t) * REC;
+            float bv[SETS][2];
+#pragma unroll
+            for (int e = 0; e < SETS; ++e) {
+                bv[e][0] = r[b_off + 32 * e];
+                bv[e][1] = r[b_off + 32 * e + 16];
+            }
+#pragma unroll
+            for (int tt = 0; tt < T; ++tt) {
+                const float av = r[a_off + 8 * tt];
+#pragma unroll
+                for (int e = 0; e < SETS; ++e)
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+                        acc[e][p][b][tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[e][p], acc[e][p][b][tt], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// X tiles of consumer wave cw -> LDS rows [4 cw + b][S + 1] float2. Register r of lane (g, i)
+// is row 4 g + r = (j = 2 g + r / 2, Z part r % 2) of sample 16 j + i.
+template <int J, int SETS, int T>
+__device__ __forceinline__ void td_store_x(const Chirp& c, float2* X, int cw, int lane,
+                                           const f32x4 (&acc)[SETS][2][4][T]) {
+    const int ldx = c.S + 1;
+    const int g = lane >> 4, i = lane & 15;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int tt = 0; tt < T; ++tt)
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {
+                const int j = 8 * tt + 2 * g + (r >> 1);
+                if (j >= J) continue;
+                const int n = 16 * j + i;
+                float2 xs[SETS];
+#pragma unroll
+                for (int e = 0; e < SETS; ++e)
+                    xs[e] = make_float2(acc[e][0][b][tt][r] - acc[e][1][b][tt][r + 1],
+                                        acc[e][1][b][tt][r] + acc[e][0][b][tt][r + 1]);
+                float2 val = xs[0];
+                if constexpr (SETS == 2) {
+                    // - (2 pi i n / N) X2
+                    const float k = c.two_pi_N * (float)n;
+                    val.x += k * xs[1].y;
+                    val.y -= k * xs[1].x;
+                }
+                X[(4 * cw + b) * ldx + n] = val;
+            }
+}
+
+// Single-buffer form (JVP): all 4 waves produce one pair each per chunk, then consume.
+template <int J, bool JVP, bool FRESNEL>
 __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const float* __restrict__ pos, int A,
                                                         const float* __restrict__ ctr,
                                                         const float* __restrict__ nrm,
@@ -128,133 +292,98 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int a0 = blockIdx.x * AT;
     fill_twiddles(E, c.N, tid, 256);
-
-    // producer: one pair per thread per chunk
     const int pal = tid >> 4, psp = tid & 15;
     const int pa = min(a0 + pal, A - 1);
     const bool p_ok = a0 + pal < A;
     const float px = pos[3 * pa], py = pos[3 * pa + 1], pz = pos[3 * pa + 2];
-    float* mine = rec + tid * REC;
 
-    // consumer lane roles (16x16x4): kk = surfel-in-step (kk >> 1) and part (kk & 1)
-    const int kk = lane >> 4, x = lane & 15;
-    const int part = kk & 1, sl = kk >> 1;
-    const int jrow = x >> 1, prow = x & 1;
-    const unsigned flip = (part && !prow) ? 0x80000000u : 0u;
-    const int a_off = (prow ^ part) * J + jrow;  // Z component for row (jrow, prow)
-    const int b_off = 2 * J + part * 16 + x;     // (e W) component for column x
-
-    f32x4 acc[SETS][4][T];
+    f32x4 acc[SETS][2][4][T];
 #pragma unroll
     for (int e = 0; e < SETS; ++e)
 #pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int tt = 0; tt < T; ++tt) acc[e][p][b][tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    SurfIn<JVP> in = load_surfel<JVP>(min(psp, S - 1), ctr, nrm, w, dctr, dnrm, dw);
+    for (int s0 = 0; s0 < S; s0 += SC) {
+        const int s = s0 + psp;
+        const SurfIn<JVP> next = load_surfel<JVP>(min(s + SC, S - 1), ctr, nrm, w, dctr, dnrm, dw);
+        __syncthreads();  // previous chunk consumed (and the twiddle table written)
+        td_produce<J, JVP, FRESNEL>(c, E, p_ok && s < S, px, py, pz, in, rec + tid * REC);
+        __syncthreads();
+        td_consume<J, SETS, T, SC>(rec, wave, lane, acc);
+        in = next;
+    }
+    __syncthreads();
+    float2* X = reinterpret_cast<float2*>(rec);
+    td_store_x<J, SETS, T>(c, X, wave, lane, acc);
+    __syncthreads();
+    dft16(X, c.S + 1, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(AT, A - a0), wave, lane);
+}
+
+// Warp-specialized, double-buffered form (render): waves 0-3 consume chunk k on MFMA while
+// waves 4-7 produce chunk k + 1 on the VALU, so every SIMD holds one wave of each and the
+// matrix and vector pipes work at the same time. One barrier per chunk.
+template <int J, bool FRESNEL>
+__global__ __launch_bounds__(512) void td_render_ws_kernel(const Chirp c, const float* __restrict__ pos, int A,
+                                                           const float* __restrict__ ctr,
+                                                           const float* __restrict__ nrm,
+                                                           const float2* __restrict__ w, int S,
+                                                           float2* __restrict__ out) {
+    constexpr int AT = kTdApertures, SC = kTdChunk, REC = TdRec<J, false>::size;
+    constexpr int T = (2 * J + 15) / 16;
+    extern __shared__ float smem[];
+    float2* E = reinterpret_cast<float2*>(smem);  // [N]
+    float* buf = smem + 2 * c.N;                  // [2][AT * SC][REC]; later X [AT][S + 1] float2
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const bool producer = wave >= 4;
+    const int a0 = blockIdx.x * AT;
+    fill_twiddles(E, c.N, tid, 512);
+    const int ptid = tid - 256;
+    const int pal = (ptid >> 4) & 15, psp = ptid & 15;
+    const int pa = min(a0 + pal, A - 1);
+    const bool p_ok = a0 + pal < A;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (producer) {
+        px = pos[3 * pa];
+        py = pos[3 * pa + 1];
+        pz = pos[3 * pa + 2];
+    }
+    f32x4 acc[1][2][4][T];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
-            for (int tt = 0; tt < T; ++tt) acc[e][b][tt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int tt = 0; tt < T; ++tt) acc[0][p][b][tt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int s0 = 0; s0 < S; s0 += SC) {
-        __syncthreads();  // previous chunk consumed (and the twiddle table written)
-        {
-            const int s = s0 + psp;
-            float2 coef[SETS];
-#pragma unroll
-            for (int e = 0; e < SETS; ++e) coef[e] = make_float2(0.f, 0.f);
-            float2 v = make_float2(1.f, 0.f), z1 = make_float2(1.f, 0.f);
-            if (p_ok && s < S) {
-                const float nx = nrm[3 * s], ny = nrm[3 * s + 1], nz = nrm[3 * s + 2];
-                const Pair q = pair_geometry(c, px, py, pz, ctr[3 * s], ctr[3 * s + 1], ctr[3 * s + 2], nx, ny, nz);
-                if (q.amp != 0.0f) {
-                    if constexpr (JVP) {
-                        const float dcx = dctr[3 * s], dcy = dctr[3 * s + 1], dcz = dctr[3 * s + 2];
-                        const float dr = -(q.ux * dcx + q.uy * dcy + q.uz * dcz);
-                        const float dcos = ((q.cosine * q.ux - nx) * dcx + (q.cosine * q.uy - ny) * dcy +
-                                            (q.cosine * q.uz - nz) * dcz) * q.inv_r +
-                                           q.ux * dnrm[3 * s] + q.uy * dnrm[3 * s + 1] + q.uz * dnrm[3 * s + 2];
-                        const float damp = -q.amp * dr * q.inv_r + q.amp_cos * dcos;
-                        const float2 ws = w[s];
-                        coef[0] = cmul(q.rot, cmul(make_float2(damp, q.amp * q.rate * dr), ws) + q.amp * dw[s]);
-                        coef[SETS - 1] = (-q.amp * c.k0pm_f * dr) * cmul(q.rot, ws);
-                    } else {
-                        coef[0] = q.amp * cmul(w[s], q.rot);
-                    }
-                    v = rotation(E, c.N, q.lobe, q.delta, 1);
-                    z1 = rotation(E, c.N, q.lobe, q.delta, 16);
-                }
+    const int chunks = (S + SC - 1) / SC;
+    SurfIn<false> in;
+    if (producer) in = load_surfel<false>(min(psp, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
+    __syncthreads();  // twiddle table
+    for (int k = 0; k <= chunks; ++k) {
+        if (producer) {
+            if (k < chunks) {
+                const int s = k * SC + psp;
+                const SurfIn<false> next = load_surfel<false>(min(s + SC, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
+                td_produce<J, false, FRESNEL>(c, E, p_ok && s < S, px, py, pz, in,
+                                              buf + ((k & 1) * AT * SC + ptid) * REC);
+                in = next;
             }
-            // Z_j = z1^j by a binary product tree
-            float2 Z[J];
-            Z[0] = make_float2(1.f, 0.f);
-            Z[1] = z1;
-#pragma unroll
-            for (int j = 2; j < J; ++j) Z[j] = (j & 1) ? cmul(Z[j - 1], z1) : csq(Z[j >> 1]);
-#pragma unroll
-            for (int j = 0; j < J; j += 4) {
-                *reinterpret_cast<float4*>(mine + j) = make_float4(Z[j].x, Z[j + 1].x, Z[j + 2].x, Z[j + 3].x);
-                *reinterpret_cast<float4*>(mine + J + j) = make_float4(Z[j].y, Z[j + 1].y, Z[j + 2].y, Z[j + 3].y);
-            }
-            // e W_i = e v^i, two interleaved chains by v^2
-            const float2 v2 = csq(v);
-#pragma unroll
-            for (int e = 0; e < SETS; ++e) {
-                float2 W[16];
-                W[0] = coef[e];
-                W[1] = cmul(coef[e], v);
-#pragma unroll
-                for (int i = 2; i < 16; ++i) W[i] = cmul(W[i - 2], v2);
-                float* dst = mine + 2 * J + 32 * e;
-#pragma unroll
-                for (int i = 0; i < 16; i += 4) {
-                    *reinterpret_cast<float4*>(dst + i) = make_float4(W[i].x, W[i + 1].x, W[i + 2].x, W[i + 3].x);
-                    *reinterpret_cast<float4*>(dst + 16 + i) = make_float4(W[i].y, W[i + 1].y, W[i + 2].y, W[i + 3].y);
-                }
-            }
+        } else if (k > 0) {
+            td_consume<J, 1, T, SC>(buf + ((k - 1) & 1) * AT * SC * REC, wave, lane, acc);
         }
         __syncthreads();
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const float* base = rec + ((4 * wave + b) * SC + sl) * REC;
-#pragma unroll
-            for (int t = 0; t < SC / 2; ++t) {
-                const float* r = base + 2 * t * REC;
-#pragma unroll
-                for (int tt = 0; tt < T; ++tt) {
-                    const float av = __uint_as_float(__float_as_uint(r[a_off + 8 * tt]) ^ flip);
-#pragma unroll
-                    for (int e = 0; e < SETS; ++e)
-                        acc[e][b][tt] =
-                            __builtin_amdgcn_mfma_f32_16x16x4f32(av, r[b_off + 32 * e], acc[e][b][tt], 0, 0, 0);
-                }
-            }
-        }
     }
+    float2* X = reinterpret_cast<float2*>(buf);
+    if (!producer) td_store_x<J, 1, T>(c, X, wave, lane, acc);
     __syncthreads();
-    // X tiles -> LDS [AT][S + 1] float2: register r of lane (g, i) is row 4 g + r = (j, part),
-    // so registers (0, 1) and (2, 3) are the (re, im) of samples 16 j + i.
-    const int ldx = c.S + 1;
-    float2* X = reinterpret_cast<float2*>(rec);
-    {
-        const int g = lane >> 4, i = lane & 15;
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int tt = 0; tt < T; ++tt)
-#pragma unroll
-                for (int r = 0; r < 4; r += 2) {
-                    const int j = 8 * tt + 2 * g + (r >> 1);
-                    if (j >= J) continue;
-                    const int n = 16 * j + i;
-                    float2 val = make_float2(acc[0][b][tt][r], acc[0][b][tt][r + 1]);
-                    if constexpr (JVP) {
-                        // - (2 pi i n / N) X2
-                        const float k = 2.0f * c.pi_N * (float)n;
-                        val.x += k * acc[SETS - 1][b][tt][r + 1];
-                        val.y -= k * acc[SETS - 1][b][tt][r];
-                    }
-                    X[(4 * wave + b) * ldx + n] = val;
-                }
-    }
-    __syncthreads();
-    dft16(X, ldx, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(AT, A - a0), wave, lane);
+    dft16(X, c.S + 1, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(AT, A - a0), wave & 3,
+          lane);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -298,7 +427,55 @@ __device__ __forceinline__ void td_row(int tile, int q, int h, int r, int* mat, 
     }
 }
 
+// What one lane carries from the generation of its pair to the contraction and chain rule.
 template <int J>
+struct VjpStage {
+    float B0[16], B1[16];  // MFMA B operands of apertures a and a + 1 (after the half swap)
+    float2 Z[J];           // Z_j of the own pair
+    float2 rot;
+    float amp, amp_cos, inv_r, ux, uy, uz, cosine, rate;
+};
+
+template <int J, bool FRESNEL>
+__device__ __forceinline__ void vjp_stage(const Chirp& c, const float2* E, const float* __restrict__ pos, int a,
+                                          bool ok, float cx, float cy, float cz, float nx, float ny, float nz,
+                                          VjpStage<J>& st) {
+    const Pair q = pair_geometry_t<FRESNEL>(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
+    st.rot = q.rot;
+    st.amp = ok ? q.amp : 0.0f;
+    st.amp_cos = ok ? q.amp_cos : 0.0f;
+    st.inv_r = q.inv_r;
+    st.ux = q.ux;
+    st.uy = q.uy;
+    st.uz = q.uz;
+    st.cosine = q.cosine;
+    st.rate = q.rate;
+    // w = conj(v): W_t = w^t, Z_j = w^{16 j}
+    const float2 w1 = cconj(rotation(c, E, q.lobe, q.delta, 1));
+    const float2 z1 = cconj(rotation(c, E, q.lobe, q.delta, 16));
+    float2 W[16];
+    W[0] = make_float2(1.f, 0.f);
+    W[1] = w1;
+#pragma unroll
+    for (int t = 2; t < 16; ++t) W[t] = (t & 1) ? cmul(W[t - 1], w1) : csq(W[t >> 1]);
+    st.Z[0] = make_float2(1.f, 0.f);
+    st.Z[1] = z1;
+#pragma unroll
+    for (int j = 2; j < J; ++j) st.Z[j] = (j & 1) ? cmul(st.Z[j - 1], z1) : csq(st.Z[j >> 1]);
+    // B operands: half 0 holds Re W of aperture a (own) and of a + 1, half 1 holds Im W of
+    // both: one half swap per t.
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        st.B0[t] = W[t].x;
+        st.B1[t] = W[t].y;
+        swap_halves(st.B0[t], st.B1[t]);
+    }
+}
+
+// One workgroup: 4 waves x 32 surfels, an aperture range staged G apertures at a time. Each
+// step covers apertures (a, a + 1): the two lane halves generate one pair each, then 32 MFMAs,
+// the contraction over j and the chain rule.
+template <int J, bool FRESNEL>
 __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float* __restrict__ pos, int A,
                                                      const float* __restrict__ ctr,
                                                      const float* __restrict__ nrm,
@@ -326,8 +503,11 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
 
     for (int g0 = a_begin; g0 < a_end; g0 += G) {
         __syncthreads();
-        // Stage A operands: value(t, lane L) for L = row + 32 k, row -> (mat, j, part).
-        for (int e = tid; e < G * T * 4 * 64; e += 256) {
+        // Stage A operands: value(t, lane L) for L = row + 32 k, row -> (mat, j, part). The trip
+        // count is a constant, so all loads of a thread are in flight together.
+#pragma unroll
+        for (int it = 0; it < G * T * 4 * 64 / 256; ++it) {
+            const int e = tid + 256 * it;
             const int L = e & 63, tg = (e >> 6) & 3, tile = (e >> 8) % T, ga = e / (256 * T);
             const int a = g0 + ga;
             const int row = L & 31, k = L >> 5;
@@ -351,35 +531,10 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
         }
         __syncthreads();
         for (int ga = 0; ga < G; ga += 2) {
-            const int a = g0 + ga + h;  // this lane's pair: (a, s)
-            const bool a_ok = a < a_end;
-            const int ac = min(a, A - 1);
-            const Pair q = pair_geometry(c, pos[3 * ac], pos[3 * ac + 1], pos[3 * ac + 2], cx, cy, cz, nx, ny, nz);
-            const float amp = (a_ok && s_ok) ? q.amp : 0.0f;
-            // w = conj(v): W_t = w^t, Z_j = w^{16 j}
-            float2 W[16], Z[J];
+            VjpStage<J> cur;
             {
-                const float2 w1 = cconj(rotation(E, c.N, q.lobe, q.delta, 1));
-                const float2 z1 = cconj(rotation(E, c.N, q.lobe, q.delta, 16));
-                W[0] = make_float2(1.f, 0.f);
-                W[1] = w1;
-#pragma unroll
-                for (int t = 2; t < 16; ++t) W[t] = (t & 1) ? cmul(W[t - 1], w1) : csq(W[t >> 1]);
-                Z[0] = make_float2(1.f, 0.f);
-                Z[1] = z1;
-#pragma unroll
-                for (int j = 2; j < J; ++j) Z[j] = (j & 1) ? cmul(Z[j - 1], z1) : csq(Z[j >> 1]);
-            }
-            // B operands: half 0 keeps Re W of its pair (aperture a) and receives Re W of
-            // aperture a + 1; half 1 keeps Im W (aperture a + 1) and receives Im W of aperture a.
-            float B0[16], B1[16];
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                const float send = h ? W[t].x : W[t].y;
-                const float recv = __shfl_xor(send, 32, 64);
-                const float own = h ? W[t].y : W[t].x;
-                B0[t] = h ? recv : own;  // aperture g0 + ga
-                B1[t] = h ? own : recv;  // aperture g0 + ga + 1
+                const int a = g0 + ga + h;  // this lane's pair: (a, s)
+                vjp_stage<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
             }
             f32x16 acc0[T], acc1[T];
 #pragma unroll
@@ -397,19 +552,19 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
                 for (int tg = 0; tg < 4; ++tg) {
                     const float4 x0 = A0[(tile * 4 + tg) * 64];
                     const float4 x1 = A1[(tile * 4 + tg) * 64];
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.x, B0[4 * tg], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.x, B1[4 * tg], acc1[tile], 0, 0, 0);
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.y, B0[4 * tg + 1], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.y, B1[4 * tg + 1], acc1[tile], 0, 0, 0);
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.z, B0[4 * tg + 2], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.z, B1[4 * tg + 2], acc1[tile], 0, 0, 0);
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.w, B0[4 * tg + 3], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.w, B1[4 * tg + 3], acc1[tile], 0, 0, 0);
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.x, cur.B0[4 * tg], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.x, cur.B1[4 * tg], acc1[tile], 0, 0, 0);
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.y, cur.B0[4 * tg + 1], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.y, cur.B1[4 * tg + 1], acc1[tile], 0, 0, 0);
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.z, cur.B0[4 * tg + 2], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.z, cur.B1[4 * tg + 2], acc1[tile], 0, 0, 0);
+                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.w, cur.B0[4 * tg + 3], acc0[tile], 0, 0, 0);
+                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.w, cur.B1[4 * tg + 3], acc1[tile], 0, 0, 0);
                 }
-            // Contraction P_mat = sum_j Z_j M_mat[j]. Lane half h holds j of its row set for
-            // both apertures; Z_j of aperture (g0 + ga + h) is local, the other one comes from
-            // the partner lane.
-            float2 P[2][2];  // [aperture 0/1][mat]
+            // Contraction P_mat = sum_j Z_j M_mat[j]. Half h holds the rows of j in its set J_h
+            // for both apertures; one half swap of (Z_{j of half 0}, Z_{j of half 1}) gives each
+            // half Z_{j in J_h} of aperture a and of aperture a + 1.
+            float2 P[2][2];  // [aperture a / a + 1][mat]
 #pragma unroll
             for (int x = 0; x < 2; ++x) P[x][0] = P[x][1] = make_float2(0.f, 0.f);
 #pragma unroll
@@ -419,45 +574,45 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
                     int mat, j0, j1;
                     td_row<J>(tile, rr >> 2, 0, rr & 3, &mat, &j0);
                     td_row<J>(tile, rr >> 2, 1, rr & 3, &mat, &j1);
-                    // own Z for the own aperture; the partner needs Z of its own row set
-                    const float2 zown = h ? Z[j1] : Z[j0];
-                    const float2 zsend = h ? Z[j0] : Z[j1];
-                    const float2 zrecv = make_float2(__shfl_xor(zsend.x, 32, 64), __shfl_xor(zsend.y, 32, 64));
-                    const float2 za0 = h ? zrecv : zown;  // Z_j of aperture g0 + ga
-                    const float2 za1 = h ? zown : zrecv;  // Z_j of aperture g0 + ga + 1
+                    float2 za0 = cur.Z[j0], za1 = cur.Z[j1];
+                    swap_halves(za0.x, za1.x);
+                    swap_halves(za0.y, za1.y);
                     P[0][mat] += cmul(za0, make_float2(acc0[tile][rr], acc0[tile][rr + 1]));
                     P[1][mat] += cmul(za1, make_float2(acc1[tile][rr], acc1[tile][rr + 1]));
                 }
-            // Complete the own aperture's sums with the partner's half.
-            float2 P0, P1;
-            {
-                const float2 s0 = h ? P[0][0] : P[1][0];
-                const float2 s1 = h ? P[0][1] : P[1][1];
-                const float2 r0 = make_float2(__shfl_xor(s0.x, 32, 64), __shfl_xor(s0.y, 32, 64));
-                const float2 r1 = make_float2(__shfl_xor(s1.x, 32, 64), __shfl_xor(s1.y, 32, 64));
-                P0 = (h ? P[1][0] : P[0][0]) + r0;
-                P1 = (h ? P[1][1] : P[0][1]) + r1;
+            // Each half completes the sums of its own pair's aperture (a + h) with the other
+            // half's share: after the swap, P[0] + P[1] is that total on both halves.
+#pragma unroll
+            for (int mat = 0; mat < 2; ++mat) {
+                swap_halves(P[0][mat].x, P[1][mat].x);
+                swap_halves(P[0][mat].y, P[1][mat].y);
             }
-            // g0 = P0 / S, g1 = (2 pi i / (N S)) P1; chain as the cot-form VJP.
+            const float2 P0 = P[0][0] + P[1][0];
+            const float2 P1 = P[0][1] + P[1][1];
+            // g0 = P0 / S, g1 = (2 pi i / (N S)) P1; chained as surfel.slang:294-302.
             const float2 g0 = c.inv_S * P0;
-            const float2 g1 = (2.0f * c.pi_N * c.inv_S) * make_float2(-P1.y, P1.x);
-            const float2 z0 = cmulc(q.rot, g0), zz1 = cmulc(q.rot, g1);
+            const float2 g1 = (c.two_pi_N * c.inv_S) * make_float2(-P1.y, P1.x);
+            const float2 z0 = cmulc(cur.rot, g0), zz1 = cmulc(cur.rot, g1);
             const float2 cw = cmulc(ws, z0);
             const float along = cmulc(ws, zz1).x * c.k0pm_f;
-            const float gr = -cw.x * amp * q.inv_r + amp * (cw.y * q.rate - along);
-            const float gcos = cw.x * ((a_ok && s_ok) ? q.amp_cos : 0.0f);
-            gacc[0] += -gr * q.ux + gcos * (q.cosine * q.ux - nx) * q.inv_r;
-            gacc[1] += -gr * q.uy + gcos * (q.cosine * q.uy - ny) * q.inv_r;
-            gacc[2] += -gr * q.uz + gcos * (q.cosine * q.uz - nz) * q.inv_r;
-            gacc[3] += gcos * q.ux;
-            gacc[4] += gcos * q.uy;
-            gacc[5] += gcos * q.uz;
-            gacc[6] += amp * z0.x;
-            gacc[7] += amp * z0.y;
+            const float gr = -cw.x * cur.amp * cur.inv_r + cur.amp * (cw.y * cur.rate - along);
+            const float gcos = cw.x * cur.amp_cos;
+            gacc[0] += -gr * cur.ux + gcos * (cur.cosine * cur.ux - nx) * cur.inv_r;
+            gacc[1] += -gr * cur.uy + gcos * (cur.cosine * cur.uy - ny) * cur.inv_r;
+            gacc[2] += -gr * cur.uz + gcos * (cur.cosine * cur.uz - nz) * cur.inv_r;
+            gacc[3] += gcos * cur.ux;
+            gacc[4] += gcos * cur.uy;
+            gacc[5] += gcos * cur.uz;
+            gacc[6] += cur.amp * z0.x;
+            gacc[7] += cur.amp * z0.y;
         }
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j) gacc[j] += __shfl_xor(gacc[j], 32, 64);
+    for (int j = 0; j < 8; ++j) {
+        float x = gacc[j], y = gacc[j];
+        swap_halves(x, y);
+        gacc[j] = x + y;
+    }
     if (h == 0 && s_ok) {
         float4* dst = reinterpret_cast<float4*>(partial + ((size_t)s * gridDim.y + range) * 8);
         dst[0] = make_float4(gacc[0], gacc[1], gacc[2], gacc[3]);
@@ -482,11 +637,35 @@ static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, co
     const size_t rec = sizeof(float) * (size_t)kTdApertures * kTdChunk * TdRec<J, JVP>::size;
     const size_t xbuf = sizeof(float2) * (size_t)kTdApertures * (c.S + 1);
     const size_t smem = sizeof(float2) * c.N + (rec > xbuf ? rec : xbuf);
-    hipError_t e = allow_lds((const void*)td_render_kernel<J, JVP>, smem);
+    const bool fres = c.fresnel_n > 1.0f;
+    hipError_t e = allow_lds(fres ? (const void*)td_render_kernel<J, JVP, true> : (const void*)td_render_kernel<J, JVP, false>,
+                             smem);
     if (e != hipSuccess) return e;
     const dim3 grid((A + kTdApertures - 1) / kTdApertures);
-    hipLaunchKernelGGL((td_render_kernel<J, JVP>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr, dnrm, dw,
-                       S, out);
+    if (fres)
+        hipLaunchKernelGGL((td_render_kernel<J, JVP, true>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr,
+                           dnrm, dw, S, out);
+    else
+        hipLaunchKernelGGL((td_render_kernel<J, JVP, false>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr,
+                           dnrm, dw, S, out);
+    return hipGetLastError();
+}
+
+template <int J>
+static hipError_t launch_td_render_ws_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                        const float2* w, int S, float2* out, hipStream_t st) {
+    const size_t rec = 2 * sizeof(float) * (size_t)kTdApertures * kTdChunk * TdRec<J, false>::size;
+    const size_t xbuf = sizeof(float2) * (size_t)kTdApertures * (c.S + 1);
+    const size_t smem = sizeof(float2) * c.N + (rec > xbuf ? rec : xbuf);
+    const bool fres = c.fresnel_n > 1.0f;
+    hipError_t e =
+        allow_lds(fres ? (const void*)td_render_ws_kernel<J, true> : (const void*)td_render_ws_kernel<J, false>, smem);
+    if (e != hipSuccess) return e;
+    const dim3 grid((A + kTdApertures - 1) / kTdApertures);
+    if (fres)
+        hipLaunchKernelGGL((td_render_ws_kernel<J, true>), grid, dim3(512), smem, st, c, pos, A, ctr, nrm, w, S, out);
+    else
+        hipLaunchKernelGGL((td_render_ws_kernel<J, false>), grid, dim3(512), smem, st, c, pos, A, ctr, nrm, w, S, out);
     return hipGetLastError();
 }
 
@@ -542,10 +721,15 @@ static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const
         if (e != hipSuccess) return e;
     }
     const size_t smem = sizeof(float2) * c.N + sizeof(float4) * (size_t)kTdVjpGroup * (J / 8) * 4 * 64;
-    hipError_t e = allow_lds((const void*)td_vjp_kernel<J>, smem);
+    const void* fn = c.fresnel_n > 1.0f ? (const void*)td_vjp_kernel<J, true> : (const void*)td_vjp_kernel<J, false>;
+    hipError_t e = allow_lds(fn, smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(td_vjp_kernel<J>, dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A, ctr, nrm, w, S, U,
-                       v.per_range, partial);
+    if (c.fresnel_n > 1.0f)
+        hipLaunchKernelGGL((td_vjp_kernel<J, true>), dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A, ctr,
+                           nrm, w, S, U, v.per_range, partial);
+    else
+        hipLaunchKernelGGL((td_vjp_kernel<J, false>), dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A, ctr,
+                           nrm, w, S, U, v.per_range, partial);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_vjp_reduce(partial, S, v.ranges, gc, gn, gw, st);
