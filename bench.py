@@ -288,11 +288,12 @@ def scan_workload(args):
 def dsfw_workload(args):
     """DSFW steps (spatial search, merged replacement of 256 atoms, joint Gauss-Newton) on a
     float64-synthesized 128x128 scan of the ellipsoid with 30 dB noise, from random initial
-    surfels, in two forms: plain (FP32 SYRK normal equations, candidates scored in chunks,
-    host-loop search, backtracking and elimination) and mixed-precision batched (FP16 hi+lo
-    split SYRK, one-launch orientation choice, quasi-Newton search and elimination,
-    correlation rhs, 2^29-element design blocks). Both start from the same state; per-step
-    wall time between synchronizations and the NMSE after each step."""
+    surfels, in two forms: plain (bin-domain FP32 splat kernels, FP32 SYRK normal equations,
+    candidates scored in chunks, host-loop search, backtracking and elimination) and
+    mixed-precision batched (sample-domain MFMA splat, FP16 hi+lo split SYRK, one-launch
+    orientation choice, quasi-Newton search and elimination, correlation rhs, 2^29-element
+    design blocks). Both start from the same state; per-step wall time between
+    synchronizations and the NMSE after each step."""
     import statistics
 
     import torch
@@ -306,14 +307,22 @@ def dsfw_workload(args):
         adc_samples=args.adc_samples, truth_draws=draws, front_count=None if args.truth_draws else 2048
     )
     S = len(problem["initial"]["centers"])
+    # plain: bin-domain FP32 splat kernels, FP32 SYRK, host-loop search; mixed: sample-domain
+    # MFMA splat, FP16 hi+lo SYRK, one-launch scoring / search / elimination
     forms = {
-        "plain_fp32": DSFWConfig(batch=256),
-        "mixed_batched": DSFWConfig(batch=256, gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22),
+        "plain_fp32": (DSFWConfig(batch=256), 1),
+        "mixed_batched": (
+            DSFWConfig(batch=256, gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22), 0
+        ),
     }
+    from dsplat import _lib
+
+    lib = _lib.library()
     search = SpatialSearchConfig()
     steps, warmup = max(1, args.steps), max(1, args.warmup)
     out = {}
-    for name, config in forms.items():
+    for name, (config, splat_form) in forms.items():
+        lib.dsplat_set_splat_form(splat_form)
         state = initialize(problem["model"], problem["target"], problem["initial"], config,
                            parameters=problem["parameters"])  # fmt: skip
         for _ in range(warmup):
@@ -326,7 +335,9 @@ def dsfw_workload(args):
             torch.cuda.synchronize()
             times.append((time.perf_counter() - t0) * 1e3)
         error = torch.cdist(state.params["centers"], problem["truth"]["centers"]).min(1).values
+        lib.dsplat_set_splat_form(0)
         out[name] = {
+            "splat_form": "bin-domain" if splat_form else "sample-domain",
             "ms_per_step_mean": statistics.fmean(times),
             "ms_per_step_median": statistics.median(times),
             "ms": times,

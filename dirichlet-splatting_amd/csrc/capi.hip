@@ -405,7 +405,7 @@ int dsplat_project_design(const dsplat_fmcw* cfg, const float* positions, int32_
 size_t dsplat_qn_workspace(const dsplat_fmcw* cfg, int32_t num_apertures) {
     Chirp c;
     if (!make_chirp(cfg, &c) || num_apertures < 0) return 0;
-    return qn_workspace(num_apertures, c.P) + 16;
+    return qn_workspace(num_apertures, c.P, c.K) + 16;
 }
 
 int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* residual,
@@ -418,7 +418,7 @@ int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num
         return hipErrorInvalidValue;
     if (count == 0) return hipSuccess;
     if (!positions || !residual || !centers || !normals || !areas || !centers_out || !normals_out || !values_out ||
-        !workspace || workspace_bytes < qn_workspace(num_apertures, c.P))
+        !workspace || workspace_bytes < qn_workspace(num_apertures, c.P, c.K))
         return hipErrorInvalidValue;
     return launch_qn_search(c, positions, num_apertures, reinterpret_cast<const float2*>(residual), centers, normals,
                             areas, count, *spec, centers_out, normals_out, values_out, workspace, (hipStream_t)stream);
@@ -528,6 +528,21 @@ int dsplat_scan_batched(const dsplat_fmcw* cfg, int32_t num_scans, const float* 
     chunk = (chunk + 7) / 8 * 8;
     return launch_scan(c, set, centers, normals, num_candidates, chunk, static_cast<float*>(workspace), out,
                        (hipStream_t)stream);
+}
+
+int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stream) {
+    if (n < 0 || lda < n || !info) return hipErrorInvalidValue;
+    if (n == 0) return fill_zero(reinterpret_cast<float*>(info), 1, (hipStream_t)stream);
+    if (!a) return hipErrorInvalidValue;
+    return launch_cholesky(a, n, lda, info, (hipStream_t)stream);
+}
+
+int dsplat_cholesky_solve(const double* l, int32_t n, int32_t lda, double* b, int32_t nrhs, int32_t ldb,
+                          void* stream) {
+    if (n < 0 || nrhs < 0 || lda < n || ldb < n || n > kCholMaxSolve) return hipErrorInvalidValue;
+    if (n == 0 || nrhs == 0) return hipSuccess;
+    if (!l || !b) return hipErrorInvalidValue;
+    return launch_cholesky_solve(l, n, lda, b, nrhs, ldb, (hipStream_t)stream);
 }
 
 }  // extern "C"

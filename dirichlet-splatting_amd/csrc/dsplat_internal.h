@@ -110,7 +110,7 @@ bool project_supported(const Chirp& c, int rows);
 hipError_t launch_project(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                           const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st);
 typedef dsplat_qn_spec QnSpec;
-size_t qn_workspace(int A, int P);
+size_t qn_workspace(int A, int P, int K);
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,
                             const float* nrm, const float* areas, int count, const QnSpec& sp, float* oc, float* on,
                             float* ov, void* workspace, hipStream_t st);
@@ -143,6 +143,10 @@ hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* c
                          hipStream_t st);
 hipError_t launch_vjp_reduce(const float* partial, int S, int nblk, float* gc, float* gn, float* gw,
                              hipStream_t st);
+
+constexpr int kCholMaxSolve = 8192;  // LDS holds the solution vector
+hipError_t launch_cholesky(double* A, int n, int lda, int* info, hipStream_t st);
+hipError_t launch_cholesky_solve(const double* L, int n, int lda, double* B, int nrhs, int ldb, hipStream_t st);
 
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st);

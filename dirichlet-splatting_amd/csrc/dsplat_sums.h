@@ -12,11 +12,11 @@ struct Sums {
 };
 
 // Bin sums of one pair over the observed bins: T is the derivative cot table [2N] (T_0 = 0),
-// R the aperture's residual at bin k in R[k * STRIDE] (zero-padded to Kp), Us its P class
-// sums (bins k = r mod P) in Us[r * STRIDE]. GRAD adds g1 and rx.
-template <int P, bool GRAD, int STRIDE>
+// R the aperture's residual at bin k in R[k * stride] (zero-padded to Kp), Us its P class
+// sums (bins k = r mod P) in Us[r * stride]. GRAD adds g1 and rx.
+template <int P, bool GRAD>
 __device__ __forceinline__ Sums bin_sums(const Chirp& c, const float* __restrict__ T, const float2* __restrict__ R,
-                                         const float2* __restrict__ Us, const Pair& q) {
+                                         const float2* __restrict__ Us, const Pair& q, int STRIDE) {
     const float t = tan_small(c.pi_N * q.delta);
     const float* Tb = T + (c.N - q.lobe);
     float2 F[P], G[P];

@@ -23,15 +23,22 @@ namespace dsplat {
 constexpr int kQnThreads = 256;
 constexpr int kQnMaxValues = 21;  // corr (2), norm, dcorr (2 x 6), dnorm (6)
 
-// Residual class sums us[a][r] = sum over bins k = r mod P of res[a][k].
-__global__ __launch_bounds__(256) void class_sums_kernel(const float2* __restrict__ res, int A, int K, int P,
-                                                         float2* __restrict__ us) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= A * P) return;
-    const int a = i / P, r = i - a * P;
-    float2 s = make_float2(0.f, 0.f);
-    for (int k = r; k < K; k += P) s += res[(size_t)a * K + k];
-    us[i] = s;
+// Aperture-minor copies for the per-thread aperture loops of qn_sums, so that the lanes of
+// a wave (consecutive apertures) read consecutive addresses: resT[k][a] = res[a][k] and the
+// class sums usT[r][a] = sum over bins k = r mod P of res[a][k].
+__global__ __launch_bounds__(256) void qn_layout_kernel(const float2* __restrict__ res, int A, int K, int P,
+                                                        float2* __restrict__ resT, float2* __restrict__ usT) {
+    const int a = blockIdx.x * 256 + threadIdx.x;
+    if (a >= A) return;
+    const int k0 = blockIdx.y * 16;
+    for (int k = k0; k < min(K, k0 + 16); ++k) resT[(size_t)k * A + a] = res[(size_t)a * K + k];
+    if (blockIdx.y == 0) {
+        for (int r = 0; r < P; ++r) {
+            float2 s = make_float2(0.f, 0.f);
+            for (int k = r; k < K; k += P) s += res[(size_t)a * K + k];
+            usT[(size_t)r * A + a] = s;
+        }
+    }
 }
 
 // Per-thread sums over apertures tid, tid + 256, ... then a float64 workgroup reduction;
@@ -48,7 +55,7 @@ __device__ __forceinline__ void qn_sums(const Chirp& c, const float* __restrict_
     for (int a = tid; a < A; a += kQnThreads) {
         const Pair q = pair_geometry(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cen[0], cen[1], cen[2], nor[0],
                                      nor[1], nor[2]);
-        const Sums z = bin_sums<P, (W > 0), 1>(c, T, res + (size_t)a * c.K, us + (size_t)a * P, q);
+        const Sums z = bin_sums<P, (W > 0)>(c, T, res + a, us + a, q, A);
         const float2 corr = cmulc(q.rot, z.g0);
         v[0] += q.amp * corr.x;
         v[1] += q.amp * corr.y;
@@ -290,7 +297,7 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
     }
 }
 
-size_t qn_workspace(int A, int P) { return sizeof(float2) * (size_t)A * P; }
+size_t qn_workspace(int A, int P, int K) { return sizeof(float2) * (size_t)A * (P + K); }
 
 template <int P, int W>
 static hipError_t launch_qn_pw(const Chirp& c, const float* pos, int A, const float2* res, const float2* us,
@@ -313,10 +320,13 @@ static hipError_t launch_qn_p(const Chirp& c, const float* pos, int A, const flo
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,
                             const float* nrm, const float* areas, int count, const QnSpec& sp, float* oc, float* on,
                             float* ov, void* workspace, hipStream_t st) {
-    float2* us = static_cast<float2*>(workspace);
-    hipLaunchKernelGGL(class_sums_kernel, dim3((A * c.P + 255) / 256), dim3(256), 0, st, res, A, c.K, c.P, us);
+    float2* us = static_cast<float2*>(workspace);  // [P][A]
+    float2* resT = us + (size_t)c.P * A;            // [K][A]
+    hipLaunchKernelGGL(qn_layout_kernel, dim3((A + 255) / 256, (c.K + 15) / 16), dim3(256), 0, st, res, A, c.K, c.P,
+                       resT, us);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    res = resT;
     switch (c.P) {
         case 1: return launch_qn_p<1>(c, pos, A, res, us, ctr, nrm, areas, count, sp, oc, on, ov, st);
         case 2: return launch_qn_p<2>(c, pos, A, res, us, ctr, nrm, areas, count, sp, oc, on, ov, st);

@@ -56,7 +56,7 @@ __device__ __forceinline__ void load_tile(const Chirp& c, const CorTile& tile, i
 // Bin sums of one pair (lane = aperture al of the tile). GRAD adds g1 and rx.
 template <int P, bool GRAD>
 __device__ __forceinline__ Sums pair_sums(const Chirp& c, const CorTile& tile, int al, const Pair& q) {
-    return bin_sums<P, GRAD, kCorApertures>(c, tile.T, tile.R + al, tile.Us + al, q);
+    return bin_sums<P, GRAD>(c, tile.T, tile.R + al, tile.Us + al, q, kCorApertures);
 }
 
 template <int NV>

@@ -45,6 +45,9 @@ class CertificateSearchConfig:
     backtracks: int = 8
 
 
+QN_MIN_BATCH = 256  # candidates per one-launch quasi-Newton search (one workgroup each)
+
+
 def candidate_grid(lower, upper, spacing, device="cuda"):
     """Regular grid including both box corners; the actual pitch never exceeds spacing."""
     axes = [
@@ -154,7 +157,10 @@ def search_certificate(
     coarse_scores = score(feasible) if initial_scores is None else initial_scores
     ids = select_peaks(coarse_scores, feasible["centers"], config.peaks, config.separation)
     coarse = {key: value[ids] for key, value in feasible.items()}
-    if batched and hasattr(model, "qn_search") and not center_spec.normalize:
+    # One workgroup per candidate fills the GPU only with enough candidates; fewer (the
+    # full-aperture refinement of a few dozen peaks) run the host loop, whose every launch
+    # spreads all apertures over the chip.
+    if batched and hasattr(model, "qn_search") and not center_spec.normalize and len(ids) >= QN_MIN_BATCH:
         spec = _qn_spec(controls, center_spec, normal_spec, masks, config, mode, residual)
         centers, normals, value = model.qn_search(coarse, residual, spec)
         return _rerank(coarse, dict(coarse, centers=centers, normals=normals), coarse_scores[ids], value, full_model,

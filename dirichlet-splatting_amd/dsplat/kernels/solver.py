@@ -22,3 +22,36 @@ def backward_elimination(inverse, b, count, limited=None, limit=0):
         _lib.ptr(flags), int(limit), _lib.ptr(removed), _lib.ptr(workspace), size, _lib.stream(inverse.device),
     )  # fmt: skip
     return removed
+
+
+def _native(h):
+    return h.is_cuda and h.dtype == torch.float64 and h.ndim == 2 and h.shape[0] == h.shape[1]
+
+
+def cholesky_ex(h):
+    """(L, info) like torch.linalg.cholesky_ex for a real float64 [n, n] device matrix: L lower
+    with zeros above the diagonal, info a 0-d int32 tensor (first non-positive pivot, 1-based;
+    0 on success). Other dtypes and shapes go to torch.linalg.cholesky_ex."""
+    if not _native(h):
+        return torch.linalg.cholesky_ex(h)
+    factor = h.contiguous().clone()
+    info = torch.empty((), dtype=torch.int32, device=h.device)
+    n = len(factor)
+    _lib.call("dsplat_cholesky", _lib.ptr(factor), n, n, _lib.ptr(info), _lib.stream(h.device))
+    return factor, info
+
+
+def cholesky_solve(rhs, factor):
+    """torch.cholesky_solve(rhs, factor) (lower factor) for real float64 [n, m] right-hand
+    sides in one workgroup; other cases go to torch. (The solver path keeps
+    torch.cholesky_solve, whose multi-workgroup products are faster at K ~ 2,000.)"""
+    n = len(factor)
+    if not (_native(factor) and rhs.dtype == torch.float64 and rhs.ndim == 2 and n <= 8192):
+        return torch.cholesky_solve(rhs, factor)
+    # [m, n], a copy (the kernel works in place): column j of the solution is row j
+    x = rhs.t().clone(memory_format=torch.contiguous_format)
+    _lib.call(
+        "dsplat_cholesky_solve", _lib.ptr(factor.contiguous()), n, n, _lib.ptr(x), rhs.shape[1], n,
+        _lib.stream(factor.device),
+    )  # fmt: skip
+    return x.t()

@@ -14,6 +14,7 @@ from .certificate import certificate
 from .kernels import solver as elimination
 from .parameters import periodic_delta
 from .kernels.blas import gemm_tn
+from .kernels.solver import cholesky_ex
 from .varpro import (
     FOLD, SYRK_MODE, Fit, accept, fit_coefficients, rendered_loss, row_chunk, screen_bar, shifted_cholesky, varpro,
 )
@@ -36,7 +37,7 @@ def utility(gram, coefficients):
     redundant exactly when its coordinate takes part in the null space, and the other
     columns use the pseudoinverse (replacement.py:46-59).
     """
-    factor, info = torch.linalg.cholesky_ex(gram)
+    factor, info = cholesky_ex(gram)
     diag = gram.diag().real
     if int(info) == 0:
         inverse_diag = torch.cholesky_inverse(factor).diag().real

@@ -19,6 +19,7 @@ import torch
 
 from .kernels.blas import syrk
 from .kernels.reduce import energy
+from .kernels.solver import cholesky_ex
 
 
 class Fit(NamedTuple):
@@ -55,7 +56,7 @@ def shifted_cholesky(h, floor):
     shift = floor * largest_eigenvalue(matrix)
     for _ in range(SHIFT_RETRIES):
         matrix.diagonal().add_(shift)
-        factor, info = torch.linalg.cholesky_ex(matrix)
+        factor, info = cholesky_ex(matrix)
         if int(info) == 0:
             return factor
         shift = shift * 9
@@ -67,7 +68,7 @@ def solve(h, rhs):
     truncates, len(h) eps lambda_max, which moves x only along near-null directions
     (varpro.py:62-79). An all-zero system gives zeros."""
     q = rhs[:, None] if rhs.ndim == 1 else rhs
-    factor, info = torch.linalg.cholesky_ex(h)
+    factor, info = cholesky_ex(h)
     if int(info) != 0:
         factor = shifted_cholesky(h, len(h) * torch.finfo(h.real.dtype).eps)
         if factor is None:

@@ -289,6 +289,19 @@ int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num
  * FP32 accumulation, float64 split-K reduction. Deterministic. Workspace from
  * dsplat_syrk_workspace. Replaces kernels/cublas.py:31-44 (syrk: cublasSsyrk_v2 through
  * ctypes). */
+/* Float64 Cholesky factor of the K x K systems, in place: on return the lower triangle of a
+ * (row-major, leading dimension lda) holds L with A = L L^T and the strictly upper part is
+ * zero; only the lower triangle is read. *info (device int) = 1-based index of the first
+ * non-positive pivot, 0 on success. Replaces torch.linalg.cholesky_ex (cuSOLVER potrf) of
+ * varpro.py:44-79 (solve, shifted_cholesky). */
+int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stream);
+
+/* Solve L L^T X = B in place for nrhs right-hand sides, column j of B at b + j * ldb, with
+ * L from dsplat_cholesky; n <= 8192. Replaces torch.cholesky_solve (cuSOLVER potrs) of
+ * varpro.py:62-79. */
+int dsplat_cholesky_solve(const double* l, int32_t n, int32_t lda, double* b, int32_t nrhs, int32_t ldb,
+                          void* stream);
+
 /* Backward elimination of `count` unknowns from a least-squares solution, in one launch:
  * inverse [n, n] float64 (of the regularized real Gram), b [n] = inverse q. Each step
  * removes the unknown with the smallest b_j^2 / W_jj (ties to the lower index) among those
