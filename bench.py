@@ -23,13 +23,35 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "dirichlet-splatting_amd"))
 
-# Algorithmic FP32 work per (a, k, s) evaluation: the cot-form kernel.
+# Algorithmic FP32 work per (a, k, s) evaluation.
+# Bin-domain (cotangent) kernels, VALU:
 #   forward 13 FLOP: cot by angle addition (3 + 3), cot (1), complex accumulate (4 + 2)
 #   VJP     19 FLOP: the forward's 13 plus the (1 + cot^2) u class sum (6)
 #   scan    11 FLOP: cot by angle addition (5), residual accumulate (4), |cot|^2 (2)
+# Sample-domain kernels, MFMA (128 samples = 128 bins at pad 2, so per (a, n, s) = per eval):
+#   forward  8 FLOP: one complex MAC of X[a, n] += (c v^n)_s
+#   VJP     16 FLOP: two complex MACs, sum w^n U_n and sum n w^n U_n
 FWD_FLOP, VJP_FLOP, SCAN_FLOP = 13, 19, 11
-PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector peak
+TD_FWD_FLOP, TD_VJP_FLOP = 8, 16
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak (vector and dense MFMA)
 PEAK_HBM_GBS = 8000.0
+TRAFFIC = ROOT / "profiles" / "splat_traffic_128x128x2048.json"
+
+
+def measured_traffic(kernels):
+    """HBM bytes per launch of a kernel group from the committed rocprofv3 --pmc record
+    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by what the same counters read
+    for a plain copy of known size), or None if the record does not cover the group."""
+    if not TRAFFIC.exists():
+        return None
+    record = json.loads(TRAFFIC.read_text())
+    try:
+        per = [record["kernels"][k] for k in kernels]
+    except KeyError:
+        return None
+    fetch = sum(p["fetch_kib"] for p in per) * record["fetch_scale"]
+    write = sum(p["write_kib"] for p in per) * record["write_scale"]
+    return 1024.0 * (fetch + write)
 
 
 def parse():
@@ -452,9 +474,18 @@ def main():
     vjp_ms = sum(m[1].elapsed_time(m[2]) for m in marks) / args.steps
     ms_per_step = wall * 1e3 / args.steps
 
+    from dsplat import _lib
+
+    sample_domain = _lib.library().dsplat_splat_form() == 0 and cfg.adc_samples in (128, 256)
+    fwd_flop, vjp_flop = (TD_FWD_FLOP, TD_VJP_FLOP) if sample_domain else (FWD_FLOP, VJP_FLOP)
+    groups = {  # the launches each timed call makes
+        "render": ["td_render_kernel<8, false, false>"] if sample_domain else ["render_kernel<2, 8>"],
+        "vjp": ["dft_rows_kernel", "td_vjp_kernel<8, false>", "vjp_reduce"] if sample_domain
+        else ["vjp_kernel<2>", "vjp_reduce"],
+    }  # fmt: skip
     kernels = {
-        "render": (fwd_ms, FWD_FLOP * evals),
-        "vjp": (vjp_ms, VJP_FLOP * evals),
+        "render": (fwd_ms, fwd_flop * evals),
+        "vjp": (vjp_ms, vjp_flop * evals),
     }
     name = max(kernels, key=lambda k: kernels[k][0])
     ms, flop = kernels[name]
@@ -482,15 +513,17 @@ def main():
             "parallelism": "single GPU",
         },
         "kernels_ms": {"render": fwd_ms, "vjp": vjp_ms},
+        "splat_form": "sample-domain MFMA" if sample_domain else "bin-domain VALU",
         "roofline": {
             "kernel": name,
-            "bound": "valu",
+            "launches": groups[name],
+            "bound": "mfma" if sample_domain else "valu",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "flop_per_eval": FWD_FLOP if name == "render" else VJP_FLOP,
+            "traffic": measured_traffic(groups[name]),
+            "flop_per_eval": fwd_flop if name == "render" else vjp_flop,
         },
     }
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
