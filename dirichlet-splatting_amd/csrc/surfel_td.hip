@@ -427,11 +427,71 @@ __device__ __forceinline__ void td_row(int tile, int q, int h, int r, int* mat, 
     }
 }
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// A operands of the VJP GEMM for v_mfma_f32_32x32x16_f16 (A[i][8 h + e] -> lane i + 32 h,
+// element e), as FP16 hi + lo splits of the per-aperture scaled values, built once per
+// aperture from U: [A][T tiles][2 K halves][2 (hi, lo)][64 lanes] x 8 halves, and the scale
+// exponent e_a per aperture (values were multiplied by 2^-e_a, max |value| below 2^12).
+// Row i -> (mat, j, part) as the accumulator layout (td_row); K = 2 t + k over t < 16 and
+// k = 0 (B = Re W_t) or 1 (B = Im W_t): (re) Re U, (im) Im U for k = 0; (re) -Im U, (im) Re U
+// for k = 1, with U = U[16 j + t] (mat 0) or (16 j + t) U[16 j + t] (mat 1).
+template <int J>
+__global__ __launch_bounds__(64 * (J / 8)) void vjp_operands_kernel(const float2* __restrict__ U, int S,
+                                                                    uint4* __restrict__ out,
+                                                                    float* __restrict__ scale_exp) {
+    constexpr int T = J / 8;
+    __shared__ float red[T];
+    const int a = blockIdx.x, tid = threadIdx.x, tile = tid >> 6, L = tid & 63;
+    const int row = L & 31, h = L >> 5;
+    int mat, j;
+    td_row<J>(tile, row >> 3, (row >> 2) & 1, row & 3, &mat, &j);
+    const int part = row & 1;
+    float v[16];
+    float mx = 0.0f;
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int K = 16 * kh + 8 * h + e, t = K >> 1, k = K & 1;
+            const int n = 16 * j + t;
+            float2 z = U[(size_t)a * S + n];
+            if (mat) z = (float)n * z;
+            const float x = k == 0 ? (part ? z.y : z.x) : (part ? z.x : -z.y);
+            v[8 * kh + e] = x;
+            mx = fmaxf(mx, fabsf(x));
+        }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if (L == 0) red[tile] = mx;
+    __syncthreads();
+    for (int t = 0; t < T; ++t) mx = fmaxf(mx, red[t]);
+    int ex = 0;
+    if (mx > 0.0f) frexpf(mx, &ex);  // mx < 2^ex
+    ex -= 12;
+    if (tid == 0) scale_exp[a] = (float)ex;
+    const float sc = ldexpf(1.0f, -ex);
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh) {
+        f16x8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float x = v[8 * kh + e] * sc;
+            const _Float16 xh = (_Float16)x;
+            hi[e] = xh;
+            lo[e] = (_Float16)(x - (float)xh);
+        }
+        uint4* dst = out + ((((size_t)a * T + tile) * 2 + kh) * 2) * 64 + L;
+        dst[0] = __builtin_bit_cast(uint4, hi);
+        dst[64] = __builtin_bit_cast(uint4, lo);
+    }
+}
+
 // What one lane carries from the generation of its pair to the contraction and chain rule.
 template <int J>
 struct VjpStage {
-    float B0[16], B1[16];  // MFMA B operands of apertures a and a + 1 (after the half swap)
-    float2 Z[J];           // Z_j of the own pair
+    f16x8 Bh[2][2], Bl[2][2];  // [aperture a / a + 1][K half]: FP16 hi / lo of the B operands
+    float2 Z[J];               // Z_j of the own pair
     float2 rot;
     float amp, amp_cos, inv_r, ux, uy, uz, cosine, rate;
 };
@@ -462,31 +522,49 @@ __device__ __forceinline__ void vjp_stage(const Chirp& c, const float2* E, const
     st.Z[1] = z1;
 #pragma unroll
     for (int j = 2; j < J; ++j) st.Z[j] = (j & 1) ? cmul(st.Z[j - 1], z1) : csq(st.Z[j >> 1]);
-    // B operands: half 0 holds Re W of aperture a (own) and of a + 1, half 1 holds Im W of
-    // both: one half swap per t.
+    // B operands: lane half h supplies K = 16 kh + 8 h + e, i.e. W_t with t = 8 kh + 4 h + e / 2
+    // (Re for even e, Im for odd e). Half h generated aperture a + h; one half swap of
+    // (W_{8 kh + q}, W_{8 kh + 4 + q}) gives each half its t of aperture a and of a + 1.
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        st.B0[t] = W[t].x;
-        st.B1[t] = W[t].y;
-        swap_halves(st.B0[t], st.B1[t]);
-    }
+    for (int kh = 0; kh < 2; ++kh)
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq) {
+            float xr = W[8 * kh + qq].x, yr = W[8 * kh + 4 + qq].x;
+            float xi = W[8 * kh + qq].y, yi = W[8 * kh + 4 + qq].y;
+            swap_halves(xr, yr);
+            swap_halves(xi, yi);
+            const float vals[2][2] = {{xr, xi}, {yr, yi}};  // [aperture][re, im]
+#pragma unroll
+            for (int ap = 0; ap < 2; ++ap)
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const float x = vals[ap][p];
+                    const _Float16 xh = (_Float16)x;
+                    st.Bh[ap][kh][2 * qq + p] = xh;
+                    st.Bl[ap][kh][2 * qq + p] = (_Float16)(x - (float)xh);
+                }
+        }
 }
 
 // One workgroup: 4 waves x 32 surfels, an aperture range staged G apertures at a time. Each
-// step covers apertures (a, a + 1): the two lane halves generate one pair each, then 32 MFMAs,
-// the contraction over j and the chain rule.
+// step covers apertures (a, a + 1): the two lane halves generate one pair each, then
+// 3 x 2 x T FP16 hi + lo MFMAs per aperture (hi hi + hi lo + lo hi, FP32 accumulation), the
+// contraction over j and the chain rule.
 template <int J, bool FRESNEL>
 __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float* __restrict__ pos, int A,
                                                      const float* __restrict__ ctr,
                                                      const float* __restrict__ nrm,
                                                      const float2* __restrict__ w, int S,
-                                                     const float2* __restrict__ U, int per_range,
+                                                     const uint4* __restrict__ Aexp,
+                                                     const float* __restrict__ scale_exp, int per_range,
                                                      float* __restrict__ partial) {
     constexpr int T = J / 8;  // 32-row tiles per aperture
-    constexpr int G = kTdVjpGroup;
+    constexpr int G = kTdVjpGroup / T;
+    constexpr int PER = T * 2 * 2 * 64;  // uint4 per aperture
     extern __shared__ float smem[];
-    float2* E = reinterpret_cast<float2*>(smem);  // [N]
-    float4* Aop = reinterpret_cast<float4*>(E + c.N);  // [G][T][4 t-groups][64 lanes] float4
+    float2* E = reinterpret_cast<float2*>(smem);         // [N]
+    float* Sc = reinterpret_cast<float*>(E + c.N);       // [G] scales 2^e_a
+    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);      // [G][T][2][2][64]
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
     const int s = blockIdx.x * kTdVjpSurfels + 32 * wave + m;
@@ -503,32 +581,15 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
 
     for (int g0 = a_begin; g0 < a_end; g0 += G) {
         __syncthreads();
-        // Stage A operands: value(t, lane L) for L = row + 32 k, row -> (mat, j, part). The trip
-        // count is a constant, so all loads of a thread are in flight together.
+        // Stage the group's operands: a straight copy (apertures past the range are zeroed).
 #pragma unroll
-        for (int it = 0; it < G * T * 4 * 64 / 256; ++it) {
+        for (int it = 0; it < G * PER / 256; ++it) {
             const int e = tid + 256 * it;
-            const int L = e & 63, tg = (e >> 6) & 3, tile = (e >> 8) % T, ga = e / (256 * T);
-            const int a = g0 + ga;
-            const int row = L & 31, k = L >> 5;
-            const int q = row >> 3, hh = (row >> 2) & 1, r = row & 3;
-            int mat, j;
-            td_row<J>(tile, q, hh, r, &mat, &j);
-            const int part = r & 1;
-            float vals[4] = {0.f, 0.f, 0.f, 0.f};
-            if (a < a_end) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int t = 4 * tg + u;
-                    const int n = 16 * j + t;
-                    float2 z = U[(size_t)a * c.S + n];
-                    if (mat) z = (float)n * z;
-                    // k = 0 (B = Re W): (re) Re U, (im) Im U;  k = 1 (B = Im W): (re) -Im U, (im) Re U
-                    vals[u] = k == 0 ? (part ? z.y : z.x) : (part ? z.x : -z.y);
-                }
-            }
-            Aop[e] = make_float4(vals[0], vals[1], vals[2], vals[3]);
+            const int a = g0 + e / PER;
+            const uint4 x = Aexp[(size_t)min(a, A - 1) * PER + e % PER];
+            Aop[e] = a < a_end ? x : make_uint4(0u, 0u, 0u, 0u);
         }
+        if (tid < G) Sc[tid] = ldexpf(1.0f, (int)scale_exp[min(g0 + tid, A - 1)]);
         __syncthreads();
         for (int ga = 0; ga < G; ga += 2) {
             VjpStage<J> cur;
@@ -536,31 +597,26 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
                 const int a = g0 + ga + h;  // this lane's pair: (a, s)
                 vjp_stage<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
             }
-            f32x16 acc0[T], acc1[T];
+            f32x16 acc[2][T];
 #pragma unroll
-            for (int tile = 0; tile < T; ++tile)
+            for (int ap = 0; ap < 2; ++ap)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    acc0[tile][r] = 0.f;
-                    acc1[tile][r] = 0.f;
-                }
-            const float4* A0 = Aop + ((ga * T) * 4) * 64 + lane;
-            const float4* A1 = Aop + (((ga + 1) * T) * 4) * 64 + lane;
+                for (int tile = 0; tile < T; ++tile)
 #pragma unroll
-            for (int tile = 0; tile < T; ++tile)
+                    for (int r = 0; r < 16; ++r) acc[ap][tile][r] = 0.f;
 #pragma unroll
-                for (int tg = 0; tg < 4; ++tg) {
-                    const float4 x0 = A0[(tile * 4 + tg) * 64];
-                    const float4 x1 = A1[(tile * 4 + tg) * 64];
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.x, cur.B0[4 * tg], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.x, cur.B1[4 * tg], acc1[tile], 0, 0, 0);
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.y, cur.B0[4 * tg + 1], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.y, cur.B1[4 * tg + 1], acc1[tile], 0, 0, 0);
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.z, cur.B0[4 * tg + 2], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.z, cur.B1[4 * tg + 2], acc1[tile], 0, 0, 0);
-                    acc0[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0.w, cur.B0[4 * tg + 3], acc0[tile], 0, 0, 0);
-                    acc1[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1.w, cur.B1[4 * tg + 3], acc1[tile], 0, 0, 0);
-                }
+            for (int ap = 0; ap < 2; ++ap)
+#pragma unroll
+                for (int tile = 0; tile < T; ++tile)
+#pragma unroll
+                    for (int kh = 0; kh < 2; ++kh) {
+                        const uint4* src = Aop + (((ga + ap) * T + tile) * 2 + kh) * 2 * 64 + lane;
+                        const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
+                        const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
+                        acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap][kh], acc[ap][tile], 0, 0, 0);
+                        acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap][kh], acc[ap][tile], 0, 0, 0);
+                        acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap][kh], acc[ap][tile], 0, 0, 0);
+                    }
             // Contraction P_mat = sum_j Z_j M_mat[j]. Half h holds the rows of j in its set J_h
             // for both apertures; one half swap of (Z_{j of half 0}, Z_{j of half 1}) gives each
             // half Z_{j in J_h} of aperture a and of aperture a + 1.
@@ -577,13 +633,17 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
                     float2 za0 = cur.Z[j0], za1 = cur.Z[j1];
                     swap_halves(za0.x, za1.x);
                     swap_halves(za0.y, za1.y);
-                    P[0][mat] += cmul(za0, make_float2(acc0[tile][rr], acc0[tile][rr + 1]));
-                    P[1][mat] += cmul(za1, make_float2(acc1[tile][rr], acc1[tile][rr + 1]));
+                    P[0][mat] += cmul(za0, make_float2(acc[0][tile][rr], acc[0][tile][rr + 1]));
+                    P[1][mat] += cmul(za1, make_float2(acc[1][tile][rr], acc[1][tile][rr + 1]));
                 }
-            // Each half completes the sums of its own pair's aperture (a + h) with the other
-            // half's share: after the swap, P[0] + P[1] is that total on both halves.
+            // undo the per-aperture operand scales, then each half completes the sums of its own
+            // pair's aperture (a + h) with the other half's share: after the swap, P[0] + P[1]
+            // is that total on both halves.
+            const float s0 = Sc[ga], s1 = Sc[ga + 1];
 #pragma unroll
             for (int mat = 0; mat < 2; ++mat) {
+                P[0][mat] = s0 * P[0][mat];
+                P[1][mat] = s1 * P[1][mat];
                 swap_halves(P[0][mat].x, P[1][mat].x);
                 swap_halves(P[0][mat].y, P[1][mat].y);
             }
@@ -697,19 +757,38 @@ TdVjpShape td_vjp_shape(int A, int S) {
     return v;
 }
 
-size_t td_vjp_workspace(const Chirp& c, int A, int S) {
+// Workspace: partials [S][ranges][8] floats | U [A][S] complex | A operands [A][T][2][2][64]
+// uint4 | scale exponents [A] floats, each part 256-byte aligned.
+struct TdVjpWorkspace {
+    size_t partial, u, ops, scales, total;
+};
+
+static TdVjpWorkspace td_vjp_layout(const Chirp& c, int A, int S) {
     const TdVjpShape v = td_vjp_shape(A, S);
-    return sizeof(float) * 8 * (size_t)S * v.ranges + sizeof(float2) * (size_t)A * c.S + 256;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    TdVjpWorkspace w;
+    w.partial = 0;
+    w.u = up(sizeof(float) * 8 * (size_t)S * v.ranges);
+    w.ops = w.u + up(sizeof(float2) * (size_t)A * c.S);
+    w.scales = w.ops + up(sizeof(uint4) * (size_t)A * (c.S / 128) * 2 * 2 * 64);
+    w.total = w.scales + up(sizeof(float) * (size_t)A);
+    return w;
 }
+
+size_t td_vjp_workspace(const Chirp& c, int A, int S) { return td_vjp_layout(c, A, S).total; }
 
 template <int J>
 static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                   const float2* w, int S, const float2* up, void* ws, float* gc, float* gn,
                                   float* gw, hipStream_t st) {
+    constexpr int T = J / 8;
     const TdVjpShape v = td_vjp_shape(A, S);
-    float* partial = static_cast<float*>(ws);
-    const size_t part_bytes = (sizeof(float) * 8 * (size_t)S * v.ranges + 255) / 256 * 256;
-    float2* U = reinterpret_cast<float2*>(static_cast<char*>(ws) + part_bytes);
+    const TdVjpWorkspace lay = td_vjp_layout(c, A, S);
+    char* base = static_cast<char*>(ws);
+    float* partial = reinterpret_cast<float*>(base + lay.partial);
+    float2* U = reinterpret_cast<float2*>(base + lay.u);
+    uint4* ops = reinterpret_cast<uint4*>(base + lay.ops);
+    float* scales = reinterpret_cast<float*>(base + lay.scales);
     // U = DFT of the cotangent rows: K bins -> S samples, sign +1
     {
         const size_t smem = sizeof(float2) * (c.N + 16 * (size_t)(c.K + 1));
@@ -720,16 +799,20 @@ static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    const size_t smem = sizeof(float2) * c.N + sizeof(float4) * (size_t)kTdVjpGroup * (J / 8) * 4 * 64;
+    hipLaunchKernelGGL(vjp_operands_kernel<J>, dim3(A), dim3(64 * T), 0, st, U, c.S, ops, scales);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t smem = sizeof(float2) * c.N + sizeof(float) * 16 +
+                        sizeof(uint4) * (size_t)(kTdVjpGroup / T) * T * 2 * 2 * 64;
     const void* fn = c.fresnel_n > 1.0f ? (const void*)td_vjp_kernel<J, true> : (const void*)td_vjp_kernel<J, false>;
-    hipError_t e = allow_lds(fn, smem);
+    e = allow_lds(fn, smem);
     if (e != hipSuccess) return e;
     if (c.fresnel_n > 1.0f)
         hipLaunchKernelGGL((td_vjp_kernel<J, true>), dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A, ctr,
-                           nrm, w, S, U, v.per_range, partial);
+                           nrm, w, S, ops, scales, v.per_range, partial);
     else
         hipLaunchKernelGGL((td_vjp_kernel<J, false>), dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A, ctr,
-                           nrm, w, S, U, v.per_range, partial);
+                           nrm, w, S, ops, scales, v.per_range, partial);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_vjp_reduce(partial, S, v.ranges, gc, gn, gw, st);
