@@ -34,6 +34,9 @@ sys.path.insert(0, str(ROOT / "dirichlet-splatting_amd"))
 #           products each, FP32 accumulation: FP32-accurate, so priced against the FP32 peak)
 FWD_FLOP, VJP_FLOP, SCAN_FLOP = 13, 19, 11
 TD_FWD_FLOP, TD_VJP_FLOP = 8, 16
+#   scan     8 FLOP: one complex MAC of sum w^n R_n per (candidate, aperture, sample); the norm
+#           polynomial rides in the same MFMA tile (FP16 hi + lo, FP32-accurate)
+TD_SCAN_FLOP = 8
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak (vector and dense MFMA)
 PEAK_HBM_GBS = 8000.0
 TRAFFIC = ROOT / "profiles" / "splat_traffic_128x128x2048.json"
@@ -257,7 +260,11 @@ def scan_workload(args):
     ms_per_step = (time.perf_counter() - t0) * 1e3 / steps
     kernel_ms, _ = timed(lambda: surfel.scan_batched(cfg, scans, centers, normals, 0.0), 3, stream)
     single_ms, _ = timed(lambda: [surfel.correlate(cfg, p, centers, normals, r, 0.0) for p, r in scans], 3, stream)
-    achieved = SCAN_FLOP * evals / (kernel_ms * 1e-3) / 1e12
+    from dsplat import _lib
+
+    sample_domain = _lib.library().dsplat_splat_form() == 0 and cfg.adc_samples == 128
+    scan_flop = TD_SCAN_FLOP if sample_domain else SCAN_FLOP
+    achieved = scan_flop * evals / (kernel_ms * 1e-3) / 1e12
     # the top candidates lie near the surface: distance of the best per scan to the nearest surfel
     best = centers[ids[:, :16].flatten()]
     near = torch.cdist(best, sc).min(1).values.reshape(3, 16).median(1).values
@@ -280,9 +287,12 @@ def scan_workload(args):
         "kernels_ms": {"scan_batched": kernel_ms, "one_launch_per_scan": single_ms},
         "autotune": tuning,
         "median_distance_top16_to_surface_m": [float(x) for x in near],
-        "roofline": {"kernel": "scan_kernel<2>", "bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "flop_per_eval": SCAN_FLOP},
+        "roofline": {"kernel": "scan_td_kernel<false>" if sample_domain else "scan_kernel<2>",
+                     "bound": "mfma" if sample_domain else "valu", "achieved": achieved,
+                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": None, "flop_per_eval": scan_flop,
+                     "peak_basis": "FP32 dense (FP32-accurate; FP16 hi+lo x3 MFMA)" if sample_domain
+                     else "FP32 vector"},
     }
     if not args.no_cpu:
         from oracle import surfel as O

@@ -502,8 +502,18 @@ static bool make_scanset(int num_scans, const float* const* positions, const flo
     return true;
 }
 
-size_t dsplat_scan_workspace(int32_t num_scans, const int32_t* apertures, int32_t num_candidates) {
-    if (num_scans < 1 || num_scans > 8 || !apertures || num_candidates <= 0) return 0;
+size_t dsplat_scan_workspace(const dsplat_fmcw* cfg, int32_t num_scans, const int32_t* apertures,
+                             int32_t num_candidates) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_scans < 1 || num_scans > 8 || !apertures || num_candidates <= 0) return 0;
+    if (td_scan_supported(c)) {
+        static const float dummy = 0.0f;
+        const float* p[kMaxScans];
+        for (int i = 0; i < kMaxScans; ++i) p[i] = &dummy;
+        ScanSet set;
+        if (!make_scanset(num_scans, p, p, apertures, &set)) return 0;
+        return td_scan_workspace(c, set, num_candidates);
+    }
     size_t blocks = 0;
     for (int i = 0; i < num_scans; ++i) blocks += (size_t)(apertures[i] + 31) / 32;
     return sizeof(float) * 3 * blocks * (size_t)num_candidates;
@@ -519,7 +529,9 @@ int dsplat_scan_batched(const dsplat_fmcw* cfg, int32_t num_scans, const float* 
         return hipErrorInvalidValue;
     if (num_candidates == 0) return hipSuccess;
     if (!centers || !normals || !out || !workspace) return hipErrorInvalidValue;
-    if (workspace_bytes < dsplat_scan_workspace(num_scans, apertures, num_candidates)) return hipErrorInvalidValue;
+    if (workspace_bytes < dsplat_scan_workspace(cfg, num_scans, apertures, num_candidates)) return hipErrorInvalidValue;
+    if (td_scan_supported(c))  // chunk = apertures per range here
+        return launch_td_scan(c, set, centers, normals, num_candidates, chunk, workspace, out, (hipStream_t)stream);
     if (chunk <= 0) {
         const int groups = (4096 + set.first_block[num_scans] - 1) / set.first_block[num_scans];
         chunk = (num_candidates + groups - 1) / groups;

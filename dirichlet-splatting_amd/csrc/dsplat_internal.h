@@ -40,6 +40,11 @@ struct ScanSet {
 hipError_t launch_scan(const Chirp& c, const ScanSet& set, const float* ctr, const float* nrm, int S, int chunk,
                        float* partial, float* out, hipStream_t st);
 size_t cor_smem(const Chirp& c);
+// Sample-domain scan (S = 128): workspace bytes and launch (chunk = apertures per range).
+bool td_scan_supported(const Chirp& c);
+size_t td_scan_workspace(const Chirp& c, const ScanSet& set, int C);
+hipError_t launch_td_scan(const Chirp& c, const ScanSet& set, const float* ctr, const float* nrm, int C, int chunk,
+                          void* ws, float* out, hipStream_t st);
 
 hipError_t launch_design(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float* areas, int S, int complex_layout, float* out, hipStream_t st);

@@ -826,4 +826,290 @@ hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* c
     return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------------------
+// Certificate scan in the sample domain (S = 128): per (candidate, aperture)
+//   corr = amp conj(rot) g0,  g0 = (1/S) sum_n w^n R[a, n]   (R = the residual row taken to
+//   the sample domain), and norm = amp^2 f(k0), f(k0) = sum_{k<K} |D(k - k0)|^2.
+// f is a trigonometric polynomial of degree S - 1 in v = e^{2 pi i k0 / N}:
+//   f = 2 Re sum_{m=0}^{S-1} phi_m v^m,  phi_0 = K / (2 S),  phi_m = (S - m) E_m / S^2,
+//   E_m = sum_{k<K} e^{-2 pi i m k / N} = 1 - i cot(pi m / N) for odd m, 0 for even m (K = N / 2),
+// so with psi = conj(phi) it is 2 Re sum_j Z_j sum_i W_i psi_{16 j + i}: the same product as g0
+// with psi in place of R. The 32 accumulator rows per aperture hold both: rows of mat 0 are
+// the residual's (scaled by 2^-e_a per aperture), rows of mat 1 the fixed psi (scaled by 2^10).
+// ---------------------------------------------------------------------------------------
+constexpr int kScanPsiExp = 10;
+constexpr int kScanCandidates = 128;  // per workgroup (4 waves x 32)
+constexpr int kScanMinRange = 512;    // apertures per range at least (bounds the partials)
+
+struct TdScanSet {
+    int count;
+    const float* positions[kMaxScans];
+    int apertures[kMaxScans];
+    int ap_off[kMaxScans + 1];       // prefix sums of apertures
+    int range_first[kMaxScans + 1];  // prefix sums of aperture ranges
+    int per_range;
+};
+
+__global__ __launch_bounds__(64) void scan_operands_kernel(const float2* __restrict__ U, int S, int N, int K,
+                                                           uint4* __restrict__ out, float* __restrict__ scale_exp) {
+    const int a = blockIdx.x, L = threadIdx.x;
+    const int row = L & 31, h = L >> 5;
+    int mat, j;
+    td_row<8>(0, row >> 3, (row >> 2) & 1, row & 3, &mat, &j);
+    const int part = row & 1;
+    float v[16];
+    float mx = 0.0f;
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int Kx = 16 * kh + 8 * h + e, t = Kx >> 1, k = Kx & 1;
+            const int m = 16 * j + t;
+            float2 z;
+            if (mat == 0) {
+                z = U[(size_t)a * S + m];
+            } else {  // psi_m = conj(phi_m)
+                double re = 0.0, im = 0.0;
+                if (m == 0) {
+                    re = (double)K / (2.0 * S);
+                } else if (m & 1) {
+                    double sn, cs;
+                    sincospi((double)m / (double)N, &sn, &cs);
+                    const double w = (double)(S - m) / ((double)S * S);
+                    re = w;
+                    im = w * cs / sn;
+                }
+                z = make_float2((float)ldexp(re, kScanPsiExp), (float)ldexp(im, kScanPsiExp));
+            }
+            const float x = k == 0 ? (part ? z.y : z.x) : (part ? z.x : -z.y);
+            v[8 * kh + e] = x;
+            if (mat == 0) mx = fmaxf(mx, fabsf(x));
+        }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    int ex = 0;
+    if (mx > 0.0f) frexpf(mx, &ex);
+    ex -= 12;
+    if (L == 0) scale_exp[a] = (float)ex;
+    const float sc = mat == 0 ? ldexpf(1.0f, -ex) : 1.0f;
+#pragma unroll
+    for (int kh = 0; kh < 2; ++kh) {
+        f16x8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float x = v[8 * kh + e] * sc;
+            const _Float16 xh = (_Float16)x;
+            hi[e] = xh;
+            lo[e] = (_Float16)(x - (float)xh);
+        }
+        uint4* dst = out + (((size_t)a * 2 + kh) * 2) * 64 + L;
+        dst[0] = __builtin_bit_cast(uint4, hi);
+        dst[64] = __builtin_bit_cast(uint4, lo);
+    }
+}
+
+// grid (candidate tiles of 128, aperture ranges of every scan back to back); partial
+// [C][ranges][3] = (corr.re, corr.im, norm) of the range.
+template <bool FRESNEL>
+__global__ __launch_bounds__(256) void scan_td_kernel(const Chirp c, const TdScanSet set,
+                                                      const float* __restrict__ ctr, const float* __restrict__ nrm,
+                                                      int C, const uint4* __restrict__ ops,
+                                                      const float* __restrict__ scale_exp,
+                                                      float* __restrict__ partial) {
+    constexpr int J = 8, G = kTdVjpGroup, PER = 2 * 2 * 64;
+    extern __shared__ float smem[];
+    float2* E = reinterpret_cast<float2*>(smem);     // [N]
+    float* Sc = reinterpret_cast<float*>(E + c.N);   // [G] scales 2^e_a
+    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);  // [G][2][2][64]
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
+    const int s = blockIdx.x * kScanCandidates + 32 * wave + m;
+    const bool s_ok = s < C;
+    const int sc = min(s, C - 1);
+    const int range = blockIdx.y, nrange = gridDim.y;
+    int k = 0;
+    while (k + 1 < set.count && range >= set.range_first[k + 1]) ++k;
+    const int A = set.apertures[k];
+    const float* __restrict__ pos = set.positions[k];
+    const uint4* __restrict__ kops = ops + (size_t)set.ap_off[k] * PER;
+    const float* __restrict__ kexp = scale_exp + set.ap_off[k];
+    const int a_begin = (range - set.range_first[k]) * set.per_range;
+    const int a_end = min(A, a_begin + set.per_range);
+    fill_twiddles(E, c.N, tid, 256);
+
+    const float cx = ctr[3 * sc], cy = ctr[3 * sc + 1], cz = ctr[3 * sc + 2];
+    const float nx = nrm[3 * sc], ny = nrm[3 * sc + 1], nz = nrm[3 * sc + 2];
+    const float psi_scale = ldexpf(1.0f, -kScanPsiExp);
+    float acc_re = 0.f, acc_im = 0.f, acc_n = 0.f;
+
+    for (int g0 = a_begin; g0 < a_end; g0 += G) {
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < G * PER / 256; ++it) {
+            const int e = tid + 256 * it;
+            const int a = g0 + e / PER;
+            const uint4 x = kops[(size_t)min(a, A - 1) * PER + e % PER];
+            Aop[e] = a < a_end ? x : make_uint4(0u, 0u, 0u, 0u);
+        }
+        if (tid < G) Sc[tid] = ldexpf(1.0f, (int)kexp[min(g0 + tid, A - 1)]);
+        __syncthreads();
+        for (int ga = 0; ga < G; ga += 2) {
+            VjpStage<J> cur;
+            {
+                const int a = g0 + ga + h;  // this lane's pair: (a, s)
+                vjp_stage<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
+            }
+            f32x16 acc[2];
+#pragma unroll
+            for (int ap = 0; ap < 2; ++ap) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[ap][r] = 0.f;
+#pragma unroll
+                for (int kh = 0; kh < 2; ++kh) {
+                    const uint4* src = Aop + ((ga + ap) * 2 + kh) * 2 * 64 + lane;
+                    const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
+                    const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
+                    acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap][kh], acc[ap], 0, 0, 0);
+                    acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap][kh], acc[ap], 0, 0, 0);
+                    acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap][kh], acc[ap], 0, 0, 0);
+                }
+            }
+            float2 P[2][2];  // [aperture a / a + 1][mat 0: residual, mat 1: psi]
+#pragma unroll
+            for (int x = 0; x < 2; ++x) P[x][0] = P[x][1] = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int rr = 0; rr < 16; rr += 2) {
+                int mat, j0, j1;
+                td_row<J>(0, rr >> 2, 0, rr & 3, &mat, &j0);
+                td_row<J>(0, rr >> 2, 1, rr & 3, &mat, &j1);
+                float2 za0 = cur.Z[j0], za1 = cur.Z[j1];
+                swap_halves(za0.x, za1.x);
+                swap_halves(za0.y, za1.y);
+                P[0][mat] += cmul(za0, make_float2(acc[0][rr], acc[0][rr + 1]));
+                P[1][mat] += cmul(za1, make_float2(acc[1][rr], acc[1][rr + 1]));
+            }
+            P[0][0] = Sc[ga] * P[0][0];
+            P[1][0] = Sc[ga + 1] * P[1][0];
+#pragma unroll
+            for (int mat = 0; mat < 2; ++mat) {
+                swap_halves(P[0][mat].x, P[1][mat].x);
+                swap_halves(P[0][mat].y, P[1][mat].y);
+            }
+            const float2 g0 = c.inv_S * (P[0][0] + P[1][0]);
+            const float f = 2.0f * psi_scale * (P[0][1].x + P[1][1].x);
+            const float2 corr = cmulc(cur.rot, g0);
+            acc_re = fmaf(cur.amp, corr.x, acc_re);
+            acc_im = fmaf(cur.amp, corr.y, acc_im);
+            acc_n = fmaf(cur.amp * cur.amp, f, acc_n);
+        }
+    }
+    {
+        float x = acc_re, y = acc_re;
+        swap_halves(x, y);
+        acc_re = x + y;
+        x = acc_im, y = acc_im;
+        swap_halves(x, y);
+        acc_im = x + y;
+        x = acc_n, y = acc_n;
+        swap_halves(x, y);
+        acc_n = x + y;
+    }
+    if (h == 0 && s_ok) {
+        float* dst = partial + ((size_t)s * nrange + range) * 3;
+        dst[0] = acc_re;
+        dst[1] = acc_im;
+        dst[2] = acc_n;
+    }
+}
+
+// out[scan][s][3] = sum of partial[s][r][3] over the scan's ranges, float64, fixed order.
+__global__ __launch_bounds__(256) void scan_td_reduce(const float* __restrict__ partial, int C, int nrange,
+                                                      const TdScanSet set, float* __restrict__ out) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int k = blockIdx.y;
+    if (row >= C) return;
+    const int r0 = set.range_first[k], r1 = set.range_first[k + 1];
+    for (int f = 0; f < 3; ++f) {
+        double acc = 0.0;
+        for (int r = r0 + lane; r < r1; r += 64) acc += partial[((size_t)row * nrange + r) * 3 + f];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) out[((size_t)k * C + row) * 3 + f] = (float)acc;
+    }
+}
+
+bool td_scan_supported(const Chirp& c) { return splat_form() == 0 && c.S == 128 && td_supported(c); }
+
+static TdScanSet td_scan_set(const ScanSet& set, int chunk) {
+    TdScanSet t;
+    t.count = set.count;
+    t.per_range = chunk < kScanMinRange ? kScanMinRange : (chunk + kTdVjpGroup - 1) / kTdVjpGroup * kTdVjpGroup;
+    t.ap_off[0] = 0;
+    t.range_first[0] = 0;
+    for (int i = 0; i < set.count; ++i) {
+        t.positions[i] = set.positions[i];
+        t.apertures[i] = set.apertures[i];
+        t.ap_off[i + 1] = t.ap_off[i] + set.apertures[i];
+        t.range_first[i + 1] = t.range_first[i] + (set.apertures[i] + t.per_range - 1) / t.per_range;
+    }
+    return t;
+}
+
+// Workspace: partials [C][ranges][3] (ranges at the smallest range length) | U [sum A][S]
+// complex | operands [sum A][256] uint4 | scale exponents [sum A], each 256-byte aligned.
+struct TdScanWorkspace {
+    size_t partial, u, ops, scales, total;
+};
+
+static TdScanWorkspace td_scan_layout(const Chirp& c, const ScanSet& set, int C) {
+    const TdScanSet t = td_scan_set(set, kScanMinRange);
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t apertures = t.ap_off[set.count];
+    TdScanWorkspace w;
+    w.partial = 0;
+    w.u = up(sizeof(float) * 3 * (size_t)C * t.range_first[set.count]);
+    w.ops = w.u + up(sizeof(float2) * apertures * c.S);
+    w.scales = w.ops + up(sizeof(uint4) * apertures * 256);
+    w.total = w.scales + up(sizeof(float) * apertures);
+    return w;
+}
+
+size_t td_scan_workspace(const Chirp& c, const ScanSet& set, int C) { return td_scan_layout(c, set, C).total; }
+
+hipError_t launch_td_scan(const Chirp& c, const ScanSet& set, const float* ctr, const float* nrm, int C, int chunk,
+                          void* ws, float* out, hipStream_t st) {
+    const TdScanSet t = td_scan_set(set, chunk);
+    const TdScanWorkspace lay = td_scan_layout(c, set, C);
+    char* base = static_cast<char*>(ws);
+    float* partial = reinterpret_cast<float*>(base + lay.partial);
+    float2* U = reinterpret_cast<float2*>(base + lay.u);
+    uint4* ops = reinterpret_cast<uint4*>(base + lay.ops);
+    float* scales = reinterpret_cast<float*>(base + lay.scales);
+    const size_t dsmem = sizeof(float2) * (c.N + 16 * (size_t)(c.K + 1));
+    hipError_t e = allow_lds((const void*)dft_rows_kernel, dsmem);
+    if (e != hipSuccess) return e;
+    for (int i = 0; i < set.count; ++i) {
+        const int A = set.apertures[i];
+        hipLaunchKernelGGL(dft_rows_kernel, dim3((A + 15) / 16), dim3(256), dsmem, st, set.residuals[i], A, c.K, c.N,
+                           1.0f, 1.0f, c.S, U + (size_t)t.ap_off[i] * c.S);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    const int apertures = t.ap_off[set.count];
+    hipLaunchKernelGGL(scan_operands_kernel, dim3(apertures), dim3(64), 0, st, U, c.S, c.N, c.K, ops, scales);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const int nrange = t.range_first[set.count];
+    const dim3 grid((C + kScanCandidates - 1) / kScanCandidates, nrange);
+    const size_t smem = sizeof(float2) * c.N + sizeof(float) * 16 + sizeof(uint4) * (size_t)kTdVjpGroup * 256;
+    const bool fres = c.fresnel_n > 1.0f;
+    e = allow_lds(fres ? (const void*)scan_td_kernel<true> : (const void*)scan_td_kernel<false>, smem);
+    if (e != hipSuccess) return e;
+    if (fres)
+        hipLaunchKernelGGL(scan_td_kernel<true>, grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales, partial);
+    else
+        hipLaunchKernelGGL(scan_td_kernel<false>, grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales, partial);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(scan_td_reduce, dim3((C + 3) / 4, set.count), dim3(256), 0, st, partial, C, nrange, t, out);
+    return hipGetLastError();
+}
+
 }  // namespace dsplat

@@ -133,7 +133,7 @@ SIGNATURES = {
     "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_gemm_tn_workspace": (c_size, [c_int, c_int, c_int, c_int]),
     "dsplat_gemm_tn": (c_int, [c_ptr, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
-    "dsplat_scan_workspace": (c_size, [c_int, c_ptr, c_int]),
+    "dsplat_scan_workspace": (c_size, [c_ptr, c_int, c_ptr, c_int]),
     "dsplat_scan_batched": (
         c_int, [c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]
     ),

@@ -330,7 +330,7 @@ def correlate_gradient(config, positions, centers, normals, residual, fresnel_n,
     )
 
 
-SCAN_CHUNKS = (512, 2048, 8192, 32768)  # candidates per workgroup tried by the autotuner
+SCAN_CHUNKS = (512, 2048, 8192, 32768)  # autotuned: candidates per workgroup (bin domain) or apertures per range (sample domain)
 
 
 def scan_batched(config, scans, centers, normals, fresnel_n, chunk=None):
@@ -348,10 +348,10 @@ def scan_batched(config, scans, centers, normals, fresnel_n, chunk=None):
     pos_ptrs = (ctypes.c_void_p * n)(*[p.data_ptr() for p in positions])
     res_ptrs = (ctypes.c_void_p * n)(*[r.data_ptr() for r in residuals])
     apertures = (ctypes.c_int32 * n)(*[len(p) for p in positions])
-    size = _lib.library().dsplat_scan_workspace(n, apertures, count)
+    cfg = _lib.fmcw(config, fresnel_n)
+    size = _lib.library().dsplat_scan_workspace(_lib.ctypes.byref(cfg), n, apertures, count)
     workspace = centers.new_empty(max(1, size // 4))
     out = centers.new_empty((n, count, 3))
-    cfg = _lib.fmcw(config, fresnel_n)
     centers, normals = centers.contiguous(), normals.contiguous()
 
     def run(value):

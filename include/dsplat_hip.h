@@ -141,15 +141,17 @@ int dsplat_surfel_orient_select(const dsplat_fmcw* cfg, const float* positions, 
                                 int32_t oversample, int32_t real_part, double* best_value,
                                 int32_t* best_index, void* stream);
 
-/* Workspace bytes of dsplat_scan_batched. */
-size_t dsplat_scan_workspace(int32_t num_scans, const int32_t* apertures, int32_t num_candidates);
+/* Workspace bytes of dsplat_scan_batched for this chirp (0 if unsupported). */
+size_t dsplat_scan_workspace(const dsplat_fmcw* cfg, int32_t num_scans, const int32_t* apertures,
+                             int32_t num_candidates);
 
 /* Certificate scan of candidates against up to 8 scans of one chirp with uneven aperture
  * counts, in one launch: out [num_scans, C, 3] = (corr.re, corr.im, norm) of every
  * candidate against each scan's residual, without area. positions[i] [A_i, 3] and
  * residuals[i] [A_i, K] complex are device pointers held in host arrays; apertures[i] =
- * A_i (host). chunk = candidates per workgroup (0: a fill heuristic; the Python side
- * autotunes it). No counterpart launch in the reference, which scores one scan at a
+ * A_i (host). chunk = candidates per workgroup of the bin-domain kernel, or apertures per
+ * range of the sample-domain kernel used at 128 ADC samples (0: a fill heuristic; the
+ * Python side autotunes it). No counterpart launch in the reference, which scores one scan at a
  * time through kernels/surfel.py:242-260 (correlate) in certificate.py:11-31. */
 int dsplat_scan_batched(const dsplat_fmcw* cfg, int32_t num_scans, const float* const* positions,
                         const float* const* residuals, const int32_t* apertures, const float* centers,
