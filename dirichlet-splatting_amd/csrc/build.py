@@ -27,6 +27,9 @@ FLAGS = [
     # Packed FP32 VALU forms are kept out of every kernel.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # MFMA results stay in ArchVGPRs: no accumulator-to-VGPR copies after each tile and
+    # one register file for the occupancy budget.
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall",
     "-Wno-unused-function",
 ]
