@@ -51,6 +51,16 @@ struct Chirp {
     float c2, c4;        // Taylor coefficients of the lobe ratio
 };
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: linear id b
+// runs on XCD b % 8. The returned logical id gives XCD x the contiguous logical ids
+// [x total / 8, (x + 1) total / 8), so workgroups that are neighbours in logical order (and
+// read the same operands) share one L2. Identity when total is not a multiple of 8.
+__device__ __forceinline__ int xcd_remap(int b, int total) {
+    constexpr int kXcds = 8;
+    if (total % kXcds) return b;
+    return (b % kXcds) * (total / kXcds) + b / kXcds;
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

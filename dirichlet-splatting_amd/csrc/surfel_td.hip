@@ -105,6 +105,23 @@ __device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const f
     }
 }
 
+// Straight copy of ITERS x 256 uint4 from global memory to LDS by the direct global -> LDS
+// loads of gfx950 (global_load_lds_dwordx4): the data never passes through VGPRs and no load
+// waits for another. Element e = tid + 256 it lands in dst[e]: the wave's slot dst + (e & ~63)
+// goes in M0 and lane l writes 16 bytes at M0 + 16 l. The caller's barrier follows the wait.
+template <int ITERS, typename Src>
+__device__ __forceinline__ void stage_to_lds(uint4* dst, int tid, Src src_of) {
+    const int wave_base = __builtin_amdgcn_readfirstlane(tid & ~63);  // uniform: kept in SGPRs
+#pragma unroll
+    for (int it = 0; it < ITERS; ++it) {
+        const int e = tid + 256 * it;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src_of(e),
+                                         (__attribute__((address_space(3))) void*)(dst + wave_base + 256 * it), 16,
+                                         0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
 // ---------------------------------------------------------------------------------------
 // Forward
 // ---------------------------------------------------------------------------------------
@@ -567,10 +584,12 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
     uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);      // [G][T][2][2][64]
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
-    const int s = blockIdx.x * kTdVjpSurfels + 32 * wave + m;
+    // the surfel tiles of one aperture range run on one XCD and read its operands through one L2
+    const int lin = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = lin % gridDim.x, range = lin / gridDim.x;
+    const int s = tile * kTdVjpSurfels + 32 * wave + m;
     const bool s_ok = s < S;
     const int sc = min(s, S - 1);
-    const int range = blockIdx.y;
     const int a_begin = range * per_range, a_end = min(A, a_begin + per_range);
     fill_twiddles(E, c.N, tid, 256);
 
@@ -581,14 +600,11 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
 
     for (int g0 = a_begin; g0 < a_end; g0 += G) {
         __syncthreads();
-        // Stage the group's operands: a straight copy (apertures past the range are zeroed).
-#pragma unroll
-        for (int it = 0; it < G * PER / 256; ++it) {
-            const int e = tid + 256 * it;
-            const int a = g0 + e / PER;
-            const uint4 x = Aexp[(size_t)min(a, A - 1) * PER + e % PER];
-            Aop[e] = a < a_end ? x : make_uint4(0u, 0u, 0u, 0u);
-        }
+        // Stage the group's operands. Apertures past the range take a clamped aperture's
+        // (finite) operands: their pairs have amp = amp_cos = 0, which zeroes every gradient term.
+        stage_to_lds<G * PER / 256>(Aop, tid, [&](int e) {
+            return Aexp + (size_t)min(g0 + e / PER, A - 1) * PER + e % PER;
+        });
         if (tid < G) Sc[tid] = ldexpf(1.0f, (int)scale_exp[min(g0 + tid, A - 1)]);
         __syncthreads();
         for (int ga = 0; ga < G; ga += 2) {
@@ -944,13 +960,10 @@ __global__ __launch_bounds__(256) void scan_td_kernel(const Chirp c, const TdSca
 
     for (int g0 = a_begin; g0 < a_end; g0 += G) {
         __syncthreads();
-#pragma unroll
-        for (int it = 0; it < G * PER / 256; ++it) {
-            const int e = tid + 256 * it;
-            const int a = g0 + e / PER;
-            const uint4 x = kops[(size_t)min(a, A - 1) * PER + e % PER];
-            Aop[e] = a < a_end ? x : make_uint4(0u, 0u, 0u, 0u);
-        }
+        // past the range: a clamped aperture's operands, zeroed by amp = 0 of those pairs
+        stage_to_lds<G * PER / 256>(Aop, tid, [&](int e) {
+            return kops + (size_t)min(g0 + e / PER, A - 1) * PER + e % PER;
+        });
         if (tid < G) Sc[tid] = ldexpf(1.0f, (int)kexp[min(g0 + tid, A - 1)]);
         __syncthreads();
         for (int ga = 0; ga < G; ga += 2) {
