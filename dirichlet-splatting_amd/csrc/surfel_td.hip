@@ -105,6 +105,25 @@ __device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const f
     }
 }
 
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// FP16 hi + lo of the pair (x, y) for MFMA operands: hi = RNE in one packed conversion, then
+// lo = RNE(x - hi) (the difference is exact in FP32) by v_fma_mix{lo,hi}_f16, which read hi
+// as FP16 from its half of the packed register, compute -hi + x in FP32 and round the result
+// into one half of lo: three VALU instructions per pair of values. The compiler does not see
+// the asm as a VALU write and so inserts no wait states before an MFMA that reads lo; the
+// s_nop 1 provides the two that a VALU-written VGPR needs before it is an MFMA source.
+__device__ __forceinline__ void split_f16x2(float x, float y, f16x2& hi, f16x2& lo) {
+    hi = f16x2{(_Float16)x, (_Float16)y};
+    const uint32_t h = __builtin_bit_cast(uint32_t, hi);
+    uint32_t l;
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(h), "v"(x));
+    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\ts_nop 1"
+        : "+v"(l) : "v"(h), "v"(y));
+    lo = __builtin_bit_cast(f16x2, l);
+}
+
 // Straight copy of ITERS x 256 uint4 from global memory to LDS by the direct global -> LDS
 // loads of gfx950 (global_load_lds_dwordx4): the data never passes through VGPRs and no load
 // waits for another. Element e = tid + 256 it lands in dst[e]: the wave's slot dst + (e & ~63)
@@ -133,6 +152,9 @@ struct TdRec {
     static constexpr int sets = JVP ? 2 : 1;
     static constexpr int used = 2 * J + 32 * sets;
     static constexpr int size = used + 2;
+    static constexpr int shift = 0;  // dwords added after every group of 4 records
+    __device__ static int at(int t) { return t * size + shift * (t >> 2); }
+    static size_t bytes(int n) { return sizeof(float) * ((size_t)n * size + shift * (size_t)(n / 4)); }
 };
 
 // Per-surfel inputs of a producer, loaded one chunk ahead.
@@ -233,12 +255,13 @@ __device__ __forceinline__ void td_consume(const float* rec, int cw, int lane, f
     const int kk = lane >> 4, x = lane & 15;
     const int a_off = (x & 1) * J + (x >> 1);  // Z part x % 2 of j = x / 2
     const int b_off = 2 * J + x;               // (e W) re of i = x; im at + 16
-    const float* base = rec + ((4 * cw) * SC + kk) * REC;
+    constexpr int GROUP = 4 * REC + TdRec<J, SETS == 2>::shift;  // dwords per group of 4 records
+    const float* base = rec + kk * REC + (4 * cw) * (SC / 4) * GROUP;
 #pragma unroll
     for (int t = 0; t < SC / 4; ++t) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
-            const float* r = base + (b * SC + 4 * t) * REC;
+            const float* r = base + (b * (SC / 4) + t) * GROUP;
             float bv[SETS][2];
 #pragma unroll
             for (int e = 0; e < SETS; ++e) {
@@ -300,11 +323,11 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
                                                         const float* __restrict__ dnrm,
                                                         const float2* __restrict__ dw, int S,
                                                         float2* __restrict__ out) {
-    constexpr int AT = kTdApertures, SC = kTdChunk, REC = TdRec<J, JVP>::size, SETS = TdRec<J, JVP>::sets;
+    constexpr int AT = kTdApertures, SC = kTdChunk, SETS = TdRec<J, JVP>::sets;
     constexpr int T = (2 * J + 15) / 16;
     extern __shared__ float smem[];
     float2* E = reinterpret_cast<float2*>(smem);  // [N]
-    float* rec = smem + 2 * c.N;                  // [AT * SC][REC]; later X [AT][S + 1] float2
+    float* rec = smem + 2 * c.N;                  // AT * SC records; later X [AT][S + 1] float2
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int a0 = blockIdx.x * AT;
@@ -329,7 +352,7 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
         const int s = s0 + psp;
         const SurfIn<JVP> next = load_surfel<JVP>(min(s + SC, S - 1), ctr, nrm, w, dctr, dnrm, dw);
         __syncthreads();  // previous chunk consumed (and the twiddle table written)
-        td_produce<J, JVP, FRESNEL>(c, E, p_ok && s < S, px, py, pz, in, rec + tid * REC);
+        td_produce<J, JVP, FRESNEL>(c, E, p_ok && s < S, px, py, pz, in, rec + TdRec<J, JVP>::at(tid));
         __syncthreads();
         td_consume<J, SETS, T, SC>(rec, wave, lane, acc);
         in = next;
@@ -339,68 +362,6 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
     td_store_x<J, SETS, T>(c, X, wave, lane, acc);
     __syncthreads();
     dft16(X, c.S + 1, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(AT, A - a0), wave, lane);
-}
-
-// Warp-specialized, double-buffered form (render): waves 0-3 consume chunk k on MFMA while
-// waves 4-7 produce chunk k + 1 on the VALU, so every SIMD holds one wave of each and the
-// matrix and vector pipes work at the same time. One barrier per chunk.
-template <int J, bool FRESNEL>
-__global__ __launch_bounds__(512) void td_render_ws_kernel(const Chirp c, const float* __restrict__ pos, int A,
-                                                           const float* __restrict__ ctr,
-                                                           const float* __restrict__ nrm,
-                                                           const float2* __restrict__ w, int S,
-                                                           float2* __restrict__ out) {
-    constexpr int AT = kTdApertures, SC = kTdChunk, REC = TdRec<J, false>::size;
-    constexpr int T = (2 * J + 15) / 16;
-    extern __shared__ float smem[];
-    float2* E = reinterpret_cast<float2*>(smem);  // [N]
-    float* buf = smem + 2 * c.N;                  // [2][AT * SC][REC]; later X [AT][S + 1] float2
-
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const bool producer = wave >= 4;
-    const int a0 = blockIdx.x * AT;
-    fill_twiddles(E, c.N, tid, 512);
-    const int ptid = tid - 256;
-    const int pal = (ptid >> 4) & 15, psp = ptid & 15;
-    const int pa = min(a0 + pal, A - 1);
-    const bool p_ok = a0 + pal < A;
-    float px = 0.f, py = 0.f, pz = 0.f;
-    if (producer) {
-        px = pos[3 * pa];
-        py = pos[3 * pa + 1];
-        pz = pos[3 * pa + 2];
-    }
-    f32x4 acc[1][2][4][T];
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int tt = 0; tt < T; ++tt) acc[0][p][b][tt] = f32x4{0.f, 0.f, 0.f, 0.f};
-
-    const int chunks = (S + SC - 1) / SC;
-    SurfIn<false> in;
-    if (producer) in = load_surfel<false>(min(psp, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
-    __syncthreads();  // twiddle table
-    for (int k = 0; k <= chunks; ++k) {
-        if (producer) {
-            if (k < chunks) {
-                const int s = k * SC + psp;
-                const SurfIn<false> next = load_surfel<false>(min(s + SC, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
-                td_produce<J, false, FRESNEL>(c, E, p_ok && s < S, px, py, pz, in,
-                                              buf + ((k & 1) * AT * SC + ptid) * REC);
-                in = next;
-            }
-        } else if (k > 0) {
-            td_consume<J, 1, T, SC>(buf + ((k - 1) & 1) * AT * SC * REC, wave, lane, acc);
-        }
-        __syncthreads();
-    }
-    float2* X = reinterpret_cast<float2*>(buf);
-    if (!producer) td_store_x<J, 1, T>(c, X, wave, lane, acc);
-    __syncthreads();
-    dft16(X, c.S + 1, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(AT, A - a0), wave & 3,
-          lane);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -444,7 +405,6 @@ __device__ __forceinline__ void td_row(int tile, int q, int h, int r, int* mat, 
     }
 }
 
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 // A operands of the VJP GEMM for v_mfma_f32_32x32x16_f16 (A[i][8 h + e] -> lane i + 32 h,
 // element e), as FP16 hi + lo splits of the per-aperture scaled values, built once per
@@ -552,14 +512,14 @@ __device__ __forceinline__ void vjp_stage(const Chirp& c, const float2* E, const
             swap_halves(xi, yi);
             const float vals[2][2] = {{xr, xi}, {yr, yi}};  // [aperture][re, im]
 #pragma unroll
-            for (int ap = 0; ap < 2; ++ap)
-#pragma unroll
-                for (int p = 0; p < 2; ++p) {
-                    const float x = vals[ap][p];
-                    const _Float16 xh = (_Float16)x;
-                    st.Bh[ap][kh][2 * qq + p] = xh;
-                    st.Bl[ap][kh][2 * qq + p] = (_Float16)(x - (float)xh);
-                }
+            for (int ap = 0; ap < 2; ++ap) {
+                f16x2 hi, lo;
+                split_f16x2(vals[ap][0], vals[ap][1], hi, lo);
+                st.Bh[ap][kh][2 * qq] = hi.x;
+                st.Bh[ap][kh][2 * qq + 1] = hi.y;
+                st.Bl[ap][kh][2 * qq] = lo.x;
+                st.Bl[ap][kh][2 * qq + 1] = lo.y;
+            }
         }
 }
 
@@ -710,7 +670,7 @@ template <int J, bool JVP>
 static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                      const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S,
                                      float2* out, hipStream_t st) {
-    const size_t rec = sizeof(float) * (size_t)kTdApertures * kTdChunk * TdRec<J, JVP>::size;
+    const size_t rec = TdRec<J, JVP>::bytes(kTdApertures * kTdChunk);
     const size_t xbuf = sizeof(float2) * (size_t)kTdApertures * (c.S + 1);
     const size_t smem = sizeof(float2) * c.N + (rec > xbuf ? rec : xbuf);
     const bool fres = c.fresnel_n > 1.0f;
@@ -724,24 +684,6 @@ static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, co
     else
         hipLaunchKernelGGL((td_render_kernel<J, JVP, false>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr,
                            dnrm, dw, S, out);
-    return hipGetLastError();
-}
-
-template <int J>
-static hipError_t launch_td_render_ws_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                                        const float2* w, int S, float2* out, hipStream_t st) {
-    const size_t rec = 2 * sizeof(float) * (size_t)kTdApertures * kTdChunk * TdRec<J, false>::size;
-    const size_t xbuf = sizeof(float2) * (size_t)kTdApertures * (c.S + 1);
-    const size_t smem = sizeof(float2) * c.N + (rec > xbuf ? rec : xbuf);
-    const bool fres = c.fresnel_n > 1.0f;
-    hipError_t e =
-        allow_lds(fres ? (const void*)td_render_ws_kernel<J, true> : (const void*)td_render_ws_kernel<J, false>, smem);
-    if (e != hipSuccess) return e;
-    const dim3 grid((A + kTdApertures - 1) / kTdApertures);
-    if (fres)
-        hipLaunchKernelGGL((td_render_ws_kernel<J, true>), grid, dim3(512), smem, st, c, pos, A, ctr, nrm, w, S, out);
-    else
-        hipLaunchKernelGGL((td_render_ws_kernel<J, false>), grid, dim3(512), smem, st, c, pos, A, ctr, nrm, w, S, out);
     return hipGetLastError();
 }
 
