@@ -66,10 +66,11 @@ def parse():
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
     parser.add_argument(
-        "--workload", choices=("splat", "gram", "scan", "dsfw", "copy"), default="splat",
+        "--workload", choices=("splat", "gram", "scan", "dsfw", "synth", "copy"), default="splat",
         help="splat: the fwd+VJP metric (default); gram: the split-precision Gram of the same scene; "
         "scan: the batched certificate scan of 64^3 candidates against 64^2, 96^2 and 128^2 scans; "
         "dsfw: DSFW steps, plain FP32 form against the mixed-precision batched form; "
+        "synth: the float64 truth scan of the DSFW fit by the ADC signal chain on the FP64 matrix cores; "
         "copy: a plain 1 GiB device copy, the reference size for the HBM byte counters",
     )  # fmt: skip
     parser.add_argument("--adc-samples", type=int, default=128, help="dsfw: ADC samples (bins = samples)")
@@ -206,6 +207,121 @@ def gram_workload(args):
                      "peak": peak[best], "unit": "TFLOP/s", "frac": modes[best]["mfma_tflops"] / peak[best],
                      "traffic": None},
     }
+
+
+def torch_synthesis(positions, centers, normals, amplitudes, cfg, aperture_block=256, surfel_block=512):
+    """The float64 beat-tone chain of groundtruth.py:58-90 as torch tensor operations over
+    aperture and surfel blocks (the reference's own formulation): the comparison leg of the
+    synthesis workload."""
+    import math
+
+    import torch
+
+    t = torch.arange(cfg.adc_samples, device=centers.device, dtype=torch.float64) / cfg.sample_rate_hz
+    centers, normals, amplitudes = centers.double(), normals.double(), amplitudes.to(torch.complex128)
+    rows = []
+    for a0 in range(0, len(positions), aperture_block):
+        block = positions[a0 : a0 + aperture_block].double()
+        adc = torch.zeros((len(block), 1, cfg.adc_samples), dtype=torch.complex128, device=centers.device)
+        for s0 in range(0, len(centers), surfel_block):
+            sl = slice(s0, s0 + surfel_block)
+            offset = block[:, None, :] - centers[None, sl]
+            r = offset.norm(dim=-1).clamp(min=1e-9)
+            cosine = ((offset / r[..., None]) * normals[None, sl]).sum(-1).clamp(min=0)
+            gain = cosine.square() * (cfg.lambda_m / (8 * math.pi)) / r
+            delay = 2 * r / cfg.c0_m_s
+            phase = 2 * math.pi * (cfg.fc_hz * delay + cfg.slope_hz_s * delay * (cfg.adc_start_time_s - 0.5 * delay))
+            echo = torch.polar(gain, phase) * amplitudes[sl]
+            beat = 2 * cfg.slope_hz_s * r / cfg.c0_m_s
+            tones = torch.polar(torch.ones_like(r)[..., None], 2 * math.pi * beat[..., None] * t)
+            adc += torch.bmm(echo[:, None, :], tones)
+        rows.append(torch.fft.fft(adc[:, 0], n=cfg.n_fft)[:, : cfg.num_range_bins] / cfg.adc_samples)
+    return torch.cat(rows)
+
+
+def synth_workload(args):
+    """The float64 truth scan of the DSFW fit: 128x128 apertures at 2.5 mm, 256 ADC samples
+    (pad 2, 256 bins), the front faces of 4000 area-uniform ellipsoid draws with amplitude
+    sqrt(area), made by the ADC signal chain on the FP64 matrix cores (dsplat_synthesize).
+    Also times the reference's torch float64 block loop on the same inputs and checks the
+    kernel against the float64 numpy oracle on a sample of apertures."""
+    import numpy as np
+    import torch
+    from threadpoolctl import threadpool_limits
+
+    from dsplat import FMCWConfig, workloads
+    from dsplat.groundtruth import planar_array
+    from dsplat.kernels import surfel
+
+    dev = torch.device("cuda", 0)
+    cfg = FMCWConfig(adc_samples=256)
+    centers, normals = (x.to(dev) for x in workloads.ellipsoid_samples(4000, 0, keep_all_draws=True))
+    area = workloads.ellipsoid_area(*workloads.SEMI_AXES) / 4000
+    positions = planar_array(128, workloads.PITCH).to(dev)
+    amplitudes = torch.full((len(centers),), area**0.5, dtype=torch.complex128, device=dev)
+    A, S, T = len(positions), len(centers), cfg.adc_samples
+    evals = A * T * S
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        return surfel.synthesize(cfg, positions, centers, normals, amplitudes, 0.0)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    steps = max(1, min(args.steps, 20))
+    kernel_ms, out = timed(step, steps, stream)
+    sample = torch.arange(0, A, 1024, device=dev)  # 16 apertures across the array
+    torch_ms, ref = timed(lambda: torch_synthesis(positions[sample], centers, normals, amplitudes, cfg), 1, stream)
+    torch_ms_full = torch_ms * A / len(sample)
+    from oracle import surfel as O
+
+    want = O.synthesize(positions[sample].cpu().numpy(), centers.cpu().numpy(), normals.cpu().numpy(),
+                        amplitudes.cpu().numpy(), cfg)  # fmt: skip
+    got = out[sample].cpu().numpy()
+    rel = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    rel_torch = float(np.linalg.norm(ref.cpu().numpy() - want) / np.linalg.norm(want))
+    achieved = 8 * evals / (kernel_ms * 1e-3) / 1e12
+    result = {
+        "metric": "FP64 scan synthesis evals/s (aperture·sample·surfel)",
+        "value": evals / (kernel_ms * 1e-3),
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": max(1, args.warmup),
+        "ms_per_step": kernel_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"float64 FMCW truth scan: 128x128 apertures @ 2.5 mm, 256 ADC samples (256 bins), "
+                   f"{S} front-facing surfels of 4000 ellipsoid draws", "apertures": A, "samples": T,
+                   "surfels": S, "evals_per_step": evals, "parallelism": "single GPU"},
+        "kernels_ms": {"synthesize": kernel_ms, "torch_float64_blocks": torch_ms_full},
+        "speedup_vs_torch_float64": torch_ms_full / kernel_ms,
+        "rel_l2_vs_float64_oracle": {"kernel": rel, "torch_float64_blocks": rel_torch, "apertures": len(sample)},
+        "roofline": {"kernel": "synthesize_kernel<16>", "bound": "mfma", "achieved": achieved, "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None, "flop_per_eval": 8,
+                     "peak_basis": "FP64 dense MFMA (one complex MAC per (aperture, sample, surfel))"},
+    }
+    if not args.no_cpu:
+        pos, cc, cn = positions.cpu().numpy(), centers.cpu().numpy(), normals.cpu().numpy()
+        amps = amplitudes.cpu().numpy()
+        with threadpool_limits(limits=1):
+            t = time.perf_counter()
+            O.synthesize(pos[:2], cc, cn, amps, cfg)
+            probe = time.perf_counter() - t
+            count = int(min(A, max(2, 2 * args.cpu_seconds / max(probe, 1e-3))))
+            ids = np.linspace(0, A - 1, count).astype(np.int64)
+            t = time.perf_counter()
+            O.synthesize(pos[ids], cc, cn, amps, cfg)
+            seconds = time.perf_counter() - t
+        result["cpu_baseline"] = {
+            "value": count * T * S / seconds, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": f"float64 numpy oracle synthesis of {count} of {A} apertures ({count * T * S:.3e} evals, "
+            f"{seconds:.1f} s, 1 thread)",
+        }  # fmt: skip
+    return result
 
 
 def scan_workload(args):
@@ -443,6 +559,9 @@ def main():
         return
     if args.workload == "dsfw":
         print(json.dumps(dsfw_workload(args)), flush=True)
+        return
+    if args.workload == "synth":
+        print(json.dumps(synth_workload(args)), flush=True)
         return
     if args.workload == "copy":
         print(json.dumps(copy_workload(args)), flush=True)

@@ -48,6 +48,8 @@ bool make_chirp(const dsplat_fmcw* cfg, Chirp* out) {
     c.two_over_c0 = 2.0 / cfg->c0_m_s;
     c.inv_N = 1.0 / N;
     c.two_pi_N = (float)(2.0 * M_PI / N);
+    c.lam_d = cfg->lambda_m;
+    c.fresnel_d = cfg->fresnel_n;
     c.lam = (float)cfg->lambda_m;
     c.fresnel_n = (float)cfg->fresnel_n;
     c.inv_S = (float)(1.0 / S);
@@ -387,6 +389,21 @@ int dsplat_range_response(double* out, double low, double high, int32_t count, i
     if (!out) return hipErrorInvalidValue;
     return launch_range_response(reinterpret_cast<double2*>(out), low, high, count, num_bins, samples, n_fft,
                                  (hipStream_t)stream);
+}
+
+int dsplat_synthesize(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* centers,
+                      const float* normals, const double* amplitudes, int32_t num_surfels, double* out,
+                      void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0 || !synthesize_supported(c))
+        return hipErrorInvalidValue;
+    if (num_apertures == 0) return hipSuccess;
+    if (!positions || !out || (num_surfels > 0 && (!centers || !normals || !amplitudes))) return hipErrorInvalidValue;
+    if (num_surfels == 0) return fill_zero(reinterpret_cast<float*>(out), 4 * (size_t)num_apertures * c.K,
+                                           (hipStream_t)stream);
+    return launch_synthesize(c, positions, num_apertures, centers, normals,
+                             reinterpret_cast<const double2*>(amplitudes), num_surfels,
+                             reinterpret_cast<double2*>(out), (hipStream_t)stream);
 }
 
 // ------------------------------------------------------------------------------------ blas

@@ -41,6 +41,7 @@ struct Chirp {
     double k0pm, fc, slope, t0, c0;
     double two_over_c0;  // 2 / c0: round-trip delay per metre
     double inv_N;        // 1 / N
+    double lam_d, fresnel_d;  // wavelength and Fresnel index in float64 (scan synthesis)
     float lam, fresnel_n;
     float inv_S;         // 1 / S
     float pi_P, pi_N;    // pi / P, pi / N

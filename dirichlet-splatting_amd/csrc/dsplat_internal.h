@@ -99,6 +99,9 @@ hipError_t launch_direction(const float* inv, const float* grad, const float* nr
 hipError_t launch_trial(const float* ctr, const float* nrm, const float* dir, const float* ctl, int count, int width,
                         float fraction, float* oc, float* on, hipStream_t st);
 hipError_t launch_energy(const float2* x, const float2* y, size_t n, double* part, float* out, hipStream_t st);
+bool synthesize_supported(const Chirp& c);
+hipError_t launch_synthesize(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                             const double2* amp, int S, double2* out, hipStream_t st);
 hipError_t launch_range_response(double2* out, double low, double high, int count, int bins, int samples, int nfft,
                                  hipStream_t st);
 

@@ -1,8 +1,6 @@
 """Synthetic ground truth: planar arrays and float64 FMCW scans of known surfels
 (groundtruth.py:16-90 of the reference)."""
 
-import math
-
 import numpy as np
 import torch
 
@@ -41,33 +39,22 @@ def synthesize(
     2 slope r / c0 sampled by the ADC, the tones are summed, zero-padded to n_fft, FFT'd
     and cut to the positive bins, divided by adc_samples (groundtruth.py:58-90). It never
     evaluates the Dirichlet kernel, so a fit to its scans does not reuse the forward model.
+
+    Runs as one float64 matrix-core kernel (dsplat_synthesize) on the device of the
+    surfels, which must be a GPU; geometry is taken as float32, as the reference's callers
+    pass it. aperture_block and surfel_block are the reference's memory tiling and are
+    accepted for compatibility: the kernel tiles on its own.
     """
     from .fmcw import Scan
+    from .kernels import surfel as kernels
 
     device = surfel_centers.device
-    c = config
-    t = torch.arange(c.adc_samples, device=device, dtype=torch.float64) / c.sample_rate_hz
-    centers, normals = surfel_centers.double(), surfel_normals.double()
-    amplitudes = surfel_amplitudes.to(torch.complex128)
-    rows = []
-    for a0 in range(0, len(aperture_positions), aperture_block):
-        block = aperture_positions[a0 : a0 + aperture_block].to(device, torch.float64)
-        adc = torch.zeros((len(block), 1, c.adc_samples), dtype=torch.complex128, device=device)
-        for s0 in range(0, len(centers), surfel_block):
-            sl = slice(s0, s0 + surfel_block)
-            offset = block[:, None, :] - centers[None, sl]
-            r = offset.norm(dim=-1).clamp(min=1e-9)
-            cosine = ((offset / r[..., None]) * normals[None, sl]).sum(-1).clamp(min=0)
-            gain = cosine.square() * (c.lambda_m / (8 * math.pi)) / r
-            if fresnel_n > 1:
-                root = (fresnel_n * fresnel_n - 1 + cosine.square()).sqrt()
-                gain = gain * (root - cosine) / (root + cosine)
-            delay = 2 * r / c.c0_m_s
-            carrier = 2 * math.pi * (c.fc_hz * delay + c.slope_hz_s * delay * (c.adc_start_time_s - 0.5 * delay))
-            echo = torch.polar(gain, carrier) * amplitudes[sl]
-            beat = 2 * c.slope_hz_s * r / c.c0_m_s
-            tones = torch.polar(torch.ones_like(r)[..., None], 2 * math.pi * beat[..., None] * t)
-            adc += torch.bmm(echo[:, None, :], tones)
-        spectrum = torch.fft.fft(adc[:, 0], n=c.n_fft)[:, : c.num_range_bins]
-        rows.append((spectrum / c.adc_samples).to(torch.complex64))
-    return Scan(torch.cat(rows), aperture_positions.to(device), config)
+    spectrum = kernels.synthesize(
+        config,
+        aperture_positions.to(device, torch.float32).contiguous(),
+        surfel_centers.to(torch.float32).contiguous(),
+        surfel_normals.to(torch.float32).contiguous(),
+        surfel_amplitudes.to(device),
+        fresnel_n,
+    )
+    return Scan(spectrum.to(torch.complex64), aperture_positions.to(device), config)

@@ -147,6 +147,27 @@ def render(config, positions, centers, normals, weights, fresnel_n):
     return torch.view_as_complex(output)
 
 
+def synthesize(config, positions, centers, normals, amplitudes, fresnel_n):
+    """Float64 scan [A, K] complex128 of surfels with complex amplitudes by the ADC signal
+    chain (groundtruth.py:58-90): beat tones, zero-padded FFT, positive bins, / adc_samples."""
+    _check(positions, centers, normals)
+    output = torch.empty((len(positions), config.num_range_bins), dtype=torch.complex128, device=centers.device)
+    cfg = _lib.fmcw(config, fresnel_n)
+    _lib.call(
+        "dsplat_synthesize",
+        _lib.ctypes.byref(cfg),
+        _lib.ptr(positions.contiguous()),
+        len(positions),
+        _lib.ptr(centers.contiguous()),
+        _lib.ptr(normals.contiguous()),
+        _lib.ptr(torch.view_as_real(amplitudes.to(torch.complex128).contiguous())),
+        len(centers),
+        _lib.ptr(torch.view_as_real(output)),
+        _lib.stream(centers.device),
+    )
+    return output
+
+
 def design(config, positions, centers, normals, areas, fresnel_n, real):
     """Dictionary rows of whole apertures, including sqrt(area) (kernels/surfel.py:199-220).
 

@@ -237,6 +237,17 @@ size_t dsplat_energy_workspace(size_t count);
 int dsplat_energy(const float* value, const float* reference, size_t count, float* out,
                   void* workspace, size_t workspace_bytes, void* stream);
 
+/* Float64 FMCW scan of surfels with complex amplitudes by the ADC signal chain, not the
+ * Dirichlet kernel: each aperture-surfel echo gain e^{i carrier} b_s (cfg->fresnel_n > 1 adds
+ * the Fresnel factor) is a beat tone of 2 slope r / c0 sampled at the ADC rate; the tones are
+ * summed, zero-padded to n_fft, transformed, cut to the positive bins and divided by
+ * samples. amplitudes [S] complex128 (interleaved doubles); out [A, K] complex128.
+ * samples at most 256. Replaces groundtruth.py:58-90
+ * (synthesize: a torch float64 loop over aperture and surfel blocks). */
+int dsplat_synthesize(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                      const float* centers, const float* normals, const double* amplitudes,
+                      int32_t num_surfels, double* out, void* stream);
+
 /* out [bins, count] complex128: float64 D(bin - center) for count centers spread over
  * [low, high]. Replaces kernels/response.py:8-20 (range_response). */
 int dsplat_range_response(double* out, double low, double high, int32_t count, int32_t num_bins,
