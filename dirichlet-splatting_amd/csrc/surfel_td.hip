@@ -146,11 +146,12 @@ __device__ __forceinline__ void stage_to_lds(uint4* dst, int tid, Src src_of) {
 // ---------------------------------------------------------------------------------------
 template <int J, bool JVP>
 struct TdRec {
-    // floats per pair record: Z re[J], Z im[J], (e W) re[16], (e W) im[16] per coefficient set,
-    // plus 2 floats so the record stride is 2 mod 4 dwords: the 8-byte writes of 16
-    // consecutive producer lanes then fall on distinct LDS banks.
+    // floats per pair record. Forward: Z re[J], Z im[J], (c W) re[16], (c W) im[16]. JVP (two
+    // coefficient sets sharing the powers): (e1 Z) re[J], (e1 Z) im[J], (e2 Z) re[J],
+    // (e2 Z) im[J], W re[16], W im[16]. Plus 2 floats so the record stride is 2 mod 4 dwords:
+    // the 8-byte writes of 16 consecutive producer lanes then fall on distinct LDS banks.
     static constexpr int sets = JVP ? 2 : 1;
-    static constexpr int used = 2 * J + 32 * sets;
+    static constexpr int used = 2 * J * sets + 32;
     static constexpr int size = used + 2;
     static constexpr int shift = 0;  // dwords added after every group of 4 records
     __device__ static int at(int t) { return t * size + shift * (t >> 2); }
@@ -214,6 +215,32 @@ __device__ __forceinline__ void td_produce(const Chirp& c, const float2* E, bool
     }
     const float2 v = rotation(c, E, q.lobe, q.delta, 1);
     const float2 z1 = rotation(c, E, q.lobe, q.delta, 16);
+    const float2 v2 = csq(v);
+    if constexpr (JVP) {
+        // e Z_j = e z1^j for both sets by chains from e; W_i = v^i once, two chains by v^2
+#pragma unroll
+        for (int e = 0; e < SETS; ++e) {
+            float2 a0 = coef[e];
+            float* dst = mine + 2 * J * e;
+#pragma unroll
+            for (int j = 0; j < J; j += 2) {
+                const float2 a1 = cmul(a0, z1);
+                *reinterpret_cast<float2*>(dst + j) = make_float2(a0.x, a1.x);
+                *reinterpret_cast<float2*>(dst + J + j) = make_float2(a0.y, a1.y);
+                a0 = cmul(a1, z1);
+            }
+        }
+        float2 w0 = make_float2(1.f, 0.f), w1 = v;
+        float* dst = mine + 2 * J * SETS;
+#pragma unroll
+        for (int i = 0; i < 16; i += 2) {
+            *reinterpret_cast<float2*>(dst + i) = make_float2(w0.x, w1.x);
+            *reinterpret_cast<float2*>(dst + 16 + i) = make_float2(w0.y, w1.y);
+            w0 = i == 0 ? v2 : cmul(w0, v2);
+            w1 = cmul(w1, v2);
+        }
+        return;
+    }
     // Z_j = z1^j by a binary product tree
     float2 Z[J];
     Z[0] = make_float2(1.f, 0.f);
@@ -225,21 +252,17 @@ __device__ __forceinline__ void td_produce(const Chirp& c, const float2* E, bool
         *reinterpret_cast<float2*>(mine + j) = make_float2(Z[j].x, Z[j + 1].x);
         *reinterpret_cast<float2*>(mine + J + j) = make_float2(Z[j].y, Z[j + 1].y);
     }
-    // e W_i = e v^i, two interleaved chains by v^2
-    const float2 v2 = csq(v);
+    // c W_i = c v^i, two interleaved chains by v^2
+    float2 W[16];
+    W[0] = coef[0];
+    W[1] = cmul(coef[0], v);
 #pragma unroll
-    for (int e = 0; e < SETS; ++e) {
-        float2 W[16];
-        W[0] = coef[e];
-        W[1] = cmul(coef[e], v);
+    for (int i = 2; i < 16; ++i) W[i] = cmul(W[i - 2], v2);
+    float* dst = mine + 2 * J;
 #pragma unroll
-        for (int i = 2; i < 16; ++i) W[i] = cmul(W[i - 2], v2);
-        float* dst = mine + 2 * J + 32 * e;
-#pragma unroll
-        for (int i = 0; i < 16; i += 2) {
-            *reinterpret_cast<float2*>(dst + i) = make_float2(W[i].x, W[i + 1].x);
-            *reinterpret_cast<float2*>(dst + 16 + i) = make_float2(W[i].y, W[i + 1].y);
-        }
+    for (int i = 0; i < 16; i += 2) {
+        *reinterpret_cast<float2*>(dst + i) = make_float2(W[i].x, W[i + 1].x);
+        *reinterpret_cast<float2*>(dst + 16 + i) = make_float2(W[i].y, W[i + 1].y);
     }
 }
 
@@ -247,14 +270,15 @@ __device__ __forceinline__ void td_produce(const Chirp& c, const float2* E, bool
 // v_mfma_f32_16x16x4f32 (A[i][k] -> lane i + 16 k, B[k][j] -> lane j + 16 k,
 // C[4 (lane / 16) + r][lane % 16] in register r). The four real products of the complex one
 // are kept apart, so no operand needs a sign: K index kk = surfel 4 t + kk; row x =
-// (j = x / 2, Z part x % 2); column tile (set e, W part p), column x = i. Then
-//   Re X = C(re)[(j, re)] - C(im)[(j, im)],  Im X = C(im)[(j, re)] + C(re)[(j, im)].
+// (j = x / 2, Z part x % 2) of the set's A block; column tile (set e, W part p), column x = i.
+// Then Re X = C(re)[(j, re)] - C(im)[(j, im)],  Im X = C(im)[(j, re)] + C(re)[(j, im)].
+// Forward: A = Z, B = c W; JVP: A = e Z per set, B = W shared.
 template <int J, int SETS, int T, int SC>
 __device__ __forceinline__ void td_consume(const float* rec, int cw, int lane, f32x4 (&acc)[SETS][2][4][T]) {
     constexpr int REC = TdRec<J, SETS == 2>::size;
     const int kk = lane >> 4, x = lane & 15;
     const int a_off = (x & 1) * J + (x >> 1);  // Z part x % 2 of j = x / 2
-    const int b_off = 2 * J + x;               // (e W) re of i = x; im at + 16
+    const int b_off = 2 * J * SETS + x;        // W re of i = x; im at + 16
     constexpr int GROUP = 4 * REC + TdRec<J, SETS == 2>::shift;  // dwords per group of 4 records
     const float* base = rec + kk * REC + (4 * cw) * (SC / 4) * GROUP;
 #pragma unroll
@@ -262,21 +286,16 @@ __device__ __forceinline__ void td_consume(const float* rec, int cw, int lane, f
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const float* r = base + (b * (SC / 4) + t) * GROUP;
-            float bv[SETS][2];
+            const float bv[2] = {r[b_off], r[b_off + 16]};
 #pragma unroll
-            for (int e = 0; e < SETS; ++e) {
-                bv[e][0] = r[b_off + 32 * e];
-                bv[e][1] = r[b_off + 32 * e + 16];
-            }
+            for (int tt = 0; tt < T; ++tt)
 #pragma unroll
-            for (int tt = 0; tt < T; ++tt) {
-                const float av = r[a_off + 8 * tt];
-#pragma unroll
-                for (int e = 0; e < SETS; ++e)
+                for (int e = 0; e < SETS; ++e) {
+                    const float av = r[a_off + 2 * J * e + 8 * tt];
 #pragma unroll
                     for (int p = 0; p < 2; ++p)
-                        acc[e][p][b][tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[e][p], acc[e][p][b][tt], 0, 0, 0);
-            }
+                        acc[e][p][b][tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p], acc[e][p][b][tt], 0, 0, 0);
+                }
         }
     }
 }

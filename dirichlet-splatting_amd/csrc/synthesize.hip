@@ -41,16 +41,25 @@ __device__ __forceinline__ double2 cis_cycles(double x) {
     return make_double2(c, s);
 }
 
-// Producer: the record of pair (aperture at p, surfel s) in float64, as groundtruth.py:69-84.
+// Per-surfel inputs of a producer, loaded one chunk ahead.
+struct SySurfel {
+    float cx, cy, cz, nx, ny, nz;
+    double2 amp;
+};
+
+__device__ __forceinline__ SySurfel sy_load(int s, const float* __restrict__ ctr, const float* __restrict__ nrm,
+                                            const double2* __restrict__ amp) {
+    return SySurfel{ctr[3 * s], ctr[3 * s + 1], ctr[3 * s + 2], nrm[3 * s], nrm[3 * s + 1], nrm[3 * s + 2], amp[s]};
+}
+
+// Producer: the record of pair (aperture at p, surfel in) in float64, as groundtruth.py:69-84.
 template <int J>
 __device__ __forceinline__ void sy_produce(const Chirp& c, bool ok, double px, double py, double pz,
-                                           const float* __restrict__ ctr, const float* __restrict__ nrm,
-                                           const double2* __restrict__ amp, int s, double* mine) {
-    const double dx = px - (double)ctr[3 * s], dy = py - (double)ctr[3 * s + 1], dz = pz - (double)ctr[3 * s + 2];
+                                           const SySurfel& in, double* mine) {
+    const double dx = px - (double)in.cx, dy = py - (double)in.cy, dz = pz - (double)in.cz;
     const double r = fmax(sqrt(fma(dx, dx, fma(dy, dy, dz * dz))), 1e-9);
     const double inv = 1.0 / r;
-    const double cosine =
-        fmax(dx * inv * (double)nrm[3 * s] + dy * inv * (double)nrm[3 * s + 1] + dz * inv * (double)nrm[3 * s + 2], 0.0);
+    const double cosine = fmax(dx * inv * (double)in.nx + dy * inv * (double)in.ny + dz * inv * (double)in.nz, 0.0);
     double gain = cosine * cosine * (c.lam_d / (8.0 * kPiD)) * inv;
     if (c.fresnel_d > 1.0) {
         const double root = sqrt(c.fresnel_d * c.fresnel_d - 1.0 + cosine * cosine);
@@ -58,7 +67,7 @@ __device__ __forceinline__ void sy_produce(const Chirp& c, bool ok, double px, d
     }
     const double delay = c.two_over_c0 * r;
     const double2 rot = cis_cycles(c.fc * delay + c.slope * delay * (c.t0 - 0.5 * delay));
-    double2 e = zmul(rot, amp[s]);
+    double2 e = zmul(rot, in.amp);
     const double g = ok ? gain : 0.0;
     e = make_double2(g * e.x, g * e.y);
     const double f = r * c.k0pm * c.inv_N;  // cycles per ADC sample
@@ -147,12 +156,15 @@ __global__ __launch_bounds__(128) void synthesize_kernel(const Chirp c, const fl
         for (int p = 0; p < 2; ++p)
 #pragma unroll
             for (int tt = 0; tt < T; ++tt) acc[b][p][tt] = f64x4{0.0, 0.0, 0.0, 0.0};
+    SySurfel in = sy_load(min(psp, S - 1), ctr, nrm, amp);
     for (int s0 = 0; s0 < S; s0 += SC) {
         const int s = s0 + psp;
+        const SySurfel next = sy_load(min(s + SC, S - 1), ctr, nrm, amp);
         __syncthreads();  // previous chunk consumed (and the twiddle table written)
-        sy_produce<J>(c, p_ok && s < S, px, py, pz, ctr, nrm, amp, min(s, S - 1), rec + tid * SyRec<J>::size);
+        sy_produce<J>(c, p_ok && s < S, px, py, pz, in, rec + tid * SyRec<J>::size);
         __syncthreads();
         sy_consume<J>(rec, wave, lane, acc);
+        in = next;
     }
     __syncthreads();
     // X rows to LDS: registers h and h + 2 of lane (g, i) in tile tt are the Re- and Im-part
