@@ -604,6 +604,10 @@ def main():
     vjp_ms = sum(m[1].elapsed_time(m[2]) for m in marks) / args.steps
     ms_per_step = wall * 1e3 / args.steps
 
+    # the forward-mode product (not part of the metric), timed after the steps
+    dc, dn, dw = (work[k].to(dev) for k in ("dcenters", "dnormals", "dweights"))
+    jvp_ms, _ = timed(lambda: surfel.jvp(cfg, positions, centers, normals, weights, dc, dn, dw, 0.0), 5, stream)
+
     from dsplat import _lib
 
     sample_domain = _lib.library().dsplat_splat_form() == 0 and cfg.adc_samples in (128, 256)
@@ -642,7 +646,7 @@ def main():
             "evals_per_step": evals,
             "parallelism": "single GPU",
         },
-        "kernels_ms": {"render": fwd_ms, "vjp": vjp_ms},
+        "kernels_ms": {"render": fwd_ms, "vjp": vjp_ms, "jvp": jvp_ms},
         "splat_form": "sample-domain MFMA" if sample_domain else "bin-domain VALU",
         "roofline": {
             "kernel": name,

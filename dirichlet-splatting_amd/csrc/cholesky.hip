@@ -18,34 +18,41 @@ namespace dsplat {
 constexpr int kCholBlock = 64;
 constexpr int kCholLd = kCholBlock + 1;  // LDS row stride (doubles): conflict-free columns
 
-// Unblocked Cholesky of the w x w lower block D (LDS, stride kCholLd) by 256 threads:
-// thread t owns row i = t / 4 and the columns k = t % 4 + 4 q of it. Returns the first
-// failing pivot index (or -1) on every thread.
+// Unblocked Cholesky of the w x w lower block D (LDS, stride kCholLd) by wave 0 alone: lane i
+// keeps row i in registers; at step j every lane takes the pivot from LDS, lane i > j scales
+// its entry and publishes it in the LDS column buffer, and every lane subtracts
+// l_ij l_kj from its entries k > j. One wave's LDS accesses complete in order, so the steps
+// need no barrier; the other waves wait at one barrier at the end. Entries above the
+// diagonal of a row are updated too and never read. Returns the first failing pivot index
+// (or -1) on every thread.
 __device__ int factor_block(double* D, int w) {
     __shared__ int fail;
-    const int tid = threadIdx.x, i = tid >> 2, k0 = tid & 3;
-    if (tid == 0) fail = -1;
-    __syncthreads();
-    for (int j = 0; j < w; ++j) {
-        if (tid == 0) {
-            const double d = D[j * kCholLd + j];
-            if (!(d > 0.0) && fail < 0) fail = j;
-            D[j * kCholLd + j] = sqrt(d);
-        }
-        __syncthreads();
-        if (i > j && i < w && k0 == (j & 3)) D[i * kCholLd + j] /= D[j * kCholLd + j];
-        __syncthreads();
-        // trailing lower part: D[i][k] -= D[i][j] D[k][j], j < k <= i
-        if (i > j && i < w) {
-            const double dij = D[i * kCholLd + j];
+    __shared__ double col[kCholBlock];
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const int i = tid;
+        double x[kCholBlock];
 #pragma unroll
-            for (int q = 0; q < kCholBlock / 4; ++q) {
-                const int k = k0 + 4 * q;
-                if (k > j && k <= i) D[i * kCholLd + k] -= dij * D[k * kCholLd + j];
+        for (int k = 0; k < kCholBlock; ++k) x[k] = D[i * kCholLd + k];
+        int first = -1;
+#pragma unroll
+        for (int j = 0; j < kCholBlock; ++j) {
+            if (j < w) {  // uniform
+                col[i] = x[j];  // column j after the updates of steps < j
+                const double d = col[j];
+                if (!(d > 0.0) && first < 0) first = j;
+                const double ljj = sqrt(d);
+                x[j] = i == j ? ljj : x[j] / ljj;
+                col[i] = x[j];
+#pragma unroll
+                for (int k = j + 1; k < kCholBlock; ++k) x[k] = fma(-x[j], col[k], x[k]);
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kCholBlock; ++k) D[i * kCholLd + k] = x[k];
+        if (tid == 0) fail = first;
     }
+    __syncthreads();
     return fail;
 }
 
