@@ -53,13 +53,35 @@ struct Chirp {
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2: linear id b
-// runs on XCD b % 8. The returned logical id gives XCD x the contiguous logical ids
-// [x total / 8, (x + 1) total / 8), so workgroups that are neighbours in logical order (and
-// read the same operands) share one L2. Identity when total is not a multiple of 8.
+// runs on XCD b % 8. The returned logical id gives each XCD one contiguous range of logical
+// ids (XCD x holds total / 8 of them, one more for x < total % 8), so workgroups that are
+// neighbours in logical order (and read the same operands) share one L2.
 __device__ __forceinline__ int xcd_remap(int b, int total) {
     constexpr int kXcds = 8;
-    if (total % kXcds) return b;
-    return (b % kXcds) * (total / kXcds) + b / kXcds;
+    const int x = b % kXcds, i = b / kXcds, per = total / kXcds, rem = total % kXcds;
+    return x < rem ? x * (per + 1) + i : rem * (per + 1) + (x - rem) * per + i;
+}
+
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+// FP16 hi + lo of the pair (x, y): hi = RNE in one packed conversion, then lo = RNE(x - hi)
+// (the difference is exact in FP32) by v_fma_mix{lo,hi}_f16, which read hi as FP16 from its
+// half of the packed register, compute -hi + x in FP32 and round the result into one half of
+// lo: three VALU instructions per pair of values. The compiler does not see the asm as a VALU
+// write, so it inserts no wait states before an MFMA that reads lo: TO_MFMA adds the two a
+// VALU-written VGPR needs before it is an MFMA source (lo stored to LDS needs none).
+template <bool TO_MFMA>
+__device__ __forceinline__ void split_f16_pair(float x, float y, f16x2& hi, f16x2& lo) {
+    hi = f16x2{(_Float16)x, (_Float16)y};
+    const uint32_t h = __builtin_bit_cast(uint32_t, hi);
+    uint32_t l;
+    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(h), "v"(x));
+    if constexpr (TO_MFMA)
+        asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\ts_nop 1"
+            : "+v"(l) : "v"(h), "v"(y));
+    else
+        asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(h), "v"(y));
+    lo = __builtin_bit_cast(f16x2, l);
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -106,23 +106,7 @@ __device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const f
 }
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
-// FP16 hi + lo of the pair (x, y) for MFMA operands: hi = RNE in one packed conversion, then
-// lo = RNE(x - hi) (the difference is exact in FP32) by v_fma_mix{lo,hi}_f16, which read hi
-// as FP16 from its half of the packed register, compute -hi + x in FP32 and round the result
-// into one half of lo: three VALU instructions per pair of values. The compiler does not see
-// the asm as a VALU write and so inserts no wait states before an MFMA that reads lo; the
-// s_nop 1 provides the two that a VALU-written VGPR needs before it is an MFMA source.
-__device__ __forceinline__ void split_f16x2(float x, float y, f16x2& hi, f16x2& lo) {
-    hi = f16x2{(_Float16)x, (_Float16)y};
-    const uint32_t h = __builtin_bit_cast(uint32_t, hi);
-    uint32_t l;
-    asm("v_fma_mixlo_f16 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(h), "v"(x));
-    asm("v_fma_mixhi_f16 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]\n\ts_nop 1"
-        : "+v"(l) : "v"(h), "v"(y));
-    lo = __builtin_bit_cast(f16x2, l);
-}
 
 // Straight copy of ITERS x 256 uint4 from global memory to LDS by the direct global -> LDS
 // loads of gfx950 (global_load_lds_dwordx4): the data never passes through VGPRs and no load
@@ -533,7 +517,7 @@ __device__ __forceinline__ void vjp_stage(const Chirp& c, const float2* E, const
 #pragma unroll
             for (int ap = 0; ap < 2; ++ap) {
                 f16x2 hi, lo;
-                split_f16x2(vals[ap][0], vals[ap][1], hi, lo);
+                split_f16_pair<true>(vals[ap][0], vals[ap][1], hi, lo);
                 st.Bh[ap][kh][2 * qq] = hi.x;
                 st.Bh[ap][kh][2 * qq + 1] = hi.y;
                 st.Bl[ap][kh][2 * qq] = lo.x;

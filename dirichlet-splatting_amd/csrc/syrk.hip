@@ -107,6 +107,18 @@ struct Staged {
 __device__ __forceinline__ void stage_load(Staged& s, const float* __restrict__ rows, int n, int k0, int k_end,
                                            int ci, int cj, bool two, int tid) {
     const int c = tid & (kTile - 1), rg = tid >> 7;
+    if (k0 + kChunk <= k_end && ci + kTile <= n && (!two || cj + kTile <= n)) {
+        // a full chunk of full panels (the common case): plain loads, no per-load predicate;
+        // a diagonal tile (one panel) loads it twice and never reads the second copy
+        const float* pi = rows + (size_t)(k0 + rg * 16) * n + ci + c;
+        const float* pj = rows + (size_t)(k0 + rg * 16) * n + (two ? cj : ci) + c;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            s.v[0][e] = pi[(size_t)e * n];
+            s.v[1][e] = pj[(size_t)e * n];
+        }
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         const int k = k0 + rg * 16 + e;
@@ -121,12 +133,15 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
                                                    int per_split, const unsigned* __restrict__ colmax,
                                                    float* __restrict__ partial) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    // the tiles of one row range run on one XCD: the panels they share come through one L2
+    const int lin = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int tile = lin % gridDim.x, split = lin / gridDim.x;
     int ci, cj;
-    tile_origin(blockIdx.x, n, col0, &ci, &cj);
+    tile_origin(tile, n, col0, &ci, &cj);
     const bool two = ci != cj;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wi = wave >> 1, wj = wave & 1, h = lane >> 5, l32 = lane & 31;
-    const int k_begin = blockIdx.y * per_split, k_end = min(m, k_begin + per_split);
+    const int k_begin = split * per_split, k_end = min(m, k_begin + per_split);
     const int c = tid & (kTile - 1), rg = tid >> 7;
 
     f32x16 acc[2][2];
@@ -209,12 +224,22 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
             for (int p = 0; p < 2; ++p) {
                 const float sc = p == 0 ? scale_i : scale_j;
                 v8 hi[2], lo[2];
+                if constexpr (MODE == 2) {
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const float x = MODE == 2 ? st.v[p][e] * sc : st.v[p][e];
-                    const h16 xh = (h16)x;
-                    hi[e >> 3][e & 7] = xh;
-                    lo[e >> 3][e & 7] = (h16)(x - (float)xh);
+                    for (int e = 0; e < 16; e += 2) {
+                        f16x2 h2, l2;
+                        split_f16_pair<false>(st.v[p][e] * sc, st.v[p][e + 1] * sc, h2, l2);
+                        hi[e >> 3][e & 7] = h2.x, hi[e >> 3][(e & 7) + 1] = h2.y;
+                        lo[e >> 3][e & 7] = l2.x, lo[e >> 3][(e & 7) + 1] = l2.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const float x = st.v[p][e];
+                        const h16 xh = (h16)x;
+                        hi[e >> 3][e & 7] = xh;
+                        lo[e >> 3][e & 7] = (h16)(x - (float)xh);
+                    }
                 }
                 v8* dh = reinterpret_cast<v8*>(col_ptr(b, 0, p, c) + rg * 16);
                 v8* dl = reinterpret_cast<v8*>(col_ptr(b, 1, p, c) + rg * 16);
@@ -260,7 +285,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
         }
     }
 
-    float* dst = partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * kTile * kTile;
+    float* dst = partial + ((size_t)split * gridDim.x + tile) * kTile * kTile;
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
