@@ -86,23 +86,25 @@ __global__ __launch_bounds__(256) void project_kernel(const Chirp c, const float
         for (int x = 0; x < ROWS / 32; ++x)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[x][r] = 0.0f;
-        for (int t = 0; t < steps; ++t) {
-            const int b = t & 1;
-            // 4 bins x (re, im) of this thread's column, as hi / lo
+        // 4 bins x (re, im) of this thread's column for step t, as hi / lo, into X[t & 1]
+        auto generate = [&](int t) {
             f16x8 hi, lo;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int k = t * 8 + gh * 4 + e;
                 const float2 v = column_atom<P>(c, T, col, k, k % P);
-                const _Float16 hr = (_Float16)v.x, hm = (_Float16)v.y;
-                hi[2 * e] = hr;
-                hi[2 * e + 1] = hm;
-                lo[2 * e] = (_Float16)(v.x - (float)hr);
-                lo[2 * e + 1] = (_Float16)(v.y - (float)hm);
+                f16x2 h2, l2;
+                split_f16_pair<false>(v.x, v.y, h2, l2);
+                hi[2 * e] = h2.x, hi[2 * e + 1] = h2.y;
+                lo[2 * e] = l2.x, lo[2 * e + 1] = l2.y;
             }
-            *reinterpret_cast<f16x8*>(xat(b, 0, gc) + 8 * gh) = hi;
-            *reinterpret_cast<f16x8*>(xat(b, 1, gc) + 8 * gh) = lo;
-            __syncthreads();
+            *reinterpret_cast<f16x8*>(xat(t & 1, 0, gc) + 8 * gh) = hi;
+            *reinterpret_cast<f16x8*>(xat(t & 1, 1, gc) + 8 * gh) = lo;
+        };
+        generate(0);
+        __syncthreads();
+        for (int t = 0; t < steps; ++t) {
+            const int b = t & 1;
             const int colx = wave * 32 + l32;
             const f16x8 xh = *reinterpret_cast<const f16x8*>(xat(b, 0, colx) + 8 * h);
             const f16x8 xl = *reinterpret_cast<const f16x8*>(xat(b, 1, colx) + 8 * h);
@@ -115,6 +117,10 @@ __global__ __launch_bounds__(256) void project_kernel(const Chirp c, const float
                 acc[x] = mfma_f16(uh, xl, acc[x]);
                 acc[x] = mfma_f16(ul, xh, acc[x]);
             }
+            // the next step's atoms are generated while these products run on the matrix core;
+            // X[b ^ 1] was last read in step t - 1, before the barrier that ended it
+            if (t + 1 < steps) generate(t + 1);
+            __syncthreads();
         }
         // (Re, Im) row pairs times the column weight; C[row][col] -> acc[r] with
         // row = 8 (r / 4) + 4 h + r % 4, so rows 2j and 2j + 1 are acc[r], acc[r + 1], r even
