@@ -173,7 +173,7 @@ __device__ __forceinline__ Pair pair_geometry_t(const Chirp& c, float px, float 
     const double dz = (double)pz - (double)cz;
     // float64 range from a float32 square root and one float64 Newton correction
     const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
-    const float r0 = sqrtf((float)r2);
+    const float r0 = __builtin_amdgcn_sqrtf((float)r2);  // v_sqrt_f32 (1 ulp): the Newton step fixes it
     const double r = fma(fma(-(double)r0, (double)r0, r2), (double)(0.5f * rcp(r0)), (double)r0) + 1e-12;
     q.r = (float)r;
     float inv = rcp(q.r);
@@ -187,7 +187,7 @@ __device__ __forceinline__ Pair pair_geometry_t(const Chirp& c, float px, float 
     float f = 1.0f, df = 0.0f;
     if constexpr (FRESNEL) {
         const float n = c.fresnel_n;
-        const float root = sqrtf(n * n - 1.0f + q.cosine * q.cosine);
+        const float root = __builtin_amdgcn_sqrtf(n * n - 1.0f + q.cosine * q.cosine);
         const float inv_sum = rcp(root + q.cosine);
         f = (root - q.cosine) * inv_sum;
         df = -2.0f * f * rcp(root);

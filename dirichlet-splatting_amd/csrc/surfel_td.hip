@@ -265,22 +265,31 @@ __device__ __forceinline__ void td_consume(const float* rec, int cw, int lane, f
     const int b_off = 2 * J * SETS + x;        // W re of i = x; im at + 16
     constexpr int GROUP = 4 * REC + TdRec<J, SETS == 2>::shift;  // dwords per group of 4 records
     const float* base = rec + kk * REC + (4 * cw) * (SC / 4) * GROUP;
+    // all operands of a K step are read before its products, so the LDS reads overlap
 #pragma unroll
     for (int t = 0; t < SC / 4; ++t) {
+        float bv[4][2], av[4][T][SETS];
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const float* r = base + (b * (SC / 4) + t) * GROUP;
-            const float bv[2] = {r[b_off], r[b_off + 16]};
+            bv[b][0] = r[b_off];
+            bv[b][1] = r[b_off + 16];
 #pragma unroll
             for (int tt = 0; tt < T; ++tt)
 #pragma unroll
-                for (int e = 0; e < SETS; ++e) {
-                    const float av = r[a_off + 2 * J * e + 8 * tt];
+                for (int e = 0; e < SETS; ++e) av[b][tt][e] = r[a_off + 2 * J * e + 8 * tt];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int tt = 0; tt < T; ++tt)
+#pragma unroll
+                for (int e = 0; e < SETS; ++e)
 #pragma unroll
                     for (int p = 0; p < 2; ++p)
-                        acc[e][p][b][tt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p], acc[e][p][b][tt], 0, 0, 0);
-                }
-        }
+                        acc[e][p][b][tt] =
+                            __builtin_amdgcn_mfma_f32_16x16x4f32(av[b][tt][e], bv[b][p], acc[e][p][b][tt], 0, 0, 0);
     }
 }
 
