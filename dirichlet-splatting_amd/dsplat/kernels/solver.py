@@ -6,7 +6,7 @@ from .. import _lib
 
 
 def backward_elimination(inverse, b, count, limited=None, limit=0):
-    """Removed unknown indices [count] int64 of the one-launch backward elimination;
+    """Removed unknown indices [count] int64 of the on-device backward elimination;
     inverse [n, n] float64 real, b [n] = inverse q (replacement.py:104-139)."""
     n = len(inverse)
     removed = torch.zeros(count, dtype=torch.int64, device=inverse.device)
@@ -14,7 +14,7 @@ def backward_elimination(inverse, b, count, limited=None, limit=0):
         return removed
     size = _lib.library().dsplat_elimination_workspace(n, count)
     if size == 0:
-        raise ValueError(f"backward elimination kernel supports n <= 4096, count <= n; got {n}, {count}")
+        raise ValueError(f"backward elimination kernel supports count <= n and count <= 4096; got {n}, {count}")
     workspace = torch.empty(size // 8 + 1, dtype=torch.float64, device=inverse.device)
     flags = None if limited is None else limited.to(torch.int32).contiguous()
     _lib.call(

@@ -315,13 +315,14 @@ int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stre
 int dsplat_cholesky_solve(const double* l, int32_t n, int32_t lda, double* b, int32_t nrhs, int32_t ldb,
                           void* stream);
 
-/* Backward elimination of `count` unknowns from a least-squares solution, in one launch:
- * inverse [n, n] float64 (of the regularized real Gram), b [n] = inverse q. Each step
- * removes the unknown with the smallest b_j^2 / W_jj (ties to the lower index) among those
- * still allowed, keeping the exact inverse of the rest by rank-one corrections; at most
- * `limit` unknowns with limited[j] != 0 leave (limited may be NULL). removed [count] int64
- * in order. n <= 4096, count <= n. Replaces the host loop of
- * replacement.py:104-139 (backward_elimination). */
+/* Backward elimination of `count` unknowns from a least-squares solution, one launch per
+ * step on the stream (no host synchronization): inverse [n, n] float64 (of the regularized
+ * real Gram), b [n] = inverse q. Each step removes the unknown with the smallest
+ * b_j^2 / W_jj (ties to the lower index) among those still allowed, keeping the exact
+ * inverse of the rest by rank-one corrections; at most `limit` unknowns with
+ * limited[j] != 0 leave (limited may be NULL). removed [count] int64 in order.
+ * count <= n, count <= 4096. Replaces the host loop of replacement.py:104-139
+ * (backward_elimination). */
 size_t dsplat_elimination_workspace(int32_t n, int32_t count);
 int dsplat_backward_elimination(const double* inverse, const double* b, int32_t n, int32_t count,
                                 const int32_t* limited, int32_t limit, int64_t* removed, void* workspace,

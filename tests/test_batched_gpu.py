@@ -40,7 +40,9 @@ def test_batched_orientation_choice_equals_the_chunked_loop(fmcw, mode, use_tabl
     assert (chosen == values.argmax(1)).float().mean() > 0.99
 
 
-@pytest.mark.parametrize("n,count,limit", [(300, 40, None), (300, 40, 5), (2304, 256, None), (37, 37, None)])
+@pytest.mark.parametrize(
+    "n,count,limit", [(300, 40, None), (300, 40, 5), (2304, 256, None), (37, 37, None), (2500, 100, 0)]
+)
 def test_elimination_kernel_equals_the_host_loop(n, count, limit):
     from dsplat.replacement import backward_elimination
 
