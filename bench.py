@@ -113,6 +113,76 @@ def cpu_baseline(work, target_seconds):
     }
 
 
+def torch_fp32_splat(positions, centers, normals, weights, cfg):
+    """The reference's float32 direct-form splat as torch tensor operations, differentiable by
+    autograd: the column model of surfel.slang:63-142 (amp = cos^2 lambda / (8 pi r) for cos > 0,
+    the carrier phase 2 pi (fc tau + slope tau (t0 - tau / 2)) in float32) times the closed-form
+    Dirichlet kernel of common.slang:10-45, D(x) = sin(pi S x / N) / (S sin(pi x / N))
+    e^{-i pi (S - 1) x / N} with x = k - k0 wrapped to [-N/2, N/2) and its Taylor form near 0."""
+    import math
+
+    import torch
+
+    S, N, K = cfg.adc_samples, cfg.n_fft, cfg.num_range_bins
+    offset = positions[:, None, :] - centers[None]
+    r = offset.norm(dim=-1)
+    cosine = ((offset / r[..., None]) * normals[None]).sum(-1)
+    amp = torch.where(cosine > 0, cosine.square() * (cfg.lambda_m / (8 * math.pi)) / r, torch.zeros_like(r))
+    tau = 2 * r / cfg.c0_m_s
+    phase = 2 * math.pi * (cfg.fc_hz * tau + cfg.slope_hz_s * tau * (cfg.adc_start_time_s - 0.5 * tau))
+    coef = torch.polar(amp, phase) * weights[None]
+    x = torch.arange(K, dtype=r.dtype)[None, None] - (r / cfg.range_bin_spacing_m)[..., None]
+    x = torch.remainder(x + N / 2, N) - N / 2
+    small = x.abs() < 1e-3
+    den = torch.where(small, torch.ones_like(x), S * torch.sin(math.pi * x / N))
+    ratio = torch.where(small, torch.ones_like(x), torch.sin(math.pi * S * x / N) / den)
+    D = torch.polar(ratio.abs(), -math.pi * (S - 1) * x / N) * torch.sign(ratio)
+    return (coef[..., None] * D).sum(1)
+
+
+def cpu_torch_fp32(work, target_seconds):
+    """The reference's float32 model in torch on every host core: render + autograd VJP on a
+    bounded aperture sample, and its relative L2 error against the float64 oracle."""
+    import numpy as np
+    import torch
+
+    from oracle import surfel as O
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    cfg = work["config"]
+    pos, c, n, w, up = (work[k] for k in ("positions", "centers", "normals", "weights", "cotangent"))
+
+    def run(ids):
+        t0 = time.perf_counter()
+        outs = []
+        for chunk in np.array_split(ids, max(1, len(ids) // 8)):
+            cc = c.clone().requires_grad_(True)
+            nn = n.clone().requires_grad_(True)
+            ww = w.clone().requires_grad_(True)
+            out = torch_fp32_splat(pos[chunk], cc, nn, ww, cfg)
+            (up[chunk].conj() * out).real.sum().backward()
+            outs.append(out.detach())
+        return time.perf_counter() - t0, torch.cat(outs)
+
+    probe = np.arange(0, len(pos), len(pos) // 8)[:8]
+    run(probe)  # first-call set-up
+    seconds, _ = run(probe)
+    count = int(min(len(pos), max(8, 8 * target_seconds / max(seconds, 1e-3))))
+    ids = np.linspace(0, len(pos) - 1, count).astype(np.int64)
+    seconds, out = run(ids)
+    check = np.arange(0, count, max(1, count // 32))[:32]  # the float64 oracle on 32 of them
+    want = O.render(pos[ids[check]].numpy(), c.numpy(), n.numpy(), w.numpy(), cfg)
+    rel = float(np.linalg.norm(out[check].numpy() - want) / np.linalg.norm(want))
+    model = next((ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")), "?")
+    evals = count * cfg.num_range_bins * len(c)
+    return {
+        "value": evals / seconds, "unit": "evals/s", "threads": torch.get_num_threads(), "cpu": model,
+        "torch": torch.__version__, "rel_l2_forward_vs_float64": rel,
+        "sample": f"float32 torch restatement of the reference model, render + autograd VJP of {count} of "
+        f"{len(pos)} apertures ({evals:.3e} evals, {seconds:.1f} s)",
+    }  # fmt: skip
+
+
 def timed(fn, repeats, stream):
     """Mean milliseconds of fn() between HIP events on the launch stream, after one warm-up."""
     import torch
@@ -664,6 +734,7 @@ def main():
     }
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         result["cpu_baseline"] = cpu_baseline(work, args.cpu_seconds)
+        result["cpu_torch_fp32"] = cpu_torch_fp32(work, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 
