@@ -87,6 +87,10 @@ __device__ __forceinline__ void split_f16_pair(float x, float y, f16x2& hi, f16x
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// acc + a b as four FMAs (a separate product and sum would take two more instructions)
+__device__ __forceinline__ float2 cmac(float2 acc, float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x)), fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y)));
+}
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // conj(a) * b
     return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }

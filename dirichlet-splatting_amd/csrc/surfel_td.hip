@@ -621,8 +621,8 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
                     float2 za0 = cur.Z[j0], za1 = cur.Z[j1];
                     swap_halves(za0.x, za1.x);
                     swap_halves(za0.y, za1.y);
-                    P[0][mat] += cmul(za0, make_float2(acc[0][tile][rr], acc[0][tile][rr + 1]));
-                    P[1][mat] += cmul(za1, make_float2(acc[1][tile][rr], acc[1][tile][rr + 1]));
+                    P[0][mat] = cmac(P[0][mat], za0, make_float2(acc[0][tile][rr], acc[0][tile][rr + 1]));
+                    P[1][mat] = cmac(P[1][mat], za1, make_float2(acc[1][tile][rr], acc[1][tile][rr + 1]));
                 }
             // undo the per-aperture operand scales, then each half completes the sums of its own
             // pair's aperture (a + h) with the other half's share: after the swap, P[0] + P[1]
@@ -952,8 +952,8 @@ __global__ __launch_bounds__(256) void scan_td_kernel(const Chirp c, const TdSca
                 float2 za0 = cur.Z[j0], za1 = cur.Z[j1];
                 swap_halves(za0.x, za1.x);
                 swap_halves(za0.y, za1.y);
-                P[0][mat] += cmul(za0, make_float2(acc[0][rr], acc[0][rr + 1]));
-                P[1][mat] += cmul(za1, make_float2(acc[1][rr], acc[1][rr + 1]));
+                P[0][mat] = cmac(P[0][mat], za0, make_float2(acc[0][rr], acc[0][rr + 1]));
+                P[1][mat] = cmac(P[1][mat], za1, make_float2(acc[1][rr], acc[1][rr + 1]));
             }
             P[0][0] = Sc[ga] * P[0][0];
             P[1][0] = Sc[ga + 1] * P[1][0];
