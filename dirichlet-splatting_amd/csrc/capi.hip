@@ -419,6 +419,32 @@ int dsplat_project_design(const dsplat_fmcw* cfg, const float* positions, int32_
                           (hipStream_t)stream);
 }
 
+int32_t dsplat_project_form(const dsplat_fmcw* cfg, int32_t rows) {
+    Chirp c;
+    if (!make_chirp(cfg, &c)) return -1;
+    if (project_td_supported(c, rows)) return 0;
+    return project_supported(c, rows) ? 1 : -1;
+}
+
+int dsplat_sample_basis(const dsplat_fmcw* cfg, const float* basis, int32_t rows, float* out, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || !project_td_supported(c, rows)) return hipErrorInvalidValue;
+    if (!basis || !out) return hipErrorInvalidValue;
+    return launch_sample_basis(c, basis, rows, out, (hipStream_t)stream);
+}
+
+int dsplat_project_sampled(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                           const float* centers, const float* normals, const float* areas, int32_t num_surfels,
+                           const float* sample_basis, int32_t rows, float* out, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0 || !project_td_supported(c, rows))
+        return hipErrorInvalidValue;
+    if (num_apertures == 0 || num_surfels == 0) return hipSuccess;
+    if (!positions || !centers || !normals || !areas || !sample_basis || !out) return hipErrorInvalidValue;
+    return launch_project_td(c, positions, num_apertures, centers, normals, areas, num_surfels, sample_basis, rows,
+                             out, (hipStream_t)stream);
+}
+
 size_t dsplat_qn_workspace(const dsplat_fmcw* cfg, int32_t num_apertures) {
     Chirp c;
     if (!make_chirp(cfg, &c) || num_apertures < 0) return 0;

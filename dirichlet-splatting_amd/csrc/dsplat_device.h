@@ -339,4 +339,35 @@ __device__ __forceinline__ void amp_gradient(const Pair& q, float nx, float ny, 
     da[2] = (q.amp * q.uz + q.amp_cos * (q.cosine * q.uz - nz)) * q.inv_r;
 }
 
+// ---- sample-domain helpers (n = 16 j + i power factorizations) ----
+
+__device__ __forceinline__ float2 csq(float2 a) { return make_float2(a.x * a.x - a.y * a.y, 2.0f * a.x * a.y); }
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+
+// v_permlane32_swap: lanes 32-63 of a trade places with lanes 0-31 of b. With a = x and
+// b = y held by both lane halves, afterwards a = (x of half 0, y of half 0) and
+// b = (x of half 1, y of half 1) across the two halves.
+__device__ __forceinline__ void swap_halves(float& a, float& b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+// e^{2 pi i m / N} for m in [0, N), exactly rounded from float64.
+__device__ __forceinline__ void fill_twiddles(float2* E, int N, int tid, int nthreads) {
+    for (int m = tid; m < N; m += nthreads) {
+        double s, c;
+        sincospi(2.0 * (double)m / (double)N, &s, &c);
+        E[m] = make_float2((float)c, (float)s);
+    }
+}
+
+// e^{2 pi i (q lobe + q delta) / N} = E[q lobe mod N] * e^{2 pi i q delta / N}, |q delta| <= q / 2.
+// The small rotation needs |2 pi q delta / N| <= pi / 4: q <= N / 4.
+__device__ __forceinline__ float2 rotation(const Chirp& ch, const float2* E, int lobe, float delta, int q) {
+    float s, c;
+    sincos_small(ch.two_pi_N * (float)q * delta, &s, &c);
+    return cmul(E[(q * lobe) & (ch.N - 1)], make_float2(c, s));
+}
+
 }  // namespace dsplat

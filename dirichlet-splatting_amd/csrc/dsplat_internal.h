@@ -117,6 +117,10 @@ hipError_t launch_gram(const Chirp& c, int mode, const float* pos, int A, const 
 bool project_supported(const Chirp& c, int rows);
 hipError_t launch_project(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                           const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st);
+bool project_td_supported(const Chirp& c, int rows);
+hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, float* out, hipStream_t st);
+hipError_t launch_project_td(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                             const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st);
 typedef dsplat_qn_spec QnSpec;
 size_t qn_workspace(int A, int P, int K);
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,

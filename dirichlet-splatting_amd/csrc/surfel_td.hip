@@ -43,35 +43,6 @@ constexpr int kTdChunk = 16;      // forward: surfels per LDS chunk
 constexpr int kTdVjpSurfels = 128;
 constexpr int kTdVjpGroup = 16;   // VJP: apertures staged per LDS fill
 
-__device__ __forceinline__ float2 csq(float2 a) { return make_float2(a.x * a.x - a.y * a.y, 2.0f * a.x * a.y); }
-__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
-
-// v_permlane32_swap: lanes 32-63 of a trade places with lanes 0-31 of b. With a = x and
-// b = y held by both lane halves, afterwards a = (x of half 0, y of half 0) and
-// b = (x of half 1, y of half 1) across the two halves.
-__device__ __forceinline__ void swap_halves(float& a, float& b) {
-    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
-    a = __uint_as_float(r[0]);
-    b = __uint_as_float(r[1]);
-}
-
-// e^{2 pi i m / N} for m in [0, N), exactly rounded from float64.
-__device__ __forceinline__ void fill_twiddles(float2* E, int N, int tid, int nthreads) {
-    for (int m = tid; m < N; m += nthreads) {
-        double s, c;
-        sincospi(2.0 * (double)m / (double)N, &s, &c);
-        E[m] = make_float2((float)c, (float)s);
-    }
-}
-
-// e^{2 pi i (q lobe + q delta) / N} = E[q lobe mod N] * e^{2 pi i q delta / N}, |q delta| <= q / 2.
-// The small rotation needs |2 pi q delta / N| <= pi / 4: q <= N / 4.
-__device__ __forceinline__ float2 rotation(const Chirp& ch, const float2* E, int lobe, float delta, int q) {
-    float s, c;
-    sincos_small(ch.two_pi_N * (float)q * delta, &s, &c);
-    return cmul(E[(q * lobe) & (ch.N - 1)], make_float2(c, s));
-}
-
 // out[r][q] = scale sum_{p < Lin} e^{sign 2 pi i p q / N} X[r][p] for the 16 rows of X (LDS,
 // row stride ldx float2) and q < Lout (multiple of 8), one 16 x 8-bin tile per wave at a time
 // on v_mfma_f32_16x16x4f32. Lane layouts (gfx950): A[i][k] -> lane i + 16 k, B[k][j] -> lane

@@ -124,6 +124,9 @@ SIGNATURES = {
          c_ptr],
     ),
     "dsplat_project_design": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr]),
+    "dsplat_project_form": (c_int, [c_ptr, c_int]),
+    "dsplat_sample_basis": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
+    "dsplat_project_sampled": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr]),
     "dsplat_qn_workspace": (c_size, [c_ptr, c_int]),
     "dsplat_qn_search": (
         c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]

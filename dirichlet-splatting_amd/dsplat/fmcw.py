@@ -246,10 +246,11 @@ class FMCWModel:
         bins = self.shape[1]
         apertures = max(1, chunk // bins)
         basis = self.basis.interleaved
+        sampled = surfel.kernel.sample_basis(config, basis, self.fresnel_n)  # once per pass
         return (
             surfel.kernel.project_design(
                 config, positions[a0 : a0 + apertures], params["centers"], params["normals"], params["areas"], basis,
-                self.fresnel_n,
+                self.fresnel_n, sampled,
             )  # fmt: skip
             for a0 in range(0, len(positions), apertures)
         )

@@ -280,17 +280,35 @@ def correlate_orientations(config, positions, centers, normals, bank, residual, 
     return torch.complex(output[..., 0], output[..., 1]), output[..., 2]
 
 
-def project_design(config, positions, centers, normals, areas, basis, fresnel_n):
+def sample_basis(config, basis, fresnel_n=0.0):
+    """Sample-domain form [rows, 2 * adc_samples] of an interleaved real row basis [rows, 2K]
+    (B = U^H F, F[k][n] = e^{-2 pi i n k / N}, float64 sums) for project_design's `sampled`
+    argument, or None when only the bin-domain projection serves this chirp."""
+    rows = basis.shape[0]
+    cfg = _lib.fmcw(config, fresnel_n)
+    if _lib.library().dsplat_project_form(_lib.ctypes.byref(cfg), rows) != 0:
+        return None
+    out = torch.empty((rows, 2 * int(config.adc_samples)), dtype=torch.float32, device=basis.device)
+    _lib.call(
+        "dsplat_sample_basis", _lib.ctypes.byref(cfg), _lib.ptr(basis.float().contiguous()), rows, _lib.ptr(out),
+        _lib.stream(basis.device),
+    )  # fmt: skip
+    return out
+
+
+def project_design(config, positions, centers, normals, areas, basis, fresnel_n, sampled=None):
     """Row-projected real design [rows * A, S] (row j * A + a) of a basis [rows, 2K] in its
-    interleaved real form, atoms generated on chip (fmcw.py:243-261 blocks + reduce_major)."""
+    interleaved real form, atoms generated on chip (fmcw.py:243-261 blocks + reduce_major).
+    With `sampled` (sample_basis of the same basis) the sample-domain kernel runs instead."""
     _check(positions, centers, normals)
     rows = basis.shape[0]
     out = centers.new_empty((rows * len(positions), len(centers)))
     cfg = _lib.fmcw(config, fresnel_n)
+    name, rows_basis = ("dsplat_project_design", basis) if sampled is None else ("dsplat_project_sampled", sampled)
     _lib.call(
-        "dsplat_project_design", _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
+        name, _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
         _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()),
-        len(centers), _lib.ptr(basis.float().contiguous()), rows, _lib.ptr(out), _lib.stream(centers.device),
+        len(centers), _lib.ptr(rows_basis.float().contiguous()), rows, _lib.ptr(out), _lib.stream(centers.device),
     )  # fmt: skip
     return out
 

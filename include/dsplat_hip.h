@@ -264,6 +264,28 @@ int dsplat_project_design(const dsplat_fmcw* cfg, const float* positions, int32_
                           int32_t num_surfels, const float* basis, int32_t rows, float* out,
                           void* stream);
 
+/* Which projection kernel serves cfg and rows: 0 for dsplat_project_sampled (sample-domain
+ * splat form in effect and 128 or 256 ADC samples), 1 for dsplat_project_design only, -1 for
+ * neither (invalid configuration or odd rows). */
+int32_t dsplat_project_form(const dsplat_fmcw* cfg, int32_t rows);
+
+/* Sample-domain form of a row basis: out [rows, 2 * samples] = basis [rows, 2K] times the real
+ * 2x2-block form of F[k][n] = e^{-2 pi i n k / N}, summed in float64 and rounded once; row
+ * pairs keep the interleaved (Re, Im) layout. The input of dsplat_project_sampled; a basis is
+ * converted once and reused for every block. Same role as dsplat_project_design's basis
+ * (fmcw.py:95-98 reduce_major). */
+int dsplat_sample_basis(const dsplat_fmcw* cfg, const float* basis, int32_t rows, float* out, void* stream);
+
+/* dsplat_project_design's output (same layout, same rows) from the sample-domain basis of
+ * dsplat_sample_basis: the atoms' sample-domain powers v^n are projected on the matrix cores
+ * and never formed per bin. hipErrorInvalidValue unless dsplat_project_form is 0. Replaces the
+ * design + reduce_major product of the streamed blocks (fmcw.py:243-261,
+ * surfel.slang:313-378). */
+int dsplat_project_sampled(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                           const float* centers, const float* normals, const float* areas,
+                           int32_t num_surfels, const float* sample_basis, int32_t rows, float* out,
+                           void* stream);
+
 /* ---- Batched quasi-Newton certificate search. */
 
 /* Per-search constants of dsplat_qn_search, from the parameter specs (certificate.py:75-86

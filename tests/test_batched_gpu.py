@@ -170,6 +170,58 @@ def test_projected_design_equals_the_projected_dense_design(pad, fresnel_n):
     assert err < 2e-6, err
 
 
+@pytest.mark.parametrize(
+    "samples,pad,fresnel_n,count,spread",
+    [(128, 2, 0.0, 300, 0.03), (128, 2, 1.6, 515, 0.15), (256, 2, 0.0, 260, 0.03), (128, 4, 0.0, 257, 0.05),
+     (256, 1, 1.6, 100, 0.1), (128, 1, 0.0, 33, 0.01)],
+)  # fmt: skip
+def test_sampled_projection_matches_the_float64_projected_design(samples, pad, fresnel_n, count, spread):
+    """Sample-domain projection (B = U^H F on the powers v^n) against the float64 oracle's real
+    design times the same float32 basis; its sample basis against B formed from the complex
+    U^H and e^{-2 pi i n k / N} directly. 25 apertures leave a last unit of one aperture; the
+    surfel counts leave partial 256-column tiles; the wide spreads give ranks above 64 (a
+    second 128-row pass)."""
+    import numpy as np
+    from dsplat import FMCWConfig
+    from dsplat.fmcw import response_basis
+    from dsplat.kernels import surfel
+    from oracle import surfel as O
+
+    cfg = FMCWConfig(adc_samples=samples, pad_factor=pad)
+    positions = aperture_grid(5)
+    p = surfels(count, seed=samples + pad + count, spread=spread)
+    p["areas"] = p["areas"] * torch.linspace(0.5, 2.0, count, device="cuda")
+    k0 = torch.cdist(p["centers"], positions) / cfg.range_bin_spacing_m
+    basis = response_basis(cfg, float(k0.min()) - 1, float(k0.max()) + 1, "cuda")
+    assert basis is not None
+    rows, K, N = basis.interleaved.shape[0], cfg.num_range_bins, cfg.n_fft
+    sampled = surfel.sample_basis(cfg, basis.interleaved, fresnel_n)
+    assert sampled is not None and sampled.shape == (rows, 2 * samples)
+
+    b = basis.interleaved.double().cpu().numpy()
+    uh = b[0::2, 0::2] - 1j * b[0::2, 1::2]  # row 2j holds (Re, -Im) of U^H
+    f = np.exp(-2j * np.pi * np.outer(np.arange(K), np.arange(samples)) / N)
+    bs = uh @ f
+    want_b = np.empty((rows, 2 * samples))
+    want_b[0::2, 0::2], want_b[0::2, 1::2] = bs.real, -bs.imag
+    want_b[1::2, 0::2], want_b[1::2, 1::2] = bs.imag, bs.real
+    got_b = sampled.double().cpu().numpy()
+    assert np.linalg.norm(got_b - want_b) / np.linalg.norm(want_b) < 1e-7
+
+    got = surfel.project_design(
+        cfg, positions, p["centers"], p["normals"], p["areas"], basis.interleaved, fresnel_n, sampled
+    ).double().cpu().numpy()
+    dense = O.design(
+        positions.cpu().numpy(), p["centers"].cpu().numpy(), p["normals"].cpu().numpy(), p["areas"].cpu().numpy(),
+        cfg, fresnel_n, True,
+    )  # fmt: skip
+    want = (b @ dense.reshape(2 * K, -1)).reshape(rows * len(positions), count)
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    assert err < 2e-6, (rows, err)
+    binned = surfel.project_design(cfg, positions, p["centers"], p["normals"], p["areas"], basis.interleaved, fresnel_n)
+    assert np.linalg.norm(binned.double().cpu().numpy() - got) / np.linalg.norm(want) < 3e-6
+
+
 def test_fused_normal_equations_match_the_plain_stream(fmcw):  # noqa: F811
     from dsplat import DSFWConfig
     from dsplat.varpro import normal_equations
