@@ -373,7 +373,7 @@ hipError_t launch_correlate(const Chirp& c, int W, const float* pos, int A, cons
 // per-block partials. The aperture sums accumulate in float32 in aperture order.
 constexpr int kOrientMax = 25;  // own normal + up to 24 bank normals
 
-template <int P, bool TABLE>
+template <int P, bool TABLE, bool FRESNEL>
 __global__ __launch_bounds__(256) void orient_best_kernel(
     const Chirp c, const float* __restrict__ pos, int A, const float* __restrict__ ctr, const float* __restrict__ nrm,
     const float* __restrict__ areas, const float* __restrict__ bank, int B, int S, const float2* __restrict__ res,
@@ -391,22 +391,23 @@ __global__ __launch_bounds__(256) void orient_best_kernel(
 #pragma unroll
     for (int j = 0; j < kOrientMax; ++j) acc[j][0] = acc[j][1] = acc[j][2] = 0.0f;
     for (int i = 3 * (B + 1) + tid; i < 3 * kOrientMax; i += 256) dirs[i] = 0.0f;  // unused slots: zero normals
-    const float fres = c.fresnel_n;
+    // amp = lam / (8 pi r) cos^2 F(cos) for cos > 0, else 0 (scattering.slang:16-38), with the
+    // Fresnel factor F = (root - cos) / (root + cos) = (root - cos)^2 / (n^2 - 1),
+    // root = sqrt(n^2 - 1 + cos^2): no division per normal, and cos <= 0 enters as 0.
+    const float n2m1 = FRESNEL ? c.fresnel_n * c.fresnel_n - 1.0f : 1.0f;
+    const float amp_scale = c.lam * (1.0f / (8.0f * kPi)) / n2m1;
     auto accumulate = [&](const Pair& q, float2 corr, float zn) {
+        const float pre = amp_scale * q.inv_r;
         // every slot is evaluated (a zero normal adds nothing), so the sums stay in registers
 #pragma unroll
         for (int j = 0; j < kOrientMax; ++j) {
             const float mx = j == 0 ? nx : dirs[3 * j], my = j == 0 ? ny : dirs[3 * j + 1];
             const float mz = j == 0 ? nz : dirs[3 * j + 2];
-            const float cosine = q.ux * mx + q.uy * my + q.uz * mz;
-            float amp = 0.0f;
-            if (cosine > 0.0f) {
-                float f = 1.0f;
-                if (fres > 1.0f) {
-                    const float root = sqrtf(fres * fres - 1.0f + cosine * cosine);
-                    f = (root - cosine) / (root + cosine);
-                }
-                amp = c.lam * q.inv_r * (1.0f / (8.0f * kPi)) * cosine * cosine * f;
+            const float cp = fmaxf(q.ux * mx + q.uy * my + q.uz * mz, 0.0f);
+            float amp = pre * cp * cp;
+            if constexpr (FRESNEL) {
+                const float d = __builtin_amdgcn_sqrtf(fmaf(cp, cp, n2m1)) - cp;  // v_sqrt_f32, 1 ulp
+                amp *= d * d;
             }
             acc[j][0] = fmaf(amp, corr.x, acc[j][0]);
             acc[j][1] = fmaf(amp, corr.y, acc[j][1]);
@@ -417,7 +418,8 @@ __global__ __launch_bounds__(256) void orient_best_kernel(
     if constexpr (TABLE) {
         const int size = c.N * oversample;
         for (int a = 0; a < A; ++a) {
-            const Pair q = pair_geometry(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
+            const Pair q =
+                pair_geometry_t<FRESNEL>(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
             const double x = (double)q.r * c.k0pm * oversample;
             const double fl = floor(x);
             const float tt = (float)(x - fl);
@@ -444,7 +446,8 @@ __global__ __launch_bounds__(256) void orient_best_kernel(
             const int na = min(kCorApertures, A - a0);
             for (int al = 0; al < na; ++al) {
                 const int a = a0 + al;
-                const Pair q = pair_geometry(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
+                const Pair q =
+                    pair_geometry_t<FRESNEL>(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
                 const Sums z = pair_sums<P, false>(c, tile, al, q);
                 accumulate(q, cmulc(q.rot, z.g0), z.nrm);
             }
@@ -470,23 +473,36 @@ __global__ __launch_bounds__(256) void orient_best_kernel(
     best_index[s] = index;
 }
 
+template <int P, bool F>
+static hipError_t launch_orient_best_pf(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                        const float* bank, int B, int S, const float2* res, const float2* table,
+                                        const float* norm_table, int oversample, const OrientPick& pick,
+                                        hipStream_t st) {
+    const dim3 grid((S + 255) / 256);
+    hipError_t e;
+    if (table) {
+        hipLaunchKernelGGL((orient_best_kernel<P, true, F>), grid, dim3(256), 0, st, c, pos, A, ctr, nrm, pick.areas,
+                           bank, B, S, res, table, norm_table, oversample, pick.real_part, pick.value, pick.index);
+    } else {
+        const size_t smem = cor_smem(c);
+        if ((e = allow_lds((const void*)orient_best_kernel<P, false, F>, smem)) != hipSuccess) return e;
+        hipLaunchKernelGGL((orient_best_kernel<P, false, F>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm,
+                           pick.areas, bank, B, S, res, table, norm_table, oversample, pick.real_part, pick.value,
+                           pick.index);
+    }
+    return hipGetLastError();
+}
+
 template <int P>
 static hipError_t launch_orient_best_p(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                        const float* bank, int B, int S, const float2* res, const float2* table,
                                        const float* norm_table, int oversample, const OrientPick& pick,
                                        hipStream_t st) {
-    const dim3 grid((S + 255) / 256);
-    hipError_t e;
-    if (table) {
-        hipLaunchKernelGGL((orient_best_kernel<P, true>), grid, dim3(256), 0, st, c, pos, A, ctr, nrm, pick.areas,
-                           bank, B, S, res, table, norm_table, oversample, pick.real_part, pick.value, pick.index);
-    } else {
-        const size_t smem = cor_smem(c);
-        if ((e = allow_lds((const void*)orient_best_kernel<P, false>, smem)) != hipSuccess) return e;
-        hipLaunchKernelGGL((orient_best_kernel<P, false>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, pick.areas,
-                           bank, B, S, res, table, norm_table, oversample, pick.real_part, pick.value, pick.index);
-    }
-    return hipGetLastError();
+    if (c.fresnel_n > 1.0f)
+        return launch_orient_best_pf<P, true>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample,
+                                              pick, st);
+    return launch_orient_best_pf<P, false>(c, pos, A, ctr, nrm, bank, B, S, res, table, norm_table, oversample, pick,
+                                           st);
 }
 
 hipError_t launch_orient_best(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,

@@ -9,11 +9,18 @@ from test_solver_gpu import aperture_grid, close, fmcw, fmcw_specs, surfels  # n
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("mode,use_table", [("complex", False), ("real", False), ("complex", True)])
-def test_batched_orientation_choice_equals_the_chunked_loop(fmcw, mode, use_table):  # noqa: F811
+@pytest.mark.parametrize(
+    "mode,use_table,fresnel_n",
+    [("complex", False, 0.0), ("real", False, 0.0), ("complex", True, 0.0), ("complex", True, 1.6),
+     ("real", False, 1.6)],
+)  # fmt: skip
+def test_batched_orientation_choice_equals_the_chunked_loop(fmcw, mode, use_table, fresnel_n):  # noqa: F811
+    from dataclasses import replace
+
     from dsplat.certificate import fibonacci_sphere, orient_candidates
 
     model, params, _ = fmcw
+    model = replace(model, fresnel_n=fresnel_n)
     g = torch.Generator().manual_seed(11)
     count = 3000
     candidates = dict(
