@@ -35,14 +35,26 @@ __global__ __launch_bounds__(256) void range_gate_kernel(const float* __restrict
 // Greedy best-first separated peaks (search.slang:27-64): repeatedly take the best finite
 // score (ties to the lower index) and drop every candidate closer than the separation.
 // The same selection is made here from the candidates sorted by (score desc, index asc):
-// a candidate is taken iff no earlier taken one lies within the separation. Blocks of 256
-// sorted candidates are resolved at a time: each thread tests its candidate against all
-// peaks taken so far, the block's pairwise conflicts go into bitmasks, and one thread
-// walks the block in order. Distances are formed exactly as in the serial definition,
-// d = c_candidate - c_taken accumulated by fmaf over the coordinates.
+// a candidate is taken iff no earlier taken one lies within the separation. Distances are
+// formed exactly as in the serial definition, d = c_candidate - c_taken accumulated by fmaf
+// over the coordinates.
+//
+// One workgroup. Windows of 2048 sorted candidates are first tested in parallel against the
+// peaks taken so far, and the survivors go, in sorted order, into an LDS queue: a candidate
+// close to a taken peak can never be taken, so this only skips work. Blocks of 256 queued
+// survivors are then resolved at a time: each is tested again against every peak taken
+// since, each records which later candidates of the block clash with it (bitmasks), and
+// one wave walks the block in order, one turn per taken peak. Past the first peaks most
+// sorted candidates sit next to a taken one, so most windows add few survivors and the
+// blocks (several barriers each) stay few.
 
 constexpr int kPeakBlock = 256;
 constexpr int kPeakLdsFloats = 8192;  // taken centers cached in LDS (32 KB); the rest in the spill
+constexpr int kPeakPer = 4;           // window candidates per thread
+constexpr int kPeakChainLds = 4096;   // bucket chain links of the first taken peaks in LDS
+constexpr int kPeakWindow = kPeakPer * kPeakBlock;
+constexpr int kPeakQueue = kPeakWindow + kPeakBlock;  // fewer than 256 left over plus one window
+constexpr int kPeakMaxDim = 8;
 
 __global__ __launch_bounds__(256) void peak_keys(const float* __restrict__ scores, int count,
                                                  float* __restrict__ keys, int* __restrict__ ids) {
@@ -55,10 +67,12 @@ __global__ __launch_bounds__(256) void peak_keys(const float* __restrict__ score
 
 __device__ __forceinline__ float peak_d2(const float* a, const float* b, int dim) {
     float d2 = 0.0f;
-    for (int j = 0; j < dim; ++j) {
-        const float d = a[j] - b[j];
-        d2 = fmaf(d, d, d2);
-    }
+#pragma unroll
+    for (int j = 0; j < kPeakMaxDim; ++j)
+        if (j < dim) {
+            const float d = a[j] - b[j];
+            d2 = fmaf(d, d, d2);
+        }
     return d2;
 }
 
@@ -86,11 +100,15 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
                                                            int* __restrict__ chain, int* __restrict__ out) {
     __shared__ float taken_lds[kPeakLdsFloats];
     __shared__ int head[kPeakBuckets];
-    __shared__ float block_c[kPeakBlock * 8];
+    __shared__ int chain_lds[kPeakChainLds];  // the rest in `chain`
+    __shared__ float block_c[kPeakBlock * kPeakMaxDim];
     __shared__ unsigned long long conflict[kPeakBlock][kPeakBlock / 64];
     __shared__ unsigned long long chosen[kPeakBlock / 64];
     __shared__ int alive[kPeakBlock];
-    __shared__ int n_taken, finished;
+    __shared__ int q_id[kPeakQueue];                   // queued survivors: candidate index
+    __shared__ float q_c[kPeakQueue * kPeakMaxDim];    // and its coordinates
+    __shared__ int offsets[kPeakPer * (kPeakBlock / 64) + 1];
+    __shared__ int n_taken, finished, window_end;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lds_cap = kPeakLdsFloats / dim;
     const bool grid = dim <= 3;  // taken lookups through the cell grid, else a linear scan
@@ -103,51 +121,127 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
     auto cell = [&](const float* p, int j) {  // clamped far from int overflow
         return j < dim ? (int)fminf(fmaxf(floorf(p[j] * inv_h), -1e9f), 1e9f) : 0;
     };
-    // the block's candidate of this thread, loaded one block ahead
-    auto fetch = [&](int base, bool* valid, int* id, float* c) {
-        const int p = base + tid;
-        *valid = p < count && keys[p] != -INFINITY;
-        *id = *valid ? order[p] : -1;
-        for (int j = 0; j < dim; ++j) c[j] = *valid ? ctr[(size_t)*id * dim + j] : 0.0f;
-    };
-    bool valid;
-    int id;
-    float c[8];
-    fetch(0, &valid, &id, c);
-    __syncthreads();
-    for (int base = 0; base < count; base += kPeakBlock) {
-        bool ok = valid;
-        const int n = n_taken;
-        if (sep2 > 0.0f && grid) {
+    // whether c lies within the separation of a peak taken so far (n of them)
+    auto clashes = [&](const float* c, int n) {
+        if (!(sep2 > 0.0f)) return false;
+        if (grid) {
             const int ix = cell(c, 0), iy = cell(c, 1), iz = cell(c, 2);
-            for (int dz = dim > 2 ? -1 : 0; dz <= (dim > 2 ? 1 : 0) && ok; ++dz)
-                for (int dy = dim > 1 ? -1 : 0; dy <= (dim > 1 ? 1 : 0) && ok; ++dy)
-                    for (int dx = -1; dx <= 1 && ok; ++dx)
-                        for (int q = head[peak_bucket(ix + dx, iy + dy, iz + dz)]; q >= 0 && ok; q = chain[q]) {
+            for (int dz = dim > 2 ? -1 : 0; dz <= (dim > 2 ? 1 : 0); ++dz)
+                for (int dy = dim > 1 ? -1 : 0; dy <= (dim > 1 ? 1 : 0); ++dy)
+                    for (int dx = -1; dx <= 1; ++dx)
+                        for (int q = head[peak_bucket(ix + dx, iy + dy, iz + dz)]; q >= 0;
+                             q = q < kPeakChainLds ? chain_lds[q] : chain[q]) {
                             const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
-                            if (peak_d2(c, t, dim) < sep2) ok = false;
+                            if (peak_d2(c, t, dim) < sep2) return true;
                         }
-        } else if (sep2 > 0.0f) {
-            for (int q = 0; q < n && ok; ++q) {
-                const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
-                if (peak_d2(c, t, dim) < sep2) ok = false;
-            }
+            return false;
         }
+        for (int q = 0; q < n; ++q) {
+            const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
+            if (peak_d2(c, t, dim) < sep2) return true;
+        }
+        return false;
+    };
+    __syncthreads();
+    int scan = 0, q_head = 0, q_len = 0;  // uniform
+    bool scan_done = count <= 0;
+    for (;;) {
+        // refill the queue until it holds a block or the finite candidates are exhausted
+        while (q_len - q_head < kPeakBlock && !scan_done) {
+            const int left = q_len - q_head;
+            int keep_id = -1;
+            float keep_c[kPeakMaxDim];
+            if (tid < left) {
+                keep_id = q_id[q_head + tid];
+#pragma unroll
+                for (int j = 0; j < kPeakMaxDim; ++j)
+                    if (j < dim) keep_c[j] = q_c[(q_head + tid) * dim + j];
+            }
+            const int n = n_taken;
+            int w_id[kPeakPer];
+            float w_c[kPeakPer][kPeakMaxDim];
+            bool w_ok[kPeakPer], ended = false;
+#pragma unroll
+            for (int i = 0; i < kPeakPer; ++i) {
+                const int p = scan + i * kPeakBlock + tid;
+                const bool valid = p < count && keys[p] != -INFINITY;  // sorted: -inf (non-finite) last
+                ended |= !valid;
+                w_id[i] = valid ? order[p] : -1;
+#pragma unroll
+                for (int j = 0; j < kPeakMaxDim; ++j)
+                    if (j < dim) w_c[i][j] = valid ? ctr[(size_t)w_id[i] * dim + j] : 0.0f;
+                w_ok[i] = valid;
+            }
+#pragma unroll
+            for (int i = 0; i < kPeakPer; ++i) w_ok[i] = w_ok[i] && !clashes(w_c[i], n);
+            if (tid == 0) window_end = 0;
+            __syncthreads();  // every thread has read the leftovers and n_taken
+            if (ended) window_end = 1;
+            if (tid < left) {
+                q_id[tid] = keep_id;
+#pragma unroll
+                for (int j = 0; j < kPeakMaxDim; ++j)
+                    if (j < dim) q_c[tid * dim + j] = keep_c[j];
+            }
+            unsigned long long bal[kPeakPer];
+#pragma unroll
+            for (int i = 0; i < kPeakPer; ++i) {
+                bal[i] = __ballot(w_ok[i]);
+                if (lane == 0) offsets[i * (kPeakBlock / 64) + wave] = __popcll(bal[i]);
+            }
+            __syncthreads();
+            if (tid == 0) {  // exclusive scan in sorted order: chunk-major, wave-minor
+                int sum = 0;
+                for (int i = 0; i < kPeakPer * (kPeakBlock / 64); ++i) {
+                    const int v = offsets[i];
+                    offsets[i] = sum;
+                    sum += v;
+                }
+                offsets[kPeakPer * (kPeakBlock / 64)] = sum;
+            }
+            __syncthreads();
+            const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+            for (int i = 0; i < kPeakPer; ++i)
+                if (w_ok[i]) {
+                    const int e = left + offsets[i * (kPeakBlock / 64) + wave] + __popcll(bal[i] & below);
+                    q_id[e] = w_id[i];
+#pragma unroll
+                    for (int j = 0; j < kPeakMaxDim; ++j)
+                        if (j < dim) q_c[e * dim + j] = w_c[i][j];
+                }
+            q_head = 0;
+            q_len = left + offsets[kPeakPer * (kPeakBlock / 64)];
+            scan += kPeakWindow;
+            scan_done = window_end != 0 || scan >= count;
+            __syncthreads();  // the queue is written; window_end and offsets may be reused
+        }
+        if (q_head >= q_len) break;  // nothing left to resolve
+        // one block of queued survivors, in sorted order
+        const int e = q_head + tid;
+        const bool valid = e < q_len;
+        const int id = valid ? q_id[e] : -1;
+        float c[kPeakMaxDim];
+#pragma unroll
+        for (int j = 0; j < kPeakMaxDim; ++j)
+            if (j < dim) c[j] = valid ? q_c[e * dim + j] : 0.0f;
+        const int n = n_taken;
+        const bool ok = valid && !clashes(c, n);
         alive[tid] = ok;
-        for (int j = 0; j < dim; ++j) block_c[tid * dim + j] = c[j];
+#pragma unroll
+        for (int j = 0; j < kPeakMaxDim; ++j)
+            if (j < dim) block_c[tid * dim + j] = c[j];
         __syncthreads();
+        // the later candidates of the block that clash with this one
         unsigned long long row[kPeakBlock / 64] = {0ull, 0ull, 0ull, 0ull};
         if (ok)
-            for (int q = 0; q < tid; ++q)
+            for (int q = tid + 1; q < kPeakBlock; ++q)
                 if (alive[q] && peak_d2(c, block_c + q * dim, dim) < sep2) row[q >> 6] |= 1ull << (q & 63);
         for (int w = 0; w < kPeakBlock / 64; ++w) conflict[tid][w] = row[w];
         __syncthreads();
-        bool next_valid;
-        int next_id;
-        float next_c[8];
-        fetch(base + kPeakBlock, &next_valid, &next_id, next_c);
-        // wave 0 walks the block in order on scalar bit masks: a candidate is taken iff it
-        // is alive and clashes with none taken before it in this block
+        // wave 0 walks the block in order on scalar bit masks: the first live candidate is
+        // taken and every later one that clashes with it leaves, so the walk takes one turn
+        // per taken peak, however many near-duplicates the block holds
         if (wave == 0) {
             unsigned long long rows[kPeakBlock / 64][kPeakBlock / 64];
             unsigned long long live[kPeakBlock / 64];
@@ -161,25 +255,20 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
             int k = n;
 #pragma unroll
             for (int slot = 0; slot < kPeakBlock / 64; ++slot) {
-                unsigned long long m = live[slot];
-                while (m != 0ull && k < peaks) {
-                    const int l = __builtin_ctzll(m);
-                    m &= m - 1ull;
-                    unsigned long long clash = 0ull;
-#pragma unroll
-                    for (int w = 0; w < kPeakBlock / 64; ++w) clash |= readlane64(rows[slot][w], l) & mine[w];
-                    if (clash != 0ull) continue;
+                while (live[slot] != 0ull && k < peaks) {
+                    const int l = __builtin_ctzll(live[slot]);
+                    live[slot] &= live[slot] - 1ull;
                     mine[slot] |= 1ull << l;
                     ++k;
+#pragma unroll
+                    for (int w = slot; w < kPeakBlock / 64; ++w) live[w] &= ~readlane64(rows[slot][w], l);
                 }
             }
             if (lane == 0) {
 #pragma unroll
                 for (int w = 0; w < kPeakBlock / 64; ++w) chosen[w] = mine[w];
                 n_taken = k;
-                // sorted keys: a non-finite key ends the finite candidates
-                const int last = min(count, base + kPeakBlock) - 1;
-                finished = k >= peaks || keys[last] == -INFINITY;
+                finished = k >= peaks;
             }
         }
         __syncthreads();
@@ -190,14 +279,20 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
             for (int w = 0; w < (tid >> 6); ++w) k += __popcll(chosen[w]);
             out[k] = id;
             float* dst = k < lds_cap ? taken_lds + k * dim : spill + (size_t)(k - lds_cap) * dim;
-            for (int j = 0; j < dim; ++j) dst[j] = c[j];
-            if (sep2 > 0.0f && grid) chain[k] = atomicExch(&head[peak_bucket(cell(c, 0), cell(c, 1), cell(c, 2))], k);
+#pragma unroll
+            for (int j = 0; j < kPeakMaxDim; ++j)
+                if (j < dim) dst[j] = c[j];
+            if (sep2 > 0.0f && grid) {
+                const int next = atomicExch(&head[peak_bucket(cell(c, 0), cell(c, 1), cell(c, 2))], k);
+                if (k < kPeakChainLds)
+                    chain_lds[k] = next;
+                else
+                    chain[k] = next;
+            }
         }
         __syncthreads();
         if (finished) break;
-        valid = next_valid;
-        id = next_id;
-        for (int j = 0; j < dim; ++j) c[j] = next_c[j];
+        q_head = min(q_head + kPeakBlock, q_len);
     }
     for (int k = n_taken + tid; k < peaks; k += kPeakBlock) out[k] = -1;
 }

@@ -111,6 +111,23 @@ def test_peaks_kernel_matches_serial_greedy_selection(dim, count, separation):
     assert torch.isnan(scores[0])  # the scores were not modified
 
 
+@pytest.mark.parametrize("dim,clusters,size,count", [(3, 1500, 100, 1000), (2, 400, 60, 500), (5, 300, 40, 200)])
+def test_peaks_kernel_skips_clustered_near_duplicates(dim, clusters, size, count):
+    """The certificate-scan regime: every strong peak comes with many lower-scored neighbours
+    inside the separation, so most sorted candidates are suppressed by a taken peak and the
+    prefiltered windows add few survivors; the linear scan for more than 3 coordinates."""
+    from dsplat.kernels import search
+
+    g = torch.Generator().manual_seed(dim * clusters + size)
+    hubs = torch.rand(clusters, dim, generator=g)
+    centers = (hubs[:, None] + 0.004 * (2 * torch.rand(clusters, size, dim, generator=g) - 1)).reshape(-1, dim)
+    scores = torch.rand(clusters * size, generator=g)
+    scores[::89] = float("nan")
+    got = search.peaks(scores.cuda(), centers.cuda(), count, 0.02).cpu().tolist()
+    want = Se.peaks(scores.numpy(), centers.numpy(), count, 0.02)
+    assert got == want
+
+
 def test_peaks_kernel_stops_when_every_candidate_is_suppressed():
     from dsplat.kernels import search
 
