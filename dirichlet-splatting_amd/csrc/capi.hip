@@ -435,14 +435,14 @@ int dsplat_sample_basis(const dsplat_fmcw* cfg, const float* basis, int32_t rows
 
 int dsplat_project_sampled(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
                            const float* centers, const float* normals, const float* areas, int32_t num_surfels,
-                           const float* sample_basis, int32_t rows, float* out, void* stream) {
+                           const float* sample_basis, int32_t rows, float* out, uint32_t* column_max, void* stream) {
     Chirp c;
     if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0 || !project_td_supported(c, rows))
         return hipErrorInvalidValue;
     if (num_apertures == 0 || num_surfels == 0) return hipSuccess;
     if (!positions || !centers || !normals || !areas || !sample_basis || !out) return hipErrorInvalidValue;
     return launch_project_td(c, positions, num_apertures, centers, normals, areas, num_surfels, sample_basis, rows,
-                             out, (hipStream_t)stream);
+                             out, column_max, (hipStream_t)stream);
 }
 
 size_t dsplat_qn_workspace(const dsplat_fmcw* cfg, int32_t num_apertures) {
@@ -490,7 +490,16 @@ int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* ou
     if (m < 0 || n < 0 || mode < 0 || mode > 2) return hipErrorInvalidValue;
     if (m == 0 || n == 0) return hipSuccess;
     if (!rows || !out || !workspace || workspace_bytes < syrk_workspace(m, n, -1, mode)) return hipErrorInvalidValue;
-    return launch_syrk(rows, m, n, -1, mode, out, workspace, (hipStream_t)stream);
+    return launch_syrk(rows, m, n, -1, mode, nullptr, out, workspace, (hipStream_t)stream);
+}
+
+int dsplat_syrk_scaled(const float* rows, int32_t m, int32_t n, int32_t mode, const uint32_t* column_max, float* out,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+    if (m < 0 || n < 0 || mode < 0 || mode > 2) return hipErrorInvalidValue;
+    if (m == 0 || n == 0) return hipSuccess;
+    if (!rows || !out || !workspace || !column_max || workspace_bytes < syrk_workspace(m, n, -1, mode))
+        return hipErrorInvalidValue;
+    return launch_syrk(rows, m, n, -1, mode, column_max, out, workspace, (hipStream_t)stream);
 }
 
 size_t dsplat_gemm_tn_workspace(int32_t m, int32_t n, int32_t col0, int32_t mode) {
@@ -504,7 +513,17 @@ int dsplat_gemm_tn(const float* rows, int32_t m, int32_t n, int32_t col0, int32_
     if (m == 0 || col0 == n) return hipSuccess;
     if (!rows || !out || !workspace || workspace_bytes < syrk_workspace(m, n, col0, mode))
         return hipErrorInvalidValue;
-    return launch_syrk(rows, m, n, col0, mode, out, workspace, (hipStream_t)stream);
+    return launch_syrk(rows, m, n, col0, mode, nullptr, out, workspace, (hipStream_t)stream);
+}
+
+int dsplat_gemm_tn_scaled(const float* rows, int32_t m, int32_t n, int32_t col0, int32_t mode,
+                          const uint32_t* column_max, float* out, void* workspace, size_t workspace_bytes,
+                          void* stream) {
+    if (m < 0 || n < 0 || col0 < 0 || col0 > n || mode < 0 || mode > 2) return hipErrorInvalidValue;
+    if (m == 0 || col0 == n) return hipSuccess;
+    if (!rows || !out || !workspace || !column_max || workspace_bytes < syrk_workspace(m, n, col0, mode))
+        return hipErrorInvalidValue;
+    return launch_syrk(rows, m, n, col0, mode, column_max, out, workspace, (hipStream_t)stream);
 }
 
 size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels) {

@@ -120,7 +120,8 @@ hipError_t launch_project(const Chirp& c, const float* pos, int A, const float* 
 bool project_td_supported(const Chirp& c, int rows);
 hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, float* out, hipStream_t st);
 hipError_t launch_project_td(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                             const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st);
+                             const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
+                             hipStream_t st);
 typedef dsplat_qn_spec QnSpec;
 size_t qn_workspace(int A, int P, int K);
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,
@@ -130,8 +131,8 @@ size_t elimination_workspace(int n, int count);
 hipError_t launch_elimination(const double* W, const double* b, int n, int count, const int* limited, int limit,
                               long long* removed, void* workspace, hipStream_t st);
 size_t syrk_workspace(int m, int n, int col0, int mode);
-hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, float* out, void* workspace,
-                       hipStream_t st);
+hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, const unsigned* column_max, float* out,
+                       void* workspace, hipStream_t st);
 
 // Splat form: 0 = sample-domain MFMA kernels where the chirp allows them (S in {128, 256}),
 // 1 = bin-domain cotangent kernels everywhere. Process-wide, set through the C ABI.

@@ -62,9 +62,10 @@ __device__ __forceinline__ f32x16 mfma_f16(f16x8 a, f16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
-// One aperture of one unit: v, z = v^16 and the weight w / S of this lane's pair.
+// One aperture of one unit: v, z = v^16 and the weight w / S of this lane's pair. Returns
+// the largest |value| this lane stored (0 for a padding column).
 template <int J, int NB>
-__device__ __forceinline__ void project_aperture(const _Float16* Bh, const _Float16* Bl, int bs, int h, int l32,
+__device__ __forceinline__ float project_aperture(const _Float16* Bh, const _Float16* Bl, int bs, int h, int l32,
                                                  float2 v, float2 z, float2 w, int row0, int rows, int a, int A,
                                                  int s, int S, float* __restrict__ out) {
     // W of this lane's residues i = 4 h + e (even K steps) and 8 + 4 h + e (odd steps)
@@ -124,7 +125,7 @@ __device__ __forceinline__ void project_aperture(const _Float16* Bh, const _Floa
     }
     // (Re, Im) row pairs times w; C[row][col] -> acc[r] with row = 8 (r / 4) + 4 h + r % 4, so
     // rows 2j and 2j + 1 are acc[r], acc[r + 1] for even r
-    if (s >= S) return;
+    if (s >= S) return 0.0f;
     // The row stride passes through an empty asm so that the 64-bit row offsets are formed
     // here, per aperture, in scalar registers, instead of being hoisted out of every loop
     // into dozens of live registers.
@@ -132,6 +133,7 @@ __device__ __forceinline__ void project_aperture(const _Float16* Bh, const _Floa
     asm volatile("" : "+s"(stride));
     float* p = out + ((size_t)(row0 + 4 * h) * A + a) * S + s;
     const bool full = row0 + 32 * NB <= rows;  // uniform: no per-row test on whole passes
+    float top = 0.0f;
 #pragma unroll
     for (int x = 0; x < NB; ++x)
 #pragma unroll
@@ -139,10 +141,21 @@ __device__ __forceinline__ void project_aperture(const _Float16* Bh, const _Floa
             const int dr = x * 32 + 8 * (r >> 2) + (r & 3);  // row - row0 - 4 h, even
             if (full || row0 + 4 * h + dr < rows) {           // rows is even: row + 1 < rows too
                 const float hr = acc[x][r], hm = acc[x][r + 1];
-                p[dr * stride] = w.x * hr - w.y * hm;
-                p[(dr + 1) * stride] = w.x * hm + w.y * hr;
+                const float re = w.x * hr - w.y * hm, im = w.x * hm + w.y * hr;
+                p[dr * stride] = re;
+                p[(dr + 1) * stride] = im;
+                top = fmaxf(top, fmaxf(fabsf(re), fabsf(im)));
             }
         }
+    return top;
+}
+
+// colmax[s] = max(colmax[s], top of both lane halves) as float bits (non-negative floats
+// order like unsigned ints).
+__device__ __forceinline__ void flush_column_max(unsigned* colmax, float top, int h, int s, int S) {
+    float other = top;
+    swap_halves(top, other);  // top: half 0's value in every lane, other: half 1's
+    if (colmax && h == 0 && s < S) atomicMax(colmax + s, __float_as_uint(fmaxf(top, other)));
 }
 
 template <int J, bool FRESNEL, int ROWS>
@@ -151,7 +164,8 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
                                                                 const float* __restrict__ nrm,
                                                                 const float* __restrict__ areas, int S,
                                                                 const float* __restrict__ basis, int rows,
-                                                                float* __restrict__ out) {
+                                                                float* __restrict__ out,
+                                                                unsigned* __restrict__ colmax) {
     constexpr int KS = 32 * J;  // real values per basis row: (re, im) of 16 J samples
     constexpr int BS = KS + 8;  // LDS row stride in halves
     constexpr int NB = ROWS / 32;
@@ -183,8 +197,11 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
     const int u_end = min(units, (int)blockIdx.x * per + per);
     int tile = -1, s = 0;
     float cx = 0.f, cy = 0.f, cz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f, root_area = 0.f;
+    float top = 0.0f;  // largest |value| of this lane's column in the current tile
     for (int u = blockIdx.x * per; u < u_end; ++u) {
         if (u / pairs != tile) {
+            if (tile >= 0) flush_column_max(colmax, top, h, s, S);
+            top = 0.0f;
             tile = u / pairs;
             s = tile * kPtCols + wave * 32 + l32;
             const int sc = min(s, S - 1);
@@ -211,11 +228,12 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
         }
 #pragma unroll 1
         for (int e = 0; e < 2 && a0 + e < A; ++e)
-            project_aperture<J, NB>(Bh, Bl, BS, h, l32, make_float2(e ? f1[0] : f0[0], e ? f1[1] : f0[1]),
-                                    make_float2(e ? f1[2] : f0[2], e ? f1[3] : f0[3]),
-                                    make_float2(e ? f1[4] : f0[4], e ? f1[5] : f0[5]), row0, rows, a0 + e, A, s, S,
-                                    out);
+            top = fmaxf(top, project_aperture<J, NB>(Bh, Bl, BS, h, l32, make_float2(e ? f1[0] : f0[0], e ? f1[1] : f0[1]),
+                                                     make_float2(e ? f1[2] : f0[2], e ? f1[3] : f0[3]),
+                                                     make_float2(e ? f1[4] : f0[4], e ? f1[5] : f0[5]), row0, rows,
+                                                     a0 + e, A, s, S, out));
     }
+    if (tile >= 0) flush_column_max(colmax, top, h, s, S);
 }
 
 static size_t project_td_lds(const Chirp& c, int rows_tile) {
@@ -234,7 +252,8 @@ bool project_td_supported(const Chirp& c, int rows) {
 
 template <int J, bool F, int ROWS>
 static hipError_t launch_ptd(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                             const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st) {
+                             const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
+                             hipStream_t st) {
     const void* kernel = (const void*)project_td_kernel<J, F, ROWS>;
     const size_t smem = project_td_lds(c, ROWS);
     hipError_t e = allow_lds(kernel, smem);
@@ -249,18 +268,19 @@ static hipError_t launch_ptd(const Chirp& c, const float* pos, int A, const floa
     if (blocks < 1) blocks = 1;
     if (blocks > units) blocks = units;
     hipLaunchKernelGGL((project_td_kernel<J, F, ROWS>), dim3((unsigned)blocks, passes), dim3(kPtThreads), smem, st,
-                       c, pos, A, ctr, nrm, areas, S, basis, rows, out);
+                       c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax);
     return hipGetLastError();
 }
 
 template <int J, bool F>
 static hipError_t launch_ptd_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                               const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st) {
+                               const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
+                               hipStream_t st) {
     if constexpr (J == 8) {
         if (project_td_tile(c, rows) == 128)
-            return launch_ptd<J, F, 128>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, st);
+            return launch_ptd<J, F, 128>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
     }
-    return launch_ptd<J, F, 64>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, st);
+    return launch_ptd<J, F, 64>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
 }
 
 hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, float* out, hipStream_t st) {
@@ -270,13 +290,14 @@ hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, flo
 }
 
 hipError_t launch_project_td(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                             const float* areas, int S, const float* basis, int rows, float* out, hipStream_t st) {
+                             const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
+                             hipStream_t st) {
     const bool fresnel = c.fresnel_n > 1.0f;
     if (c.S == 128)
-        return fresnel ? launch_ptd_j<8, true>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, st)
-                       : launch_ptd_j<8, false>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, st);
-    return fresnel ? launch_ptd_j<16, true>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, st)
-                   : launch_ptd_j<16, false>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, st);
+        return fresnel ? launch_ptd_j<8, true>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st)
+                       : launch_ptd_j<8, false>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
+    return fresnel ? launch_ptd_j<16, true>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st)
+                   : launch_ptd_j<16, false>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
 }
 
 }  // namespace dsplat

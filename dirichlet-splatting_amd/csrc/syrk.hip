@@ -361,19 +361,20 @@ static hipError_t launch_syrk_mode(const float* rows, int m, int n, int col0, co
     return hipGetLastError();
 }
 
-hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, float* out, void* workspace,
-                       hipStream_t st) {
+hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, const unsigned* column_max, float* out,
+                       void* workspace, hipStream_t st) {
     if (m <= 0 || n <= 0 || col0 >= n) return hipSuccess;
     const SyrkShape s = syrk_shape(m, n, col0);
     float* partial = static_cast<float*>(workspace);
-    unsigned* colmax = nullptr;
+    const unsigned* colmax = mode == 2 ? column_max : nullptr;
     hipError_t e;
-    if (mode == 2) {
-        colmax = reinterpret_cast<unsigned*>(partial + (size_t)s.splits * s.tiles * kTile * kTile);
-        if ((e = fill_zero(reinterpret_cast<float*>(colmax), n, st)) != hipSuccess) return e;
+    if (mode == 2 && !column_max) {
+        unsigned* colmax_ws = reinterpret_cast<unsigned*>(partial + (size_t)s.splits * s.tiles * kTile * kTile);
+        colmax = colmax_ws;
+        if ((e = fill_zero(reinterpret_cast<float*>(colmax_ws), n, st)) != hipSuccess) return e;
         const int rows_per = (m + 4095) / 4096 > 256 ? (m + 4095) / 4096 : 256;
         hipLaunchKernelGGL(syrk_colmax, dim3((n + 255) / 256, (m + rows_per - 1) / rows_per), dim3(256), 0, st,
-                           rows, m, n, rows_per, colmax);
+                           rows, m, n, rows_per, colmax_ws);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     switch (mode) {

@@ -126,7 +126,9 @@ SIGNATURES = {
     "dsplat_project_design": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr]),
     "dsplat_project_form": (c_int, [c_ptr, c_int]),
     "dsplat_sample_basis": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
-    "dsplat_project_sampled": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr]),
+    "dsplat_project_sampled": (
+        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, c_ptr]
+    ),
     "dsplat_qn_workspace": (c_size, [c_ptr, c_int]),
     "dsplat_qn_search": (
         c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
@@ -135,8 +137,10 @@ SIGNATURES = {
     "dsplat_backward_elimination": (c_int, [c_ptr, c_ptr, c_int, c_int, c_ptr, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_syrk_workspace": (c_size, [c_int, c_int, c_int]),
     "dsplat_syrk": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
+    "dsplat_syrk_scaled": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_gemm_tn_workspace": (c_size, [c_int, c_int, c_int, c_int]),
     "dsplat_gemm_tn": (c_int, [c_ptr, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]),
+    "dsplat_gemm_tn_scaled": (c_int, [c_ptr, c_int, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_scan_workspace": (c_size, [c_ptr, c_int, c_ptr, c_int]),
     "dsplat_scan_batched": (
         c_int, [c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]

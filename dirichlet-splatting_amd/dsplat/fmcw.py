@@ -236,10 +236,12 @@ class FMCWModel:
                 a, y = self.design(params, rows), flat[rows]
                 yield (a, y) if basis is None else (basis.reduce(a), basis.reduce(y))
 
-    def projected_blocks(self, params, chunk):
+    def projected_blocks(self, params, chunk, column_max=False):
         """Real row-projected design blocks of whole apertures on the shared row basis, atoms
         generated on chip and never written (the blocks of `blocks` without y), or None
-        when there is no basis to project on."""
+        when there is no basis to project on. With column_max, (block, maxima) pairs: the
+        float bits of every column's largest |entry| (int32 [S]) where the sample-domain
+        projection forms them on the way, else None."""
         if self.basis is None:
             return None
         config, positions = self.sensor
@@ -247,13 +249,19 @@ class FMCWModel:
         apertures = max(1, chunk // bins)
         basis = self.basis.interleaved
         sampled = surfel.kernel.sample_basis(config, basis, self.fresnel_n)  # once per pass
-        return (
-            surfel.kernel.project_design(
+        count = len(params["centers"])
+
+        def block(a0):
+            maxima = None
+            if column_max and sampled is not None:
+                maxima = torch.zeros(count, dtype=torch.int32, device=basis.device)
+            a = surfel.kernel.project_design(
                 config, positions[a0 : a0 + apertures], params["centers"], params["normals"], params["areas"], basis,
-                self.fresnel_n, sampled,
+                self.fresnel_n, sampled, maxima,
             )  # fmt: skip
-            for a0 in range(0, len(positions), apertures)
-        )
+            return (a, maxima) if column_max else a
+
+        return (block(a0) for a0 in range(0, len(positions), apertures))
 
     def _apertures(self, rows):
         apertures, inverse = torch.unique(rows // self.shape[1], return_inverse=True)

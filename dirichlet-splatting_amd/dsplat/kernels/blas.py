@@ -9,10 +9,18 @@ from .. import _lib
 SYRK_MODES = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
 
 
-def syrk(rows, out, mode="fp32"):
+def _column_max(column_max, n):
+    if column_max.dtype != torch.int32 or column_max.shape != (n,) or not column_max.is_contiguous():
+        raise ValueError("column_max must be a contiguous int32 [n] tensor of float bits")
+    return _lib.ptr(column_max)
+
+
+def syrk(rows, out, mode="fp32", column_max=None):
     """Add rows^T rows to the upper triangle of out [n, n]; the lower triangle is untouched
     (kernels/cublas.py:31-44). rows [m, n] float32, out float32 contiguous. mode: fp32
-    (FP32 MFMA), bf16x3 or fp16x3 (hi + lo split, three MFMA products, FP32 accumulation)."""
+    (FP32 MFMA), bf16x3 or fp16x3 (hi + lo split, three MFMA products, FP32 accumulation).
+    column_max: the float bits of max |rows[:, c]| per column (int32 [n], as the sample-domain
+    projection writes them), so the FP16 mode skips its own pass over rows."""
     if rows.dtype != torch.float32 or out.dtype != torch.float32:
         raise TypeError("syrk takes float32 tensors")
     if not out.is_contiguous() or out.shape != (rows.shape[1], rows.shape[1]):
@@ -22,14 +30,17 @@ def syrk(rows, out, mode="fp32"):
     m, n = rows.shape
     size = _lib.library().dsplat_syrk_workspace(m, n, code)
     workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=rows.device)
-    _lib.call(
-        "dsplat_syrk", _lib.ptr(rows), m, n, code, _lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(rows.device)
-    )
+    tail = (_lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(rows.device))
+    if column_max is None:
+        _lib.call("dsplat_syrk", _lib.ptr(rows), m, n, code, *tail)
+    else:
+        _lib.call("dsplat_syrk_scaled", _lib.ptr(rows), m, n, code, _column_max(column_max, n), *tail)
     return out
 
 
-def gemm_tn(rows, col0, out, mode="fp32"):
-    """out [n, n - col0] += rows^T rows[:, col0:] on the SYRK tiles (rows [m, n] float32)."""
+def gemm_tn(rows, col0, out, mode="fp32", column_max=None):
+    """out [n, n - col0] += rows^T rows[:, col0:] on the SYRK tiles (rows [m, n] float32);
+    column_max as for syrk."""
     if rows.dtype != torch.float32 or out.dtype != torch.float32:
         raise TypeError("gemm_tn takes float32 tensors")
     m, n = rows.shape
@@ -39,10 +50,11 @@ def gemm_tn(rows, col0, out, mode="fp32"):
     rows = rows.contiguous()
     size = _lib.library().dsplat_gemm_tn_workspace(m, n, col0, code)
     workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=rows.device)
-    _lib.call(
-        "dsplat_gemm_tn", _lib.ptr(rows), m, n, int(col0), code, _lib.ptr(out), _lib.ptr(workspace), size,
-        _lib.stream(rows.device),
-    )  # fmt: skip
+    tail = (_lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(rows.device))
+    if column_max is None:
+        _lib.call("dsplat_gemm_tn", _lib.ptr(rows), m, n, int(col0), code, *tail)
+    else:
+        _lib.call("dsplat_gemm_tn_scaled", _lib.ptr(rows), m, n, int(col0), code, _column_max(column_max, n), *tail)
     return out
 
 

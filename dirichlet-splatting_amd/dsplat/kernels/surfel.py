@@ -296,20 +296,29 @@ def sample_basis(config, basis, fresnel_n=0.0):
     return out
 
 
-def project_design(config, positions, centers, normals, areas, basis, fresnel_n, sampled=None):
+def project_design(config, positions, centers, normals, areas, basis, fresnel_n, sampled=None, column_max=None):
     """Row-projected real design [rows * A, S] (row j * A + a) of a basis [rows, 2K] in its
     interleaved real form, atoms generated on chip (fmcw.py:243-261 blocks + reduce_major).
-    With `sampled` (sample_basis of the same basis) the sample-domain kernel runs instead."""
+    With `sampled` (sample_basis of the same basis) the sample-domain kernel runs instead,
+    and then `column_max` (zeroed int32 [S]) receives the float bits of max |out[:, s]|."""
     _check(positions, centers, normals)
     rows = basis.shape[0]
     out = centers.new_empty((rows * len(positions), len(centers)))
     cfg = _lib.fmcw(config, fresnel_n)
-    name, rows_basis = ("dsplat_project_design", basis) if sampled is None else ("dsplat_project_sampled", sampled)
-    _lib.call(
-        name, _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
-        _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()),
-        len(centers), _lib.ptr(rows_basis.float().contiguous()), rows, _lib.ptr(out), _lib.stream(centers.device),
+    head = (
+        _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions), _lib.ptr(centers.contiguous()),
+        _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()), len(centers),
     )  # fmt: skip
+    if sampled is None:
+        if column_max is not None:
+            raise ValueError("column maxima come with the sample-domain projection only")
+        _lib.call("dsplat_project_design", *head, _lib.ptr(basis.float().contiguous()), rows, _lib.ptr(out),
+                  _lib.stream(centers.device))  # fmt: skip
+    else:
+        if column_max is not None and (column_max.dtype != torch.int32 or column_max.shape != (len(centers),)):
+            raise ValueError("column_max must be an int32 [S] tensor")
+        _lib.call("dsplat_project_sampled", *head, _lib.ptr(sampled.float().contiguous()), rows, _lib.ptr(out),
+                  _lib.ptr(column_max), _lib.stream(centers.device))  # fmt: skip
     return out
 
 

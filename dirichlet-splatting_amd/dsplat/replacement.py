@@ -167,10 +167,10 @@ def _pooled_cross(model, pool, count, target, config):
     part = torch.zeros((total, total - count), dtype=torch.float32, device=target.device)
     folded = torch.zeros_like(part, dtype=torch.float64)
     chunk = row_chunk(model, total, config)
-    fused = model.projected_blocks(pool, chunk) if hasattr(model, "projected_blocks") else None
-    stream = fused if fused is not None else (a for a, _ in model.blocks(pool, target, True, chunk))
-    for index, a in enumerate(stream):
-        gemm_tn(a.contiguous(), count, part, mode)
+    fused = model.projected_blocks(pool, chunk, column_max=True) if hasattr(model, "projected_blocks") else None
+    stream = fused if fused is not None else ((a, None) for a, _ in model.blocks(pool, target, True, chunk))
+    for index, (a, maxima) in enumerate(stream):
+        gemm_tn(a.contiguous(), count, part, mode, maxima)
         if (index + 1) % FOLD == 0:
             folded += part
             part.zero_()

@@ -103,13 +103,15 @@ def normal_equations(model, params, target, config):
     part = torch.zeros((count, count), dtype=torch.float32, device=device)
     total = torch.zeros((count, count), dtype=torch.float64, device=device)
     rhs = torch.zeros(count, dtype=torch.float64, device=device)
+    blocks = ((a, y, None) for a, y in stream)
     if batched and hasattr(model, "projected_blocks"):
-        # projected blocks straight from the atoms on chip; y enters through the correlation
-        fused = model.projected_blocks(params, row_chunk(model, count, config))
+        # projected blocks straight from the atoms on chip (with their column maxima for
+        # the FP16 split); y enters through the correlation
+        fused = model.projected_blocks(params, row_chunk(model, count, config), column_max=True)
         if fused is not None:
-            stream = ((a, None) for a in fused)
-    for index, (a, y) in enumerate(stream):
-        syrk(a.contiguous(), part, mode)
+            blocks = ((a, None, maxima) for a, maxima in fused)
+    for index, (a, y, maxima) in enumerate(blocks):
+        syrk(a.contiguous(), part, mode, maxima)
         if not batched:
             rhs += a.T @ y
         if (index + 1) % FOLD == 0:

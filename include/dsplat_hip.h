@@ -278,13 +278,15 @@ int dsplat_sample_basis(const dsplat_fmcw* cfg, const float* basis, int32_t rows
 
 /* dsplat_project_design's output (same layout, same rows) from the sample-domain basis of
  * dsplat_sample_basis: the atoms' sample-domain powers v^n are projected on the matrix cores
- * and never formed per bin. hipErrorInvalidValue unless dsplat_project_form is 0. Replaces the
- * design + reduce_major product of the streamed blocks (fmcw.py:243-261,
- * surfel.slang:313-378). */
+ * and never formed per bin. column_max (NULL or [num_surfels], zeroed by the caller) receives
+ * max(column_max[s], max |out[., s]|) as float bits: the column maxima that FP16 mode of
+ * dsplat_syrk_scaled / dsplat_gemm_tn_scaled takes, without another pass over out.
+ * hipErrorInvalidValue unless dsplat_project_form is 0. Replaces the design + reduce_major
+ * product of the streamed blocks (fmcw.py:243-261, surfel.slang:313-378). */
 int dsplat_project_sampled(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
                            const float* centers, const float* normals, const float* areas,
                            int32_t num_surfels, const float* sample_basis, int32_t rows, float* out,
-                           void* stream);
+                           uint32_t* column_max, void* stream);
 
 /* ---- Batched quasi-Newton certificate search. */
 
@@ -353,6 +355,12 @@ int dsplat_backward_elimination(const double* inverse, const double* b, int32_t 
 size_t dsplat_syrk_workspace(int32_t m, int32_t n, int32_t mode);
 int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* out, void* workspace,
                 size_t workspace_bytes, void* stream);
+/* dsplat_syrk with the column maxima of rows given (float bits of an upper bound of
+ * max |rows[., c]| per column, as dsplat_project_sampled writes them): FP16 mode (2) then
+ * skips its own pass over rows; other modes ignore them. Same workspace. Replaces the same
+ * cuBLAS call (kernels/cublas.py:31-44) on projected blocks. */
+int dsplat_syrk_scaled(const float* rows, int32_t m, int32_t n, int32_t mode, const uint32_t* column_max,
+                       float* out, void* workspace, size_t workspace_bytes, void* stream);
 
 /* out [n, n - col0] (row-major) += rows^T rows[:, col0:], rows [m, n] float32: the cross
  * products of the trailing columns with all columns, on the SYRK kernel's tiles and modes.
@@ -360,6 +368,11 @@ int dsplat_syrk(const float* rows, int32_t m, int32_t n, int32_t mode, float* ou
 size_t dsplat_gemm_tn_workspace(int32_t m, int32_t n, int32_t col0, int32_t mode);
 int dsplat_gemm_tn(const float* rows, int32_t m, int32_t n, int32_t col0, int32_t mode, float* out,
                    void* workspace, size_t workspace_bytes, void* stream);
+/* dsplat_gemm_tn with the column maxima of rows given, as dsplat_syrk_scaled. Replaces the
+ * same streamed addmm_ (replacement.py:178-199) on projected blocks. */
+int dsplat_gemm_tn_scaled(const float* rows, int32_t m, int32_t n, int32_t col0, int32_t mode,
+                          const uint32_t* column_max, float* out, void* workspace, size_t workspace_bytes,
+                          void* stream);
 
 size_t dsplat_gram_workspace(int32_t num_apertures, int32_t num_surfels);
 

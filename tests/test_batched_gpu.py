@@ -215,9 +215,19 @@ def test_sampled_projection_matches_the_float64_projected_design(samples, pad, f
     got_b = sampled.double().cpu().numpy()
     assert np.linalg.norm(got_b - want_b) / np.linalg.norm(want_b) < 1e-7
 
-    got = surfel.project_design(
-        cfg, positions, p["centers"], p["normals"], p["areas"], basis.interleaved, fresnel_n, sampled
-    ).double().cpu().numpy()
+    maxima = torch.zeros(count, dtype=torch.int32, device="cuda")
+    projected = surfel.project_design(
+        cfg, positions, p["centers"], p["normals"], p["areas"], basis.interleaved, fresnel_n, sampled, maxima
+    )
+    # the column maxima formed on the way are exactly those of the written block, and the
+    # FP16 SYRK given them equals the one that finds them itself
+    assert torch.equal(maxima.view(torch.float32), projected.abs().amax(0))
+    from dsplat.kernels.blas import syrk
+
+    own = syrk(projected, torch.zeros(count, count, device="cuda"), "fp16x3")
+    given = syrk(projected, torch.zeros(count, count, device="cuda"), "fp16x3", maxima)
+    assert torch.equal(own, given)
+    got = projected.double().cpu().numpy()
     dense = O.design(
         positions.cpu().numpy(), p["centers"].cpu().numpy(), p["normals"].cpu().numpy(), p["areas"].cpu().numpy(),
         cfg, fresnel_n, True,
