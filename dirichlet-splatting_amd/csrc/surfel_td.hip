@@ -331,12 +331,20 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
                 for (int tt = 0; tt < T; ++tt) acc[e][p][b][tt] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     SurfIn<JVP> in = load_surfel<JVP>(min(psp, S - 1), ctr, nrm, w, dctr, dnrm, dw);
+    __syncthreads();  // twiddle table written
+    // Wave w produces the records of apertures 4 w .. 4 w + 3 and is their only consumer, so
+    // the chunk loop needs no workgroup barrier: the LDS executes one wave's operations in
+    // issue order, and the wave barriers keep the compiler from moving the record writes of a
+    // chunk across the reads of the previous one (or the reads ahead of the writes).
     for (int s0 = 0; s0 < S; s0 += SC) {
         const int s = s0 + psp;
         const SurfIn<JVP> next = load_surfel<JVP>(min(s + SC, S - 1), ctr, nrm, w, dctr, dnrm, dw);
-        __syncthreads();  // previous chunk consumed (and the twiddle table written)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
         td_produce<J, JVP, FRESNEL>(c, E, p_ok && s < S, px, py, pz, in, rec + TdRec<J, JVP>::at(tid));
-        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         td_consume<J, SETS, T, SC>(rec, wave, lane, acc);
         in = next;
     }
@@ -345,6 +353,214 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
     td_store_x<J, SETS, T>(c, X, wave, lane, acc);
     __syncthreads();
     dft16(X, c.S + 1, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(AT, A - a0), wave, lane);
+}
+
+// ---------------------------------------------------------------------------------------
+// Forward on FP16 hi + lo MFMA (128 samples)
+// ---------------------------------------------------------------------------------------
+// FP32 products issue on the SIMD cycles that the pair generation needs for its VALU work;
+// FP16 products co-issue with it. td_render_h16_kernel forms the sums over surfels on
+// v_mfma_f32_16x16x32_f16 as hi hi + hi lo + lo hi of FP16 splits with FP32 accumulation.
+//  * Samples are split symmetrically, n = 16 j + 7 + i with i in [-7, 8]: v^n = Y_j W_i with
+//    Y_j = c v^(16 j + 7) and W_i = v^i, and W_-i = conj(W_i). With the four real sums
+//    P1 = sum Re Y Re W, P2 = sum Im Y Im W, P3 = sum Im Y Re W, P4 = sum Re Y Im W over the
+//    surfels, X[16 j + 7 + i] = (P1 - P2) + i (P3 + P4) and X[16 j + 7 - i] = (P1 + P2) +
+//    i (P3 - P4): nine W columns (W_0 = 1) give all 16 samples of a row j, and a pair
+//    supplies 32 generated values (Y_0..7, W_1..8) for 128 complex products.
+//  * A workgroup is 4 waves x 2 apertures; surfels stream in chunks of 32. Lane (h, m) of
+//    wave w generates pair (aperture 2 w + m / 16, chunk surfel t + 16 h), t = m % 16.
+//    Its values go in row pairs (rowbase, rowbase + 1) through one v_permlane32_swap, after
+//    which half h holds row rowbase + h of surfels t and t + 16, and one packed split gives
+//    the hi and lo words [row][t] = (surfel t, surfel t + 16).
+//  * Rows per aperture: 0..15 Y (row 2 j + part), 16 + x Re W, 32 + x Im W for the B column
+//    x: W_(x+1) for x < 8, W_0 (constant) at x = 8, zero for x > 8 (constant rows are
+//    written once). Consumer lane (x, g) reads the words 4 g .. 4 g + 3 of its row: K slot
+//    8 g + 2 u + b of A and B alike is surfel 4 g + u + 16 b.
+//  * Row R holds 16 hi words then 16 lo words, in the other order when (R / 2) is odd, and
+//    word group g sits at group g ^ ((R / 4) % 4): the 16 lanes of a 128-bit read cover all
+//    64 banks. The second aperture's rows start 16 words further, so the 64 lanes of a
+//    producer write fall on distinct banks.
+//  * Y is scaled by 2^sh with one exponent per wave, so that the largest value of the chunks
+//    so far lies below 2^13.5 (FP16 keeps 22 bits of every value); sh only decreases, the
+//    accumulators are rescaled by 2^(sh_new - sh_old) when it does, and X = acc 2^-sh.
+//  * Records are wave-private, so the chunk loop has no workgroup barrier.
+constexpr int kH16Apertures = 8;
+constexpr int kH16Chunk = 32;
+constexpr int kH16Rows = 48;
+constexpr int kH16ApWords = kH16Rows * 32 + 16;  // [row][hi | lo 16 words] + bank shift
+constexpr int kH16Words = 2 * kH16ApWords;       // per wave: two apertures
+
+// word offset of the (hi | lo) half of row R, word group g, word u within the group
+__host__ __device__ constexpr int h16_word(int R, int lo, int g, int u) {
+    return R * 32 + 16 * (((R >> 1) & 1) ^ lo) + 4 * (g ^ ((R >> 2) & 3)) + u;
+}
+
+// Maximum of x over the wave (uniform result): quad, half-row and row mirrors by DPP, then
+// the four rows by readlane.
+__device__ __forceinline__ int wave_max_i(int x) {
+    x = max(x, __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false));   // quad_perm [1, 0, 3, 2]
+    x = max(x, __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false));   // quad_perm [2, 3, 0, 1]
+    x = max(x, __builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    x = max(x, __builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, false));  // row_mirror
+    return max(max(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+               max(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
+}
+
+// first row of the producer row pair p: Y rows 2 p (p < 8), Re W 16 + 2 (p - 8), Im W 32 + 2 (p - 12)
+__host__ __device__ constexpr int h16_rowbase(int p) { return p < 12 ? 2 * p : 2 * p + 8; }
+
+template <bool FRESNEL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void td_render_h16_kernel(
+    const Chirp c, const float* __restrict__ pos, int A, const float* __restrict__ ctr,
+    const float* __restrict__ nrm, const float2* __restrict__ w, int S, float2* __restrict__ out) {
+    extern __shared__ float smem[];
+    float2* E = reinterpret_cast<float2*>(smem);  // [N]
+    float* region = smem + 2 * c.N;               // 4 waves of records; later X [16][S + 1] float2
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
+    uint32_t* rec = reinterpret_cast<uint32_t*>(region) + wave * kH16Words;
+    const int a0 = blockIdx.x * kH16Apertures;
+    const int pb = m >> 4, t = m & 15;
+    const int a = a0 + 2 * wave + pb;
+    const bool p_ok = a < A;
+    const int pa = min(a, A - 1);
+    const float px = pos[3 * pa], py = pos[3 * pa + 1], pz = pos[3 * pa + 2];
+    fill_twiddles(E, c.N, tid, 256);
+    // constant B rows of both apertures: Re W_0 = 1 at column 8, zeros at Im column 8 and at
+    // columns 9..15 (words written once; the chunk loop writes columns 0..7 only)
+    for (int i = lane; i < 2 * 2 * 8 * 32; i += 64) {
+        const int b = i >> 9, r = (i >> 5) & 15, wd = i & 31;
+        const int row = (r < 8 ? 16 : 32) + 8 + (r & 7);
+        const bool one = r == 0 && wd < 16 && !(((row >> 1) & 1));   // hi half of the Re W_0 row
+        const bool one_sw = r == 0 && wd >= 16 && (((row >> 1) & 1));  // (hi half stored second)
+        rec[b * kH16ApWords + row * 32 + wd] = (one || one_sw) ? 0x3C003C00u : 0u;  // (1.0, 1.0) FP16
+    }
+
+    // producer: lane word offsets for the four swizzle keys (row pair p uses key (rowbase / 4) % 4)
+    int wofs[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wofs[k] = pb * kH16ApWords + h * 32 + 4 * ((t >> 2) ^ k) + (t & 3);
+    // consumer: row x (A) or 16 + x, 32 + x (B) of aperture b, word group g
+    const int x = lane & 15, g = lane >> 4;
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) acc[b][p] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int e_run = 0, sh = 127;
+
+    SurfIn<false> in = load_surfel<false>(min(t + 16 * h, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
+    __syncthreads();  // twiddle table and constant rows written
+    for (int s0 = 0; s0 < S; s0 += kH16Chunk) {
+        const int s = s0 + t + 16 * h;
+        const SurfIn<false> next =
+            load_surfel<false>(min(s + kH16Chunk, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
+        const Pair q = pair_geometry_t<FRESNEL>(c, px, py, pz, in.cx, in.cy, in.cz, in.nx, in.ny, in.nz);
+        const float amp = p_ok && s < S ? q.amp : 0.0f;
+        float2 coef = amp * cmul(in.w, q.rot);
+        // wave-uniform running exponent of max |Re c|, |Im c| (biased); |Y_j| = |c|
+        const uint32_t bits = max(__float_as_uint(coef.x) & 0x7fffffffu, __float_as_uint(coef.y) & 0x7fffffffu);
+        const int e_chunk = wave_max_i((int)(bits >> 23));
+        if (e_chunk > e_run) {
+            e_run = e_chunk;
+            const int sh_new = min(139 - e_run, 127);
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[b][p][r] = ldexpf(acc[b][p][r], sh_new - sh);
+            sh = sh_new;
+        }
+        coef = make_float2(ldexpf(coef.x, sh), ldexpf(coef.y, sh));
+        const float2 v = rotation(c, E, q.lobe, q.delta, 1);
+        const float2 z1 = rotation(c, E, q.lobe, q.delta, 16);
+        float val[32];
+        {
+            // W_1..8 by a product tree, Y_j = c W_7 z1^j by two chains over z1^2
+            float2 W[9];
+            W[1] = v;
+#pragma unroll
+            for (int i = 2; i <= 8; ++i) W[i] = (i & 1) ? cmul(W[i - 1], v) : csq(W[i >> 1]);
+            const float2 z2 = csq(z1);
+            float2 Y[8];
+            Y[0] = cmul(coef, W[7]);
+            Y[1] = cmul(Y[0], z1);
+#pragma unroll
+            for (int j = 2; j < 8; ++j) Y[j] = cmul(Y[j - 2], z2);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                val[2 * j] = Y[j].x;
+                val[2 * j + 1] = Y[j].y;
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                val[16 + i] = W[i + 1].x;
+                val[24 + i] = W[i + 1].y;
+            }
+        }
+        // the previous chunk's reads were issued before these writes (in-order LDS per wave)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int p = 0; p < 16; ++p) {
+            float x0 = val[2 * p], x1 = val[2 * p + 1];
+            swap_halves(x0, x1);  // half h: row rowbase(p) + h of surfels t (x0) and t + 16 (x1)
+            f16x2 hi, lo;
+            split_f16_pair<false>(x0, x1, hi, lo);
+            const int R = h16_rowbase(p);  // even: h only moves the row by one
+            uint32_t* dst = rec + wofs[(R >> 2) & 3];
+            dst[h16_word(R, 0, 0, 0) - 4 * (0 ^ ((R >> 2) & 3))] = __builtin_bit_cast(uint32_t, hi);
+            dst[h16_word(R, 1, 0, 0) - 4 * (0 ^ ((R >> 2) & 3))] = __builtin_bit_cast(uint32_t, lo);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        f16x8 op[2][2][3];  // [aperture][hi, lo][A, B re, B im]
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int hl = 0; hl < 2; ++hl)
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    const int R = 16 * o + x;
+                    op[b][hl][o] = __builtin_bit_cast(
+                        f16x8, *reinterpret_cast<const uint4*>(rec + b * kH16ApWords + h16_word(R, hl, g, 0)));
+                }
+        // the four accumulators interleaved, so no product waits for the one before it
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+                    acc[b][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(op[b][k == 2][0], op[b][k == 1][1 + p],
+                                                                       acc[b][p], 0, 0, 0);
+        in = next;
+    }
+    __syncthreads();  // every wave's records consumed: the region becomes X
+    float2* X = reinterpret_cast<float2*>(region);
+    const int ldx = c.S + 1;
+    {
+        // register r of lane (g, x) is row 4 g + r = (j = 2 g + r / 2, Y part r % 2) of column
+        // x, i.e. W_(x + 1) for x < 8 and W_0 for x = 8
+        const int i = x < 8 ? x + 1 : 0;
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {
+                const int n0 = 16 * (2 * g + (r >> 1)) + 7;
+                const float P1 = acc[b][0][r], P3 = acc[b][0][r + 1], P4 = acc[b][1][r], P2 = acc[b][1][r + 1];
+                float2* row = X + (2 * wave + b) * ldx;
+                if (x <= 8) row[n0 + i] = make_float2(ldexpf(P1 - P2, -sh), ldexpf(P3 + P4, -sh));
+                if (x < 7) row[n0 - i] = make_float2(ldexpf(P1 + P2, -sh), ldexpf(P3 - P4, -sh));
+            }
+        // rows 8..15 of the 16-row DFT tile are not written out; keep them zero
+        for (int i2 = tid; i2 < kH16Apertures * ldx; i2 += 256) X[kH16Apertures * ldx + i2] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    dft16(X, ldx, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(kH16Apertures, A - a0), wave,
+          lane);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -670,9 +886,26 @@ static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, co
     return hipGetLastError();
 }
 
+static hipError_t launch_td_render_h16(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                       const float2* w, int S, float2* out, hipStream_t st) {
+    const size_t region = sizeof(uint32_t) * 4 * (size_t)kH16Words;
+    const size_t xbuf = sizeof(float2) * 16 * (size_t)(c.S + 1);
+    const size_t smem = sizeof(float2) * c.N + (region > xbuf ? region : xbuf);
+    const bool fres = c.fresnel_n > 1.0f;
+    const void* fn = fres ? (const void*)td_render_h16_kernel<true> : (const void*)td_render_h16_kernel<false>;
+    hipError_t e = allow_lds(fn, smem);
+    if (e != hipSuccess) return e;
+    const dim3 grid((A + kH16Apertures - 1) / kH16Apertures);
+    if (fres)
+        hipLaunchKernelGGL((td_render_h16_kernel<true>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, S, out);
+    else
+        hipLaunchKernelGGL((td_render_h16_kernel<false>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, S, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_td_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                             const float2* w, int S, float2* out, hipStream_t st) {
-    if (c.S == 128) return launch_td_render_j<8, false>(c, pos, A, ctr, nrm, w, nullptr, nullptr, nullptr, S, out, st);
+    if (c.S == 128) return launch_td_render_h16(c, pos, A, ctr, nrm, w, S, out, st);
     if (c.S == 256) return launch_td_render_j<16, false>(c, pos, A, ctr, nrm, w, nullptr, nullptr, nullptr, S, out, st);
     return hipErrorInvalidValue;
 }
