@@ -76,6 +76,54 @@ __device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const f
     }
 }
 
+// dft16 for 8 rows: the 16 MFMA rows take the two halves of the samples, row i the samples
+// p < Lin / 2 of X row i % 8 (i < 8) or p + Lin / 2 (i >= 8), so the K loop is half as long.
+// The second half's result is turned by e^{sign 2 pi i (Lin / 2) q / N} (Re and Im of a bin
+// sit in neighbouring lanes: one quad swap) and added to the first half's across the lane
+// halves (rows i and i + 8 are lanes l and l + 32). Lin must be a multiple of 4.
+__device__ __forceinline__ void dft8_split(const float2* X, int ldx, int Lin, const float2* E, int N, float sign,
+                                           float scale, int Lout, float2* out, int ldo, int rows, int wave,
+                                           int lane) {
+    const int kk = lane >> 4, x = lane & 15;
+    const int pin = kk & 1, dp = kk >> 1, pout = x & 1;
+    const int half = Lin >> 1;
+    const float* Xf = reinterpret_cast<const float*>(X);
+    const float* xrow = Xf + 2 * ((x & 7) * ldx + dp + (x >> 3) * half) + pin;
+    for (int ct = wave; ct * 8 < Lout; ct += 4) {
+        const int q = ct * 8 + (x >> 1);
+        f32x4 d = {0.f, 0.f, 0.f, 0.f};
+        int m = (dp * q) & (N - 1);
+        const int step = (2 * q) & (N - 1);
+        for (int t = 0; t < half / 2; ++t) {
+            const float a = xrow[4 * t];
+            const float2 e = E[m];
+            const float fr = e.x * scale, fi = sign * e.y * scale;
+            const float b = pin ? (pout ? fr : -fi) : (pout ? fi : fr);
+            d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
+            m = (m + step) & (N - 1);
+        }
+        const float2 rho = E[(half * q) & (N - 1)];
+        const float rr = rho.x, ri = sign * rho.y;
+        float res[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const float other = __shfl_xor(d[r], 1, 64);
+            const float re = pout ? other : d[r], im = pout ? d[r] : other;
+            float turned = pout ? fmaf(re, ri, im * rr) : fmaf(re, rr, -im * ri);
+            float recv = turned;
+            swap_halves(turned, recv);  // lanes 0..31: recv = turned of lane + 32
+            res[r] = d[r] + recv;
+        }
+        if (kk < 2) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 4 * kk + r;
+                if (row < rows) reinterpret_cast<float*>(out + (size_t)row * ldo + q)[pout] = res[r];
+            }
+        }
+    }
+}
+
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 
@@ -415,7 +463,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     const float* __restrict__ nrm, const float2* __restrict__ w, int S, float2* __restrict__ out) {
     extern __shared__ float smem[];
     float2* E = reinterpret_cast<float2*>(smem);  // [N]
-    float* region = smem + 2 * c.N;               // 4 waves of records; later X [16][S + 1] float2
+    float* region = smem + 2 * c.N;               // 4 waves of records; later X [8][S + 1] float2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
     uint32_t* rec = reinterpret_cast<uint32_t*>(region) + wave * kH16Words;
     const int a0 = blockIdx.x * kH16Apertures;
@@ -555,12 +603,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
                 if (x <= 8) row[n0 + i] = make_float2(ldexpf(P1 - P2, -sh), ldexpf(P3 + P4, -sh));
                 if (x < 7) row[n0 - i] = make_float2(ldexpf(P1 + P2, -sh), ldexpf(P3 - P4, -sh));
             }
-        // rows 8..15 of the 16-row DFT tile are not written out; keep them zero
-        for (int i2 = tid; i2 < kH16Apertures * ldx; i2 += 256) X[kH16Apertures * ldx + i2] = make_float2(0.f, 0.f);
     }
     __syncthreads();
-    dft16(X, ldx, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(kH16Apertures, A - a0), wave,
-          lane);
+    dft8_split(X, ldx, c.S, E, c.N, -1.0f, c.inv_S, c.K, out + (size_t)a0 * c.K, c.K, min(kH16Apertures, A - a0),
+               wave, lane);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -889,7 +935,7 @@ static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, co
 static hipError_t launch_td_render_h16(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                        const float2* w, int S, float2* out, hipStream_t st) {
     const size_t region = sizeof(uint32_t) * 4 * (size_t)kH16Words;
-    const size_t xbuf = sizeof(float2) * 16 * (size_t)(c.S + 1);
+    const size_t xbuf = sizeof(float2) * kH16Apertures * (size_t)(c.S + 1);
     const size_t smem = sizeof(float2) * c.N + (region > xbuf ? region : xbuf);
     const bool fres = c.fresnel_n > 1.0f;
     const void* fn = fres ? (const void*)td_render_h16_kernel<true> : (const void*)td_render_h16_kernel<false>;
