@@ -19,14 +19,17 @@
 // into LDS, then each wave accumulates X of its 4 apertures as [2J x 16] real tiles:
 //   rows (j, part), cols i, K = (surfel, part): Re X += Zr (cW)r - Zi (cW)i, Im X += Zi (cW)r + Zr (cW)i.
 // The DFT to bins (a [16 x S] x [S x K] complex GEMM with twiddles from the LDS table) runs
-// in the same workgroup, so X never leaves the chip.
+// in the same workgroup, so X never leaves the chip. This FP32 form serves 256 samples and the
+// JVP; 128 samples use td_render_h16_kernel (FP16 hi + lo products, symmetric split, below).
 //
 // VJP (td_vjp_kernel<J>): a workgroup owns 128 surfels (32 per wave, lane % 32 = surfel) and a
 // range of apertures; U (the DFT of the cotangent, from the dft_rows_kernel pre-pass) is
 // staged per aperture in the MFMA A-operand layout. The two lane halves generate two pairs
 // (apertures a and a + 1, same surfel) and swap half of their W operands, so every pair is
-// generated once. Per aperture M0[s, j] = sum_i W_i U[16 j + i] and M1 = the same with
-// n U[n] form one 32-row tile per 8 j (v_mfma_f32_32x32x2f32); P0 = sum_j Z_j M0, P1 likewise.
+// generated once. With the symmetric split n = 16 j + 7.5 +- (i + 0.5), i < 8 (vjp_operands_kernel),
+// per aperture M0[s, j] = sum_i W_i U+ + conj(W_i) U- and M1 the same with n U[n] form one
+// 32-row tile per 8 j on v_mfma_f32_32x32x16_f16 (FP16 hi + lo operands, K = 16: Re and Im of
+// the 8 W_i = w^(i + 1/2)); P0 = sum_j Y_j M0 with Y_j = w^(16 j + 7.5), P1 likewise.
 // The 8 chained gradients accumulate in registers over the aperture range (no cross-lane
 // reduction per pair) and are written as one partial per (surfel, range), summed in float64
 // by vjp_reduce.
@@ -653,11 +656,14 @@ __device__ __forceinline__ void td_row(int tile, int q, int h, int r, int* mat, 
 
 // A operands of the VJP GEMM for v_mfma_f32_32x32x16_f16 (A[i][8 h + e] -> lane i + 32 h,
 // element e), as FP16 hi + lo splits of the per-aperture scaled values, built once per
-// aperture from U: [A][T tiles][2 K halves][2 (hi, lo)][64 lanes] x 8 halves, and the scale
-// exponent e_a per aperture (values were multiplied by 2^-e_a, max |value| below 2^12).
-// Row i -> (mat, j, part) as the accumulator layout (td_row); K = 2 t + k over t < 16 and
-// k = 0 (B = Re W_t) or 1 (B = Im W_t): (re) Re U, (im) Im U for k = 0; (re) -Im U, (im) Re U
-// for k = 1, with U = U[16 j + t] (mat 0) or (16 j + t) U[16 j + t] (mat 1).
+// aperture from U: [A][T tiles][2 (hi, lo)][64 lanes] x 8 halves, and the scale exponent e_a
+// per aperture (values were multiplied by 2^-e_a, max |value| below 2^12).
+// The samples are split symmetrically with half-integer powers: n = 16 j + 7.5 +- (i + 0.5),
+// i < 8, so sum_n w^n U[n] = sum_j Y_j M_j with Y_j = w^(16 j + 7.5), W_i = w^(i + 0.5) and
+// M_j = sum_i W_i U+ + conj(W_i) U- = sum_i Re W_i (U+ + U-) + i Im W_i (U+ - U-), where
+// U+ = U[16 j + 8 + i], U- = U[16 j + 7 - i] (times n for mat 1). Row -> (mat, j, part) as
+// the accumulator layout (td_row); K = 2 i + k, k = 0 (B = Re W_i) or 1 (B = Im W_i):
+// (re) Re S, (im) Im S for k = 0; (re) -Im D, (im) Re D for k = 1 (S = U+ + U-, D = U+ - U-).
 template <int J>
 __global__ __launch_bounds__(64 * (J / 8)) void vjp_operands_kernel(const float2* __restrict__ U, int S,
                                                                     uint4* __restrict__ out,
@@ -669,20 +675,22 @@ __global__ __launch_bounds__(64 * (J / 8)) void vjp_operands_kernel(const float2
     int mat, j;
     td_row<J>(tile, row >> 3, (row >> 2) & 1, row & 3, &mat, &j);
     const int part = row & 1;
-    float v[16];
+    float v[8];
     float mx = 0.0f;
 #pragma unroll
-    for (int kh = 0; kh < 2; ++kh)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int K = 16 * kh + 8 * h + e, t = K >> 1, k = K & 1;
-            const int n = 16 * j + t;
-            float2 z = U[(size_t)a * S + n];
-            if (mat) z = (float)n * z;
-            const float x = k == 0 ? (part ? z.y : z.x) : (part ? z.x : -z.y);
-            v[8 * kh + e] = x;
-            mx = fmaxf(mx, fabsf(x));
+    for (int e = 0; e < 8; ++e) {
+        const int K = 8 * h + e, i = K >> 1, k = K & 1;
+        const int np = 16 * j + 8 + i, nm = 16 * j + 7 - i;
+        float2 up = U[(size_t)a * S + np], um = U[(size_t)a * S + nm];
+        if (mat) {
+            up = (float)np * up;
+            um = (float)nm * um;
         }
+        const float2 sum = up + um, dif = up - um;
+        const float x = k == 0 ? (part ? sum.y : sum.x) : (part ? dif.x : -dif.y);
+        v[e] = x;
+        mx = fmaxf(mx, fabsf(x));
+    }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     if (L == 0) red[tile] = mx;
@@ -693,19 +701,83 @@ __global__ __launch_bounds__(64 * (J / 8)) void vjp_operands_kernel(const float2
     ex -= 12;
     if (tid == 0) scale_exp[a] = (float)ex;
     const float sc = ldexpf(1.0f, -ex);
+    f16x8 hi, lo;
 #pragma unroll
-    for (int kh = 0; kh < 2; ++kh) {
-        f16x8 hi, lo;
+    for (int e = 0; e < 8; ++e) {
+        const float x = v[e] * sc;
+        const _Float16 xh = (_Float16)x;
+        hi[e] = xh;
+        lo[e] = (_Float16)(x - (float)xh);
+    }
+    uint4* dst = out + (((size_t)a * T + tile) * 2) * 64 + L;
+    dst[0] = __builtin_bit_cast(uint4, hi);
+    dst[64] = __builtin_bit_cast(uint4, lo);
+}
+
+// e^{2 pi i (lobe + delta) / (2 N)} = E[lobe / 2] e^{i pi ((lobe % 2) + delta) / N}: a square
+// root of rotation(..., 1), relatively accurate like it (|pi (1 + delta) / N| <= pi / 4).
+__device__ __forceinline__ float2 half_rotation(const Chirp& ch, const float2* E, int lobe, float delta) {
+    float s, c;
+    sincos_small(ch.pi_N * ((float)(lobe & 1) + delta), &s, &c);
+    return cmul(E[lobe >> 1], make_float2(c, s));
+}
+
+// The VJP's pair stage with the symmetric split above.
+template <int J>
+struct VjpStageSym {
+    f16x8 Bh[2], Bl[2];  // [aperture a / a + 1]: FP16 hi / lo of the B operands
+    float2 Z[J];         // Y_j = w^(16 j + 7.5) of the own pair
+    float2 rot;
+    float amp, amp_cos, inv_r, ux, uy, uz, cosine, rate;
+};
+
+template <int J, bool FRESNEL>
+__device__ __forceinline__ void vjp_stage_sym(const Chirp& c, const float2* E, const float* __restrict__ pos, int a,
+                                              bool ok, float cx, float cy, float cz, float nx, float ny, float nz,
+                                              VjpStageSym<J>& st) {
+    const Pair q = pair_geometry_t<FRESNEL>(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
+    st.rot = q.rot;
+    st.amp = ok ? q.amp : 0.0f;
+    st.amp_cos = ok ? q.amp_cos : 0.0f;
+    st.inv_r = q.inv_r;
+    st.ux = q.ux;
+    st.uy = q.uy;
+    st.uz = q.uz;
+    st.cosine = q.cosine;
+    st.rate = q.rate;
+    // w = conj(v): W_i = w^(i + 1/2) by two chains over w^2, Y_j = W_7 z1^j by two chains over z1^2
+    const float2 w1 = cconj(rotation(c, E, q.lobe, q.delta, 1));
+    const float2 z1 = cconj(rotation(c, E, q.lobe, q.delta, 16));
+    const float2 w2 = csq(w1);
+    float2 W[8];
+    W[0] = cconj(half_rotation(c, E, q.lobe, q.delta));
+    W[1] = cmul(W[0], w1);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float x = v[8 * kh + e] * sc;
-            const _Float16 xh = (_Float16)x;
-            hi[e] = xh;
-            lo[e] = (_Float16)(x - (float)xh);
+    for (int i = 2; i < 8; ++i) W[i] = cmul(W[i - 2], w2);
+    const float2 z2 = csq(z1);
+    st.Z[0] = W[7];
+    st.Z[1] = cmul(W[7], z1);
+#pragma unroll
+    for (int j = 2; j < J; ++j) st.Z[j] = cmul(st.Z[j - 2], z2);
+    // B operands: lane half h supplies K = 8 h + e, i.e. W_i with i = 4 h + e / 2 (Re for even
+    // e, Im for odd e). Half h generated aperture a + h; one half swap of (W_q, W_{4 + q})
+    // gives each half its i of aperture a and of a + 1.
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+        float xr = W[qq].x, yr = W[4 + qq].x;
+        float xi = W[qq].y, yi = W[4 + qq].y;
+        swap_halves(xr, yr);
+        swap_halves(xi, yi);
+        const float vals[2][2] = {{xr, xi}, {yr, yi}};  // [aperture][re, im]
+#pragma unroll
+        for (int ap = 0; ap < 2; ++ap) {
+            f16x2 hi, lo;
+            split_f16_pair<true>(vals[ap][0], vals[ap][1], hi, lo);
+            st.Bh[ap][2 * qq] = hi.x;
+            st.Bh[ap][2 * qq + 1] = hi.y;
+            st.Bl[ap][2 * qq] = lo.x;
+            st.Bl[ap][2 * qq + 1] = lo.y;
         }
-        uint4* dst = out + ((((size_t)a * T + tile) * 2 + kh) * 2) * 64 + L;
-        dst[0] = __builtin_bit_cast(uint4, hi);
-        dst[64] = __builtin_bit_cast(uint4, lo);
     }
 }
 
@@ -782,11 +854,11 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
                                                      float* __restrict__ partial) {
     constexpr int T = J / 8;  // 32-row tiles per aperture
     constexpr int G = kTdVjpGroup / T;
-    constexpr int PER = T * 2 * 2 * 64;  // uint4 per aperture
+    constexpr int PER = T * 2 * 64;  // uint4 per aperture
     extern __shared__ float smem[];
     float2* E = reinterpret_cast<float2*>(smem);         // [N]
     float* Sc = reinterpret_cast<float*>(E + c.N);       // [G] scales 2^e_a
-    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);      // [G][T][2][2][64]
+    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);      // [G][T][2][64]
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
     // the surfel tiles of one aperture range run on one XCD and read its operands through one L2
@@ -813,10 +885,10 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
         if (tid < G) Sc[tid] = ldexpf(1.0f, (int)scale_exp[min(g0 + tid, A - 1)]);
         __syncthreads();
         for (int ga = 0; ga < G; ga += 2) {
-            VjpStage<J> cur;
+            VjpStageSym<J> cur;
             {
                 const int a = g0 + ga + h;  // this lane's pair: (a, s)
-                vjp_stage<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
+                vjp_stage_sym<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
             }
             f32x16 acc[2][T];
 #pragma unroll
@@ -828,16 +900,14 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
 #pragma unroll
             for (int ap = 0; ap < 2; ++ap)
 #pragma unroll
-                for (int tile = 0; tile < T; ++tile)
-#pragma unroll
-                    for (int kh = 0; kh < 2; ++kh) {
-                        const uint4* src = Aop + (((ga + ap) * T + tile) * 2 + kh) * 2 * 64 + lane;
-                        const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
-                        const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
-                        acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap][kh], acc[ap][tile], 0, 0, 0);
-                        acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap][kh], acc[ap][tile], 0, 0, 0);
-                        acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap][kh], acc[ap][tile], 0, 0, 0);
-                    }
+                for (int tile = 0; tile < T; ++tile) {
+                    const uint4* src = Aop + ((ga + ap) * T + tile) * 2 * 64 + lane;
+                    const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
+                    const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
+                    acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap], acc[ap][tile], 0, 0, 0);
+                    acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap], acc[ap][tile], 0, 0, 0);
+                    acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap], acc[ap][tile], 0, 0, 0);
+                }
             // Contraction P_mat = sum_j Z_j M_mat[j]. Half h holds the rows of j in its set J_h
             // for both apertures; one half swap of (Z_{j of half 0}, Z_{j of half 1}) gives each
             // half Z_{j in J_h} of aperture a and of aperture a + 1.
@@ -977,7 +1047,7 @@ TdVjpShape td_vjp_shape(int A, int S) {
     return v;
 }
 
-// Workspace: partials [S][ranges][8] floats | U [A][S] complex | A operands [A][T][2][2][64]
+// Workspace: partials [S][ranges][8] floats | U [A][S] complex | A operands [A][T][2][64]
 // uint4 | scale exponents [A] floats, each part 256-byte aligned.
 struct TdVjpWorkspace {
     size_t partial, u, ops, scales, total;
@@ -990,7 +1060,7 @@ static TdVjpWorkspace td_vjp_layout(const Chirp& c, int A, int S) {
     w.partial = 0;
     w.u = up(sizeof(float) * 8 * (size_t)S * v.ranges);
     w.ops = w.u + up(sizeof(float2) * (size_t)A * c.S);
-    w.scales = w.ops + up(sizeof(uint4) * (size_t)A * (c.S / 128) * 2 * 2 * 64);
+    w.scales = w.ops + up(sizeof(uint4) * (size_t)A * (c.S / 128) * 2 * 64);
     w.total = w.scales + up(sizeof(float) * (size_t)A);
     return w;
 }
@@ -1023,7 +1093,7 @@ static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t smem = sizeof(float2) * c.N + sizeof(float) * 16 +
-                        sizeof(uint4) * (size_t)(kTdVjpGroup / T) * T * 2 * 2 * 64;
+                        sizeof(uint4) * (size_t)(kTdVjpGroup / T) * T * 2 * 64;
     const void* fn = c.fresnel_n > 1.0f ? (const void*)td_vjp_kernel<J, true> : (const void*)td_vjp_kernel<J, false>;
     e = allow_lds(fn, smem);
     if (e != hipSuccess) return e;
