@@ -722,9 +722,10 @@ __device__ __forceinline__ float2 half_rotation(const Chirp& ch, const float2* E
     return cmul(E[lobe >> 1], make_float2(c, s));
 }
 
-// The VJP's pair stage with the symmetric split above.
+// What one lane carries from the generation of its pair to the contraction and chain rule
+// (symmetric split above).
 template <int J>
-struct VjpStageSym {
+struct VjpStage {
     f16x8 Bh[2], Bl[2];  // [aperture a / a + 1]: FP16 hi / lo of the B operands
     float2 Z[J];         // Y_j = w^(16 j + 7.5) of the own pair
     float2 rot;
@@ -732,9 +733,9 @@ struct VjpStageSym {
 };
 
 template <int J, bool FRESNEL>
-__device__ __forceinline__ void vjp_stage_sym(const Chirp& c, const float2* E, const float* __restrict__ pos, int a,
+__device__ __forceinline__ void vjp_stage(const Chirp& c, const float2* E, const float* __restrict__ pos, int a,
                                               bool ok, float cx, float cy, float cz, float nx, float ny, float nz,
-                                              VjpStageSym<J>& st) {
+                                              VjpStage<J>& st) {
     const Pair q = pair_geometry_t<FRESNEL>(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
     st.rot = q.rot;
     st.amp = ok ? q.amp : 0.0f;
@@ -779,65 +780,6 @@ __device__ __forceinline__ void vjp_stage_sym(const Chirp& c, const float2* E, c
             st.Bl[ap][2 * qq + 1] = lo.y;
         }
     }
-}
-
-// What one lane carries from the generation of its pair to the contraction and chain rule.
-template <int J>
-struct VjpStage {
-    f16x8 Bh[2][2], Bl[2][2];  // [aperture a / a + 1][K half]: FP16 hi / lo of the B operands
-    float2 Z[J];               // Z_j of the own pair
-    float2 rot;
-    float amp, amp_cos, inv_r, ux, uy, uz, cosine, rate;
-};
-
-template <int J, bool FRESNEL>
-__device__ __forceinline__ void vjp_stage(const Chirp& c, const float2* E, const float* __restrict__ pos, int a,
-                                          bool ok, float cx, float cy, float cz, float nx, float ny, float nz,
-                                          VjpStage<J>& st) {
-    const Pair q = pair_geometry_t<FRESNEL>(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx, ny, nz);
-    st.rot = q.rot;
-    st.amp = ok ? q.amp : 0.0f;
-    st.amp_cos = ok ? q.amp_cos : 0.0f;
-    st.inv_r = q.inv_r;
-    st.ux = q.ux;
-    st.uy = q.uy;
-    st.uz = q.uz;
-    st.cosine = q.cosine;
-    st.rate = q.rate;
-    // w = conj(v): W_t = w^t, Z_j = w^{16 j}
-    const float2 w1 = cconj(rotation(c, E, q.lobe, q.delta, 1));
-    const float2 z1 = cconj(rotation(c, E, q.lobe, q.delta, 16));
-    float2 W[16];
-    W[0] = make_float2(1.f, 0.f);
-    W[1] = w1;
-#pragma unroll
-    for (int t = 2; t < 16; ++t) W[t] = (t & 1) ? cmul(W[t - 1], w1) : csq(W[t >> 1]);
-    st.Z[0] = make_float2(1.f, 0.f);
-    st.Z[1] = z1;
-#pragma unroll
-    for (int j = 2; j < J; ++j) st.Z[j] = (j & 1) ? cmul(st.Z[j - 1], z1) : csq(st.Z[j >> 1]);
-    // B operands: lane half h supplies K = 16 kh + 8 h + e, i.e. W_t with t = 8 kh + 4 h + e / 2
-    // (Re for even e, Im for odd e). Half h generated aperture a + h; one half swap of
-    // (W_{8 kh + q}, W_{8 kh + 4 + q}) gives each half its t of aperture a and of a + 1.
-#pragma unroll
-    for (int kh = 0; kh < 2; ++kh)
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) {
-            float xr = W[8 * kh + qq].x, yr = W[8 * kh + 4 + qq].x;
-            float xi = W[8 * kh + qq].y, yi = W[8 * kh + 4 + qq].y;
-            swap_halves(xr, yr);
-            swap_halves(xi, yi);
-            const float vals[2][2] = {{xr, xi}, {yr, yi}};  // [aperture][re, im]
-#pragma unroll
-            for (int ap = 0; ap < 2; ++ap) {
-                f16x2 hi, lo;
-                split_f16_pair<true>(vals[ap][0], vals[ap][1], hi, lo);
-                st.Bh[ap][kh][2 * qq] = hi.x;
-                st.Bh[ap][kh][2 * qq + 1] = hi.y;
-                st.Bl[ap][kh][2 * qq] = lo.x;
-                st.Bl[ap][kh][2 * qq + 1] = lo.y;
-            }
-        }
 }
 
 // One workgroup: 4 waves x 32 surfels, an aperture range staged G apertures at a time. Each
@@ -885,10 +827,10 @@ __global__ __launch_bounds__(256) void td_vjp_kernel(const Chirp c, const float*
         if (tid < G) Sc[tid] = ldexpf(1.0f, (int)scale_exp[min(g0 + tid, A - 1)]);
         __syncthreads();
         for (int ga = 0; ga < G; ga += 2) {
-            VjpStageSym<J> cur;
+            VjpStage<J> cur;
             {
                 const int a = g0 + ga + h;  // this lane's pair: (a, s)
-                vjp_stage_sym<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
+                vjp_stage<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
             }
             f32x16 acc[2][T];
 #pragma unroll
@@ -1123,7 +1065,7 @@ hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* c
 // f is a trigonometric polynomial of degree S - 1 in v = e^{2 pi i k0 / N}:
 //   f = 2 Re sum_{m=0}^{S-1} phi_m v^m,  phi_0 = K / (2 S),  phi_m = (S - m) E_m / S^2,
 //   E_m = sum_{k<K} e^{-2 pi i m k / N} = 1 - i cot(pi m / N) for odd m, 0 for even m (K = N / 2),
-// so with psi = conj(phi) it is 2 Re sum_j Z_j sum_i W_i psi_{16 j + i}: the same product as g0
+// so with psi = conj(phi) it is 2 Re sum_n w^n psi_n: the same product as g0 (split as the VJP's)
 // with psi in place of R. The 32 accumulator rows per aperture hold both: rows of mat 0 are
 // the residual's (scaled by 2^-e_a per aperture), rows of mat 1 the fixed psi (scaled by 2^10).
 // ---------------------------------------------------------------------------------------
@@ -1140,6 +1082,8 @@ struct TdScanSet {
     int per_range;
 };
 
+// Operands as vjp_operands_kernel (symmetric half-integer split, one K half) for the rows of
+// both mats: S = U+ + U-, D = U+ - U- of the residual (mat 0) and of psi (mat 1).
 __global__ __launch_bounds__(64) void scan_operands_kernel(const float2* __restrict__ U, int S, int N, int K,
                                                            uint4* __restrict__ out, float* __restrict__ scale_exp) {
     const int a = blockIdx.x, L = threadIdx.x;
@@ -1147,34 +1091,33 @@ __global__ __launch_bounds__(64) void scan_operands_kernel(const float2* __restr
     int mat, j;
     td_row<8>(0, row >> 3, (row >> 2) & 1, row & 3, &mat, &j);
     const int part = row & 1;
-    float v[16];
+    // psi_m = conj(phi_m), scaled by 2^kScanPsiExp
+    auto psi = [&](int m) {
+        double re = 0.0, im = 0.0;
+        if (m == 0) {
+            re = (double)K / (2.0 * S);
+        } else if (m & 1) {
+            double sn, cs;
+            sincospi((double)m / (double)N, &sn, &cs);
+            const double w = (double)(S - m) / ((double)S * S);
+            re = w;
+            im = w * cs / sn;
+        }
+        return make_float2((float)ldexp(re, kScanPsiExp), (float)ldexp(im, kScanPsiExp));
+    };
+    float v[8];
     float mx = 0.0f;
 #pragma unroll
-    for (int kh = 0; kh < 2; ++kh)
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int Kx = 16 * kh + 8 * h + e, t = Kx >> 1, k = Kx & 1;
-            const int m = 16 * j + t;
-            float2 z;
-            if (mat == 0) {
-                z = U[(size_t)a * S + m];
-            } else {  // psi_m = conj(phi_m)
-                double re = 0.0, im = 0.0;
-                if (m == 0) {
-                    re = (double)K / (2.0 * S);
-                } else if (m & 1) {
-                    double sn, cs;
-                    sincospi((double)m / (double)N, &sn, &cs);
-                    const double w = (double)(S - m) / ((double)S * S);
-                    re = w;
-                    im = w * cs / sn;
-                }
-                z = make_float2((float)ldexp(re, kScanPsiExp), (float)ldexp(im, kScanPsiExp));
-            }
-            const float x = k == 0 ? (part ? z.y : z.x) : (part ? z.x : -z.y);
-            v[8 * kh + e] = x;
-            if (mat == 0) mx = fmaxf(mx, fabsf(x));
-        }
+    for (int e = 0; e < 8; ++e) {
+        const int Kx = 8 * h + e, i = Kx >> 1, k = Kx & 1;
+        const int np = 16 * j + 8 + i, nm = 16 * j + 7 - i;
+        const float2 up = mat == 0 ? U[(size_t)a * S + np] : psi(np);
+        const float2 um = mat == 0 ? U[(size_t)a * S + nm] : psi(nm);
+        const float2 sum = up + um, dif = up - um;
+        const float x = k == 0 ? (part ? sum.y : sum.x) : (part ? dif.x : -dif.y);
+        v[e] = x;
+        if (mat == 0) mx = fmaxf(mx, fabsf(x));
+    }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     int ex = 0;
@@ -1182,20 +1125,17 @@ __global__ __launch_bounds__(64) void scan_operands_kernel(const float2* __restr
     ex -= 12;
     if (L == 0) scale_exp[a] = (float)ex;
     const float sc = mat == 0 ? ldexpf(1.0f, -ex) : 1.0f;
+    f16x8 hi, lo;
 #pragma unroll
-    for (int kh = 0; kh < 2; ++kh) {
-        f16x8 hi, lo;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float x = v[8 * kh + e] * sc;
-            const _Float16 xh = (_Float16)x;
-            hi[e] = xh;
-            lo[e] = (_Float16)(x - (float)xh);
-        }
-        uint4* dst = out + (((size_t)a * 2 + kh) * 2) * 64 + L;
-        dst[0] = __builtin_bit_cast(uint4, hi);
-        dst[64] = __builtin_bit_cast(uint4, lo);
+    for (int e = 0; e < 8; ++e) {
+        const float x = v[e] * sc;
+        const _Float16 xh = (_Float16)x;
+        hi[e] = xh;
+        lo[e] = (_Float16)(x - (float)xh);
     }
+    uint4* dst = out + ((size_t)a * 2) * 64 + L;
+    dst[0] = __builtin_bit_cast(uint4, hi);
+    dst[64] = __builtin_bit_cast(uint4, lo);
 }
 
 // grid (candidate tiles of 128, aperture ranges of every scan back to back); partial
@@ -1206,11 +1146,11 @@ __global__ __launch_bounds__(256) void scan_td_kernel(const Chirp c, const TdSca
                                                       int C, const uint4* __restrict__ ops,
                                                       const float* __restrict__ scale_exp,
                                                       float* __restrict__ partial) {
-    constexpr int J = 8, G = kTdVjpGroup, PER = 2 * 2 * 64;
+    constexpr int J = 8, G = kTdVjpGroup, PER = 2 * 64;
     extern __shared__ float smem[];
     float2* E = reinterpret_cast<float2*>(smem);     // [N]
     float* Sc = reinterpret_cast<float*>(E + c.N);   // [G] scales 2^e_a
-    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);  // [G][2][2][64]
+    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);  // [G][2][64]
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
     const int s = blockIdx.x * kScanCandidates + 32 * wave + m;
@@ -1251,15 +1191,12 @@ __global__ __launch_bounds__(256) void scan_td_kernel(const Chirp c, const TdSca
             for (int ap = 0; ap < 2; ++ap) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r) acc[ap][r] = 0.f;
-#pragma unroll
-                for (int kh = 0; kh < 2; ++kh) {
-                    const uint4* src = Aop + ((ga + ap) * 2 + kh) * 2 * 64 + lane;
-                    const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
-                    const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
-                    acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap][kh], acc[ap], 0, 0, 0);
-                    acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap][kh], acc[ap], 0, 0, 0);
-                    acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap][kh], acc[ap], 0, 0, 0);
-                }
+                const uint4* src = Aop + (ga + ap) * 2 * 64 + lane;
+                const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
+                const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
+                acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap], acc[ap], 0, 0, 0);
+                acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap], acc[ap], 0, 0, 0);
+                acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap], acc[ap], 0, 0, 0);
             }
             float2 P[2][2];  // [aperture a / a + 1][mat 0: residual, mat 1: psi]
 #pragma unroll
@@ -1343,7 +1280,7 @@ static TdScanSet td_scan_set(const ScanSet& set, int chunk) {
 }
 
 // Workspace: partials [C][ranges][3] (ranges at the smallest range length) | U [sum A][S]
-// complex | operands [sum A][256] uint4 | scale exponents [sum A], each 256-byte aligned.
+// complex | operands [sum A][128] uint4 | scale exponents [sum A], each 256-byte aligned.
 struct TdScanWorkspace {
     size_t partial, u, ops, scales, total;
 };
@@ -1356,7 +1293,7 @@ static TdScanWorkspace td_scan_layout(const Chirp& c, const ScanSet& set, int C)
     w.partial = 0;
     w.u = up(sizeof(float) * 3 * (size_t)C * t.range_first[set.count]);
     w.ops = w.u + up(sizeof(float2) * apertures * c.S);
-    w.scales = w.ops + up(sizeof(uint4) * apertures * 256);
+    w.scales = w.ops + up(sizeof(uint4) * apertures * 128);
     w.total = w.scales + up(sizeof(float) * apertures);
     return w;
 }
@@ -1386,7 +1323,7 @@ hipError_t launch_td_scan(const Chirp& c, const ScanSet& set, const float* ctr, 
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int nrange = t.range_first[set.count];
     const dim3 grid((C + kScanCandidates - 1) / kScanCandidates, nrange);
-    const size_t smem = sizeof(float2) * c.N + sizeof(float) * 16 + sizeof(uint4) * (size_t)kTdVjpGroup * 256;
+    const size_t smem = sizeof(float2) * c.N + sizeof(float) * 16 + sizeof(uint4) * (size_t)kTdVjpGroup * 128;
     const bool fres = c.fresnel_n > 1.0f;
     e = allow_lds(fres ? (const void*)scan_td_kernel<true> : (const void*)scan_td_kernel<false>, smem);
     if (e != hipSuccess) return e;
