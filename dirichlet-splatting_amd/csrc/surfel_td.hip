@@ -437,9 +437,18 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
 //  * Records are wave-private, so the chunk loop has no workgroup barrier.
 constexpr int kH16Apertures = 8;
 constexpr int kH16Chunk = 32;
-constexpr int kH16Rows = 48;
-constexpr int kH16ApWords = kH16Rows * 32 + 16;  // [row][hi | lo 16 words] + bank shift
-constexpr int kH16Words = 2 * kH16ApWords;       // per wave: two apertures
+// rows per aperture: SETS x 16 Y rows, then the Re W and Im W tiles of 16 rows each
+template <bool JVP>
+struct H16Layout {
+    static constexpr int sets = JVP ? 2 : 1;
+    static constexpr int rows = 16 * sets + 32;
+    static constexpr int ap_words = rows * 32 + 16;  // [row][hi | lo 16 words] + bank shift
+    static constexpr int wave_words = 2 * ap_words;  // per wave: two apertures
+    static constexpr int pairs = rows / 2 - 8;       // row pairs written per chunk (the rest is constant)
+    // first row of producer row pair p: Y rows 2 p, Re W 16 sets + 2 (p - 8 sets), Im W 16 sets +
+    // 16 + 2 (p - 8 sets - 4)
+    __host__ __device__ static constexpr int rowbase(int p) { return p < 8 * sets + 4 ? 2 * p : 2 * p + 8; }
+};
 
 // word offset of the (hi | lo) half of row R, word group g, word u within the group
 __host__ __device__ constexpr int h16_word(int R, int lo, int g, int u) {
@@ -457,18 +466,25 @@ __device__ __forceinline__ int wave_max_i(int x) {
                max(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
 }
 
-// first row of the producer row pair p: Y rows 2 p (p < 8), Re W 16 + 2 (p - 8), Im W 32 + 2 (p - 12)
-__host__ __device__ constexpr int h16_rowbase(int p) { return p < 12 ? 2 * p : 2 * p + 8; }
+// biased exponent of max(|Re z|, |Im z|)
+__device__ __forceinline__ int cexp_bits(float2 z) {
+    return (int)(max(__float_as_uint(z.x) & 0x7fffffffu, __float_as_uint(z.y) & 0x7fffffffu) >> 23);
+}
 
-template <bool FRESNEL>
+// Forward (JVP = false) and JVP (JVP = true: two coefficient sets e1, e2 with one W set,
+// X = X1 - (2 pi i n / N) X2 as in td_render_kernel) on the layout above.
+template <bool FRESNEL, bool JVP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void td_render_h16_kernel(
     const Chirp c, const float* __restrict__ pos, int A, const float* __restrict__ ctr,
-    const float* __restrict__ nrm, const float2* __restrict__ w, int S, float2* __restrict__ out) {
+    const float* __restrict__ nrm, const float2* __restrict__ w, const float* __restrict__ dctr,
+    const float* __restrict__ dnrm, const float2* __restrict__ dw, int S, float2* __restrict__ out) {
+    using L = H16Layout<JVP>;
+    constexpr int SETS = L::sets, RW = 16 * SETS;  // RW: first Re W row
     extern __shared__ float smem[];
     float2* E = reinterpret_cast<float2*>(smem);  // [N]
     float* region = smem + 2 * c.N;               // 4 waves of records; later X [8][S + 1] float2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
-    uint32_t* rec = reinterpret_cast<uint32_t*>(region) + wave * kH16Words;
+    uint32_t* rec = reinterpret_cast<uint32_t*>(region) + wave * L::wave_words;
     const int a0 = blockIdx.x * kH16Apertures;
     const int pb = m >> 4, t = m & 15;
     const int a = a0 + 2 * wave + pb;
@@ -480,113 +496,138 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     // columns 9..15 (words written once; the chunk loop writes columns 0..7 only)
     for (int i = lane; i < 2 * 2 * 8 * 32; i += 64) {
         const int b = i >> 9, r = (i >> 5) & 15, wd = i & 31;
-        const int row = (r < 8 ? 16 : 32) + 8 + (r & 7);
-        const bool one = r == 0 && wd < 16 && !(((row >> 1) & 1));   // hi half of the Re W_0 row
-        const bool one_sw = r == 0 && wd >= 16 && (((row >> 1) & 1));  // (hi half stored second)
-        rec[b * kH16ApWords + row * 32 + wd] = (one || one_sw) ? 0x3C003C00u : 0u;  // (1.0, 1.0) FP16
+        const int row = RW + (r < 8 ? 0 : 16) + 8 + (r & 7);
+        const bool hi_half = (wd < 16) != (((row >> 1) & 1) != 0);  // hi words come second when (row / 2) is odd
+        rec[b * L::ap_words + row * 32 + wd] = (r == 0 && hi_half) ? 0x3C003C00u : 0u;  // (1.0, 1.0) FP16
     }
 
     // producer: lane word offsets for the four swizzle keys (row pair p uses key (rowbase / 4) % 4)
     int wofs[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) wofs[k] = pb * kH16ApWords + h * 32 + 4 * ((t >> 2) ^ k) + (t & 3);
-    // consumer: row x (A) or 16 + x, 32 + x (B) of aperture b, word group g
+    for (int k = 0; k < 4; ++k) wofs[k] = pb * L::ap_words + h * 32 + 4 * ((t >> 2) ^ k) + (t & 3);
+    // consumer: row 16 e + x (A of set e), RW + x (B re) or RW + 16 + x (B im) of aperture b, word group g
     const int x = lane & 15, g = lane >> 4;
 
-    f32x4 acc[2][2];
+    f32x4 acc[SETS][2][2];
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int e = 0; e < SETS; ++e)
 #pragma unroll
-        for (int p = 0; p < 2; ++p) acc[b][p] = f32x4{0.f, 0.f, 0.f, 0.f};
-    int e_run = 0, sh = 127;
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int p = 0; p < 2; ++p) acc[e][b][p] = f32x4{0.f, 0.f, 0.f, 0.f};
+    int e_run[SETS], sh[SETS];
+#pragma unroll
+    for (int e = 0; e < SETS; ++e) {
+        e_run[e] = 0;
+        sh[e] = 127;
+    }
 
-    SurfIn<false> in = load_surfel<false>(min(t + 16 * h, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
+    SurfIn<JVP> in = load_surfel<JVP>(min(t + 16 * h, S - 1), ctr, nrm, w, dctr, dnrm, dw);
     __syncthreads();  // twiddle table and constant rows written
     for (int s0 = 0; s0 < S; s0 += kH16Chunk) {
         const int s = s0 + t + 16 * h;
-        const SurfIn<false> next =
-            load_surfel<false>(min(s + kH16Chunk, S - 1), ctr, nrm, w, nullptr, nullptr, nullptr);
+        const SurfIn<JVP> next = load_surfel<JVP>(min(s + kH16Chunk, S - 1), ctr, nrm, w, dctr, dnrm, dw);
         const Pair q = pair_geometry_t<FRESNEL>(c, px, py, pz, in.cx, in.cy, in.cz, in.nx, in.ny, in.nz);
-        const float amp = p_ok && s < S ? q.amp : 0.0f;
-        float2 coef = amp * cmul(in.w, q.rot);
-        // wave-uniform running exponent of max |Re c|, |Im c| (biased); |Y_j| = |c|
-        const uint32_t bits = max(__float_as_uint(coef.x) & 0x7fffffffu, __float_as_uint(coef.y) & 0x7fffffffu);
-        const int e_chunk = wave_max_i((int)(bits >> 23));
-        if (e_chunk > e_run) {
-            e_run = e_chunk;
-            const int sh_new = min(139 - e_run, 127);
-#pragma unroll
-            for (int b = 0; b < 2; ++b)
-#pragma unroll
-                for (int p = 0; p < 2; ++p)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) acc[b][p][r] = ldexpf(acc[b][p][r], sh_new - sh);
-            sh = sh_new;
+        const bool ok = p_ok && s < S;
+        const float amp = ok ? q.amp : 0.0f;
+        float2 coef[SETS];
+        if constexpr (JVP) {
+            // surfel.slang:146-221: e1 = rot ((damp + i amp dphase) w + amp dw), e2 = -amp k0pm dr rot w
+            const float dr = -(q.ux * in.dc[0] + q.uy * in.dc[1] + q.uz * in.dc[2]);
+            const float dcos = ((q.cosine * q.ux - in.nx) * in.dc[0] + (q.cosine * q.uy - in.ny) * in.dc[1] +
+                                (q.cosine * q.uz - in.nz) * in.dc[2]) * q.inv_r +
+                               q.ux * in.dn[0] + q.uy * in.dn[1] + q.uz * in.dn[2];
+            const float damp = -amp * dr * q.inv_r + (ok ? q.amp_cos : 0.0f) * dcos;
+            coef[0] = cmul(q.rot, cmul(make_float2(damp, amp * q.rate * dr), in.w) + amp * in.dw);
+            coef[SETS - 1] = (-amp * c.k0pm_f * dr) * cmul(q.rot, in.w);
+        } else {
+            coef[0] = amp * cmul(in.w, q.rot);
         }
-        coef = make_float2(ldexpf(coef.x, sh), ldexpf(coef.y, sh));
+        // per set a wave-uniform running exponent of max |Re e|, |Im e| (biased); |Y_j| = |e|
+#pragma unroll
+        for (int e = 0; e < SETS; ++e) {
+            const int e_chunk = wave_max_i(cexp_bits(coef[e]));
+            if (e_chunk > e_run[e]) {
+                e_run[e] = e_chunk;
+                const int sh_new = min(139 - e_chunk, 127);
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) acc[e][b][p][r] = ldexpf(acc[e][b][p][r], sh_new - sh[e]);
+                sh[e] = sh_new;
+            }
+            coef[e] = make_float2(ldexpf(coef[e].x, sh[e]), ldexpf(coef[e].y, sh[e]));
+        }
         const float2 v = rotation(c, E, q.lobe, q.delta, 1);
         const float2 z1 = rotation(c, E, q.lobe, q.delta, 16);
-        float val[32];
+        float val[2 * L::pairs];
         {
-            // W_1..8 by a product tree, Y_j = c W_7 z1^j by two chains over z1^2
+            // W_1..8 by a product tree, Y_j = e W_7 z1^j by two chains over z1^2
             float2 W[9];
             W[1] = v;
 #pragma unroll
             for (int i = 2; i <= 8; ++i) W[i] = (i & 1) ? cmul(W[i - 1], v) : csq(W[i >> 1]);
             const float2 z2 = csq(z1);
-            float2 Y[8];
-            Y[0] = cmul(coef, W[7]);
-            Y[1] = cmul(Y[0], z1);
 #pragma unroll
-            for (int j = 2; j < 8; ++j) Y[j] = cmul(Y[j - 2], z2);
+            for (int e = 0; e < SETS; ++e) {
+                float2 Y[8];
+                Y[0] = cmul(coef[e], W[7]);
+                Y[1] = cmul(Y[0], z1);
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                val[2 * j] = Y[j].x;
-                val[2 * j + 1] = Y[j].y;
+                for (int j = 2; j < 8; ++j) Y[j] = cmul(Y[j - 2], z2);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    val[16 * e + 2 * j] = Y[j].x;
+                    val[16 * e + 2 * j + 1] = Y[j].y;
+                }
             }
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
-                val[16 + i] = W[i + 1].x;
-                val[24 + i] = W[i + 1].y;
+                val[RW + i] = W[i + 1].x;
+                val[RW + 8 + i] = W[i + 1].y;
             }
         }
         // the previous chunk's reads were issued before these writes (in-order LDS per wave)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int p = 0; p < 16; ++p) {
+        for (int p = 0; p < L::pairs; ++p) {
             float x0 = val[2 * p], x1 = val[2 * p + 1];
             swap_halves(x0, x1);  // half h: row rowbase(p) + h of surfels t (x0) and t + 16 (x1)
             f16x2 hi, lo;
             split_f16_pair<false>(x0, x1, hi, lo);
-            const int R = h16_rowbase(p);  // even: h only moves the row by one
+            const int R = L::rowbase(p);  // even: h only moves the row by one
             uint32_t* dst = rec + wofs[(R >> 2) & 3];
-            dst[h16_word(R, 0, 0, 0) - 4 * (0 ^ ((R >> 2) & 3))] = __builtin_bit_cast(uint32_t, hi);
-            dst[h16_word(R, 1, 0, 0) - 4 * (0 ^ ((R >> 2) & 3))] = __builtin_bit_cast(uint32_t, lo);
+            dst[h16_word(R, 0, 0, 0) - 4 * ((R >> 2) & 3)] = __builtin_bit_cast(uint32_t, hi);
+            dst[h16_word(R, 1, 0, 0) - 4 * ((R >> 2) & 3)] = __builtin_bit_cast(uint32_t, lo);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        f16x8 op[2][2][3];  // [aperture][hi, lo][A, B re, B im]
+        f16x8 op[2][2][SETS + 2];  // [aperture][hi, lo][A of each set, B re, B im]
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int hl = 0; hl < 2; ++hl)
 #pragma unroll
-                for (int o = 0; o < 3; ++o) {
+                for (int o = 0; o < SETS + 2; ++o) {
                     const int R = 16 * o + x;
                     op[b][hl][o] = __builtin_bit_cast(
-                        f16x8, *reinterpret_cast<const uint4*>(rec + b * kH16ApWords + h16_word(R, hl, g, 0)));
+                        f16x8, *reinterpret_cast<const uint4*>(rec + b * L::ap_words + h16_word(R, hl, g, 0)));
                 }
-        // the four accumulators interleaved, so no product waits for the one before it
+        // the accumulators interleaved, so no product waits for the one before it
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int e = 0; e < SETS; ++e)
 #pragma unroll
-                for (int p = 0; p < 2; ++p)
-                    acc[b][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(op[b][k == 2][0], op[b][k == 1][1 + p],
-                                                                       acc[b][p], 0, 0, 0);
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int p = 0; p < 2; ++p)
+                        acc[e][b][p] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                            op[b][k == 2][e], op[b][k == 1][SETS + p], acc[e][b][p], 0, 0, 0);
         in = next;
     }
     __syncthreads();  // every wave's records consumed: the region becomes X
@@ -601,10 +642,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
 #pragma unroll
             for (int r = 0; r < 4; r += 2) {
                 const int n0 = 16 * (2 * g + (r >> 1)) + 7;
-                const float P1 = acc[b][0][r], P3 = acc[b][0][r + 1], P4 = acc[b][1][r], P2 = acc[b][1][r + 1];
+                float2 plus[SETS], minus[SETS];
+#pragma unroll
+                for (int e = 0; e < SETS; ++e) {
+                    const f32x4& ac0 = acc[e][b][0];
+                    const f32x4& ac1 = acc[e][b][1];
+                    const float P1 = ac0[r], P3 = ac0[r + 1], P4 = ac1[r], P2 = ac1[r + 1];
+                    plus[e] = make_float2(ldexpf(P1 - P2, -sh[e]), ldexpf(P3 + P4, -sh[e]));
+                    minus[e] = make_float2(ldexpf(P1 + P2, -sh[e]), ldexpf(P3 - P4, -sh[e]));
+                }
+                float2 vp = plus[0], vm = minus[0];
+                if constexpr (JVP) {
+                    // - (2 pi i n / N) X2
+                    const float kp = c.two_pi_N * (float)(n0 + i), km = c.two_pi_N * (float)(n0 - i);
+                    vp = make_float2(vp.x + kp * plus[1].y, vp.y - kp * plus[1].x);
+                    vm = make_float2(vm.x + km * minus[1].y, vm.y - km * minus[1].x);
+                }
                 float2* row = X + (2 * wave + b) * ldx;
-                if (x <= 8) row[n0 + i] = make_float2(ldexpf(P1 - P2, -sh), ldexpf(P3 + P4, -sh));
-                if (x < 7) row[n0 - i] = make_float2(ldexpf(P1 + P2, -sh), ldexpf(P3 - P4, -sh));
+                if (x <= 8) row[n0 + i] = vp;
+                if (x < 7) row[n0 - i] = vm;
             }
     }
     __syncthreads();
@@ -944,26 +1000,30 @@ static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, co
     return hipGetLastError();
 }
 
+template <bool JVP>
 static hipError_t launch_td_render_h16(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                                       const float2* w, int S, float2* out, hipStream_t st) {
-    const size_t region = sizeof(uint32_t) * 4 * (size_t)kH16Words;
+                                       const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S,
+                                       float2* out, hipStream_t st) {
+    const size_t region = sizeof(uint32_t) * 4 * (size_t)H16Layout<JVP>::wave_words;
     const size_t xbuf = sizeof(float2) * kH16Apertures * (size_t)(c.S + 1);
     const size_t smem = sizeof(float2) * c.N + (region > xbuf ? region : xbuf);
     const bool fres = c.fresnel_n > 1.0f;
-    const void* fn = fres ? (const void*)td_render_h16_kernel<true> : (const void*)td_render_h16_kernel<false>;
+    const void* fn = fres ? (const void*)td_render_h16_kernel<true, JVP> : (const void*)td_render_h16_kernel<false, JVP>;
     hipError_t e = allow_lds(fn, smem);
     if (e != hipSuccess) return e;
     const dim3 grid((A + kH16Apertures - 1) / kH16Apertures);
     if (fres)
-        hipLaunchKernelGGL((td_render_h16_kernel<true>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, S, out);
+        hipLaunchKernelGGL((td_render_h16_kernel<true, JVP>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr,
+                           dnrm, dw, S, out);
     else
-        hipLaunchKernelGGL((td_render_h16_kernel<false>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, S, out);
+        hipLaunchKernelGGL((td_render_h16_kernel<false, JVP>), grid, dim3(256), smem, st, c, pos, A, ctr, nrm, w, dctr,
+                           dnrm, dw, S, out);
     return hipGetLastError();
 }
 
 hipError_t launch_td_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                             const float2* w, int S, float2* out, hipStream_t st) {
-    if (c.S == 128) return launch_td_render_h16(c, pos, A, ctr, nrm, w, S, out, st);
+    if (c.S == 128) return launch_td_render_h16<false>(c, pos, A, ctr, nrm, w, nullptr, nullptr, nullptr, S, out, st);
     if (c.S == 256) return launch_td_render_j<16, false>(c, pos, A, ctr, nrm, w, nullptr, nullptr, nullptr, S, out, st);
     return hipErrorInvalidValue;
 }
@@ -971,7 +1031,7 @@ hipError_t launch_td_render(const Chirp& c, const float* pos, int A, const float
 hipError_t launch_td_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S, float2* out,
                          hipStream_t st) {
-    if (c.S == 128) return launch_td_render_j<8, true>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
+    if (c.S == 128) return launch_td_render_h16<true>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
     if (c.S == 256) return launch_td_render_j<16, true>(c, pos, A, ctr, nrm, w, dctr, dnrm, dw, S, out, st);
     return hipErrorInvalidValue;
 }
