@@ -57,10 +57,10 @@ def test_vjp_workspace_per_splat_form():
     assert lib.dsplat_splat_form() == 0
     assert lib.dsplat_surfel_vjp_workspace(cfg, 0, 10) == 0
     # sample-domain form: partials of 8 floats per (surfel, 128-aperture range), the
-    # cotangent's sample-domain rows U [A, 128] complex, the FP16 hi + lo MFMA operands
-    # (4 KiB per aperture) and one scale per aperture
+    # cotangent's sample-domain rows U [A, 128] complex, the FP16 hi + lo MFMA operands of
+    # the symmetric split (one K half: 2 KiB per aperture) and one scale per aperture
     assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == (
-        2048 * 128 * 8 * 4 + 16384 * 128 * 8 + 16384 * 4096 + 16384 * 4
+        2048 * 128 * 8 * 4 + 16384 * 128 * 8 + 16384 * 2048 + 16384 * 4
     )
     try:
         assert lib.dsplat_set_splat_form(1) == 0
