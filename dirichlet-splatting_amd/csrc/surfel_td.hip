@@ -437,17 +437,26 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
 //  * Records are wave-private, so the chunk loop has no workgroup barrier.
 constexpr int kH16Apertures = 8;
 constexpr int kH16Chunk = 32;
-// rows per aperture: SETS x 16 Y rows, then the Re W and Im W tiles of 16 rows each
+// Rows are numbered virtually: SETS x 16 Y rows, then the Re W and Im W tiles of 16 rows each
+// (tile column x = row - first row of the tile). Columns 0..7 of both tiles change with the
+// pair; columns 8..15 are constant (W_0 = 1 at Re column 8, zeros elsewhere) and sit once per
+// workgroup in a shared block of 16 slots. Per aperture, slots hold the Y rows, Re W columns
+// 0..7 and Im W columns 0..7. A slot has the parity of its virtual row, so the bank pattern of
+// h16_word (which depends on the row mod 16 and the slot parity) is the virtual row's.
 template <bool JVP>
 struct H16Layout {
     static constexpr int sets = JVP ? 2 : 1;
-    static constexpr int rows = 16 * sets + 32;
-    static constexpr int ap_words = rows * 32 + 16;  // [row][hi | lo 16 words] + bank shift
-    static constexpr int wave_words = 2 * ap_words;  // per wave: two apertures
-    static constexpr int pairs = rows / 2 - 8;       // row pairs written per chunk (the rest is constant)
-    // first row of producer row pair p: Y rows 2 p, Re W 16 sets + 2 (p - 8 sets), Im W 16 sets +
-    // 16 + 2 (p - 8 sets - 4)
+    static constexpr int rw = 16 * sets;               // first virtual Re W row
+    static constexpr int slots = rw + 16;              // per aperture
+    static constexpr int ap_words = slots * 32 + 16;   // [slot][hi | lo 16 words] + bank shift
+    static constexpr int wave_words = 2 * ap_words;    // per wave: two apertures
+    static constexpr int const_words = 16 * 32;        // per workgroup
+    static constexpr int pairs = slots / 2;            // row pairs written per chunk
+    // first virtual row of producer row pair p: Y rows 2 p, Re W rw + 2 (p - 8 sets), Im W
+    // rw + 16 + 2 (p - 8 sets - 4)
     __host__ __device__ static constexpr int rowbase(int p) { return p < 8 * sets + 4 ? 2 * p : 2 * p + 8; }
+    // per-aperture slot of a changing virtual row
+    __host__ __device__ static constexpr int slot(int R) { return R < rw + 16 ? R : R - 8; }
 };
 
 // word offset of the (hi | lo) half of row R, word group g, word u within the group
@@ -474,7 +483,7 @@ __device__ __forceinline__ int cexp_bits(float2 z) {
 // Forward (JVP = false) and JVP (JVP = true: two coefficient sets e1, e2 with one W set,
 // X = X1 - (2 pi i n / N) X2 as in td_render_kernel) on the layout above.
 template <bool FRESNEL, bool JVP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) void td_render_h16_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 4))) void td_render_h16_kernel(
     const Chirp c, const float* __restrict__ pos, int A, const float* __restrict__ ctr,
     const float* __restrict__ nrm, const float2* __restrict__ w, const float* __restrict__ dctr,
     const float* __restrict__ dnrm, const float2* __restrict__ dw, int S, float2* __restrict__ out) {
@@ -485,6 +494,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     float* region = smem + 2 * c.N;               // 4 waves of records; later X [8][S + 1] float2
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
     uint32_t* rec = reinterpret_cast<uint32_t*>(region) + wave * L::wave_words;
+    uint32_t* cst = reinterpret_cast<uint32_t*>(region) + 4 * L::wave_words;  // shared constant slots
     const int a0 = blockIdx.x * kH16Apertures;
     const int pb = m >> 4, t = m & 15;
     const int a = a0 + 2 * wave + pb;
@@ -492,21 +502,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
     const int pa = min(a, A - 1);
     const float px = pos[3 * pa], py = pos[3 * pa + 1], pz = pos[3 * pa + 2];
     fill_twiddles(E, c.N, tid, 256);
-    // constant B rows of both apertures: Re W_0 = 1 at column 8, zeros at Im column 8 and at
-    // columns 9..15 (words written once; the chunk loop writes columns 0..7 only)
-    for (int i = lane; i < 2 * 2 * 8 * 32; i += 64) {
-        const int b = i >> 9, r = (i >> 5) & 15, wd = i & 31;
-        const int row = RW + (r < 8 ? 0 : 16) + 8 + (r & 7);
+    // constant slots: Re W columns 8..15 (slots 0..7), Im W columns 8..15 (slots 8..15); only the
+    // hi words of Re W_0 (column 8) are nonzero: (1.0, 1.0) in FP16
+    for (int i = tid; i < L::const_words; i += 256) {
+        const int sl = i >> 5, wd = i & 31;
+        const int row = RW + 8 + (sl < 8 ? sl : 8 + sl);  // its virtual row
         const bool hi_half = (wd < 16) != (((row >> 1) & 1) != 0);  // hi words come second when (row / 2) is odd
-        rec[b * L::ap_words + row * 32 + wd] = (r == 0 && hi_half) ? 0x3C003C00u : 0u;  // (1.0, 1.0) FP16
+        cst[i] = (sl == 0 && hi_half) ? 0x3C003C00u : 0u;
     }
 
     // producer: lane word offsets for the four swizzle keys (row pair p uses key (rowbase / 4) % 4)
     int wofs[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) wofs[k] = pb * L::ap_words + h * 32 + 4 * ((t >> 2) ^ k) + (t & 3);
-    // consumer: row 16 e + x (A of set e), RW + x (B re) or RW + 16 + x (B im) of aperture b, word group g
+    // consumer: row 16 e + x (A of set e), RW + x (B re) or RW + 16 + x (B im) of aperture b, word
+    // group g; B columns x >= 8 come from the shared constant slots
     const int x = lane & 15, g = lane >> 4;
+    int bofs[2][2];  // [tile][hi, lo]: word offset from the aperture base (x < 8) or from cst (x >= 8)
+#pragma unroll
+    for (int tl = 0; tl < 2; ++tl)
+#pragma unroll
+        for (int hl = 0; hl < 2; ++hl) {
+            const int R = RW + 16 * tl + x;
+            const int sl = x < 8 ? RW + 8 * tl + x : 8 * tl + x - 8;
+            bofs[tl][hl] = h16_word(R, hl, g, 0) + (sl - R) * 32;
+        }
 
     f32x4 acc[SETS][2][2];
 #pragma unroll
@@ -599,7 +619,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
             f16x2 hi, lo;
             split_f16_pair<false>(x0, x1, hi, lo);
             const int R = L::rowbase(p);  // even: h only moves the row by one
-            uint32_t* dst = rec + wofs[(R >> 2) & 3];
+            uint32_t* dst = rec + wofs[(R >> 2) & 3] + (L::slot(R) - R) * 32;
             dst[h16_word(R, 0, 0, 0) - 4 * ((R >> 2) & 3)] = __builtin_bit_cast(uint32_t, hi);
             dst[h16_word(R, 1, 0, 0) - 4 * ((R >> 2) & 3)] = __builtin_bit_cast(uint32_t, lo);
         }
@@ -610,13 +630,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 3))) voi
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int hl = 0; hl < 2; ++hl)
+            for (int hl = 0; hl < 2; ++hl) {
 #pragma unroll
-                for (int o = 0; o < SETS + 2; ++o) {
-                    const int R = 16 * o + x;
+                for (int o = 0; o < SETS; ++o)
                     op[b][hl][o] = __builtin_bit_cast(
-                        f16x8, *reinterpret_cast<const uint4*>(rec + b * L::ap_words + h16_word(R, hl, g, 0)));
-                }
+                        f16x8, *reinterpret_cast<const uint4*>(rec + b * L::ap_words + h16_word(16 * o + x, hl, g, 0)));
+#pragma unroll
+                for (int tl = 0; tl < 2; ++tl)
+                    op[b][hl][SETS + tl] = __builtin_bit_cast(
+                        f16x8, *reinterpret_cast<const uint4*>((x < 8 ? rec + b * L::ap_words : cst) + bofs[tl][hl]));
+            }
         // the accumulators interleaved, so no product waits for the one before it
 #pragma unroll
         for (int k = 0; k < 3; ++k)
@@ -1004,7 +1027,7 @@ template <bool JVP>
 static hipError_t launch_td_render_h16(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                        const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S,
                                        float2* out, hipStream_t st) {
-    const size_t region = sizeof(uint32_t) * 4 * (size_t)H16Layout<JVP>::wave_words;
+    const size_t region = sizeof(uint32_t) * (4 * (size_t)H16Layout<JVP>::wave_words + H16Layout<JVP>::const_words);
     const size_t xbuf = sizeof(float2) * kH16Apertures * (size_t)(c.S + 1);
     const size_t smem = sizeof(float2) * c.N + (region > xbuf ? region : xbuf);
     const bool fres = c.fresnel_n > 1.0f;
