@@ -701,7 +701,7 @@ def main():
     fwd_flop, vjp_flop = (TD_FWD_FLOP, TD_VJP_FLOP) if sample_domain else (FWD_FLOP, VJP_FLOP)
     groups = {  # the launches each timed call makes
         "render": ["td_render_h16_kernel<false>"] if sample_domain else ["render_kernel<2, 8>"],
-        "vjp": ["dft_rows_kernel", "vjp_operands_kernel<8>", "td_vjp_kernel<8, false>", "vjp_reduce"] if sample_domain
+        "vjp": ["vjp_prep_kernel", "td_vjp_kernel<8, false>", "vjp_reduce"] if sample_domain
         else ["vjp_kernel<2>", "vjp_reduce"],
     }  # fmt: skip
     kernels = {

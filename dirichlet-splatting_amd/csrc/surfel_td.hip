@@ -793,6 +793,70 @@ __global__ __launch_bounds__(64 * (J / 8)) void vjp_operands_kernel(const float2
     dst[64] = __builtin_bit_cast(uint4, lo);
 }
 
+// The VJP pre-pass for 128 samples in one launch: 16 cotangent rows are taken to the sample
+// domain in LDS (dft16, sign +1), then each wave builds one aperture's operand tiles from
+// them as vjp_operands_kernel<8> does, so U never goes through HBM.
+__global__ __launch_bounds__(256) void vjp_prep_kernel(const float2* __restrict__ up, int A, int K, int N, int S,
+                                                       uint4* __restrict__ out, float* __restrict__ scale_exp) {
+    extern __shared__ float smem[];
+    float2* E = reinterpret_cast<float2*>(smem);  // [N]
+    float2* Xin = E + N;                          // [16][K + 1]
+    float2* U = Xin + 16 * (K + 1);               // [16][S + 1]
+    const int tid = threadIdx.x, wave = tid >> 6, L = tid & 63;
+    const int r0 = blockIdx.x * 16;
+    fill_twiddles(E, N, tid, 256);
+    for (int i = tid; i < 16 * K; i += 256) {
+        const int r = i / K, p = i - r * K;
+        Xin[r * (K + 1) + p] = r0 + r < A ? up[(size_t)(r0 + r) * K + p] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    dft16(Xin, K + 1, K, E, N, 1.0f, 1.0f, S, U, S + 1, 16, wave, L);
+    __syncthreads();
+    const int row = L & 31, h = L >> 5;
+    int mat, j;
+    td_row<8>(0, row >> 3, (row >> 2) & 1, row & 3, &mat, &j);
+    const int part = row & 1;
+    for (int q = wave; q < 16; q += 4) {
+        const int a = r0 + q;  // uniform over the wave
+        if (a >= A) break;
+        const float2* Ua = U + q * (S + 1);
+        float v[8];
+        float mx = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int Kx = 8 * h + e, i = Kx >> 1, k = Kx & 1;
+            const int np = 16 * j + 8 + i, nm = 16 * j + 7 - i;
+            float2 upl = Ua[np], umi = Ua[nm];
+            if (mat) {
+                upl = (float)np * upl;
+                umi = (float)nm * umi;
+            }
+            const float2 sum = upl + umi, dif = upl - umi;
+            const float x = k == 0 ? (part ? sum.y : sum.x) : (part ? dif.x : -dif.y);
+            v[e] = x;
+            mx = fmaxf(mx, fabsf(x));
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        int ex = 0;
+        if (mx > 0.0f) frexpf(mx, &ex);  // mx < 2^ex
+        ex -= 12;
+        if (L == 0) scale_exp[a] = (float)ex;
+        const float sc = ldexpf(1.0f, -ex);
+        f16x8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const float x = v[e] * sc;
+            const _Float16 xh = (_Float16)x;
+            hi[e] = xh;
+            lo[e] = (_Float16)(x - (float)xh);
+        }
+        uint4* dst = out + ((size_t)a * 2) * 64 + L;
+        dst[0] = __builtin_bit_cast(uint4, hi);
+        dst[64] = __builtin_bit_cast(uint4, lo);
+    }
+}
+
 // e^{2 pi i (lobe + delta) / (2 N)} = E[lobe / 2] e^{i pi ((lobe % 2) + delta) / N}: a square
 // root of rotation(..., 1), relatively accurate like it (|pi (1 + delta) / N| <= pi / 4).
 __device__ __forceinline__ float2 half_rotation(const Chirp& ch, const float2* E, int lobe, float delta) {
@@ -1104,8 +1168,15 @@ static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const
     float2* U = reinterpret_cast<float2*>(base + lay.u);
     uint4* ops = reinterpret_cast<uint4*>(base + lay.ops);
     float* scales = reinterpret_cast<float*>(base + lay.scales);
-    // U = DFT of the cotangent rows: K bins -> S samples, sign +1
-    {
+    if (J == 8) {
+        // U = DFT of the cotangent rows (K bins -> S samples, sign +1) and the operand tiles in one launch
+        const size_t smem = sizeof(float2) * (c.N + 16 * (size_t)(c.K + 1) + 16 * (size_t)(c.S + 1));
+        hipError_t e = allow_lds((const void*)vjp_prep_kernel, smem);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(vjp_prep_kernel, dim3((A + 15) / 16), dim3(256), smem, st, up, A, c.K, c.N, c.S, ops,
+                           scales);
+    } else {
+        // U = DFT of the cotangent rows: K bins -> S samples, sign +1
         const size_t smem = sizeof(float2) * (c.N + 16 * (size_t)(c.K + 1));
         hipError_t e = allow_lds((const void*)dft_rows_kernel, smem);
         if (e != hipSuccess) return e;
@@ -1113,8 +1184,8 @@ static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const
                            c.S, U);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(vjp_operands_kernel<J>, dim3(A), dim3(64 * T), 0, st, U, c.S, ops, scales);
     }
-    hipLaunchKernelGGL(vjp_operands_kernel<J>, dim3(A), dim3(64 * T), 0, st, U, c.S, ops, scales);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const size_t smem = sizeof(float2) * c.N + sizeof(float) * 16 +
