@@ -6,8 +6,9 @@
 // instead the rectangle out [n, n - col0] += rows^T rows[:, col0:] (the cross products of new
 // columns against all of them, the pooled statistics' GEMM); each wave accumulates a
 // 64 x 64 block as 2 x 2 MFMA blocks of 32 x 32. Rows are staged through LDS in chunks
-// of 32, column-major within a panel (k contiguous) so each lane reads its K values with
-// 128-bit loads, double-buffered with the next chunk's global loads in flight.
+// of 32 (MODE 0) or 16 (MODES 1, 2: 48 KB of LDS, three workgroups per CU), column-major within
+// a panel (k contiguous) so each lane reads its K values with 128-bit loads, double-buffered
+// with the next chunk's global loads in flight.
 //   MODE 0: FP32 operands, v_mfma_f32_32x32x2f32 (the lane's two K rows are permuted to
 //           h * 16 + kk; the sum over K does not depend on its order).
 //   MODE 1: BF16 hi + lo split of each value, hi hi + hi lo + lo hi on
@@ -28,9 +29,11 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kTile = 128;
-constexpr int kChunk = 32;
+constexpr int kChunk = 32;            // rows per LDS chunk, MODE 0; split rows are a multiple of it
+constexpr int kChunkH = 16;           // rows per LDS chunk, MODES 1 and 2 (48 KB of LDS: 3 workgroups per CU)
 constexpr int kStrideF = kChunk + 4;  // floats per LDS column (MODE 0)
-constexpr int kStrideH = kChunk + 8;  // 16-bit values per LDS column (MODES 1, 2)
+constexpr int kStrideH = kChunkH + 8; // 16-bit values per LDS column (MODES 1, 2): 48 B, so the 16-byte
+                                      // accesses of 16 consecutive columns fall on distinct bank groups
 constexpr int kMinSplitRows = 512;
 
 template <int MODE>
@@ -99,29 +102,32 @@ __global__ __launch_bounds__(256) void syrk_colmax(const float* __restrict__ row
     atomicMax(colmax + c, __float_as_uint(top));
 }
 
-// Global -> registers: thread t holds column (t & 127) of a panel, rows 16 (t >> 7) + 0..15.
+// Global -> registers: thread t holds column (t & 127) of a panel, rows (CH / 2) (t >> 7) + 0..CH/2-1.
+template <int CH>
 struct Staged {
-    float v[2][16];  // [panel I / J][row]
+    float v[2][CH / 2];  // [panel I / J][row]
 };
 
-__device__ __forceinline__ void stage_load(Staged& s, const float* __restrict__ rows, int n, int k0, int k_end,
+template <int CH>
+__device__ __forceinline__ void stage_load(Staged<CH>& s, const float* __restrict__ rows, int n, int k0, int k_end,
                                            int ci, int cj, bool two, int tid) {
+    constexpr int R = CH / 2;
     const int c = tid & (kTile - 1), rg = tid >> 7;
-    if (k0 + kChunk <= k_end && ci + kTile <= n && (!two || cj + kTile <= n)) {
+    if (k0 + CH <= k_end && ci + kTile <= n && (!two || cj + kTile <= n)) {
         // a full chunk of full panels (the common case): plain loads, no per-load predicate;
         // a diagonal tile (one panel) loads it twice and never reads the second copy
-        const float* pi = rows + (size_t)(k0 + rg * 16) * n + ci + c;
-        const float* pj = rows + (size_t)(k0 + rg * 16) * n + (two ? cj : ci) + c;
+        const float* pi = rows + (size_t)(k0 + rg * R) * n + ci + c;
+        const float* pj = rows + (size_t)(k0 + rg * R) * n + (two ? cj : ci) + c;
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
+        for (int e = 0; e < R; ++e) {
             s.v[0][e] = pi[(size_t)e * n];
             s.v[1][e] = pj[(size_t)e * n];
         }
         return;
     }
 #pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        const int k = k0 + rg * 16 + e;
+    for (int e = 0; e < R; ++e) {
+        const int k = k0 + rg * R + e;
         const bool in = k < k_end;
         s.v[0][e] = in && ci + c < n ? rows[(size_t)k * n + ci + c] : 0.0f;
         s.v[1][e] = two && in && cj + c < n ? rows[(size_t)k * n + cj + c] : 0.0f;
@@ -159,9 +165,10 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
         if (two && cj + c < n) scale_j = ldexpf(1.0f, column_exponent(__uint_as_float(colmax[cj + c])));
     }
 
-    Staged st;
+    constexpr int CH = MODE == 0 ? kChunk : kChunkH;
+    Staged<CH> st;
     int buf = 0;
-    if (k_begin < k_end) stage_load(st, rows, n, k_begin, k_end, ci, cj, two, tid);
+    if (k_begin < k_end) stage_load<CH>(st, rows, n, k_begin, k_end, ci, cj, two, tid);
 
     if constexpr (MODE == 0) {
         float* X = reinterpret_cast<float*>(lds);  // [2 buf][2 panel][128][kStrideF]
@@ -178,9 +185,9 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
         };
         if (k_begin < k_end) store(0);
         __syncthreads();
-        for (int k0 = k_begin; k0 < k_end; k0 += kChunk) {
-            const bool more = k0 + kChunk < k_end;
-            if (more) stage_load(st, rows, n, k0 + kChunk, k_end, ci, cj, two, tid);
+        for (int k0 = k_begin; k0 < k_end; k0 += CH) {
+            const bool more = k0 + CH < k_end;
+            if (more) stage_load<CH>(st, rows, n, k0 + CH, k_end, ci, cj, two, tid);
             const int pj = two ? 1 : 0;
             float a[2][16], b[2][16];
 #pragma unroll
@@ -223,10 +230,11 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
 #pragma unroll
             for (int p = 0; p < 2; ++p) {
                 const float sc = p == 0 ? scale_i : scale_j;
-                v8 hi[2], lo[2];
+                constexpr int NV = CH / 16;  // 8-element vectors per thread and panel
+                v8 hi[NV], lo[NV];
                 if constexpr (MODE == 2) {
 #pragma unroll
-                    for (int e = 0; e < 16; e += 2) {
+                    for (int e = 0; e < CH / 2; e += 2) {
                         f16x2 h2, l2;
                         split_f16_pair<false>(st.v[p][e] * sc, st.v[p][e + 1] * sc, h2, l2);
                         hi[e >> 3][e & 7] = h2.x, hi[e >> 3][(e & 7) + 1] = h2.y;
@@ -234,27 +242,30 @@ __global__ __launch_bounds__(256) void syrk_kernel(const float* __restrict__ row
                     }
                 } else {
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) {
+                    for (int e = 0; e < CH / 2; ++e) {
                         const float x = st.v[p][e];
                         const h16 xh = (h16)x;
                         hi[e >> 3][e & 7] = xh;
                         lo[e >> 3][e & 7] = (h16)(x - (float)xh);
                     }
                 }
-                v8* dh = reinterpret_cast<v8*>(col_ptr(b, 0, p, c) + rg * 16);
-                v8* dl = reinterpret_cast<v8*>(col_ptr(b, 1, p, c) + rg * 16);
-                dh[0] = hi[0], dh[1] = hi[1];
-                dl[0] = lo[0], dl[1] = lo[1];
+                v8* dh = reinterpret_cast<v8*>(col_ptr(b, 0, p, c) + rg * (CH / 2));
+                v8* dl = reinterpret_cast<v8*>(col_ptr(b, 1, p, c) + rg * (CH / 2));
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    dh[q] = hi[q];
+                    dl[q] = lo[q];
+                }
             }
         };
         if (k_begin < k_end) store(0);
         __syncthreads();
-        for (int k0 = k_begin; k0 < k_end; k0 += kChunk) {
-            const bool more = k0 + kChunk < k_end;
-            if (more) stage_load(st, rows, n, k0 + kChunk, k_end, ci, cj, two, tid);
+        for (int k0 = k_begin; k0 < k_end; k0 += CH) {
+            const bool more = k0 + CH < k_end;
+            if (more) stage_load<CH>(st, rows, n, k0 + CH, k_end, ci, cj, two, tid);
             const int pj = two ? 1 : 0;
 #pragma unroll
-            for (int s = 0; s < kChunk / 16; ++s) {
+            for (int s = 0; s < CH / 16; ++s) {
                 v8 ah[2], al[2], bh[2], bl[2];
 #pragma unroll
                 for (int x = 0; x < 2; ++x) {
