@@ -110,6 +110,8 @@ __device__ __forceinline__ void dft8_split(const float2* X, int ldx, int Lin, co
         float res[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+            // a lane shuffle rather than update_dpp: hipcc merged four update_dpp calls on the
+            // elements of one MFMA result into one, giving every element element 0's neighbour
             const float other = __shfl_xor(d[r], 1, 64);
             const float re = pout ? other : d[r], im = pout ? d[r] : other;
             float turned = pout ? fmaf(re, ri, im * rr) : fmaf(re, rr, -im * ri);
