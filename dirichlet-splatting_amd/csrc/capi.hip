@@ -611,6 +611,13 @@ int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stre
     return launch_cholesky(a, n, lda, info, (hipStream_t)stream);
 }
 
+int dsplat_cholesky_f32(float* a, int32_t n, int32_t lda, int32_t* info, void* stream) {
+    if (n < 0 || lda < n || !info) return hipErrorInvalidValue;
+    if (n == 0) return fill_zero(reinterpret_cast<float*>(info), 1, (hipStream_t)stream);
+    if (!a) return hipErrorInvalidValue;
+    return launch_cholesky_f32(a, n, lda, info, (hipStream_t)stream);
+}
+
 int dsplat_cholesky_solve(const double* l, int32_t n, int32_t lda, double* b, int32_t nrhs, int32_t ldb,
                           void* stream) {
     if (n < 0 || nrhs < 0 || lda < n || ldb < n || n > kCholMaxSolve) return hipErrorInvalidValue;

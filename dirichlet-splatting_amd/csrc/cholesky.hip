@@ -1,4 +1,5 @@
-// Float64 Cholesky factorization and solve of the K x K VarPro / Gauss-Newton systems
+// Float64 (and float32) Cholesky factorization and the float64 solve of the K x K VarPro /
+// Gauss-Newton systems
 // (reference: torch.linalg.cholesky_ex and torch.cholesky_solve in dsplat/varpro.py:44-79,
 // which run cuSOLVER potrf / potrs; K is about 2,000-2,500 here).
 //
@@ -9,7 +10,8 @@
 //   update: one workgroup per lower trailing block, A_ij -= L_ik L_jk^T, float64 FMA with
 //           a 4 x 4 register tile per thread.
 // Only the lower triangle of the input is read. info = 1-based index of the first pivot
-// that is not positive (0 if none), as LAPACK potrf reports it; the factor is then not
+// that is not positive (0 if none), as LAPACK potrf reports it, in the factor's own precision
+// (the float32 form is the first attempt of varpro.py:62-79); the factor is then not
 // meaningful. The solve is one workgroup: blocked forward and backward substitution.
 #include "dsplat_internal.h"
 
@@ -25,13 +27,14 @@ constexpr int kCholLd = kCholBlock + 1;  // LDS row stride (doubles): conflict-f
 // need no barrier; the other waves wait at one barrier at the end. Entries above the
 // diagonal of a row are updated too and never read. Returns the first failing pivot index
 // (or -1) on every thread.
-__device__ int factor_block(double* D, int w) {
+template <typename T>
+__device__ int factor_block(T* D, int w) {
     __shared__ int fail;
-    __shared__ double col[kCholBlock];
+    __shared__ T col[kCholBlock];
     const int tid = threadIdx.x;
     if (tid < 64) {
         const int i = tid;
-        double x[kCholBlock];
+        T x[kCholBlock];
 #pragma unroll
         for (int k = 0; k < kCholBlock; ++k) x[k] = D[i * kCholLd + k];
         int first = -1;
@@ -39,9 +42,9 @@ __device__ int factor_block(double* D, int w) {
         for (int j = 0; j < kCholBlock; ++j) {
             if (j < w) {  // uniform
                 col[i] = x[j];  // column j after the updates of steps < j
-                const double d = col[j];
+                const T d = col[j];
                 if (!(d > 0.0) && first < 0) first = j;
-                const double ljj = sqrt(d);
+                const T ljj = sqrt(d);
                 x[j] = i == j ? ljj : x[j] / ljj;
                 col[i] = x[j];
 #pragma unroll
@@ -56,9 +59,10 @@ __device__ int factor_block(double* D, int w) {
     return fail;
 }
 
-__global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A, int n, int lda, int kb,
+template <typename T>
+__global__ __launch_bounds__(256) void chol_panel_kernel(T* __restrict__ A, int n, int lda, int kb,
                                                         int* __restrict__ info) {
-    __shared__ double D[kCholBlock * kCholLd];
+    __shared__ T D[kCholBlock * kCholLd];
     const int tid = threadIdx.x;
     const int c0 = kb * kCholBlock, w = min(kCholBlock, n - c0);
     const int ib = kb + blockIdx.x;
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A,
     // of row i = tid / 4 (one wave holds 16 rows) keep its 16 columns k = tid % 4 + 4 q in
     // registers and pass X[i][j] by a lane shuffle, without barriers.
     const int i = tid >> 2, k0 = tid & 3;
-    double x[kCholBlock / 4];
+    T x[kCholBlock / 4];
 #pragma unroll
     for (int q = 0; q < kCholBlock / 4; ++q) {
         const int k = k0 + 4 * q;
@@ -91,7 +95,7 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A,
 #pragma unroll
     for (int j = 0; j < kCholBlock; ++j) {
         if (j < w) {  // uniform
-            double xj = x[j >> 2] / D[j * kCholLd + j];  // valid on the owner lane k0 == j % 4
+            T xj = x[j >> 2] / D[j * kCholLd + j];  // valid on the owner lane k0 == j % 4
             xj = __shfl(xj, base + (j & 3), 64);
             if (k0 == (j & 3)) x[j >> 2] = xj;
 #pragma unroll
@@ -112,9 +116,10 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(double* __restrict__ A,
 }
 
 // A_ij -= L_ik L_jk^T for the trailing lower blocks kb < jb <= ib.
-__global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A, int n, int lda, int kb) {
-    __shared__ double Li[kCholBlock * (kCholBlock + 1)];  // [p][r]
-    __shared__ double Lj[kCholBlock * (kCholBlock + 1)];
+template <typename T>
+__global__ __launch_bounds__(256) void chol_update_kernel(T* __restrict__ A, int n, int lda, int kb) {
+    __shared__ T Li[kCholBlock * (kCholBlock + 1)];  // [p][r]
+    __shared__ T Lj[kCholBlock * (kCholBlock + 1)];
     const int tid = threadIdx.x;
     // linear index -> (ti, tj) with tj <= ti over the trailing blocks
     int t = blockIdx.x, ti = 0;
@@ -134,13 +139,13 @@ __global__ __launch_bounds__(256) void chol_update_kernel(double* __restrict__ A
     }
     __syncthreads();
     const int r4 = 4 * (tid >> 4), c4 = 4 * (tid & 15);
-    double acc[4][4];
+    T acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
     for (int p = 0; p < w; ++p) {
-        double x[4], y[4];
+        T x[4], y[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
             x[a] = Li[p * LD + r4 + a];
@@ -239,18 +244,20 @@ __global__ void chol_info_fix(int* info, int none) {
     if (*info == none) *info = 0;
 }
 
-hipError_t launch_cholesky(double* A, int n, int lda, int* info, hipStream_t st) {
+template <typename T>
+static hipError_t launch_cholesky_t(T* A, int n, int lda, int* info, hipStream_t st) {
     hipLaunchKernelGGL(chol_info_reset, dim3(1), dim3(1), 0, st, info, n + 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nblk = (n + kCholBlock - 1) / kCholBlock;
     for (int kb = 0; kb < nblk; ++kb) {
-        hipLaunchKernelGGL(chol_panel_kernel, dim3(nblk - kb), dim3(256), 0, st, A, n, lda, kb, info);
+        hipLaunchKernelGGL(chol_panel_kernel<T>, dim3(nblk - kb), dim3(256), 0, st, A, n, lda, kb, info);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
-        const int T = nblk - kb - 1;
-        if (T > 0) {
-            hipLaunchKernelGGL(chol_update_kernel, dim3(T * (T + 1) / 2), dim3(256), 0, st, A, n, lda, kb);
+        const int trail = nblk - kb - 1;
+        if (trail > 0) {
+            hipLaunchKernelGGL(chol_update_kernel<T>, dim3(trail * (trail + 1) / 2), dim3(256), 0, st, A, n, lda,
+                               kb);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
         }
@@ -258,6 +265,14 @@ hipError_t launch_cholesky(double* A, int n, int lda, int* info, hipStream_t st)
     // n + 1 marks "no failure": map it to 0
     hipLaunchKernelGGL(chol_info_fix, dim3(1), dim3(1), 0, st, info, n + 1);
     return hipGetLastError();
+}
+
+hipError_t launch_cholesky(double* A, int n, int lda, int* info, hipStream_t st) {
+    return launch_cholesky_t<double>(A, n, lda, info, st);
+}
+
+hipError_t launch_cholesky_f32(float* A, int n, int lda, int* info, hipStream_t st) {
+    return launch_cholesky_t<float>(A, n, lda, info, st);
 }
 
 hipError_t launch_cholesky_solve(const double* L, int n, int lda, double* B, int nrhs, int ldb, hipStream_t st) {

@@ -159,6 +159,7 @@ hipError_t launch_vjp_reduce(const float* partial, int S, int nblk, float* gc, f
 
 constexpr int kCholMaxSolve = 8192;  // LDS holds the solution vector
 hipError_t launch_cholesky(double* A, int n, int lda, int* info, hipStream_t st);
+hipError_t launch_cholesky_f32(float* A, int n, int lda, int* info, hipStream_t st);
 hipError_t launch_cholesky_solve(const double* L, int n, int lda, double* B, int nrhs, int ldb, hipStream_t st);
 
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,

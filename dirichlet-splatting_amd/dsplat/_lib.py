@@ -70,6 +70,7 @@ SIGNATURES = {
     "dsplat_surfel_vjp_workspace": (c_size, [c_ptr, c_int, c_int]),
     "dsplat_set_splat_form": (c_int, [c_int]),
     "dsplat_cholesky": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
+    "dsplat_cholesky_f32": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
     "dsplat_cholesky_solve": (c_int, [c_ptr, c_int, c_int, c_ptr, c_int, c_int, c_ptr]),
     "dsplat_splat_form": (c_int, []),
     "dsplat_surfel_vjp": (

@@ -29,15 +29,17 @@ def _native(h):
 
 
 def cholesky_ex(h):
-    """(L, info) like torch.linalg.cholesky_ex for a real float64 [n, n] device matrix: L lower
-    with zeros above the diagonal, info a 0-d int32 tensor (first non-positive pivot, 1-based;
-    0 on success). Other dtypes and shapes go to torch.linalg.cholesky_ex."""
-    if not _native(h):
+    """(L, info) like torch.linalg.cholesky_ex for a real float64 or float32 [n, n] device
+    matrix: L lower with zeros above the diagonal, info a 0-d int32 tensor (first non-positive
+    pivot, 1-based; 0 on success). Other dtypes and shapes go to torch.linalg.cholesky_ex."""
+    single = h.is_cuda and h.dtype == torch.float32 and h.ndim == 2 and h.shape[0] == h.shape[1]
+    if not (_native(h) or single):
         return torch.linalg.cholesky_ex(h)
     factor = h.contiguous().clone()
     info = torch.empty((), dtype=torch.int32, device=h.device)
     n = len(factor)
-    _lib.call("dsplat_cholesky", _lib.ptr(factor), n, n, _lib.ptr(info), _lib.stream(h.device))
+    entry = "dsplat_cholesky_f32" if single else "dsplat_cholesky"
+    _lib.call(entry, _lib.ptr(factor), n, n, _lib.ptr(info), _lib.stream(h.device))
     return factor, info
 
 

@@ -333,6 +333,11 @@ int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num
  * varpro.py:44-79 (solve, shifted_cholesky). */
 int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stream);
 
+/* The same factorization in float32: the first attempt of varpro.py:62-79 (solve), which
+ * the reference runs as torch.linalg.cholesky_ex (cuSOLVER potrf) on the float32 Gram.
+ * info as for dsplat_cholesky, from the float32 pivots. */
+int dsplat_cholesky_f32(float* a, int32_t n, int32_t lda, int32_t* info, void* stream);
+
 /* Solve L L^T X = B in place for nrhs right-hand sides, column j of B at b + j * ldb, with
  * L from dsplat_cholesky; n <= 8192. Replaces torch.cholesky_solve (cuSOLVER potrs) of
  * varpro.py:62-79. */

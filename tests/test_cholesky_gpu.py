@@ -71,3 +71,36 @@ def test_varpro_solve_with_the_native_factor_agrees_with_torch():
     assert torch.equal(q, keep)  # the right-hand side is not overwritten
     want = torch.linalg.solve(h.cpu(), q.cpu())
     assert float((x.cpu() - want).norm() / want.norm()) < 1e-9
+
+
+@pytest.mark.parametrize("n", [1, 63, 65, 700, 2304])
+def test_float32_factor_matches_lapack(n):
+    """The float32 form (the first attempt of the VarPro solve): within 1e-5 of LAPACK's
+    float64 factor of the same float32 matrix, zeros above the diagonal, info 0."""
+    from dsplat.kernels.solver import cholesky_ex
+
+    h = spd(n, n + 1).astype(np.float32)
+    noisy = h + np.triu(np.full((n, n), 7.0, dtype=np.float32), 1)
+    factor, info = cholesky_ex(torch.tensor(noisy, device="cuda"))
+    torch.cuda.synchronize()
+    assert factor.dtype == torch.float32
+    want = np.linalg.cholesky(h.astype(np.float64))
+    got = factor.cpu().numpy().astype(np.float64)
+    assert int(info) == 0
+    assert np.array_equal(np.triu(got, 1), np.zeros((n, n)))
+    err = np.linalg.norm(got - want) / np.linalg.norm(want)
+    print(f"n={n} float32 factor rel err {err:.2e}")
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("n,bad", [(3, 0), (200, 128)])
+def test_float32_info_is_the_first_non_positive_pivot(n, bad):
+    """A leading minor made indefinite at row `bad` fails there in float32 as in LAPACK."""
+    from dsplat.kernels.solver import cholesky_ex
+
+    h = spd(n, 7 * n + bad)
+    h[bad, bad] = -1.0  # leading minors before `bad` stay positive definite
+    h = h.astype(np.float32)
+    _, info = cholesky_ex(torch.tensor(h, device="cuda"))
+    torch.cuda.synchronize()
+    assert int(info) == bad + 1
