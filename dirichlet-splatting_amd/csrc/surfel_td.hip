@@ -412,7 +412,8 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
 // Forward on FP16 hi + lo MFMA (128 samples)
 // ---------------------------------------------------------------------------------------
 // FP32 products issue on the SIMD cycles that the pair generation needs for its VALU work;
-// FP16 products co-issue with it. td_render_h16_kernel forms the sums over surfels on
+// FP16 products co-issue with it. td_render_h16_kernel (forward, and the JVP with two
+// coefficient sets) forms the sums over surfels on
 // v_mfma_f32_16x16x32_f16 as hi hi + hi lo + lo hi of FP16 splits with FP32 accumulation.
 //  * Samples are split symmetrically, n = 16 j + 7 + i with i in [-7, 8]: v^n = Y_j W_i with
 //    Y_j = c v^(16 j + 7) and W_i = v^i, and W_-i = conj(W_i). With the four real sums
@@ -426,13 +427,15 @@ __global__ __launch_bounds__(256) void td_render_kernel(const Chirp c, const flo
 //    which half h holds row rowbase + h of surfels t and t + 16, and one packed split gives
 //    the hi and lo words [row][t] = (surfel t, surfel t + 16).
 //  * Rows per aperture: 0..15 Y (row 2 j + part), 16 + x Re W, 32 + x Im W for the B column
-//    x: W_(x+1) for x < 8, W_0 (constant) at x = 8, zero for x > 8 (constant rows are
-//    written once). Consumer lane (x, g) reads the words 4 g .. 4 g + 3 of its row: K slot
+//    x: W_(x+1) for x < 8, W_0 (constant) at x = 8, zero for x > 8 (the constant columns are
+//    held once per workgroup, H16Layout). Consumer lane (x, g) reads the words 4 g .. 4 g + 3
+//    of its row: K slot
 //    8 g + 2 u + b of A and B alike is surfel 4 g + u + 16 b.
 //  * Row R holds 16 hi words then 16 lo words, in the other order when (R / 2) is odd, and
 //    word group g sits at group g ^ ((R / 4) % 4): the 16 lanes of a 128-bit read cover all
-//    64 banks. The second aperture's rows start 16 words further, so the 64 lanes of a
-//    producer write fall on distinct banks.
+//    64 banks. The second aperture's rows start 16 words further, so the 64 lanes of one
+//    producer dword write fall on distinct banks (a paired hi / lo write still meets the other
+//    aperture's words: the 2-way conflicts that remain).
 //  * Y is scaled by 2^sh with one exponent per wave, so that the largest value of the chunks
 //    so far lies below 2^13.5 (FP16 keeps 22 bits of every value); sh only decreases, the
 //    accumulators are rescaled by 2^(sh_new - sh_old) when it does, and X = acc 2^-sh.
