@@ -1,16 +1,23 @@
 """Benchmark of the Dirichlet-splat hot path on one MI355X.
 
-A step is one forward render plus one VJP of the full-size scene: 128x128 apertures
-at 2.5 mm, 128 range bins (120 GHz, 15 GHz, 128 ADC samples, pad 2), 2048
-front-facing surfels on the 30/30/25 mm ellipsoid at 0.35 m. The metric counts
-(aperture, bin, surfel) evaluations per second of fwd+VJP; inputs are resident in
-HBM before the timed region.
+The metric has two halves.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ramp-ms T] [--no-cpu]
+* Splat: a step is one forward render plus one VJP of the full-size scene: 128x128
+  apertures at 2.5 mm, 128 range bins (120 GHz, 15 GHz, 128 ADC samples, pad 2), 2048
+  front-facing surfels on the 30/30/25 mm ellipsoid at 0.35 m. `value` counts (aperture,
+  bin, surfel) evaluations per second of fwd+VJP; inputs are resident in HBM before the
+  timed region.
+* DSFW: the full fit (float64 truth scan of the front faces of 4000 ellipsoid draws, 256 ADC
+  samples, 30 dB noise, 100 iterations, batch 256) in the plain FP32 form and in the
+  mixed-precision batched form, as the `dsfw` object: median step ms of each, the speed-up,
+  the NMSE trajectories' deviation with its perturbed-target control, final NMSE and median
+  centre error.
 
-Prints one JSON line: value, ms_per_step, a roofline object for the dominant kernel
-(HIP events on the launch stream) and the float64 oracle timed on this host's cores
-as cpu_baseline.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--ramp-ms T] [--dsfw-steps D] [--no-cpu]
+
+Prints one JSON line: value, ms_per_step, a roofline object for the dominant splat launch
+group (HIP events on the launch stream), the dsfw object, and as cpu_baseline the
+reference's float32 model in torch on every host core (the float64 oracle beside it).
 """
 
 import argparse
@@ -36,6 +43,7 @@ sys.path.insert(0, str(ROOT / "dirichlet-splatting_amd"))
 #   forward  4.5 FLOP: per (a, s, row of 16 samples) 9 W columns x 4 real products
 #   VJP      8 FLOP: per (a, s, row) 8 W_i x (Re, Im) x 2 output parts x 2 sums (w^n U, n w^n U)
 #   scan     8 FLOP: as the VJP, the residual's sum and the norm polynomial's
+FWD_FLOP, VJP_FLOP, SCAN_FLOP = 13, 19, 11
 TD_FWD_FLOP, TD_VJP_FLOP = 4.5, 8
 TD_SCAN_FLOP = 8
 # FP16 MFMA work the sample-domain kernels execute per (a, s) pair: six 16x16x32 (forward) or
@@ -45,6 +53,10 @@ PEAK_FP16_TFLOPS = 2516.6  # MI355X published dense FP16 MFMA peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak (vector and dense MFMA)
 PEAK_HBM_GBS = 8000.0
 TRAFFIC = ROOT / "profiles" / "splat_traffic_128x128x2048.json"
+# VALU instructions per 64 (aperture, surfel) pairs (rocprofv3 SQ_INSTS_VALU per dispatch / pairs * 64)
+VALU_PROFILE = ROOT / "profiles" / "pmc_valu_td_render_h16_128x128x2048.md"
+VALU_PER_64_PAIRS = {"td_render_h16_kernel<false, false>": 352, "td_vjp_kernel<8, false>": 415}
+SIMDS, CLOCK_GHZ = 1024, 2.4  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
 
 
 def measured_traffic(kernels):
@@ -71,7 +83,7 @@ def parse():
     parser.add_argument("--ramp-ms", type=float, default=250.0,
                         help="untimed steps for this much wall time before the warm-up (clocks settle)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    parser.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
+    parser.add_argument("--cpu-seconds", type=float, default=8.0, help="target float64-oracle sample time")
     parser.add_argument(
         "--workload", choices=("splat", "gram", "scan", "dsfw", "synth", "copy"), default="splat",
         help="splat: the fwd+VJP metric (default); gram: the split-precision Gram of the same scene; "
@@ -80,9 +92,13 @@ def parse():
         "synth: the float64 truth scan of the DSFW fit by the ADC signal chain on the FP64 matrix cores; "
         "copy: a plain 1 GiB device copy, the reference size for the HBM byte counters",
     )  # fmt: skip
-    parser.add_argument("--adc-samples", type=int, default=128, help="dsfw: ADC samples (bins = samples)")
-    parser.add_argument("--truth-draws", type=int, default=0,
-                        help="dsfw: ellipsoid draws with front faces kept (0: exactly 2048 front faces)")
+    parser.add_argument("--adc-samples", type=int, default=256, help="dsfw: ADC samples (bins = samples)")
+    parser.add_argument("--truth-draws", type=int, default=4000,
+                        help="dsfw: area-uniform ellipsoid draws whose front faces are the truth")
+    parser.add_argument("--front-count", type=int, default=0,
+                        help="dsfw: exactly this many front faces as the truth instead (0: use --truth-draws)")
+    parser.add_argument("--dsfw-steps", type=int, default=100,
+                        help="default run: timed DSFW steps per form (0 skips the DSFW comparison)")
     return parser.parse_args()
 
 
@@ -147,22 +163,25 @@ def torch_fp32_splat(positions, centers, normals, weights, cfg):
     return (coef[..., None] * D).sum(1)
 
 
-def cpu_torch_fp32(work, target_seconds):
-    """The reference's float32 model in torch on every host core: render + autograd VJP on a
-    bounded aperture sample, and its relative L2 error against the float64 oracle."""
+def cpu_torch_fp32(work, repeats=5):
+    """The reference's float32 model in torch on every host core, render + autograd VJP, as the
+    reported CPU baseline: the small 32x32-aperture, 200-surfel case in full and the first 1024
+    apertures of the full-size scene (all 2048 surfels, 128 bins), one warm-up then `repeats`
+    timed runs each, median wall time; plus the relative L2 error of its forward against the
+    float64 oracle on 32 of those apertures."""
     import numpy as np
     import torch
 
+    from dsplat import workloads
     from oracle import surfel as O
 
     torch.set_num_threads(os.cpu_count() or 1)
     cfg = work["config"]
-    pos, c, n, w, up = (work[k] for k in ("positions", "centers", "normals", "weights", "cotangent"))
 
-    def run(ids):
+    def run(pos, c, n, w, up):
         t0 = time.perf_counter()
         outs = []
-        for chunk in np.array_split(ids, max(1, len(ids) // 8)):
+        for chunk in torch.split(torch.arange(len(pos)), 8):
             cc = c.clone().requires_grad_(True)
             nn = n.clone().requires_grad_(True)
             ww = w.clone().requires_grad_(True)
@@ -171,22 +190,29 @@ def cpu_torch_fp32(work, target_seconds):
             outs.append(out.detach())
         return time.perf_counter() - t0, torch.cat(outs)
 
-    probe = np.arange(0, len(pos), len(pos) // 8)[:8]
-    run(probe)  # first-call set-up
-    seconds, _ = run(probe)
-    count = int(min(len(pos), max(8, 8 * target_seconds / max(seconds, 1e-3))))
-    ids = np.linspace(0, len(pos) - 1, count).astype(np.int64)
-    seconds, out = run(ids)
-    check = np.arange(0, count, max(1, count // 32))[:32]  # the float64 oracle on 32 of them
-    want = O.render(pos[ids[check]].numpy(), c.numpy(), n.numpy(), w.numpy(), cfg)
-    rel = float(np.linalg.norm(out[check].numpy() - want) / np.linalg.norm(want))
+    def median_rate(pos, c, n, w, up):
+        run(pos[:8], c, n, w, up[:8])  # first-call set-up
+        times = [run(pos, c, n, w, up)[0] for _ in range(repeats)]
+        return len(pos) * cfg.num_range_bins * len(c) / float(np.median(times)), times
+
+    small = workloads.fmcw_small()
+    small_rate, small_times = median_rate(*(small[k] for k in ("positions", "centers", "normals", "weights", "cotangent")))
+    first = slice(0, 1024)
+    pos, c, n, w, up = (work[k] for k in ("positions", "centers", "normals", "weights", "cotangent"))
+    rate, times = median_rate(pos[first], c, n, w, up[first])
+    _, out = run(pos[:1024:32], c, n, w, up[:1024:32])
+    want = O.render(pos[:1024:32].numpy(), c.numpy(), n.numpy(), w.numpy(), cfg)
+    rel = float(np.linalg.norm(out.numpy() - want) / np.linalg.norm(want))
     model = next((ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")), "?")
-    evals = count * cfg.num_range_bins * len(c)
     return {
-        "value": evals / seconds, "unit": "evals/s", "threads": torch.get_num_threads(), "cpu": model,
-        "torch": torch.__version__, "rel_l2_forward_vs_float64": rel,
-        "sample": f"float32 torch restatement of the reference model, render + autograd VJP of {count} of "
-        f"{len(pos)} apertures ({evals:.3e} evals, {seconds:.1f} s)",
+        "value": rate, "unit": "evals/s", "cores": torch.get_num_threads(), "kind": "port",
+        "sample": f"float32 torch restatement of the reference model (closed-form Dirichlet kernel, float32 carrier "
+        f"phase), render + autograd VJP of the first 1024 of {len(pos)} apertures x {cfg.num_range_bins} bins x "
+        f"{len(c)} surfels, median of {repeats} runs ({float(np.median(times)):.2f} s each)",
+        "os_cpu_count": os.cpu_count(), "cpu": model, "torch": torch.__version__,
+        "run_seconds": times, "rel_l2_forward_vs_float64": rel,
+        "small_case": {"value": small_rate, "unit": "evals/s", "run_seconds": small_times,
+                       "workload": "32x32 apertures @ 2.5 mm, 128 bins, 200 surfels in a 60 mm cube, in full"},
     }  # fmt: skip
 
 
@@ -453,9 +479,7 @@ def scan_workload(args):
     ms_per_step = (time.perf_counter() - t0) * 1e3 / steps
     kernel_ms, _ = timed(lambda: surfel.scan_batched(cfg, scans, centers, normals, 0.0), 3, stream)
     single_ms, _ = timed(lambda: [surfel.correlate(cfg, p, centers, normals, r, 0.0) for p, r in scans], 3, stream)
-    from dsplat import _lib
-
-    sample_domain = _lib.library().dsplat_splat_form() == 0 and cfg.adc_samples == 128
+    sample_domain = cfg.splat_form == "samples" and cfg.adc_samples == 128
     scan_flop = TD_SCAN_FLOP if sample_domain else SCAN_FLOP
     achieved = scan_flop * evals / (kernel_ms * 1e-3) / 1e12
     # the top candidates lie near the surface: distance of the best per scan to the nearest surfel
@@ -472,7 +496,8 @@ def scan_workload(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 per-pair phase, f64 block reduction)",
+        "dtype": "fp16 hi+lo x3 MFMA, fp32 accumulate, f64 pair phase, f64 range reduction" if sample_domain
+        else "f32 (f64 pair phase, f64 block reduction)",
         "data": "synthetic",
         "config": {"workload": "certificate scan: 262144 candidates (64^3 grid, 70 mm box) x scans of 64^2, 96^2 "
                    "and 128^2 apertures @ 2.5 mm, 128 range bins, top 256 per scan", "apertures": A,
@@ -481,7 +506,7 @@ def scan_workload(args):
         "autotune": tuning,
         "median_distance_top16_to_surface_m": [float(x) for x in near],
         "roofline": {"kernel": "scan_td_kernel<false>" if sample_domain else "scan_kernel<2>",
-                     "bound": "mfma" if sample_domain else "valu", "achieved": achieved,
+                     "bound": "valu-issue" if sample_domain else "valu", "achieved": achieved,
                      "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": None, "flop_per_eval": scan_flop,
                      "peak_basis": "FP32 dense (FP32-accurate; FP16 hi+lo x3 MFMA)" if sample_domain
@@ -511,83 +536,145 @@ def scan_workload(args):
     return result
 
 
-def dsfw_workload(args):
-    """DSFW steps (spatial search, merged replacement of 256 atoms, joint Gauss-Newton) on a
-    float64-synthesized 128x128 scan of the ellipsoid with 30 dB noise, from random initial
-    surfels, in two forms: plain (bin-domain FP32 splat kernels, FP32 SYRK normal equations,
-    candidates scored in chunks, host-loop search, backtracking and elimination) and
-    mixed-precision batched (sample-domain MFMA splat, FP16 hi+lo split SYRK, one-launch
-    orientation choice, quasi-Newton search and elimination, correlation rhs, 2^29-element
-    design blocks). Both start from the same state; per-step wall time between
-    synchronizations and the NMSE after each step."""
+DSFW_FORMS = {
+    # plain FP32: bin-domain splat kernels, FP32 SYRK normal equations, candidates scored in
+    # chunks, host-loop quasi-Newton search, backtracking and elimination
+    "plain_fp32": ("bins", dict()),
+    # mixed-precision batched: sample-domain MFMA splat and row projection, FP16 hi+lo split
+    # SYRK, one-launch orientation choice, quasi-Newton search and elimination, 2^29-element
+    # design blocks
+    "mixed_batched": ("samples", dict(gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22)),
+}
+
+
+def _relative_deviation(a, b):
+    return max(abs(x - y) / max(abs(x), 1e-30) for x, y in zip(a, b, strict=True))
+
+
+def dsfw_compare(steps, warmup=1, adc_samples=256, truth_draws=4000, front_count=None, controls=("mixed_batched",),
+                 same_state_every=5):
+    """The full DSFW fit in the plain FP32 and the mixed-precision batched forms from the same
+    problem and initial state: `warmup` untimed steps, then `steps` steps each timed between
+    device synchronizations (median and mean), the NMSE after every step, the final median
+    centre error to the truth, and two parity figures.
+
+    * Trajectory deviation: max over steps of |nmse_mixed - nmse_plain| / nmse_plain. The
+      search takes greedy decisions on near-tied scores, so any rounding-level change of the
+      inputs moves it; `controls` reruns the named forms with the target perturbed by complex
+      Gaussian noise of 1e-7 of its RMS (the float32 rounding level), whose deviation from the
+      unperturbed run of the same form measures that sensitivity.
+    * Same-state agreement: every `same_state_every`-th geometry of the plain run is fitted
+      again (VarPro: normal equations, solve, rendered objective) by both forms; the largest
+      relative difference of the two NMSEs is the mixed form's evaluation error on identical
+      states."""
     import statistics
+    from dataclasses import replace
 
     import torch
 
-    from dsplat import DSFWConfig, workloads
+    from dsplat import DSFWConfig, FMCWModel, Scan, workloads
     from dsplat.dsfw import initialize, step
     from dsplat.spatial import SpatialSearchConfig
+    from dsplat.varpro import varpro
 
-    draws = args.truth_draws or 4000
-    problem = workloads.dsfw_problem(
-        adc_samples=args.adc_samples, truth_draws=draws, front_count=None if args.truth_draws else 2048
-    )
+    problem = workloads.dsfw_problem(adc_samples=adc_samples, truth_draws=truth_draws, front_count=front_count)
+    target, positions = problem["target"], problem["model"].scan.scan_positions_m
+    chirp = problem["model"].scan.fmcw_config
     S = len(problem["initial"]["centers"])
-    # plain: bin-domain FP32 splat kernels, FP32 SYRK, host-loop search; mixed: sample-domain
-    # MFMA splat, FP16 hi+lo SYRK, one-launch scoring / search / elimination
-    forms = {
-        "plain_fp32": (DSFWConfig(batch=256), 1),
-        "mixed_batched": (
-            DSFWConfig(batch=256, gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22), 0
-        ),
-    }
-    from dsplat import _lib
-
-    lib = _lib.library()
+    noise = workloads.complex_normal(tuple(target.shape), torch.Generator().manual_seed(77)).to(target.device)
+    perturbed = target + noise * (float(target.abs().square().mean().sqrt()) * 1e-7)
     search = SpatialSearchConfig()
-    steps, warmup = max(1, args.steps), max(1, args.warmup)
-    out = {}
-    for name, (config, splat_form) in forms.items():
-        lib.dsplat_set_splat_form(splat_form)
-        state = initialize(problem["model"], problem["target"], problem["initial"], config,
-                           parameters=problem["parameters"])  # fmt: skip
+
+    def setup(name, data):
+        form, extra = DSFW_FORMS[name]
+        model = FMCWModel(Scan(data, positions, replace(chirp, splat_form=form)))
+        return model, DSFWConfig(batch=256, iterations=steps, **extra)
+
+    def run(name, data, keep_states=False):
+        model, config = setup(name, data)
+        state = initialize(model, data, problem["initial"], config, parameters=problem["parameters"])
         for _ in range(warmup):
             state = step(state, config, search)
         torch.cuda.synchronize()
-        times = []
-        for _ in range(steps):
+        times, states = [], [state.params] if keep_states else []
+        for i in range(steps):
             t0 = time.perf_counter()
             state = step(state, config, search)
             torch.cuda.synchronize()
             times.append((time.perf_counter() - t0) * 1e3)
+            if keep_states:
+                states.append(state.params)
+            if (i + 1) % 25 == 0:
+                print(f"[dsfw] {name}{' (perturbed)' if data is perturbed else ''} step {i + 1}/{steps}: "
+                      f"nmse {state.history[-1]['nmse']:.5f}, median {statistics.median(times):.1f} ms", file=sys.stderr,
+                      flush=True)  # fmt: skip
         error = torch.cdist(state.params["centers"], problem["truth"]["centers"]).min(1).values
-        lib.dsplat_set_splat_form(0)
-        out[name] = {
-            "splat_form": "bin-domain" if splat_form else "sample-domain",
-            "ms_per_step_mean": statistics.fmean(times),
+        objective = [h["objective"] for h in state.history]
+        record = {
             "ms_per_step_median": statistics.median(times),
+            "ms_per_step_mean": statistics.fmean(times),
             "ms": times,
             "nmse": [h["nmse"] for h in state.history],
+            "final_nmse": state.history[-1]["nmse"],
             "median_center_error_m": float(error.median()),
+            "objective_non_increasing": all(b <= a for a, b in zip(objective, objective[1:])),
+            "steps_with_geometry_update": sum(1 for h in state.history[1:] if h.get("lm_accepted")),
         }
-    plain, mixed = out["plain_fp32"], out["mixed_batched"]
-    deviation = max(abs(a - b) / abs(a) for a, b in zip(plain["nmse"], mixed["nmse"], strict=True))
+        return record, states
+
+    plain, states = run("plain_fp32", target, keep_states=True)
+    mixed, _ = run("mixed_batched", target)
+    out = {"plain_fp32": plain, "mixed_batched": mixed}
+    control = {}
+    for name in controls:
+        record, _ = run(name, perturbed)
+        control[name] = _relative_deviation(out[name]["nmse"], record["nmse"])
+        out[name + "_perturbed"] = record
+    # identical geometries, fitted by both forms
+    fits = {name: setup(name, target) for name in DSFW_FORMS}
+    power = float(target.abs().square().mean())
+    same = []
+    for params in states[::same_state_every] + states[-1:]:
+        nmse = [varpro(model, params, target, config).objective / power for model, config in fits.values()]
+        same.append(abs(nmse[1] - nmse[0]) / nmse[0])
+    for record in out.values():
+        del record["ms"]
     return {
-        "metric": "DSFW step ms (2048-surfel fit, 128x128 scan)" if not args.truth_draws else "DSFW step ms",
-        "value": mixed["ms_per_step_mean"],
+        "workload": f"DSFW fit: {S} truth surfels ({'front faces of %d ellipsoid draws' % truth_draws if front_count is None else '%d front faces' % front_count}), "
+        f"128x128 array @ 2.5 mm, {adc_samples} ADC samples ({adc_samples} bins), float64 synthesis + 30 dB noise, "
+        f"random initial surfels, batch 256, spatial search, joint Gauss-Newton",
+        "surfels": S, "bins": adc_samples, "steps": steps, "warmup": warmup,
+        "speedup_median": plain["ms_per_step_median"] / mixed["ms_per_step_median"],
+        "speedup_mean": plain["ms_per_step_mean"] / mixed["ms_per_step_mean"],
+        "nmse_max_relative_deviation": _relative_deviation(plain["nmse"], mixed["nmse"]),
+        "control_nmse_max_relative_deviation": control,
+        "control_note": "same form, target + 1e-7 RMS complex noise: the search's own sensitivity",
+        "same_state_nmse_max_relative_deviation": max(same),
+        "same_state_count": len(same),
+        "forms": out,
+    }  # fmt: skip
+
+
+def dsfw_workload(args):
+    """The DSFW comparison on its own (see dsfw_compare); both forms get a perturbed control."""
+    draws = args.truth_draws or 4000
+    result = dsfw_compare(args.steps, max(1, args.warmup), args.adc_samples, draws,
+                          front_count=args.front_count or None, controls=("plain_fp32", "mixed_batched"))  # fmt: skip
+    mixed = result["forms"]["mixed_batched"]
+    return {
+        "metric": "DSFW step ms (median), mixed-precision batched form",
+        "value": mixed["ms_per_step_median"],
         "unit": "ms",
         "n_gpus": 1,
-        "steps": steps,
-        "warmup": warmup,
+        "steps": result["steps"],
+        "warmup": result["warmup"],
+        "ms_per_step": mixed["ms_per_step_median"],
         "higher_is_better": False,
-        "dtype": "f32 solver, fp16 hi+lo split SYRK (mixed form)",
+        "dtype": "f32 solver (f64 shifted Cholesky fallback), fp16 hi+lo x3 MFMA SYRK and splat (mixed form)",
         "data": "synthetic",
-        "config": {"workload": f"DSFW steps: {S} surfels, 128x128 array @ 2.5 mm, {args.adc_samples} ADC samples "
-                   f"({args.adc_samples} bins), batch 256, spatial search", "surfels": S,
-                   "bins": args.adc_samples, "parallelism": "single GPU"},
-        "speedup_mixed_batched_vs_plain": plain["ms_per_step_mean"] / mixed["ms_per_step_mean"],
-        "nmse_max_relative_deviation": deviation,
-        "forms": out,
+        "config": {"workload": result["workload"], "surfels": result["surfels"], "bins": result["bins"],
+                   "parallelism": "single GPU"},
+        "dsfw": result,
     }
 
 
@@ -695,22 +782,55 @@ def main():
     dc, dn, dw = (work[k].to(dev) for k in ("dcenters", "dnormals", "dweights"))
     jvp_ms, _ = timed(lambda: surfel.jvp(cfg, positions, centers, normals, weights, dc, dn, dw, 0.0), 5, stream)
 
-    from dsplat import _lib
-
-    sample_domain = _lib.library().dsplat_splat_form() == 0 and cfg.adc_samples in (128, 256)
+    sample_domain = cfg.splat_form == "samples" and cfg.adc_samples in (128, 256)
     fwd_flop, vjp_flop = (TD_FWD_FLOP, TD_VJP_FLOP) if sample_domain else (FWD_FLOP, VJP_FLOP)
     groups = {  # the launches each timed call makes
-        "render": ["td_render_h16_kernel<false>"] if sample_domain else ["render_kernel<2, 8>"],
+        "render": ["td_render_h16_kernel<false, false>"] if sample_domain else ["render_kernel<2, 8>"],
         "vjp": ["vjp_prep_kernel", "td_vjp_kernel<8, false>", "vjp_reduce"] if sample_domain
         else ["vjp_kernel<2>", "vjp_reduce"],
     }  # fmt: skip
     kernels = {
-        "render": (fwd_ms, fwd_flop * evals),
-        "vjp": (vjp_ms, vjp_flop * evals),
+        "render": (fwd_ms, fwd_flop * evals, FWD_FLOP * evals),
+        "vjp": (vjp_ms, vjp_flop * evals, VJP_FLOP * evals),
     }
     name = max(kernels, key=lambda k: kernels[k][0])
-    ms, flop = kernels[name]
+    ms, flop, cot_flop = kernels[name]
     achieved = flop / (ms * 1e-3) / 1e12
+    pairs = A * S
+    roofline = {
+        "kernel": name,
+        "launches": groups[name],
+        "bound": "valu-issue" if sample_domain else "valu",
+        "achieved": achieved,
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP32_TFLOPS,
+        "traffic": measured_traffic(groups[name]),
+        "traffic_source": f"{TRAFFIC.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch, "
+        "scaled by the same counters on a 1 GiB copy)",
+        "flop_per_eval": fwd_flop if name == "render" else vjp_flop,
+        "peak_basis": "FP32 peak: results are FP32-accurate, executed as FP16 hi+lo x3 MFMA products"
+        if sample_domain else "FP32 vector",
+        # the cotangent-form count per (aperture, bin, surfel): 13 FLOP forward, 19 VJP
+        "cotangent_form_count": {"flop_per_eval": FWD_FLOP if name == "render" else VJP_FLOP,
+                                 "tflops": cot_flop / (ms * 1e-3) / 1e12,
+                                 "frac": cot_flop / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS},
+    }  # fmt: skip
+    if sample_domain and cfg.adc_samples == 128:
+        kernel = groups[name][0] if name == "render" else "td_vjp_kernel<8, false>"
+        issue_ms = VALU_PER_64_PAIRS[kernel] * pairs / 64 * 4 / SIMDS / CLOCK_GHZ / 1e6
+        roofline["valu_issue"] = {
+            "kernel": kernel, "instructions_per_64_pairs": VALU_PER_64_PAIRS[kernel],
+            "source": f"{VALU_PROFILE.relative_to(ROOT)} (rocprofv3 --pmc SQ_INSTS_VALU)",
+            "issue_ms_at_peak": issue_ms, "frac": issue_ms / ms,
+            "basis": f"one wave64 VALU instruction per 4 cycles on each of {SIMDS} SIMDs at {CLOCK_GHZ} GHz",
+        }  # fmt: skip
+        roofline["executed_fp16_mfma"] = {
+            "tflops": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12,
+            "frac_of_fp16_peak": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12 / PEAK_FP16_TFLOPS,
+            "note": "the matrix products co-issue with the per-pair VALU work (geometry, powers, FP16 splits), "
+            "which bounds these kernels",
+        }
     result = {
         "metric": "fwd+VJP splat evals/s (aperture·bin·surfel)",
         "value": evals / (ms_per_step * 1e-3),
@@ -723,7 +843,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 per-pair phase)",
+        "dtype": "fp16 hi+lo x3 MFMA, fp32 accumulate, f64 pair phase" if sample_domain
+        else "f32 (f64 pair phase)",
         "data": "synthetic",
         "config": {
             "workload": "FMCW surfel splat fwd+VJP: 128x128 apertures @ 2.5 mm, 128 range bins, "
@@ -736,28 +857,19 @@ def main():
         },
         "kernels_ms": {"render": fwd_ms, "vjp": vjp_ms, "jvp": jvp_ms},
         "splat_form": "sample-domain MFMA" if sample_domain else "bin-domain VALU",
-        "roofline": {
-            "kernel": name,
-            "launches": groups[name],
-            "bound": "mfma" if sample_domain else "valu",
-            "achieved": achieved,
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": measured_traffic(groups[name]),
-            "flop_per_eval": fwd_flop if name == "render" else vjp_flop,
-            "peak_basis": "FP32 peak: results are FP32-accurate, executed as FP16 hi+lo x3 MFMA products"
-            if sample_domain else "FP32 vector",
-            **({"executed_fp16_mfma": {
-                "tflops": TD_EXEC_FLOP_PER_PAIR * A * S / (ms * 1e-3) / 1e12,
-                "frac_of_fp16_peak": TD_EXEC_FLOP_PER_PAIR * A * S / (ms * 1e-3) / 1e12 / PEAK_FP16_TFLOPS,
-                "note": "the matrix products co-issue with the per-pair VALU work (geometry, powers, "
-                "FP16 splits), which bounds these kernels"}} if sample_domain else {}),
-        },
+        "roofline": roofline,
     }
+    del work["dcenters"], work["dnormals"], work["dweights"]
+    torch.cuda.empty_cache()
+    if args.dsfw_steps > 0:
+        print(f"[bench] splat: {result['value']:.4g} evals/s; DSFW comparison, {args.dsfw_steps} steps per form",
+              file=sys.stderr, flush=True)  # fmt: skip
+        result["dsfw"] = dsfw_compare(args.dsfw_steps, 1, args.adc_samples, args.truth_draws,
+                                      front_count=args.front_count or None)  # fmt: skip
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
-        result["cpu_baseline"] = cpu_baseline(work, args.cpu_seconds)
-        result["cpu_torch_fp32"] = cpu_torch_fp32(work, args.cpu_seconds)
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_torch_fp32(work)
+        result["cpu_float64_oracle"] = cpu_baseline(work, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 

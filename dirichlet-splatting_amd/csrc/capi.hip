@@ -5,9 +5,6 @@
 
 namespace dsplat {
 
-static int g_splat_form = 0;
-int splat_form() { return g_splat_form; }
-
 // Zero fill without hipMemsetAsync, so a caller may capture these calls in a HIP graph.
 __global__ void zero_fill(float* __restrict__ p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -33,12 +30,14 @@ bool make_chirp(const dsplat_fmcw* cfg, Chirp* out) {
     const int S = cfg->samples, P = cfg->pad, N = cfg->n_fft, K = cfg->num_bins;
     if (S < 1 || !(P == 1 || P == 2 || P == 4 || P == 8) || N != S * P || K != N / 2) return false;
     if (N < 32 || K > 512) return false;
+    if (cfg->splat_form != 0 && cfg->splat_form != 1) return false;
     if (!(cfg->k0_per_m > 0) || !(cfg->c0_m_s > 0) || !(cfg->lambda_m > 0)) return false;
     Chirp c;
     c.K = K;
     c.N = N;
     c.S = S;
     c.P = P;
+    c.form = cfg->splat_form;
     c.Kp = (K + 15) / 16 * 16;
     c.k0pm = cfg->k0_per_m;
     c.fc = cfg->fc_hz;
@@ -74,14 +73,6 @@ using namespace dsplat;
 extern "C" {
 
 const char* dsplat_version(void) { return "dsplat-hip 0.2.0 gfx950"; }
-
-int dsplat_set_splat_form(int32_t form) {
-    if (form != 0 && form != 1) return hipErrorInvalidValue;
-    g_splat_form = form;
-    return hipSuccess;
-}
-
-int32_t dsplat_splat_form(void) { return g_splat_form; }
 
 const char* dsplat_error_string(int code) { return hipGetErrorString((hipError_t)code); }
 

@@ -37,6 +37,7 @@ constexpr double kPiD = 3.14159265358979323846;
 // Kernel-side chirp constants (by value).
 struct Chirp {
     int K, N, S, P;
+    int form;            // splat form of the call: 0 sample domain where supported, 1 bin domain
     int Kp;              // K rounded up to a multiple of 16
     double k0pm, fc, slope, t0, c0;
     double two_over_c0;  // 2 / c0: round-trip delay per metre

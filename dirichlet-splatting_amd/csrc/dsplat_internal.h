@@ -134,9 +134,8 @@ size_t syrk_workspace(int m, int n, int col0, int mode);
 hipError_t launch_syrk(const float* rows, int m, int n, int col0, int mode, const unsigned* column_max, float* out,
                        void* workspace, hipStream_t st);
 
-// Splat form: 0 = sample-domain MFMA kernels where the chirp allows them (S in {128, 256}),
-// 1 = bin-domain cotangent kernels everywhere. Process-wide, set through the C ABI.
-int splat_form();
+// Splat form (Chirp::form, from the call's dsplat_fmcw): 0 = sample-domain MFMA kernels where
+// the chirp allows them (S in {128, 256}), 1 = bin-domain cotangent kernels everywhere.
 bool td_supported(const Chirp& c);
 bool use_td(const Chirp& c);
 struct TdVjpShape {

@@ -1069,7 +1069,7 @@ bool td_supported(const Chirp& c) {
     return (c.S == 128 || c.S == 256) && c.N >= 64 && (c.N & (c.N - 1)) == 0 && c.K % 8 == 0;
 }
 
-bool use_td(const Chirp& c) { return splat_form() == 0 && td_supported(c); }
+bool use_td(const Chirp& c) { return c.form == 0 && td_supported(c); }
 
 template <int J, bool JVP>
 static hipError_t launch_td_render_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
@@ -1141,8 +1141,9 @@ TdVjpShape td_vjp_shape(int A, int S) {
     return v;
 }
 
-// Workspace: partials [S][ranges][8] floats | U [A][S] complex | A operands [A][T][2][64]
-// uint4 | scale exponents [A] floats, each part 256-byte aligned.
+// Workspace: partials [S][ranges][8] floats | U [A][S] complex (256 samples only: at 128
+// samples vjp_prep_kernel keeps U in LDS) | A operands [A][T][2][64] uint4 | scale exponents
+// [A] floats, each part 256-byte aligned.
 struct TdVjpWorkspace {
     size_t partial, u, ops, scales, total;
 };
@@ -1153,7 +1154,7 @@ static TdVjpWorkspace td_vjp_layout(const Chirp& c, int A, int S) {
     TdVjpWorkspace w;
     w.partial = 0;
     w.u = up(sizeof(float) * 8 * (size_t)S * v.ranges);
-    w.ops = w.u + up(sizeof(float2) * (size_t)A * c.S);
+    w.ops = w.u + (c.S == 128 ? 0 : up(sizeof(float2) * (size_t)A * c.S));
     w.scales = w.ops + up(sizeof(uint4) * (size_t)A * (c.S / 128) * 2 * 64);
     w.total = w.scales + up(sizeof(float) * (size_t)A);
     return w;
@@ -1421,7 +1422,7 @@ __global__ __launch_bounds__(256) void scan_td_reduce(const float* __restrict__ 
     }
 }
 
-bool td_scan_supported(const Chirp& c) { return splat_form() == 0 && c.S == 128 && td_supported(c); }
+bool td_scan_supported(const Chirp& c) { return c.form == 0 && c.S == 128 && td_supported(c); }
 
 static TdScanSet td_scan_set(const ScanSet& set, int chunk) {
     TdScanSet t;

@@ -27,6 +27,8 @@ class Fmcw(ctypes.Structure):
         ("n_fft", c_int),
         ("samples", c_int),
         ("pad", c_int),
+        ("splat_form", c_int),
+        ("reserved", c_int),
         ("k0_per_m", c_double),
         ("fc_hz", c_double),
         ("slope_hz_s", c_double),
@@ -68,11 +70,9 @@ SIGNATURES = {
         c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr]
     ),
     "dsplat_surfel_vjp_workspace": (c_size, [c_ptr, c_int, c_int]),
-    "dsplat_set_splat_form": (c_int, [c_int]),
     "dsplat_cholesky": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
     "dsplat_cholesky_f32": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
     "dsplat_cholesky_solve": (c_int, [c_ptr, c_int, c_int, c_ptr, c_int, c_int, c_ptr]),
-    "dsplat_splat_form": (c_int, []),
     "dsplat_surfel_vjp": (
         c_int,
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
@@ -194,6 +194,9 @@ def stream(device):
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+SPLAT_FORMS = {"samples": 0, "bins": 1}
+
+
 def fmcw(config, fresnel_n=0.0):
     """The ABI struct of an FMCWConfig (kernels/surfel.py:21-32 sensor_args, plus fresnel_n)."""
     pad = int(config.pad_factor)
@@ -204,6 +207,7 @@ def fmcw(config, fresnel_n=0.0):
         n_fft=int(config.n_fft),
         samples=int(config.adc_samples),
         pad=pad,
+        splat_form=SPLAT_FORMS[getattr(config, "splat_form", "samples")],
         k0_per_m=1.0 / config.range_bin_spacing_m,
         fc_hz=float(config.fc_hz),
         slope_hz_s=float(config.slope_hz_s),

@@ -65,4 +65,4 @@ def tune(kernel, key, candidates, run, device, repeats=2):
 
 def record(kernel):
     """The tuning records of a kernel: {shape key: {best, ms per candidate}}."""
-    return {k.split("|")[1]: v for k, v in _MEMORY.items() if k.startswith(kernel + "|")}
+    return {k[len(kernel) + 1 : k.rindex("|")]: v for k, v in _MEMORY.items() if k.startswith(kernel + "|")}

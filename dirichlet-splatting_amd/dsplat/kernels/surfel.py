@@ -410,7 +410,9 @@ def scan_batched(config, scans, centers, normals, fresnel_n, chunk=None):
         )  # fmt: skip
 
     if chunk is None:
-        key = f"C{count}|A{'-'.join(str(len(p)) for p in positions)}|K{config.num_range_bins}|P{config.pad_factor}"
+        # the form and the sample count pick the kernel, and with it what chunk means
+        key = (f"C{count}|A{'-'.join(str(len(p)) for p in positions)}|K{config.num_range_bins}|P{config.pad_factor}"
+               f"|S{config.adc_samples}|F{cfg.splat_form}")
         chunk = autotune.tune("scan_batched", key, list(SCAN_CHUNKS), run, centers.device)
     run(chunk)
     return torch.complex(out[..., 0], out[..., 1]), out[..., 2]

@@ -130,8 +130,8 @@ def certificate_candidates(per_axis=64, seed=4):
     return centers, hemisphere_normals(len(centers), _gen(seed))
 
 
-def dsfw_problem(adc_samples=256, truth_draws=4000, front_count=None, snr_db=30.0, device="cuda"):
-    """A DSFW fit: a float64 scan of ellipsoid surfels on the 128x128 array plus complex
+def dsfw_problem(adc_samples=256, truth_draws=4000, front_count=None, snr_db=30.0, device="cuda", side=128):
+    """A DSFW fit: a float64 scan of ellipsoid surfels on the side x side array plus complex
     Gaussian noise (seed 5, sigma^2 = mean|y|^2 / 10^(snr/10)), and an initial guess of as
     many surfels uniform in the truth's box with 5 mm margin and random normals (seed 1).
 
@@ -151,7 +151,7 @@ def dsfw_problem(adc_samples=256, truth_draws=4000, front_count=None, snr_db=30.
         centers, normals = ellipsoid_samples(front_count, 0)
         area = ellipsoid_area(*SEMI_AXES) / (2 * front_count)
     centers, normals = centers.to(device), normals.to(device)
-    positions = planar_array(128, PITCH).to(device)
+    positions = planar_array(side, PITCH).to(device)
     amplitudes = torch.full((len(centers),), area**0.5, device=device)
     clean = synthesize(positions, centers, normals, amplitudes, config).data
     sigma = float(clean.abs().square().mean().sqrt()) * 10 ** (-snr_db / 20)

@@ -30,6 +30,13 @@ typedef struct dsplat_fmcw {
     int32_t n_fft;      /* N, FFT length */
     int32_t samples;    /* S, ADC samples */
     int32_t pad;        /* P = N / S */
+    /* Splat form of this call (no counterpart in the reference, whose Slang kernels have one
+     * form): 0 evaluates the Dirichlet kernel as its sample sum on the matrix cores where the
+     * chirp has 128 or 256 ADC samples (other chirps fall back to 1), 1 uses the bin-domain
+     * cotangent kernels. Both match the float64 closed form to ~1e-7. It selects the kernels,
+     * and the workspace sizes, of every entry point that takes this struct. */
+    int32_t splat_form;
+    int32_t reserved;   /* 0 */
     double k0_per_m;    /* 1 / range_bin_spacing_m */
     double fc_hz;       /* carrier */
     double slope_hz_s;  /* chirp slope */
@@ -42,13 +49,6 @@ typedef struct dsplat_fmcw {
 /* Library version and a readable name for a returned code. */
 const char* dsplat_version(void);
 const char* dsplat_error_string(int code);
-
-/* Splat form used by render / VJP (process-wide; no counterpart in the reference, whose
- * Slang kernels have one form): 0 (default) evaluates the Dirichlet kernel as its sample sum
- * on MFMA where the chirp has 128 or 256 ADC samples, 1 uses the bin-domain cotangent
- * kernels for every chirp. Both match the float64 closed form to ~1e-7; 1 exists to compare. */
-int dsplat_set_splat_form(int32_t form);
-int32_t dsplat_splat_form(void);
 
 /* Range profiles out[A, K] = sum_s w_s column_s(p_a).
  * positions [A,3], centers [S,3], normals [S,3], weights [S] complex, out [A,K] complex.
@@ -264,8 +264,8 @@ int dsplat_project_design(const dsplat_fmcw* cfg, const float* positions, int32_
                           int32_t num_surfels, const float* basis, int32_t rows, float* out,
                           void* stream);
 
-/* Which projection kernel serves cfg and rows: 0 for dsplat_project_sampled (sample-domain
- * splat form in effect and 128 or 256 ADC samples), 1 for dsplat_project_design only, -1 for
+/* Which projection kernel serves cfg and rows: 0 for dsplat_project_sampled (cfg->splat_form
+ * 0 and 128 or 256 ADC samples), 1 for dsplat_project_design only, -1 for
  * neither (invalid configuration or odd rows). */
 int32_t dsplat_project_form(const dsplat_fmcw* cfg, int32_t rows);
 

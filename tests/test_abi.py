@@ -54,21 +54,22 @@ def test_vjp_workspace_per_splat_form():
 
     lib = _lib.library()
     cfg = ctypes.byref(_lib.fmcw(FMCWConfig()))
-    assert lib.dsplat_splat_form() == 0
     assert lib.dsplat_surfel_vjp_workspace(cfg, 0, 10) == 0
-    # sample-domain form: partials of 8 floats per (surfel, 128-aperture range), the
-    # cotangent's sample-domain rows U [A, 128] complex, the FP16 hi + lo MFMA operands of
-    # the symmetric split (one K half: 2 KiB per aperture) and one scale per aperture
-    assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == (
-        2048 * 128 * 8 * 4 + 16384 * 128 * 8 + 16384 * 2048 + 16384 * 4
+    # sample-domain form at 128 samples: partials of 8 floats per (surfel, 128-aperture
+    # range), the FP16 hi + lo MFMA operands of the symmetric split (one K half: 2 KiB per
+    # aperture) and one scale per aperture; the cotangent's sample-domain rows stay in LDS
+    assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == 2048 * 128 * 8 * 4 + 16384 * 2048 + 16384 * 4
+    # 256 samples: the sample-domain rows U [A, 256] complex go through the workspace
+    wide = ctypes.byref(_lib.fmcw(FMCWConfig(adc_samples=256)))
+    assert lib.dsplat_surfel_vjp_workspace(wide, 16384, 2048) == (
+        2048 * 128 * 8 * 4 + 16384 * 256 * 8 + 16384 * 2 * 2048 + 16384 * 4
     )
-    try:
-        assert lib.dsplat_set_splat_form(1) == 0
-        # bin-domain form: one partial per surfel and 32-aperture block
-        assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == 2048 * 512 * 8 * 4
-    finally:
-        lib.dsplat_set_splat_form(0)
-    assert lib.dsplat_set_splat_form(2) == 1
+    # bin-domain form, chosen per call: one partial per surfel and 32-aperture block
+    bins = ctypes.byref(_lib.fmcw(FMCWConfig(splat_form="bins")))
+    assert lib.dsplat_surfel_vjp_workspace(bins, 16384, 2048) == 2048 * 512 * 8 * 4
+    bad = _lib.fmcw(FMCWConfig())
+    bad.splat_form = 2  # forms other than 0 and 1 are rejected
+    assert lib.dsplat_surfel_vjp_workspace(ctypes.byref(bad), 16384, 2048) == 0
     odd = _lib.fmcw(FMCWConfig(adc_samples=96))  # no sample-domain kernel: bin-domain sizes
     assert lib.dsplat_surfel_vjp_workspace(ctypes.byref(odd), 1024, 200) == 200 * 32 * 8 * 4
 
@@ -76,9 +77,11 @@ def test_vjp_workspace_per_splat_form():
 def test_struct_layout_matches_the_header():
     from dsplat import _lib
 
-    # four int32 then seven doubles, naturally aligned: 16 + 56 bytes
-    assert ctypes.sizeof(_lib.Fmcw) == 72
-    assert _lib.Fmcw.k0_per_m.offset == 16
+    # six int32 (the chirp sizes, the splat form, a reserved word) then seven doubles,
+    # naturally aligned: 24 + 56 bytes
+    assert ctypes.sizeof(_lib.Fmcw) == 80
+    assert _lib.Fmcw.splat_form.offset == 16
+    assert _lib.Fmcw.k0_per_m.offset == 24
 
 
 def test_kernels_refuse_host_tensors():

@@ -216,3 +216,84 @@ def test_range_response_and_energy():
     assert r.shape == (128, 11)
     assert abs(r[60, 0] - 1) < 1e-15 and abs(r[70, 10] - 1) < 1e-15
     assert L.energy([1 + 1j, 2], [1, 0]) == 5.0
+
+
+def _spd(rng, count, width):
+    """count random symmetric positive-definite [width, width] matrices."""
+    m = rng.standard_normal((count, width, width))
+    return np.einsum("cij,ckj->cik", m, m) + width * np.eye(width)[None]
+
+
+def test_search_direction_is_the_newton_step_of_a_quadratic():
+    """With inverse = H^-1 of a quadratic f(x) = 1/2 x^T H x - b^T x, one unclipped step from
+    x lands on the minimiser H^-1 b (solved by LAPACK, not by the oracle's formula)."""
+    from oracle import search as Se
+
+    rng = np.random.default_rng(11)
+    h = _spd(rng, 5, 3)
+    b = rng.standard_normal((5, 3))
+    x = rng.standard_normal((5, 3))
+    g = np.einsum("cij,cj->ci", h, x) - b
+    d = Se.direction(np.linalg.inv(h), g, np.zeros((5, 3)), np.ones(3), 1e9)
+    np.testing.assert_allclose(x + d, np.linalg.solve(h, b[..., None])[..., 0], rtol=1e-10, atol=1e-12)
+
+
+def test_search_direction_fallbacks_mask_tangent_and_radius():
+    from oracle import search as Se
+
+    rng = np.random.default_rng(12)
+    g = rng.standard_normal((6, 3))
+    # identity inverse: steepest descent, clipped to the trust radius (length exactly radius)
+    d = Se.direction(np.broadcast_to(np.eye(3), (6, 3, 3)), g, np.zeros((6, 3)), np.ones(3), 0.25)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 0.25, rtol=1e-12)
+    np.testing.assert_allclose(d / np.linalg.norm(d, axis=1, keepdims=True), -g / np.linalg.norm(g, axis=1, keepdims=True), rtol=1e-12)
+    # a negated (ascent) inverse: d . g >= 0, so the step falls back to -g
+    d = Se.direction(np.broadcast_to(-np.eye(3), (6, 3, 3)), g, np.zeros((6, 3)), np.ones(3), 1e9)
+    np.testing.assert_allclose(d, -g, rtol=1e-12)
+    # masked components stay exactly zero
+    mask = np.array([1.0, 0.0, 1.0])
+    d = Se.direction(_spd(rng, 6, 3), g, np.zeros((6, 3)), mask, 1e9)
+    assert np.all(d[:, 1] == 0)
+    # width 6: the normal part is tangent to the unit normal
+    n = rng.standard_normal((6, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    g6 = rng.standard_normal((6, 6))
+    d = Se.direction(_spd(rng, 6, 6), g6, n, np.ones(6), 1e9)
+    np.testing.assert_allclose((d[:, 3:] * n).sum(1), 0, atol=1e-12)
+    # a length below the radius is not scaled
+    small = Se.direction(np.broadcast_to(np.eye(3), (6, 3, 3)), 1e-3 * g, np.zeros((6, 3)), np.ones(3), 1.0)
+    np.testing.assert_allclose(small, -1e-3 * g, rtol=1e-12)
+
+
+def test_search_trial_clamps_renormalises_and_freezes():
+    from oracle import search as Se
+
+    rng = np.random.default_rng(13)
+    c = rng.uniform(-1, 1, (7, 3))
+    n = rng.standard_normal((7, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    d = rng.standard_normal((7, 6))
+    controls = np.array([
+        [2.0, 0.5, 1.0, 1.0, 1.0, 1.0],     # per-coordinate scale
+        [1.0, 1.0, 0.0, 1.0, 1.0, 1.0],     # trainable: the z centre coordinate is frozen
+        [-0.5, -0.5, -0.5, 0, 0, 0],        # lower bounds
+        [0.5, 0.5, 0.5, 0, 0, 0],           # upper bounds
+    ])  # fmt: skip
+    oc, on = Se.trial(c, n, d, controls, 0.5)
+    assert np.all(oc[:, 2] == c[:, 2])
+    assert np.all(oc[:, :2] >= -0.5) and np.all(oc[:, :2] <= 0.5)
+    # an interior move is exactly centre + fraction * scale * direction
+    inside = np.all(np.abs(c[:, :2] + 0.5 * controls[0, :2] * d[:, :2]) < 0.5, axis=1)
+    np.testing.assert_allclose(oc[inside, :2], (c[:, :2] + 0.5 * controls[0, :2] * d[:, :2])[inside], rtol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(on, axis=1), 1, rtol=1e-12)
+    # the new normal points along n + fraction * d_n
+    m = n + 0.5 * d[:, 3:]
+    np.testing.assert_allclose(np.cross(on, m), 0, atol=1e-12)
+    assert np.all((on * m).sum(1) > 0)
+    # width 3: normals untouched
+    _, same = Se.trial(c, n, d[:, :3], controls, 0.5)
+    assert np.array_equal(same, n)
+    # a zero fraction returns the (clamped) start
+    oc0, on0 = Se.trial(c, n, d, controls, 0.0)
+    np.testing.assert_allclose(oc0, np.where(controls[1, :3] > 0, np.clip(c, -0.5, 0.5), c))
+    np.testing.assert_allclose(on0, n, rtol=1e-12)

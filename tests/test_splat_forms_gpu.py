@@ -23,14 +23,10 @@ def say(*args):
     print(*args, file=sys.stderr, flush=True)
 
 
-@pytest.fixture(params=[0, 1], ids=["samples", "bins"])
+@pytest.fixture(params=["samples", "bins"])
 def form(request):
-    from dsplat import _lib
-
-    lib = _lib.library()
-    assert lib.dsplat_set_splat_form(request.param) == 0
-    yield request.param
-    lib.dsplat_set_splat_form(0)
+    """The splat form each call of the test passes in its chirp (FMCWConfig.splat_form)."""
+    return request.param
 
 
 def tangents(count, seed):
@@ -93,7 +89,7 @@ def check(cfg, positions, centers, normals, weights, fresnel_n, seed):
 def test_forms_match_the_oracle_across_chirps(form, samples, pad, fresnel_n):
     from dsplat import FMCWConfig
 
-    cfg = FMCWConfig(adc_samples=samples, pad_factor=pad)
+    cfg = FMCWConfig(adc_samples=samples, pad_factor=pad, splat_form=form)
     check(cfg, *scene(40, 9, seed=samples + pad), fresnel_n, seed=pad)
 
 
@@ -101,25 +97,17 @@ def test_forms_match_the_oracle_across_chirps(form, samples, pad, fresnel_n):
 def test_forms_on_ragged_sizes_and_back_faces(form, side, count, back):
     from dsplat import FMCWConfig
 
-    cfg = FMCWConfig()
+    cfg = FMCWConfig(splat_form=form)
     check(cfg, *scene(count, side, seed=side * 31 + count, back_facing=back), 0.0, seed=count)
 
 
 def test_the_two_forms_agree_on_a_larger_case():
-    from dsplat import FMCWConfig, _lib
+    from dsplat import FMCWConfig
 
-    lib = _lib.library()
-    cfg = FMCWConfig()
     args = scene(300, 24, seed=7)
     g = np.random.default_rng(3)
     up = (g.standard_normal((24 * 24, 128)) + 1j * g.standard_normal((24 * 24, 128))).astype(np.complex64)
-    results = []
-    try:
-        for f in (0, 1):
-            assert lib.dsplat_set_splat_form(f) == 0
-            results.append(run(cfg, *args, up, 0.0, tangents(300, 7)))
-    finally:
-        lib.dsplat_set_splat_form(0)
+    results = [run(FMCWConfig(splat_form=f), *args, up, 0.0, tangents(300, 7)) for f in ("samples", "bins")]
     errs = [rel_l2(a, b) for a, b in zip(results[0], results[1])]
     say("sample-domain vs bin-domain rel L2: " + " ".join(f"{e:.2e}" for e in errs))
     assert max(errs) < 2 * TOL
