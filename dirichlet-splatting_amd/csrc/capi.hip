@@ -1,9 +1,40 @@
 // C ABI of libdsplat_hip.so: argument validation and dispatch to the launch units.
 #include <math.h>
 
+#include <mutex>
+
 #include "dsplat_internal.h"
 
 namespace dsplat {
+
+// Scratch of dsplat_surfel_vjp_noalloc: file-scope device memory that every device holds once
+// its code object is loaded, until the process ends. In host code `scratch` names a host-side
+// stand-in; the device address of each device's copy comes from hipGetSymbolAddress (taken on
+// the first call on that device, with that device current) and is the only form host code
+// passes on.
+static __device__ __align__(256) unsigned char scratch[DSPLAT_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_lock;
+static void* g_scratch_address[kMaxDevices];
+
+// Device address of the current device's scratch.
+static hipError_t scratch_address(void** out) {
+    int device = -1;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (!g_scratch_address[device]) {
+        void* address = nullptr;
+        e = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!address) return hipErrorInvalidSymbol;
+        g_scratch_address[device] = address;
+    }
+    *out = g_scratch_address[device];
+    return hipSuccess;
+}
 
 // Zero fill without hipMemsetAsync, so a caller may capture these calls in a HIP graph.
 __global__ void zero_fill(float* __restrict__ p, size_t n) {
@@ -134,6 +165,31 @@ int dsplat_surfel_vjp(const dsplat_fmcw* cfg, const float* positions, int32_t nu
     return launch_vjp(c, positions, num_apertures, centers, normals, reinterpret_cast<const float2*>(weights),
                       num_surfels, reinterpret_cast<const float2*>(upstream), static_cast<float*>(workspace),
                       grad_centers, grad_normals, grad_weights, st);
+}
+
+size_t dsplat_noalloc_scratch_bytes(void) { return DSPLAT_NOALLOC_SCRATCH_BYTES; }
+
+int dsplat_surfel_vjp_noalloc(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                              const float* centers, const float* normals, const float* weights, int32_t num_surfels,
+                              const float* upstream, float* grad_centers, float* grad_normals, float* grad_weights,
+                              void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0) return hipErrorInvalidValue;
+    if (num_surfels == 0) return hipSuccess;
+    if (!grad_centers || !grad_normals || !grad_weights) return hipErrorInvalidValue;
+    if (num_apertures == 0)
+        return dsplat_surfel_vjp(cfg, positions, 0, centers, normals, weights, num_surfels, upstream, grad_centers,
+                                 grad_normals, grad_weights, nullptr, 0, stream);
+    const size_t need = dsplat_surfel_vjp_workspace(cfg, num_apertures, num_surfels);
+    if (need == 0) return hipErrorInvalidValue;
+    if (need > DSPLAT_NOALLOC_SCRATCH_BYTES) return hipErrorOutOfMemory;
+    void* address = nullptr;
+    hipError_t e = scratch_address(&address);
+    if (e != hipSuccess) return e;
+    // The VJP kernels write every partial, operand and scale they read back (nothing is
+    // accumulated into the workspace), so the previous call's contents need no clearing.
+    return dsplat_surfel_vjp(cfg, positions, num_apertures, centers, normals, weights, num_surfels, upstream,
+                             grad_centers, grad_normals, grad_weights, address, need, stream);
 }
 
 int dsplat_surfel_design(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const float* centers,

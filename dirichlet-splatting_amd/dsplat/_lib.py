@@ -77,6 +77,10 @@ SIGNATURES = {
         c_int,
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
     ),
+    "dsplat_noalloc_scratch_bytes": (c_size, []),
+    "dsplat_surfel_vjp_noalloc": (
+        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]
+    ),
     "dsplat_surfel_design": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr]),
     "dsplat_surfel_jacobian": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr]),
     "dsplat_surfel_correlate_workspace": (c_size, [c_int, c_int, c_int]),

@@ -78,6 +78,23 @@ int dsplat_surfel_vjp(const dsplat_fmcw* cfg, const float* positions, int32_t nu
                       float* grad_normals, float* grad_weights, void* workspace,
                       size_t workspace_bytes, void* stream);
 
+/* dsplat_surfel_vjp without a workspace argument, for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. Its workspace is a fixed scratch array inside the library,
+ * DSPLAT_NOALLOC_SCRATCH_BYTES per device, held from the library's load on a device until
+ * the process ends; the call allocates nothing. Returns hipErrorOutOfMemory (2) when
+ * dsplat_surfel_vjp_workspace(cfg, A, S) exceeds that size (cfg2's 128x128 apertures x 2048
+ * surfels at 128 samples fit), hipErrorInvalidValue as dsplat_surfel_vjp otherwise. Every call
+ * on one device shares that device's scratch: calls that overlap in time on one device (two
+ * streams, two host threads) race and must be ordered by the caller. The stream must belong
+ * to the current device. Same results as dsplat_surfel_vjp, bit for bit. */
+#define DSPLAT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+size_t dsplat_noalloc_scratch_bytes(void);
+int dsplat_surfel_vjp_noalloc(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                              const float* centers, const float* normals, const float* weights,
+                              int32_t num_surfels, const float* upstream, float* grad_centers,
+                              float* grad_normals, float* grad_weights, void* stream);
+
 /* Dictionary rows of whole apertures including sqrt(area): complex [A*K, S]
  * (row a*K + k) when real_layout == 0, else real [2*K*A, S] with row (2k + part)*A + a.
  * areas [S]. Replaces kernels/surfel.py:199-220 (design: design_bins). */
