@@ -315,3 +315,41 @@ def test_full_size_jvp_against_finite_differences_of_the_render():
     err = rel_l2(got.cpu().numpy(), want)
     say(f"full jvp subset rel L2 {err:.3e}")
     assert err < TOL
+
+
+@pytest.mark.parametrize("samples", [128, 256])
+def test_full_size_vjp_blocks_by_the_adjoint_identity_and_aperture_additivity(samples):
+    """128^2 apertures x 2048 surfels: every gradient block of the full-size VJP (centres,
+    normals, weights, each with its own tangent) against the JVP by the adjoint identity
+    Re<u, J v> = <gc, dc> + <gn, dn> + Re<gw, dw> (to 1e-5 of the larger of the inner product and
+    1e-2 of the sum of its terms' magnitudes), and the full-size VJP against the sum of the
+    VJPs of four aperture quarters (the gradient is additive over apertures; each quarter
+    reduces over its own aperture ranges)."""
+    from dsplat import FMCWConfig, workloads
+    from dsplat.kernels import surfel
+
+    w = workloads.ellipsoid_scene()
+    cfg = FMCWConfig(adc_samples=samples)
+    K = cfg.num_range_bins
+    positions, centers, normals, weights = (w[k].cuda() for k in ("positions", "centers", "normals", "weights"))
+    up = workloads.complex_normal((len(positions), K), torch.Generator().manual_seed(1)).cuda()
+    say(f"full-size vjp blocks, {samples} samples")
+    gc, gn, gw = surfel.vjp(cfg, positions, centers, normals, weights, up, 0.0)
+    dcs, dns, dws = (w[k].cuda() for k in ("dcenters", "dnormals", "dweights"))
+    zc, zw = torch.zeros_like(dcs), torch.zeros_like(dws)
+    u64 = up.to(torch.complex128)
+    for name, (dc, dn, dw) in {"centers": (dcs, zc, zw), "normals": (zc, dns, zw), "weights": (zc, zc, dws)}.items():
+        jv = surfel.jvp(cfg, positions, centers, normals, weights, dc, dn, dw, 0.0).to(torch.complex128)
+        terms = (u64.conj() * jv).real
+        # the inner product's own scale: the sum of its terms' magnitudes (the normals' sum
+        # cancels to ~1e-3 of it at 256 samples)
+        left, scale = float(terms.sum()), float(terms.abs().sum())
+        right = float((gc.double() * dc.double()).sum() + (gn.double() * dn.double()).sum()
+                      + (gw.to(torch.complex128).conj() * dw.to(torch.complex128)).real.sum())  # fmt: skip
+        say(f"  {name}: <u, J v> {left:.6e} vs <g, v> {right:.6e} (scale {scale:.3e})")
+        assert abs(left - right) < 1e-5 * max(abs(left), 1e-2 * scale)
+    parts = [surfel.vjp(cfg, positions[q], centers, normals, weights, up[q], 0.0) for q in torch.arange(16384).cuda().chunk(4)]
+    for full, pieces in zip((gc, gn, gw), zip(*parts)):
+        total = sum(p.to(torch.complex128 if p.is_complex() else torch.float64) for p in pieces)
+        err = float((full.to(total.dtype) - total).norm() / total.norm())
+        assert err < 1e-6, err
