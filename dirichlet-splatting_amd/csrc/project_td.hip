@@ -62,12 +62,26 @@ __device__ __forceinline__ f32x16 mfma_f16(f16x8 a, f16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
 }
 
+// Where a projected block goes. Row-major float32 out [rows * A, S] (row j * A + a), or
+// (split) BF16 hi / lo with hi + lo ~ value in 32-row column tiles, element (k, s) at
+// ((k / 32) ld + s) 32 + k % 32, where projected row j of aperture a is k = a * rows_pad + j,
+// rows_pad = rows rounded up to 32 (rows j >= rows hold zeros): the operand form of
+// syrk_split_kernel.
+struct ProjectOut {
+    float* out;
+    unsigned* colmax;
+    __bf16* hi;
+    __bf16* lo;
+    long long ld;
+    int rows_pad;
+};
+
 // One aperture of one unit: v, z = v^16 and the weight w / S of this lane's pair. Returns
 // the largest |value| this lane stored (0 for a padding column).
-template <int J, int NB>
+template <int J, int NB, bool SPLIT>
 __device__ __forceinline__ float project_aperture(const _Float16* Bh, const _Float16* Bl, int bs, int h, int l32,
                                                  float2 v, float2 z, float2 w, int row0, int rows, int a, int A,
-                                                 int s, int S, float* __restrict__ out) {
+                                                 int s, int S, const ProjectOut& o) {
     // W of this lane's residues i = 4 h + e (even K steps) and 8 + 4 h + e (odd steps)
     const float2 v2 = csq(v), v4 = csq(v2), v8 = csq(v4);
     float2 W[2][4];
@@ -126,6 +140,37 @@ __device__ __forceinline__ float project_aperture(const _Float16* Bh, const _Flo
     // (Re, Im) row pairs times w; C[row][col] -> acc[r] with row = 8 (r / 4) + 4 h + r % 4, so
     // rows 2j and 2j + 1 are acc[r], acc[r + 1] for even r
     if (s >= S) return 0.0f;
+    if constexpr (SPLIT) {
+        // rows 4 g .. 4 g + 3 of a 32-row block are (re, im) of two projected rows: 8 bytes of
+        // hi and of lo per group; the two lane halves fill 16 contiguous bytes, the four groups
+        // the column's 64 bytes of the tile
+        const size_t tile0 = ((size_t)a * o.rows_pad + row0) / 32;
+#pragma unroll
+        for (int x = 0; x < NB; ++x) {
+            if (row0 + x * 32 >= o.rows_pad) break;  // uniform
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                float val[4];
+#pragma unroll
+                for (int e = 0; e < 4; e += 2) {
+                    const float hr = acc[x][4 * g + e], hm = acc[x][4 * g + e + 1];
+                    val[e] = w.x * hr - w.y * hm;
+                    val[e + 1] = w.x * hm + w.y * hr;
+                }
+                __bf16 vh[4], vl[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    vh[e] = (__bf16)val[e];
+                    vl[e] = (__bf16)(val[e] - (float)vh[e]);
+                }
+                const size_t at = ((tile0 + x) * o.ld + s) * 32 + 8 * g + 4 * h;
+                *reinterpret_cast<uint2*>(o.hi + at) = __builtin_bit_cast(uint2, vh);
+                *reinterpret_cast<uint2*>(o.lo + at) = __builtin_bit_cast(uint2, vl);
+            }
+        }
+        return 0.0f;
+    }
+    float* __restrict__ out = o.out;
     // The row stride passes through an empty asm so that the 64-bit row offsets are formed
     // here, per aperture, in scalar registers, instead of being hoisted out of every loop
     // into dozens of live registers.
@@ -158,14 +203,14 @@ __device__ __forceinline__ void flush_column_max(unsigned* colmax, float top, in
     if (colmax && h == 0 && s < S) atomicMax(colmax + s, __float_as_uint(fmaxf(top, other)));
 }
 
-template <int J, bool FRESNEL, int ROWS>
+template <int J, bool FRESNEL, int ROWS, bool SPLIT>
 __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, const float* __restrict__ pos, int A,
                                                                 const float* __restrict__ ctr,
                                                                 const float* __restrict__ nrm,
                                                                 const float* __restrict__ areas, int S,
                                                                 const float* __restrict__ basis, int rows,
-                                                                float* __restrict__ out,
-                                                                unsigned* __restrict__ colmax) {
+                                                                const ProjectOut o) {
+    unsigned* __restrict__ colmax = o.colmax;
     constexpr int KS = 32 * J;  // real values per basis row: (re, im) of 16 J samples
     constexpr int BS = KS + 8;  // LDS row stride in halves
     constexpr int NB = ROWS / 32;
@@ -228,10 +273,11 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
         }
 #pragma unroll 1
         for (int e = 0; e < 2 && a0 + e < A; ++e)
-            top = fmaxf(top, project_aperture<J, NB>(Bh, Bl, BS, h, l32, make_float2(e ? f1[0] : f0[0], e ? f1[1] : f0[1]),
-                                                     make_float2(e ? f1[2] : f0[2], e ? f1[3] : f0[3]),
-                                                     make_float2(e ? f1[4] : f0[4], e ? f1[5] : f0[5]), row0, rows,
-                                                     a0 + e, A, s, S, out));
+            top = fmaxf(top, project_aperture<J, NB, SPLIT>(Bh, Bl, BS, h, l32,
+                                                            make_float2(e ? f1[0] : f0[0], e ? f1[1] : f0[1]),
+                                                            make_float2(e ? f1[2] : f0[2], e ? f1[3] : f0[3]),
+                                                            make_float2(e ? f1[4] : f0[4], e ? f1[5] : f0[5]), row0,
+                                                            rows, a0 + e, A, s, S, o));
     }
     if (tile >= 0) flush_column_max(colmax, top, h, s, S);
 }
@@ -250,11 +296,11 @@ bool project_td_supported(const Chirp& c, int rows) {
     return use_td(c) && rows >= 2 && rows % 2 == 0 && project_td_tile(c, rows) > 0;
 }
 
-template <int J, bool F, int ROWS>
+template <int J, bool F, int ROWS, bool SPLIT>
 static hipError_t launch_ptd(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                             const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
+                             const float* areas, int S, const float* basis, int rows, const ProjectOut& o,
                              hipStream_t st) {
-    const void* kernel = (const void*)project_td_kernel<J, F, ROWS>;
+    const void* kernel = (const void*)project_td_kernel<J, F, ROWS, SPLIT>;
     const size_t smem = project_td_lds(c, ROWS);
     hipError_t e = allow_lds(kernel, smem);
     if (e != hipSuccess) return e;
@@ -267,20 +313,32 @@ static hipError_t launch_ptd(const Chirp& c, const float* pos, int A, const floa
     long long blocks = (long long)cus * (occ > 0 ? occ : 1) / passes;
     if (blocks < 1) blocks = 1;
     if (blocks > units) blocks = units;
-    hipLaunchKernelGGL((project_td_kernel<J, F, ROWS>), dim3((unsigned)blocks, passes), dim3(kPtThreads), smem, st,
-                       c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax);
+    hipLaunchKernelGGL((project_td_kernel<J, F, ROWS, SPLIT>), dim3((unsigned)blocks, passes), dim3(kPtThreads),
+                       smem, st, c, pos, A, ctr, nrm, areas, S, basis, rows, o);
     return hipGetLastError();
 }
 
-template <int J, bool F>
+template <int J, bool F, bool SPLIT>
 static hipError_t launch_ptd_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                               const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
+                               const float* areas, int S, const float* basis, int rows, const ProjectOut& o,
                                hipStream_t st) {
     if constexpr (J == 8) {
         if (project_td_tile(c, rows) == 128)
-            return launch_ptd<J, F, 128>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
+            return launch_ptd<J, F, 128, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
     }
-    return launch_ptd<J, F, 64>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
+    return launch_ptd<J, F, 64, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+}
+
+template <bool SPLIT>
+static hipError_t launch_ptd_any(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                 const float* areas, int S, const float* basis, int rows, const ProjectOut& o,
+                                 hipStream_t st) {
+    const bool fresnel = c.fresnel_n > 1.0f;
+    if (c.S == 128)
+        return fresnel ? launch_ptd_j<8, true, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st)
+                       : launch_ptd_j<8, false, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+    return fresnel ? launch_ptd_j<16, true, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st)
+                   : launch_ptd_j<16, false, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
 }
 
 hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, float* out, hipStream_t st) {
@@ -292,12 +350,17 @@ hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, flo
 hipError_t launch_project_td(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                              const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
                              hipStream_t st) {
-    const bool fresnel = c.fresnel_n > 1.0f;
-    if (c.S == 128)
-        return fresnel ? launch_ptd_j<8, true>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st)
-                       : launch_ptd_j<8, false>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
-    return fresnel ? launch_ptd_j<16, true>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st)
-                   : launch_ptd_j<16, false>(c, pos, A, ctr, nrm, areas, S, basis, rows, out, colmax, st);
+    ProjectOut o{out, colmax, nullptr, nullptr, 0, 0};
+    return launch_ptd_any<false>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+}
+
+int project_rows_pad(int rows) { return (rows + 31) / 32 * 32; }
+
+hipError_t launch_project_split(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                                const float* areas, int S, const float* basis, int rows, __bf16* hi, __bf16* lo,
+                                long long ld, hipStream_t st) {
+    ProjectOut o{nullptr, nullptr, hi, lo, ld, project_rows_pad(rows)};
+    return launch_ptd_any<true>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
 }
 
 }  // namespace dsplat

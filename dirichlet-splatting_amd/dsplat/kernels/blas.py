@@ -58,6 +58,51 @@ def gemm_tn(rows, col0, out, mode="fp32", column_max=None):
     return out
 
 
+class SplitBlock:
+    """A block P [m, n] in the operand form of syrk_split: BF16 hi and lo [m / 32, ld, 32]
+    (32-row column tiles: element (k, c) at [k // 32, c, k % 32]; ld >= n) with P ~ hi + lo."""
+
+    def __init__(self, hi, lo, m, columns=None):
+        if hi.dtype != torch.bfloat16 or lo.dtype != torch.bfloat16 or hi.shape != lo.shape:
+            raise TypeError("a split block is two bfloat16 [m / 32, ld, 32] tensors")
+        if not (hi.is_contiguous() and lo.is_contiguous()) or hi.ndim != 3 or hi.shape[2] != 32:
+            raise ValueError("split block: contiguous [m / 32, ld, 32] tensors")
+        if m % 32 or hi.shape[0] != m // 32:
+            raise ValueError("split block: m is a multiple of 32 and hi has m / 32 tiles")
+        self.hi, self.lo, self.m = hi, lo, int(m)
+        self.columns = hi.shape[1] if columns is None else int(columns)
+
+    @staticmethod
+    def of(rows):
+        """The split form of a float32 [m, n] matrix (m a multiple of 32)."""
+        m, n = rows.shape
+        t = rows.float().reshape(m // 32, 32, n).transpose(1, 2).contiguous()
+        hi = t.bfloat16()
+        lo = (t - hi.float()).bfloat16()
+        return SplitBlock(hi, lo, m)
+
+    def dense(self):
+        """float32 [m, n] of hi + lo."""
+        t = self.hi[:, : self.columns].float() + self.lo[:, : self.columns].float()
+        return t.transpose(1, 2).reshape(self.m, self.columns)
+
+
+def syrk_split(block, out, col0=None):
+    """Upper triangle of out [n, n] += P^T P (col0 None), or out [n, n - col0] +=
+    P^T P[:, col0:], for a SplitBlock P: hi hi + hi lo + lo hi on the BF16 matrix cores with
+    FP32 accumulation (kernels/cublas.py:31-44 SSYRK; replacement.py:178-199 cross products)."""
+    n = block.columns
+    tail = n if col0 is None else n - col0
+    if out.dtype != torch.float32 or not out.is_contiguous() or out.shape != (n, tail):
+        raise ValueError("out must be a contiguous float32 [n, n - col0] tensor")
+    c0 = -1 if col0 is None else int(col0)
+    size = _lib.library().dsplat_syrk_split_workspace(block.m, n, c0)
+    workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=out.device)
+    _lib.call("dsplat_syrk_split", _lib.ptr(block.hi), _lib.ptr(block.lo), block.hi.shape[1], block.m, n, c0,
+              _lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(out.device))  # fmt: skip
+    return out
+
+
 GRAM_MODES = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
 FP16_TARGET = 2.0**14  # largest scaled atom magnitude in FP16 mode (max finite 65504)
 

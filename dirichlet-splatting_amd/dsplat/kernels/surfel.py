@@ -322,6 +322,27 @@ def project_design(config, positions, centers, normals, areas, basis, fresnel_n,
     return out
 
 
+def project_split(config, positions, centers, normals, areas, sampled, rows, fresnel_n):
+    """The sample-domain projected block of project_design in the operand form of
+    blas.syrk_split: a SplitBlock of m = A * rows_pad rows (rows_pad = rows rounded up to 32;
+    projected row j of aperture a is row a * rows_pad + j, padding rows zero), S columns."""
+    from .blas import SplitBlock
+
+    _check(positions, centers, normals)
+    cfg = _lib.fmcw(config, fresnel_n)
+    pad = _lib.library().dsplat_project_split_rows(rows)
+    m = pad * len(positions)
+    hi = torch.empty((m // 32, len(centers), 32), dtype=torch.bfloat16, device=centers.device)
+    lo = torch.empty_like(hi)
+    _lib.call(
+        "dsplat_project_sampled_split", _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
+        _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()),
+        len(centers), _lib.ptr(sampled.float().contiguous()), rows, _lib.ptr(hi), _lib.ptr(lo), len(centers),
+        _lib.stream(centers.device),
+    )  # fmt: skip
+    return SplitBlock(hi, lo, m)
+
+
 def orient_select(config, positions, centers, normals, areas, bank, residual, fresnel_n, table=None, real_part=False):
     """Best orientation per candidate in one launch: (value [S] float64, index [S] int64),
     value = |s|^2 / max(area |a|^2, 1e-60) with s = sqrt(area) a^H r (or its real part),
