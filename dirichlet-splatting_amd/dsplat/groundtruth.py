@@ -5,28 +5,34 @@ import numpy as np
 import torch
 
 
-def planar_array(side, pitch, axis=(0.0, 0.0, 1.0), standoff=0.35, *, device="cpu"):
-    """side x side aperture positions [side^2, 3] in metres, pitch apart, row-major.
+def _turn_z_onto(axis):
+    """The rotation [3, 3] (float64) that takes +z onto the unit vector axis by the shortest
+    arc: angle acos(z . axis) about z x axis (a half turn about x when axis is -z)."""
+    z = np.array([0.0, 0.0, 1.0])
+    pivot = np.cross(z, axis)
+    sine, cosine = np.linalg.norm(pivot), float(z @ axis)
+    if sine < 1e-10:
+        return np.eye(3) if cosine > 0 else np.diag([1.0, -1.0, -1.0])
+    k = pivot / sine
+    cross_k = np.array([[0.0, -k[2], k[1]], [k[2], 0.0, -k[0]], [-k[1], k[0], 0.0]])
+    return np.eye(3) + sine * cross_k + (1 - cosine) * (cross_k @ cross_k)
 
-    The array faces along axis toward (0, 0, standoff); the default is the plane z = 0
-    facing +z. The rotation maps +z onto axis by the Rodrigues formula.
+
+def planar_array(side, pitch, axis=(0.0, 0.0, 1.0), standoff=0.35, *, device="cpu"):
+    """side x side aperture positions [side^2, 3] in metres, pitch apart, row-major
+    (groundtruth.py:16-37 of the reference).
+
+    The grid is centred on the origin in the plane z = 0 and faces along axis toward the
+    point (0, 0, standoff): the plane turns about that point by the shortest rotation taking
+    +z onto axis (no turn for the default +z).
     """
-    axis = np.asarray(axis, dtype=np.float64)
-    axis = axis / np.linalg.norm(axis)
-    source = np.array([0.0, 0.0, 1.0])
-    cross = np.cross(source, axis)
-    dot = float(source @ axis)
-    if np.linalg.norm(cross) < 1e-10:
-        rotation = np.eye(3) if dot > 0 else np.diag([1.0, -1.0, -1.0])
-    else:
-        x, y, z = cross
-        skew = np.array([[0, -z, y], [z, 0, -x], [-y, x, 0]])
-        rotation = np.eye(3) + skew + skew @ skew / (1 + dot)
-    t = (np.arange(side) - (side - 1) / 2) * pitch
-    gx, gy = np.meshgrid(t, t, indexing="ij")
-    base = np.stack([gx.ravel(), gy.ravel(), np.zeros(side * side)], -1)
-    center = np.array([0.0, 0.0, standoff])
-    return torch.tensor(center + (base - center) @ rotation.T, dtype=torch.float32, device=device)
+    direction = np.asarray(axis, dtype=np.float64)
+    turn = _turn_z_onto(direction / np.linalg.norm(direction))
+    offsets = (np.arange(side) - (side - 1) / 2) * pitch
+    u, v = np.meshgrid(offsets, offsets, indexing="ij")
+    flat = np.column_stack([u.ravel(), v.ravel(), np.zeros(side * side)])
+    pivot = np.array([0.0, 0.0, standoff])
+    return torch.tensor((flat - pivot) @ turn.T + pivot, dtype=torch.float32, device=device)
 
 
 @torch.no_grad()

@@ -1,9 +1,10 @@
-"""Constraints of per-atom parameters: free components, step scales and feasible sets.
+"""Per-atom parameter constraints: which components move, the unit of a solver step, and the
+feasible set a proposal is pulled back into.
 
-Same interface as dsplat/parameters.py of the reference (parameters.py:15-83). A
-parameter is a tensor with one row per atom (centers [S, 3], areas [S]); solver steps
-are dimensionless and go through ParameterConfig: `scale` is the physical length of a
-unit step, `project` restores feasibility, frozen components keep their values.
+Public surface of the reference's dsplat/parameters.py:15-83 (ParameterConfig with mask,
+project, tangent, differential; periodic_delta). A parameter holds one row per atom: a
+vector field [S, D] (centres, normals) or a scalar field [S] (areas), which is handled as
+[S, 1]. Solver steps are dimensionless; `scale` converts one unit into physical length.
 """
 
 from dataclasses import dataclass
@@ -12,73 +13,86 @@ import torch
 import torch.nn.functional as F
 
 
+def _rows(x):
+    """A field as [S, D]: scalar fields gain a trailing axis of one."""
+    return x.unsqueeze(-1) if x.ndim == 1 else x
+
+
+def _like(rows, original):
+    """Undo _rows for the shape of original."""
+    return rows[:, 0] if original.ndim == 1 else rows
+
+
+def _onto_unit_sphere(value, reference, free):
+    """Unit vectors from value [S, D] whose frozen components (free False) are kept: the free
+    components are rescaled to the length 1 - |frozen|^2 leaves them. A free part of zero
+    length takes its direction from reference; where that is zero too, no direction exists."""
+    zero = torch.zeros((), dtype=value.dtype, device=value.device)
+    frozen_part = torch.where(free, zero, value)
+    free_part = value - frozen_part
+    direction = torch.where(free_part.norm(dim=-1, keepdim=True) > 0, free_part, torch.where(free, reference, zero))
+    length = direction.norm(dim=-1, keepdim=True)
+    budget = (1 - frozen_part.square().sum(-1, keepdim=True)).clamp(min=0).sqrt()
+    if bool(((budget > 0) & (length == 0)).any()):
+        raise ValueError("Cannot normalize a zero vector without a reference direction.")
+    return frozen_part + direction * budget / length.clamp(min=1e-30)
+
+
 @dataclass(frozen=True)
 class ParameterConfig:
-    trainable: bool | tuple[bool, ...] = True  # one flag, or one per component
-    scale: float = 1.0
-    lower: float | tuple[float, ...] | None = None
+    trainable: bool | tuple[bool, ...] = True  # one flag for the field, or one per component
+    scale: float = 1.0  # physical length of a unit solver step
+    lower: float | tuple[float, ...] | None = None  # box bounds, one or per component
     upper: float | tuple[float, ...] | None = None
-    period: tuple[float, ...] | None = None
-    normalize: bool = False  # unit vectors, such as surfel normals
+    period: tuple[float, ...] | None = None  # wrap-around per component
+    normalize: bool = False  # the field holds unit vectors (surfel normals)
 
     def mask(self, value):
-        """Boolean trainable mask [D] (D = 1 for a vector parameter)."""
-        flags = torch.as_tensor(self.trainable, dtype=torch.bool, device=value.device)
-        return flags.expand(value.shape[1] if value.ndim == 2 else 1)
+        """bool [D]: the components a step may change (D = 1 for a scalar field)."""
+        width = value.shape[-1] if value.ndim == 2 else 1
+        return torch.as_tensor(self.trainable, dtype=torch.bool, device=value.device).expand(width)
 
     def project(self, proposed, reference):
-        """Feasible values nearest to `proposed`; frozen components come from `reference`.
-
-        With normalize, the free components are rescaled so the vector has unit length,
-        taking the room the frozen components leave; a zero free part takes the
-        reference's direction (parameters.py:29-55).
-        """
-        vector = proposed.ndim == 1
-        value = proposed[:, None] if vector else proposed
+        """The feasible value nearest proposed: wrapped, clamped to the box, frozen components
+        taken from reference, unit length restored for a normalized field."""
+        free = self.mask(reference)
+        value = _rows(proposed)
         if self.period is not None:
-            value = value.remainder(value.new_tensor(self.period))
-        if self.lower is not None:
-            value = torch.maximum(value, value.new_tensor(self.lower))
-        if self.upper is not None:
-            value = torch.minimum(value, value.new_tensor(self.upper))
-        ref = reference[:, None] if vector else reference
-        mask = self.mask(reference)[None]
-        value = torch.where(mask, value, ref)
-        if self.normalize and bool(mask.any()):
-            fixed = torch.where(mask, 0, value)
-            free = value - fixed
-            has_length = free.norm(dim=-1, keepdim=True) > 0
-            free = torch.where(has_length, free, torch.where(mask, ref, 0))
-            room = (1 - fixed.square().sum(-1, keepdim=True)).clamp(min=0).sqrt()
-            length = free.norm(dim=-1, keepdim=True)
-            if bool(((room > 0) & (length == 0)).any()):
-                raise ValueError("Cannot normalize a zero vector without a reference direction.")
-            value = fixed + free * room / length.clamp(min=1e-30)
-        return value[:, 0] if vector else value
+            value = torch.remainder(value, value.new_tensor(self.period))
+        bounds = (self.lower, torch.maximum), (self.upper, torch.minimum)
+        for bound, pick in bounds:
+            if bound is not None:
+                value = pick(value, value.new_tensor(bound))
+        value = torch.where(free, value, _rows(reference))
+        if self.normalize and bool(free.any()):
+            value = _onto_unit_sphere(value, _rows(reference), free)
+        return _like(value, proposed)
 
     def tangent(self, vector, value):
-        """The part of an update [..., S, D] that project keeps at a unit vector `value`:
-        frozen components drop out, and so does the radial part within the free ones
-        (parameters.py:57-70)."""
+        """The part of an update [..., S, D] that project keeps at the unit vectors value [S, D]:
+        no frozen component, and no radial part within the free components (whose squared
+        length is 1 minus that of the frozen ones). Fields that are not normalized pass."""
         if not self.normalize:
             return vector
-        mask = self.mask(value)
-        free = torch.where(mask, value, 0)
-        room = (1 - torch.where(mask, 0, value).square().sum(-1, keepdim=True)).clamp(min=0)
-        vector = torch.where(mask, vector, 0)
-        radial = (vector * free).sum(-1, keepdim=True) * free / room.clamp(min=1e-30)
-        return torch.where(room > 0, vector - radial, 0)
+        free = self.mask(value)
+        zero = torch.zeros((), dtype=value.dtype, device=value.device)
+        moving = torch.where(free, value, zero)
+        budget = (1 - torch.where(free, zero, value).square().sum(-1, keepdim=True)).clamp(min=0)
+        step = torch.where(free, vector, zero)
+        along = (step * moving).sum(-1, keepdim=True) * moving / budget.clamp(min=1e-30)
+        return torch.where(budget > 0, step - along, zero)
 
     def differential(self, jacobian, value):
-        """Chain a dictionary derivative [..., S, D] through the retraction, mask and scale."""
+        """A dictionary derivative [..., S, D] chained through the retraction (unit fields),
+        restricted to the free components and scaled to solver units."""
         if self.normalize:
             jacobian = self.tangent(jacobian, F.normalize(value, dim=-1))
-        return jacobian[..., self.mask(value)] * self.scale
+        return self.scale * jacobian[..., self.mask(value)]
 
 
 def periodic_delta(delta, period):
-    """Differences folded into [-period/2, period/2] per axis; None leaves them unchanged."""
+    """Differences folded per axis into [-period / 2, period / 2]; period None leaves them."""
     if period is None:
         return delta
-    period = torch.as_tensor(period, dtype=delta.dtype, device=delta.device)
-    return delta - period * torch.round(delta / period)
+    span = torch.as_tensor(period, dtype=delta.dtype, device=delta.device)
+    return delta - span * torch.round(delta / span)

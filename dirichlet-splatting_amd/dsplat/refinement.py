@@ -1,9 +1,16 @@
-"""Local geometry refinement: dense reduced-VarPro LM and matrix-free joint Gauss-Newton.
+"""Local geometry refinement: dense reduced-VarPro Levenberg-Marquardt and matrix-free joint
+Gauss-Newton.
 
-Same functions as dsplat/refinement.py of the reference (refinement.py:13-227). Both
-propose moves of every trainable atom and keep one only if its refitted objective is
-lower; steps are dimensionless in units of each ParameterConfig.scale. The joint
-system's products are the model's JVP and VJP kernels.
+Public surface of the reference's dsplat/refinement.py:13-227 (trainable, reduced_system,
+dense_slide, joint_system, joint_slide). Both refinements propose a move of every trainable
+atom and keep it only if the refitted objective is lower; steps are dimensionless, in units
+of each ParameterConfig.scale.
+
+  dense  the Gauss-Newton normal matrix of the reduced objective (coefficients eliminated
+         exactly) on sampled observation rows, solved densely with diagonal damping
+  joint  geometry and coefficients together: J^H J as a JVP followed by a VJP, solved by
+         Jacobi-preconditioned CG on an aperture subset; the diagonal comes from Rademacher
+         probes
 """
 
 import torch
@@ -12,13 +19,30 @@ from .varpro import accept, fit_coefficients, pcg, screen_bar, solve
 
 
 def trainable(params, specs):
+    """(key, spec) of every field with at least one free component."""
     return [(key, spec) for key, spec in specs.items() if spec.mask(params[key]).any()]
+
+
+def _moved(params, layout, delta, fraction):
+    """params with each trainable field moved by fraction * scale * its section of delta
+    [S, free components] and projected back onto its constraints."""
+    proposal = {key: value.clone() for key, value in params.items()}
+    column = 0
+    for key, spec in layout:
+        value = proposal[key]
+        grid = value[:, None] if value.ndim == 1 else value.clone()
+        free = spec.mask(value)
+        width = int(free.sum())
+        grid[:, free] += fraction * spec.scale * delta[:, column : column + width]
+        proposal[key] = spec.project(grid[:, 0] if value.ndim == 1 else grid, params[key])
+        column += width
+    return proposal
 
 
 @torch.no_grad()
 def reduced_system(model, params, target, specs, config, rows):
-    """Gauss-Newton normal matrix and gradient of the ridge-VarPro objective on some rows,
-    with the exact coefficient differential eliminated (refinement.py:20-52)."""
+    """Gauss-Newton normal matrix and gradient of the ridge-VarPro objective on the given rows,
+    the coefficients' exact differential eliminated (refinement.py:20-52)."""
     a = model.design(params, rows)
     y = target.flatten()[rows]
     m = len(a)
@@ -28,22 +52,22 @@ def reduced_system(model, params, target, specs, config, rows):
     layout = trainable(params, specs)
     atoms = torch.arange(len(b), device=a.device)
     raw = model.jacobian(params, rows, atoms, [key for key, _ in layout])
-    da = torch.cat([spec.differential(raw[key], params[key]) for key, spec in layout], -1)
+    d_atoms = torch.cat([spec.differential(raw[key], params[key]) for key, spec in layout], -1)
     h = (gram.real if config.real else gram) + config.ridge * torch.eye(len(b), dtype=b.dtype, device=a.device)
-    j = (da * b[None, :, None]).reshape(m, -1)
+    j = (d_atoms * b[None, :, None]).reshape(m, -1)
     cross = a.H @ j / m
     columns = torch.arange(j.shape[1], device=a.device)
-    cross[atoms.repeat_interleave(da.shape[-1]), columns] += (da.conj() * residual[:, None, None]).sum(0).flatten() / m
-    db = -solve(h, cross.real if config.real else cross)
-    reduced = j + a @ db.to(a.dtype)
-    normal = (reduced.H @ reduced / m + config.ridge * db.H @ db).real
-    gradient = (reduced.H @ residual / m + config.ridge * db.H @ b).real
+    cross[atoms.repeat_interleave(d_atoms.shape[-1]), columns] += (d_atoms.conj() * residual[:, None, None]).sum(0).flatten() / m
+    d_b = -solve(h, cross.real if config.real else cross)
+    reduced = j + a @ d_b.to(a.dtype)
+    normal = (reduced.H @ reduced / m + config.ridge * d_b.H @ d_b).real
+    gradient = (reduced.H @ residual / m + config.ridge * d_b.H @ b).real
     return normal, gradient
 
 
 def dense_slide(state, config, screen=None):
-    """Damped LM steps on lm_rows sampled rows with a per-atom trust radius
-    (refinement.py:55-98); the state and the number of accepted steps."""
+    """Damped LM steps on lm_rows sampled rows with a per-atom trust radius and halving
+    backtracks (refinement.py:55-98); returns the state and the accepted steps."""
     layout = trainable(state.params, state.parameters)
     rows = state.model.sample_rows(config.lm_rows, config.seed + state.iteration, state.target.device)
     accepted = 0
@@ -56,17 +80,7 @@ def dense_slide(state, config, screen=None):
             break
         bar = screen_bar(screen, state.params, config)
         for fraction in (1.0, 0.5, 0.25):
-            proposal = {key: value.clone() for key, value in state.params.items()}
-            offset = 0
-            for key, spec in layout:
-                value = proposal[key]
-                matrix = value[:, None] if value.ndim == 1 else value.clone()
-                mask = spec.mask(value)
-                width = int(mask.sum())
-                matrix[:, mask] += fraction * spec.scale * delta[:, offset : offset + width]
-                proposal[key] = spec.project(matrix[:, 0] if value.ndim == 1 else matrix, state.params[key])
-                offset += width
-            trial = accept(state, proposal, config, bar)
+            trial = accept(state, _moved(state.params, layout, delta, fraction), config, bar)
             if trial is not None:
                 state, accepted = trial, accepted + 1
                 break
@@ -75,100 +89,120 @@ def dense_slide(state, config, screen=None):
     return state, accepted
 
 
-def joint_system(model, params, coefficients, target, specs, config, seed):
-    """Scaled Gauss-Newton system of all trainable geometry and every coefficient:
-    matvec (J^H J by JVP then VJP), gradient, damped Jacobi diagonal (Rademacher probes)
-    and proposal(vector, fraction) (refinement.py:104-190)."""
-    b = coefficients.to(target.dtype)
-    complex_mode = not config.real
-    count = len(b)
-    layout, offset = [], 0
-    for key, spec in trainable(params, specs):
-        mask = spec.mask(params[key])
-        width = count * int(mask.sum())
-        layout.append((key, mask, slice(offset, offset + width)))
-        offset += width
-    geometry = offset
-    size = geometry + (2 * count if complex_mode else count)
-    amplitude = b.abs()
-    coefficient_scale = amplitude.clamp(min=amplitude.square().mean().sqrt() * 0.1 + 1e-20)
-    power = target.abs().square().mean().clamp(min=1e-30)
-    normalization = target.numel() * power
-    ridge = config.ridge / power
-    ridge_diagonal = amplitude.new_zeros(size)
-    ridge_gradient = amplitude.new_zeros(size)
-    if complex_mode:
-        ridge_diagonal[geometry:] = (ridge * coefficient_scale.square()).repeat_interleave(2)
-        ridge_gradient[geometry:] = torch.view_as_real(b * (ridge * coefficient_scale)).flatten()
-    else:
-        ridge_diagonal[geometry:] = ridge * coefficient_scale.square()
-        ridge_gradient[geometry:] = ridge * coefficient_scale * b.real
+class _JointSystem:
+    """The scaled Gauss-Newton system of every trainable geometry component and every
+    coefficient on one model (refinement.py:104-190 of the reference).
 
-    def unpack(vector):
-        tangent = {key: torch.zeros_like(value) for key, value in params.items()}
-        for key, mask, section in layout:
+    Unknowns are dimensionless: geometry in units of each spec's scale, coefficients relative
+    to max(|b|, 0.1 RMS |b|); the data term is normalized by M |y|^2 / M (the target power)."""
+
+    def __init__(self, model, params, coefficients, target, specs, config, seed):
+        self.model, self.params, self.specs, self.config = model, params, specs, config
+        self.b = coefficients.to(target.dtype)
+        self.complex_mode = not config.real
+        count = len(self.b)
+        self.count = count
+        self.sections, offset = [], 0
+        for key, spec in trainable(params, specs):
+            free = spec.mask(params[key])
+            width = count * int(free.sum())
+            self.sections.append((key, free, slice(offset, offset + width)))
+            offset += width
+        self.geometry = offset
+        self.size = offset + (2 * count if self.complex_mode else count)
+        self.amplitude = self.b.abs()
+        self.coefficient_scale = self.amplitude.clamp(min=self.amplitude.square().mean().sqrt() * 0.1 + 1e-20)
+        power = target.abs().square().mean().clamp(min=1e-30)
+        self.normalization = target.numel() * power
+        ridge = config.ridge / power
+        self.ridge_diagonal = self.amplitude.new_zeros(self.size)
+        self.ridge_gradient = self.amplitude.new_zeros(self.size)
+        tail = slice(self.geometry, None)
+        if self.complex_mode:
+            self.ridge_diagonal[tail] = (ridge * self.coefficient_scale.square()).repeat_interleave(2)
+            self.ridge_gradient[tail] = torch.view_as_real(self.b * (ridge * self.coefficient_scale)).flatten()
+        else:
+            self.ridge_diagonal[tail] = ridge * self.coefficient_scale.square()
+            self.ridge_gradient[tail] = ridge * self.coefficient_scale * self.b.real
+        self.gradient = self.adjoint(model.render(params, self.b) - target) / self.normalization + self.ridge_gradient
+        self.diagonal = self._probe_diagonal(target, seed)
+        self.damping = config.damping * self.diagonal
+
+    def _probe_diagonal(self, target, seed):
+        """Jacobi diagonal of J^H J from cg_probes Rademacher probes (complex, unit modulus
+        parts), plus the ridge, floored at 1e-6 of its largest entry."""
+        generator = torch.Generator(device=self.b.device).manual_seed(seed)
+        total = torch.zeros_like(self.gradient)
+        for _ in range(self.config.cg_probes):
+            signs = torch.randint(0, 2, (*target.shape, 2), device=self.b.device, generator=generator)
+            probe = torch.view_as_complex((2 * signs - 1).to(target.real.dtype))
+            total += self.adjoint(probe).square() / self.normalization
+        diagonal = total / self.config.cg_probes + self.ridge_diagonal
+        return diagonal.clamp(min=diagonal.max().clamp(min=1e-30) * 1e-6)
+
+    def unpack(self, vector):
+        """A dimensionless vector as (parameter tangents, coefficient change)."""
+        tangent = {key: torch.zeros_like(value) for key, value in self.params.items()}
+        for key, free, section in self.sections:
             value = tangent[key]
-            matrix = value[:, None] if value.ndim == 1 else value
-            matrix[:, mask] = vector[section].reshape(count, -1) * specs[key].scale
-            tangent[key] = specs[key].tangent(value, params[key])
-        tail = vector[geometry:]
-        if complex_mode:
-            return tangent, torch.complex(tail[0::2], tail[1::2]) * coefficient_scale
-        return tangent, (tail * coefficient_scale).to(b.dtype)
+            grid = value[:, None] if value.ndim == 1 else value
+            grid[:, free] = vector[section].reshape(self.count, -1) * self.specs[key].scale
+            tangent[key] = self.specs[key].tangent(value, self.params[key])
+        tail = vector[self.geometry :]
+        if self.complex_mode:
+            return tangent, torch.complex(tail[0::2], tail[1::2]) * self.coefficient_scale
+        return tangent, (tail * self.coefficient_scale).to(self.b.dtype)
 
-    def adjoint(upstream):
-        grads, coefficient = model.vjp(params, b, upstream)
-        out = amplitude.new_zeros(size)
-        for key, mask, section in layout:
-            value = specs[key].tangent(grads[key], params[key])
-            matrix = value[:, None] if value.ndim == 1 else value
-            out[section] = (matrix[:, mask] * specs[key].scale).flatten()
-        tail = coefficient * coefficient_scale
-        out[geometry:] = torch.view_as_real(tail).flatten() if complex_mode else tail.real
+    def adjoint(self, upstream):
+        """J^H upstream in the dimensionless unknowns (the model's VJP, chained)."""
+        grads, by_coefficient = self.model.vjp(self.params, self.b, upstream)
+        out = self.amplitude.new_zeros(self.size)
+        for key, free, section in self.sections:
+            value = self.specs[key].tangent(grads[key], self.params[key])
+            grid = value[:, None] if value.ndim == 1 else value
+            out[section] = (grid[:, free] * self.specs[key].scale).flatten()
+        tail = by_coefficient * self.coefficient_scale
+        out[self.geometry :] = torch.view_as_real(tail).flatten() if self.complex_mode else tail.real
         return out
 
-    gradient = adjoint(model.render(params, b) - target) / normalization + ridge_gradient
-    generator = torch.Generator(device=b.device).manual_seed(seed)
-    diagonal = torch.zeros_like(gradient)
-    for _ in range(config.cg_probes):
-        signs = torch.randint(0, 2, (*target.shape, 2), device=b.device, generator=generator)
-        probe = torch.view_as_complex((2 * signs - 1).to(target.real.dtype))
-        diagonal += adjoint(probe).square() / normalization
-    diagonal = diagonal / config.cg_probes + ridge_diagonal
-    diagonal = diagonal.clamp(min=diagonal.max().clamp(min=1e-30) * 1e-6)
-    damping = config.damping * diagonal
+    def matvec(self, vector):
+        """(J^H J / normalization + ridge + damping) vector: the model's JVP, then its VJP."""
+        return (self.adjoint(self.model.jvp(self.params, self.b, *self.unpack(vector))) / self.normalization
+                + (self.ridge_diagonal + self.damping) * vector)  # fmt: skip
 
-    def matvec(vector):
-        return adjoint(model.jvp(params, b, *unpack(vector))) / normalization + (ridge_diagonal + damping) * vector
+    def proposal(self, vector, fraction):
+        """The geometry moved by fraction of vector, the whole move first shortened so that no
+        atom leaves the trust radius, then projected onto the constraints."""
+        tangent, _ = self.unpack(vector)
+        length2 = self.amplitude.new_zeros(self.count)
+        for key, free, _ in self.sections:
+            grid = tangent[key][:, None] if tangent[key].ndim == 1 else tangent[key]
+            length2 += (grid[:, free] / self.specs[key].scale).square().sum(1)
+        shrink = (self.config.trust_radius / length2.sqrt().max().clamp(min=1e-30)).clamp(max=1)
+        return {key: self.specs[key].project(value + tangent[key] * (shrink * fraction), value)
+                for key, value in self.params.items()}  # fmt: skip
 
-    def proposal(vector, fraction):
-        tangent, _ = unpack(vector)
-        lengths = amplitude.new_zeros(count)
-        for key, mask, _ in layout:
-            matrix = tangent[key][:, None] if tangent[key].ndim == 1 else tangent[key]
-            lengths += (matrix[:, mask] / specs[key].scale).square().sum(1)
-        scale = (config.trust_radius / lengths.sqrt().max().clamp(min=1e-30)).clamp(max=1)
-        return {
-            key: specs[key].project(value + tangent[key] * (scale * fraction), value) for key, value in params.items()
-        }
 
-    return matvec, gradient, diagonal + damping, proposal
+def joint_system(model, params, coefficients, target, specs, config, seed):
+    """(matvec, gradient, damped Jacobi diagonal, proposal(vector, fraction)) of the scaled
+    joint Gauss-Newton system (refinement.py:104-190)."""
+    system = _JointSystem(model, params, coefficients, target, specs, config, seed)
+    return system.matvec, system.gradient, system.diagonal + system.damping, system.proposal
 
 
 @torch.no_grad()
 def joint_slide(state, config, screen=None):
-    """PCG steps of the joint system on joint_apertures sampled apertures; acceptance
-    uses every observation (refinement.py:193-227)."""
+    """PCG steps of the joint system on joint_apertures sampled apertures, each accepted on
+    every observation with halving backtracks (refinement.py:193-227)."""
     model, target = state.model, state.target
     if 0 < config.joint_apertures < model.apertures:
         ids, model = model.sample_apertures(config.joint_apertures, config.seed + state.iteration)
         target = target[ids]
     accepted = 0
     for inner in range(config.lm_steps):
-        matvec, gradient, diagonal, proposal = joint_system(
-            model, state.params, state.coefficients, target, state.parameters, config,
-            config.seed + state.iteration + inner,
-        )  # fmt: skip
+        matvec, gradient, diagonal, proposal = joint_system(model, state.params, state.coefficients, target,
+                                                            state.parameters, config,
+                                                            config.seed + state.iteration + inner)  # fmt: skip
         delta = pcg(matvec, -gradient, diagonal, iterations=config.cg_iterations, tolerance=config.cg_tolerance)
         if not (torch.isfinite(delta).all() and delta @ gradient < 0):
             break

@@ -1,8 +1,13 @@
 """Dirichlet atoms on a spectral grid: the signal-level measurement model.
 
-Same interface as dsplat/spectral.py of the reference (spectral.py:16-120). An atom
-has a center [D] in bin units and a complex coefficient; its spectrum is the
-separable windowed Dirichlet kernel of every axis observed at every grid bin.
+Public surface of the reference's dsplat/spectral.py:16-120 (SpectralGrid, SpectralModel).
+An atom has a centre [D] in bin units and a complex coefficient; its spectrum is the
+separable windowed Dirichlet kernel of every axis, observed at every bin of the grid.
+
+The bin lattice of a grid is made once per device and kept resident. Correlations a^H r
+run as the spectral VJP kernel (the coefficient gradient of Re<r, A b> is A^H r), so no
+dictionary block is materialized for them; atoms that sit exactly on the grid bins use the
+FFT shortcut instead (the correlation is a circular cross-correlation then).
 """
 
 from dataclasses import dataclass
@@ -12,14 +17,25 @@ import torch
 
 from .kernels import spectral as kernel
 
+_LATTICES: dict = {}
+
+
+def _lattice(shape, device):
+    """Every bin's coordinates [size, D] (row-major), float32, cached per (shape, device)."""
+    key = (tuple(shape), str(torch.device(device)))
+    if key not in _LATTICES:
+        axes = torch.meshgrid(*(torch.arange(n, device=device, dtype=torch.float32) for n in shape), indexing="ij")
+        _LATTICES[key] = torch.stack(axes, -1).reshape(-1, len(shape))
+    return _LATTICES[key]
+
 
 @dataclass(frozen=True)
 class SpectralGrid:
-    shape: tuple[int, ...]  # FFT length per axis, one to three axes
-    samples: tuple[int, ...] | None = None  # observed samples per axis; None: the FFT length
+    shape: tuple[int, ...]  # FFT length per axis (one to three axes)
+    samples: tuple[int, ...] | None = None  # observed samples per axis; None means the FFT length
     window: str = "rect"  # rect, hann, hamming or kaiser
-    beta: float = 14.0  # Kaiser shape
-    radius: float = 0.0  # support in bins; 0 is the exact, untruncated response
+    beta: float = 14.0  # Kaiser shape parameter
+    radius: float = 0.0  # support in bins; 0 keeps the exact, untruncated response
 
     def __post_init__(self):
         if self.samples is None:
@@ -31,15 +47,18 @@ class SpectralGrid:
 
     def bins(self, device="cuda"):
         """Every bin's coordinates [size, D], row-major."""
-        axes = [torch.arange(n, device=device, dtype=torch.float32) for n in self.shape]
-        return torch.stack(torch.meshgrid(*axes, indexing="ij"), -1).reshape(-1, len(axes))
+        return _lattice(self.shape, device).clone()
+
+
+# Dictionary entries per streamed block of the norm pass of truncated atoms.
+_NORM_BLOCK = 2**24
 
 
 @dataclass(frozen=True)
 class SpectralModel:
-    """Atoms with centers [S, D] on one grid: render, jvp, vjp, design, jacobian,
-    streamed blocks, correlations and row sampling (spectral.py:37-120). It has no
-    apertures, so aperture-subset options never apply."""
+    """Atoms with centres [S, D] on one grid: render, jvp, vjp, design, jacobian, streamed
+    blocks, correlations and row sampling (spectral.py:37-120 of the reference). A grid has no
+    apertures, so the solver's aperture-subset options never apply to it."""
 
     grid: SpectralGrid
     row_alignment = 1
@@ -56,61 +75,72 @@ class SpectralModel:
     def prepare(self, specs):
         return self
 
+    def _bins(self, params):
+        return _lattice(self.grid.shape, params["centers"].device)
+
     def design(self, params, rows):
-        centers = params["centers"]
-        return kernel.design(self.grid, centers, self.grid.bins(centers.device)[rows])
+        return kernel.design(self.grid, params["centers"], self._bins(params)[rows])
 
     def jacobian(self, params, rows, atoms, keys):
-        centers = params["centers"]
-        bins = self.grid.bins(centers.device)[rows]
-        return {"centers": kernel.design(self.grid, centers[atoms], bins, derivatives=True)[1]}
+        _, derivative = kernel.design(self.grid, params["centers"][atoms], self._bins(params)[rows], derivatives=True)
+        return {"centers": derivative}
 
     def render(self, params, coefficients):
-        centers = params["centers"]
-        bins = self.grid.bins(centers.device)
-        return kernel.render(self.grid, centers, coefficients, bins).reshape(self.shape)
+        return kernel.render(self.grid, params["centers"], coefficients, self._bins(params)).reshape(self.shape)
 
     def jvp(self, params, coefficients, tangent, delta_coefficients):
-        centers = params["centers"]
-        bins = self.grid.bins(centers.device)
-        out = kernel.jvp(self.grid, centers, coefficients, tangent["centers"], delta_coefficients, bins)
-        return out.reshape(self.shape)
+        bins = self._bins(params)
+        flat = kernel.jvp(self.grid, params["centers"], coefficients, tangent["centers"], delta_coefficients, bins)
+        return flat.reshape(self.shape)
 
     def vjp(self, params, coefficients, upstream):
-        centers = params["centers"]
-        bins = self.grid.bins(centers.device)
-        gc, gb = kernel.vjp(self.grid, centers, coefficients, upstream.flatten(), bins)
-        return {"centers": gc}, gb
+        by_center, by_coefficient = kernel.vjp(self.grid, params["centers"], coefficients, upstream.flatten(),
+                                               self._bins(params))  # fmt: skip
+        return {"centers": by_center}, by_coefficient
+
+    def _norms(self, centers):
+        """|a_s|^2 per atom. Untruncated atoms observed on the whole grid all have the same
+        energy (Parseval over the periodic kernel), taken from one atom; truncated ones are
+        summed over streamed dictionary blocks."""
+        bins = _lattice(self.grid.shape, centers.device)
+        if self.grid.radius <= 0:
+            one = kernel.design(self.grid, centers[:1], bins)
+            return one.abs().square().sum().expand(len(centers)).clone()
+        norms = centers.new_zeros(len(centers))
+        step = max(1, _NORM_BLOCK // max(1, len(centers)))
+        for first in range(0, self.size, step):
+            norms += kernel.design(self.grid, centers, bins[first : first + step]).abs().square().sum(0)
+        return norms
 
     def correlate(self, params, residual):
-        """a^H r and |a|^2 for every atom; an FFT correlation when the atoms are exactly
-        the grid bins (spectral.py:90-106), else streamed dictionary blocks."""
+        """a^H r and |a|^2 for every atom (spectral.py:90-106 of the reference)."""
         centers = params["centers"]
-        bins = self.grid.bins(centers.device)
+        bins = self._bins(params)
+        flat = residual.reshape(-1)
         if centers.shape == bins.shape and torch.equal(centers, bins):
-            atom = kernel.design(self.grid, torch.zeros_like(centers[:1]), bins).reshape(self.shape)
-            spectrum = torch.fft.fftn(residual.reshape(self.shape)) * torch.fft.fftn(atom).conj()
-            norm = atom.abs().square().sum().expand(len(centers))
-            return torch.fft.ifftn(spectrum).flatten(), norm
-        corr = residual.new_zeros(len(centers))
-        norm = corr.real.clone()
-        chunk = max(1, 2**24 // len(centers))
-        flat = residual.flatten()
-        for start in range(0, self.size, chunk):
-            a = self.design(params, slice(start, start + chunk))
-            corr += a.H @ flat[start : start + chunk]
-            norm += a.abs().square().sum(0)
-        return corr, norm
+            # atoms on every bin: corr[c] = sum_k conj(atom(k - c)) r(k), a circular
+            # cross-correlation with the atom at the origin
+            origin = kernel.design(self.grid, torch.zeros_like(centers[:1]), bins).reshape(self.shape)
+            spectrum = torch.fft.fftn(flat.reshape(self.shape)) * torch.fft.fftn(origin).conj()
+            energy = origin.abs().square().sum()
+            return torch.fft.ifftn(spectrum).reshape(-1), energy.expand(len(centers))
+        # the coefficient gradient of Re<r, A b> is A^H r, whatever b is
+        _, corr = kernel.vjp(self.grid, centers, torch.ones(len(centers), dtype=flat.dtype, device=flat.device),
+                             flat, bins)  # fmt: skip
+        return corr, self._norms(centers)
 
     def blocks(self, params, target, real, chunk):
-        flat = target.flatten()
-        for start in range(0, self.size, chunk):
-            rows = slice(start, start + chunk)
+        """Least-squares blocks (design rows, observations) of `chunk` bins; the real form
+        stacks the real and imaginary rows of each bin."""
+        flat = target.reshape(-1)
+        for first in range(0, self.size, chunk):
+            rows = slice(first, first + chunk)
             a, y = self.design(params, rows), flat[rows]
             if real:
-                a = torch.view_as_real(a).permute(0, 2, 1).reshape(-1, a.shape[1])
+                a = torch.view_as_real(a).transpose(1, 2).reshape(-1, a.shape[1])
                 y = torch.view_as_real(y).reshape(-1)
             yield a, y
 
     def sample_rows(self, count, seed, device):
+        """Evenly spaced bins (the grid is deterministic; seed is unused)."""
         return torch.linspace(0, self.size - 1, min(count, self.size), device=device).long()

@@ -1,15 +1,23 @@
-"""Variable projection: streamed normal equations, coefficient elimination and acceptance.
+"""Variable projection: the normal equations of one geometry, the coefficient solve, and the
+acceptance test of a proposed geometry.
 
-Objective mean(|A b - y|^2) + ridge |b|^2 (varpro.py:1-6 of the reference). Complex
-coefficients use the full Gram G = A^H A / M, signed-real ones Re(G) and Re(q). A new
-geometry is accepted only if its refitted objective is lower.
+Public surface of the reference's dsplat/varpro.py:17-228 (Fit, largest_eigenvalue,
+shifted_cholesky, solve, row_chunk, normal_equations, fit_coefficients, rendered_loss,
+varpro, exchange_varpro, screen_bar, accept, pcg). The objective is
+mean(|A b - y|^2) + ridge |b|^2; complex coefficients use G = A^H A / M, signed real ones
+Re(G) and Re(q). A geometry is accepted only when its refitted, freshly rendered objective is
+lower.
 
-Gram backends (DSFWConfig.gram) of the real-coefficient normal equations: the row
-stream of the reference (row-projected design blocks, a SYRK per block, folded into an
-FP64 total every FOLD blocks; varpro.py:82-120) with the SYRK on
-  "plain"   FP32 MFMA operands (the reference's cuBLAS SSYRK)
-  "bf16x3"  BF16 hi + lo split operands, three MFMA products, FP32 accumulation
-  "fp16x3"  FP16 hi + lo split operands with per-column scaling, likewise
+The real normal equations run on the MFMA products of kernels.blas, chosen by DSFWConfig.gram:
+
+  plain   float32 row-projected blocks (design, then the range subspace), FP32 SYRK: the
+          reference's stream over cuBLAS SSYRK
+  bf16x3  with the batched search on a sample-domain chirp: blocks written by the projection
+          kernel itself as BF16 hi / lo operands, one split SYRK per block (hi hi + hi lo +
+          lo hi); otherwise the float32 blocks split inside the SYRK
+  fp16x3  float32 blocks with per-column power-of-two scales, FP16 hi / lo inside the SYRK
+
+Block products accumulate in float32 and fold into a float64 total every FOLD blocks.
 """
 
 from dataclasses import replace
@@ -17,13 +25,13 @@ from typing import NamedTuple
 
 import torch
 
-from .kernels.blas import syrk
+from .kernels.blas import gemm_tn, syrk, syrk_split
 from .kernels.reduce import energy
 from .kernels.solver import cholesky_ex
 
 
 class Fit(NamedTuple):
-    """VarPro solution of one geometry; the field names match DSFWState."""
+    """The VarPro solution of one geometry; the field names are DSFWState's."""
 
     coefficients: torch.Tensor
     prediction: torch.Tensor
@@ -32,26 +40,34 @@ class Fit(NamedTuple):
     rhs: torch.Tensor
 
 
-POWER_STEPS = 24
-SHIFT_RETRIES = 4
-FOLD = 8  # float32 SYRK blocks per float64 fold of the Gram
+POWER_STEPS = 24  # power iterations of the largest-eigenvalue estimate
+SHIFT_RETRIES = 4  # diagonal shifts tried before a system counts as empty
+FOLD = 8  # float32 block products summed before each float64 fold
+
+SYRK_MODE = {"plain": "fp32", "fp32": "fp32", "bf16x3": "bf16x3", "fp16x3": "fp16x3"}
+
+
+# --- Solves --------------------------------------------------------------------------------
 
 
 def largest_eigenvalue(h):
-    """Power-iteration estimate of lambda_max of a PSD h from a fixed, reproducible start."""
-    golden = 0.6180339887
-    v = ((torch.arange(len(h), device=h.device, dtype=torch.float64) * golden) % 1 + 0.5).to(h.dtype)
-    top = v.norm()
+    """Power-iteration estimate of lambda_max of a positive semidefinite h, from a fixed
+    non-constant start (fractional parts of multiples of the golden ratio, plus 1/2), so that
+    the estimate is reproducible."""
+    index = torch.arange(len(h), device=h.device, dtype=torch.float64)
+    vector = (torch.remainder(index * 0.6180339887, 1) + 0.5).to(h.dtype)
+    top = vector.norm()
     for _ in range(POWER_STEPS):
-        v = h @ v
-        top = v.norm()
-        v = v / top.clamp(min=torch.finfo(top.dtype).tiny)
+        vector = h @ vector
+        top = vector.norm()
+        vector = vector / top.clamp(min=torch.finfo(top.dtype).tiny)
     return top
 
 
 def shifted_cholesky(h, floor):
-    """FP64 Cholesky factor of h + shift I, shift = floor lambda_max(h), growing 1, 10, 91...
-    times while factorization fails; None after SHIFT_RETRIES failures (varpro.py:44-59)."""
+    """Cholesky factor of h + shift I in double precision, or None. The shift starts at floor
+    lambda_max(h) and is added again, nine times larger each time, while the factorization
+    fails (1, 10, 91, ... times the first), at most SHIFT_RETRIES times."""
     matrix = h.to(torch.complex128 if h.is_complex() else torch.float64, copy=True)
     shift = floor * largest_eigenvalue(matrix)
     for _ in range(SHIFT_RETRIES):
@@ -64,79 +80,124 @@ def shifted_cholesky(h, floor):
 
 
 def solve(h, rhs):
-    """Hermitian solve; a singular system is shifted to the level where the pseudoinverse
-    truncates, len(h) eps lambda_max, which moves x only along near-null directions
-    (varpro.py:62-79). An all-zero system gives zeros."""
-    q = rhs[:, None] if rhs.ndim == 1 else rhs
+    """Hermitian solve h x = rhs. A singular system (a column no aperture sees, two atoms in
+    one place) fails the first Cholesky and is shifted to the level where a pseudoinverse
+    truncates, len(h) eps lambda_max: that moves x only along near-null directions, which do
+    not change the prediction. An all-zero system gives zeros."""
+    columns = rhs[:, None] if rhs.ndim == 1 else rhs
     factor, info = cholesky_ex(h)
     if int(info) != 0:
         factor = shifted_cholesky(h, len(h) * torch.finfo(h.real.dtype).eps)
         if factor is None:
             return torch.zeros_like(rhs)
-    x = torch.cholesky_solve(q.to(factor.dtype), factor).to(q.dtype)
+    x = torch.cholesky_solve(columns.to(factor.dtype), factor).to(columns.dtype)
     return x[:, 0] if rhs.ndim == 1 else x
 
 
+def fit_coefficients(gram, rhs, config):
+    """The minimizer of b^H G b - 2 Re(b^H q) + ridge |b|^2 (real b for real coefficients)."""
+    if config.real:
+        gram, rhs = gram.real, rhs.real
+    return solve(gram + config.ridge * torch.eye(len(gram), dtype=gram.dtype, device=gram.device), rhs)
+
+
+# --- Normal equations ----------------------------------------------------------------------
+
+
 def row_chunk(model, columns, config):
-    """Rows per streamed block: at most dictionary_elements / columns, whole apertures."""
-    chunk = min(config.row_chunk, max(1, config.dictionary_elements // columns))
-    return max(model.row_alignment, chunk - chunk % model.row_alignment)
+    """Observation rows per streamed block: at most dictionary_elements / columns and
+    row_chunk, whole apertures."""
+    rows = min(config.row_chunk, max(1, config.dictionary_elements // columns))
+    return max(model.row_alignment, rows - rows % model.row_alignment)
 
 
-SYRK_MODE = {"plain": "fp32", "fp32": "fp32", "bf16x3": "bf16x3", "fp16x3": "fp16x3"}
+class _Folded:
+    """A float32 accumulator [rows, cols] that folds into a float64 total every FOLD blocks."""
+
+    def __init__(self, rows, cols, device):
+        self.part = torch.zeros((rows, cols), dtype=torch.float32, device=device)
+        self.total = torch.zeros((rows, cols), dtype=torch.float64, device=device)
+        self.blocks = 0
+
+    def added(self):
+        """Call after each block's product went into self.part."""
+        self.blocks += 1
+        if self.blocks % FOLD == 0:
+            self.total += self.part
+            self.part.zero_()
+
+    def result(self):
+        self.total += self.part
+        self.part.zero_()
+        return self.total
+
+
+def block_products(model, params, target, config, out, col0=None):
+    """Accumulate the real block products of the streamed design into `out` (a _Folded of
+    [S, S] upper triangle, or [S, S - col0] cross products of the trailing columns) with the
+    configured backend. Returns the float64 A^T y of the streamed blocks, or None for the cross
+    products and in the batched form (which takes A^H y from one correlation)."""
+    count = len(params["centers"])
+    chunk = row_chunk(model, count, config)
+    mode = SYRK_MODE[getattr(config, "gram", "plain")]
+    batched = getattr(config, "batched", False) and hasattr(model, "correlate")
+    if batched and mode == "bf16x3" and hasattr(model, "split_blocks"):
+        split = model.split_blocks(params, chunk)
+        if split is not None:
+            for block in split:
+                syrk_split(block, out.part, col0)
+                out.added()
+            return None
+    if batched and hasattr(model, "projected_blocks"):
+        fused = model.projected_blocks(params, chunk, column_max=True)
+        if fused is not None:
+            for a, maxima in fused:
+                _block_product(a, out.part, mode, maxima, col0)
+                out.added()
+            return None
+    rhs = None if batched or col0 is not None else torch.zeros(count, dtype=torch.float64, device=target.device)
+    for a, y in model.blocks(params, target, True, chunk):
+        _block_product(a.contiguous(), out.part, mode, None, col0)
+        if rhs is not None:
+            rhs += a.T @ y
+        out.added()
+    return rhs
+
+
+def _block_product(a, out, mode, maxima, col0):
+    if col0 is None:
+        syrk(a.contiguous(), out, mode, maxima)
+    else:
+        gemm_tn(a.contiguous(), col0, out, mode, maxima)
 
 
 def normal_equations(model, params, target, config):
-    """G = A^H A / M and q = A^H y / M (Re of both for real coefficients)."""
+    """G = A^H A / M and q = A^H y / M (their real parts for real coefficients), streamed over
+    row-projected blocks of whole apertures (varpro.py:91-120 of the reference)."""
     count = len(params["centers"])
-    stream = model.blocks(params, target, config.real, row_chunk(model, count, config))
     if not config.real:
         gram = target.new_zeros((count, count))
         rhs = target.new_zeros(count)
-        for a, y in stream:
+        for a, y in model.blocks(params, target, False, row_chunk(model, count, config)):
             gram.addmm_(a.H, a)
             rhs.addmv_(a.H, y)
         return (gram + gram.H) / (2 * model.size), rhs / model.size
-    mode = SYRK_MODE[getattr(config, "gram", "plain")]
-    batched = getattr(config, "batched", False) and hasattr(model, "correlate")
-    device = target.device
-    part = torch.zeros((count, count), dtype=torch.float32, device=device)
-    total = torch.zeros((count, count), dtype=torch.float64, device=device)
-    rhs = torch.zeros(count, dtype=torch.float64, device=device)
-    blocks = ((a, y, None) for a, y in stream)
-    if batched and hasattr(model, "projected_blocks"):
-        # projected blocks straight from the atoms on chip (with their column maxima for
-        # the FP16 split); y enters through the correlation
-        fused = model.projected_blocks(params, row_chunk(model, count, config), column_max=True)
-        if fused is not None:
-            blocks = ((a, None, maxima) for a, maxima in fused)
-    for index, (a, y, maxima) in enumerate(blocks):
-        syrk(a.contiguous(), part, mode, maxima)
-        if not batched:
-            rhs += a.T @ y
-        if (index + 1) % FOLD == 0:
-            total += part
-            part.zero_()
-    total += part
-    upper = torch.triu(total)
-    gram = (upper + torch.triu(total, 1).T) / model.size
-    if batched:
-        # A^H y in one correlation launch over every aperture and bin: the row basis holds
-        # every atom, so projecting y first changes nothing
+    acc = _Folded(count, count, target.device)
+    rhs = block_products(model, params, target, config, acc)
+    total = acc.result()
+    gram = (torch.triu(total) + torch.triu(total, 1).T) / model.size
+    if rhs is None:
+        # A^H y by one correlation over every aperture and bin: the range subspace holds every
+        # atom, so projecting y first changes nothing
         rhs = model.correlate(params, target)[0].real.double()
     return gram.float(), (rhs / model.size).float()
 
 
-def fit_coefficients(gram, rhs, config):
-    """Minimizer of b^H G b - 2 Re(b^H q) + ridge |b|^2 in the configured domain."""
-    if config.real:
-        gram, rhs = gram.real, rhs.real
-    eye = torch.eye(len(gram), dtype=gram.dtype, device=gram.device)
-    return solve(gram + config.ridge * eye, rhs)
+# --- Objective and acceptance --------------------------------------------------------------
 
 
 def rendered_loss(model, params, coefficients, target, ridge):
-    """Prediction and the complete objective, from a fresh render (varpro.py:132-136)."""
+    """The prediction and the complete objective, from a fresh render (varpro.py:132-136)."""
     prediction = model.render(params, coefficients)
     loss = energy(prediction, target) / model.size + ridge * coefficients.abs().square().sum()
     return prediction, float(loss)
@@ -152,11 +213,11 @@ def varpro(model, params, target, config):
 
 @torch.no_grad()
 def exchange_varpro(state, proposal, slot, config):
-    """VarPro after one atom changed: only its Gram column and rhs entry are recomputed,
-    from one rendered column correlated against every atom (varpro.py:147-163)."""
+    """VarPro after one atom moved: its Gram row and column and its rhs entry come from one
+    rendered column correlated against every atom; the rest is the state's
+    (varpro.py:147-163)."""
     model, target = state.model, state.target
-    atom = {key: value[slot : slot + 1] for key, value in proposal.items()}
-    column = model.render(atom, target.real.new_ones(1))
+    column = model.render({key: value[slot : slot + 1] for key, value in proposal.items()}, target.real.new_ones(1))
     cross = model.correlate(proposal, column)[0] / model.size
     value = (column.conj() * target).sum() / model.size
     if config.real:
@@ -170,59 +231,61 @@ def exchange_varpro(state, proposal, slot, config):
 
 
 def screen_bar(screen, params, config):
-    """(model, target, subset objective of params) for an aperture subset, or None."""
+    """(model, target, subset objective of params) of an aperture subset, or None: one bar
+    per state, against which every proposal from that state is screened."""
     if screen is None:
         return None
     model, target = screen
     return model, target, varpro(model, params, target, config).objective
 
 
+def _changed_atoms(state, proposal):
+    moved = torch.zeros(len(state.coefficients), dtype=torch.bool, device=state.target.device)
+    for key, value in proposal.items():
+        moved |= (value != state.params[key]).reshape(len(moved), -1).any(-1)
+    return moved.nonzero().flatten().tolist()
+
+
 @torch.no_grad()
 def accept(state, proposal, config, screen=None):
-    """The state with proposal installed if its refitted objective is lower, else None.
-
-    A screen first rejects proposals clearly worse on its aperture subset; one changed
-    atom refits only its Gram column (varpro.py:178-203).
-    """
-    changed = torch.zeros(len(state.coefficients), dtype=torch.bool, device=state.target.device)
-    for key, value in proposal.items():
-        changed |= (value != state.params[key]).reshape(len(changed), -1).any(-1)
-    slots = changed.nonzero().flatten().tolist()
+    """The state with proposal installed if its refitted objective is lower, else None. A
+    screen (screen_bar of state.params) first turns away proposals clearly worse on its
+    aperture subset; a single moved atom refits only its Gram column (varpro.py:178-203)."""
+    slots = _changed_atoms(state, proposal)
     if not slots:
         return None
     if screen is not None:
         model, target, bar = screen
         if varpro(model, proposal, target, config).objective > bar + 1e-4 * state.power:
             return None
-    if len(slots) == 1:
-        fit = exchange_varpro(state, proposal, slots[0], config)
-    else:
-        fit = varpro(state.model, proposal, state.target, config)
+    fit = (exchange_varpro(state, proposal, slots[0], config) if len(slots) == 1
+           else varpro(state.model, proposal, state.target, config))  # fmt: skip
     if not fit.objective < state.objective:
         return None
     return replace(state, params=proposal, **fit._asdict())
 
 
 def pcg(operator, rhs, diagonal, *, iterations=32, tolerance=1e-3):
-    """Jacobi-preconditioned conjugate gradients from zero (varpro.py:206-228)."""
+    """Jacobi-preconditioned conjugate gradients from zero: stops at the first direction of
+    non-positive curvature or once the residual is tolerance times the right-hand side
+    (varpro.py:206-228)."""
     x = torch.zeros_like(rhs)
-    r = rhs.clone()
-    target_norm = tolerance * rhs.norm()
-    z = r / diagonal
-    p = z.clone()
-    rz = r @ z
+    residual = rhs.clone()
+    goal = tolerance * rhs.norm()
+    z = residual / diagonal
+    direction = z.clone()
+    rz = residual @ z
     for _ in range(iterations):
-        hp = operator(p)
-        curvature = p @ hp
+        image = operator(direction)
+        curvature = direction @ image
         if not curvature > 0:
             break
-        alpha = rz / curvature
-        x += alpha * p
-        r -= alpha * hp
-        if r.norm() <= target_norm:
+        step = rz / curvature
+        x += step * direction
+        residual -= step * image
+        if residual.norm() <= goal:
             break
-        z = r / diagonal
-        rz_next = r @ z
-        p = z + (rz_next / rz) * p
-        rz = rz_next
+        z = residual / diagonal
+        rz, previous = residual @ z, rz
+        direction = z + (rz / previous) * direction
     return x
