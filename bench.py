@@ -540,9 +540,9 @@ DSFW_FORMS = {
     # plain FP32: bin-domain splat kernels, FP32 SYRK normal equations, candidates scored in
     # chunks, host-loop quasi-Newton search, backtracking and elimination
     "plain_fp32": ("bins", dict()),
-    # mixed-precision batched: sample-domain MFMA splat and row projection, FP16 hi+lo split
-    # SYRK, one-launch orientation choice, quasi-Newton search and elimination, 2^29-element
-    # design blocks
+    # mixed-precision batched: sample-domain MFMA splat, the projection writing split BF16
+    # blocks for the split SYRK (hi hi + hi lo + lo hi), one-launch orientation choice,
+    # quasi-Newton search and elimination, 2^29-element design blocks
     "mixed_batched": ("samples", dict(gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22)),
 }
 
@@ -670,7 +670,8 @@ def dsfw_workload(args):
         "warmup": result["warmup"],
         "ms_per_step": mixed["ms_per_step_median"],
         "higher_is_better": False,
-        "dtype": "f32 solver (f64 shifted Cholesky fallback), fp16 hi+lo x3 MFMA SYRK and splat (mixed form)",
+        "dtype": "f32 solver (f64 shifted Cholesky fallback), bf16 hi+lo x3 MFMA normal equations, fp16 hi+lo x3 "
+        "MFMA splat at 128 samples, fp32 MFMA splat at 256 samples (mixed form)",
         "data": "synthetic",
         "config": {"workload": result["workload"], "surfels": result["surfels"], "bins": result["bins"],
                    "parallelism": "single GPU"},

@@ -496,17 +496,19 @@ int32_t dsplat_project_split_rows(int32_t rows) { return rows > 0 ? project_rows
 
 int dsplat_project_sampled_split(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
                                  const float* centers, const float* normals, const float* areas,
-                                 int32_t num_surfels, const float* sample_basis, int32_t rows, void* hi, void* lo,
-                                 int64_t ld, void* stream) {
+                                 int32_t num_surfels, const float* sample_basis, int32_t rows, int32_t format,
+                                 const float* column_exponents, void* hi, void* lo, int64_t ld, void* stream) {
     Chirp c;
     if (!make_chirp(cfg, &c) || num_apertures < 0 || num_surfels < 0 || !project_td_supported(c, rows))
         return hipErrorInvalidValue;
+    if (format != 0 && format != 1) return hipErrorInvalidValue;
     if (num_apertures == 0 || num_surfels == 0) return hipSuccess;
     if (!positions || !centers || !normals || !areas || !sample_basis || !hi || !lo) return hipErrorInvalidValue;
+    if (format == 1 && !column_exponents) return hipErrorInvalidValue;
     if (ld < num_surfels) return hipErrorInvalidValue;
     if (((uintptr_t)hi | (uintptr_t)lo) % 16 != 0) return hipErrorInvalidValue;
     return launch_project_split(c, positions, num_apertures, centers, normals, areas, num_surfels, sample_basis, rows,
-                                static_cast<__bf16*>(hi), static_cast<__bf16*>(lo), ld, (hipStream_t)stream);
+                                format, format == 1 ? column_exponents : nullptr, hi, lo, ld, (hipStream_t)stream);
 }
 
 size_t dsplat_syrk_split_workspace(int32_t m, int32_t n, int32_t col0) {
@@ -514,16 +516,16 @@ size_t dsplat_syrk_split_workspace(int32_t m, int32_t n, int32_t col0) {
     return syrk_split_workspace(m, n, col0 < 0 ? -1 : col0);
 }
 
-int dsplat_syrk_split(const void* hi, const void* lo, int64_t ld, int32_t m, int32_t n, int32_t col0, float* out,
-                      void* workspace, size_t workspace_bytes, void* stream) {
-    if (m < 0 || n < 0 || m % 32 != 0 || col0 > n) return hipErrorInvalidValue;
+int dsplat_syrk_split(int32_t format, const void* hi, const void* lo, int64_t ld, int32_t m, int32_t n, int32_t col0,
+                      const float* column_exponents, float* out, void* workspace, size_t workspace_bytes,
+                      void* stream) {
+    if (m < 0 || n < 0 || m % 32 != 0 || col0 > n || (format != 0 && format != 1)) return hipErrorInvalidValue;
     if (m == 0 || n == 0 || col0 == n) return hipSuccess;
     if (col0 < 0) col0 = -1;
     if (!hi || !lo || !out || !workspace || ld < n) return hipErrorInvalidValue;
     if (((uintptr_t)hi | (uintptr_t)lo) % 16 != 0) return hipErrorInvalidValue;
     if (workspace_bytes < syrk_split_workspace(m, n, col0)) return hipErrorInvalidValue;
-    return launch_syrk_split(static_cast<const __bf16*>(hi), static_cast<const __bf16*>(lo), ld, m, n, col0, out,
-                             workspace, (hipStream_t)stream);
+    return launch_syrk_split(format, hi, lo, ld, m, n, col0, column_exponents, out, workspace, (hipStream_t)stream);
 }
 
 size_t dsplat_qn_workspace(const dsplat_fmcw* cfg, int32_t num_apertures) {

@@ -125,16 +125,17 @@ hipError_t launch_project_td(const Chirp& c, const float* pos, int A, const floa
 // Split-operand form of the projected blocks (BF16 hi / lo, column-major, rows of aperture a
 // at a * project_rows_pad(rows) + j) and the SYRK that consumes it.
 int project_rows_pad(int rows);
+// format 0: BF16 hi / lo; 1: FP16 hi / lo of column s scaled by 2^col_exp[s].
 hipError_t launch_project_split(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                                const float* areas, int S, const float* basis, int rows, __bf16* hi, __bf16* lo,
-                                long long ld, hipStream_t st);
+                                const float* areas, int S, const float* basis, int rows, int format,
+                                const float* col_exp, void* hi, void* lo, long long ld, hipStream_t st);
 struct SplitShape {
     int tiles, splits, per_split;
 };
 SplitShape syrk_split_shape(int m, int n, int col0);
 size_t syrk_split_workspace(int m, int n, int col0);
-hipError_t launch_syrk_split(const __bf16* hi, const __bf16* lo, long long ld, int m, int n, int col0, float* out,
-                             void* workspace, hipStream_t st);
+hipError_t launch_syrk_split(int format, const void* hi, const void* lo, long long ld, int m, int n, int col0,
+                             const float* col_exp, float* out, void* workspace, hipStream_t st);
 typedef dsplat_qn_spec QnSpec;
 size_t qn_workspace(int A, int P, int K);
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,

@@ -4,12 +4,15 @@
 // replacement.py:178-199).
 //
 // P [m, n] arrives as two BF16 arrays hi, lo with P ~ hi + lo (hi = RNE(P), lo = RNE(P - hi)),
+// or as two FP16 arrays of P with column c scaled by 2^e_c (so the FP16 range holds it; the
+// products are unscaled by 2^-(e_i + e_j) in the reduction),
 // stored in 32-row column tiles: element (k, c) at ((k / 32) ldt + c) 32 + k % 32 (ldt >= n),
 // so the 32 rows of a column are 64 contiguous bytes and the 256 columns of a panel for one
 // stage are 16 contiguous KB. The projection kernel writes them in that form (project_td.hip),
 // so this kernel does no per-element work besides the products: per 32 rows it stages the two
 // 256-column panels of its tile (hi and lo: 64 KB) through LDS and runs hi hi + hi lo + lo hi
-// on v_mfma_f32_32x32x16_bf16 with FP32 accumulation (relative product error about 2^-17).
+// on v_mfma_f32_32x32x16_{bf16,f16} with FP32 accumulation (relative product error about 2^-17
+// in BF16, 2^-22 in FP16).
 //
 // A workgroup (8 waves) owns a 256 x 256 tile and a range of rows (split-K); wave (wi, wj)
 // accumulates rows wi * 128 .. + 127 and columns wj * 64 .. + 63 of the tile as 4 x 2 blocks
@@ -27,6 +30,14 @@ namespace dsplat {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x16 mma_split(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mma_split(f16x8 a, f16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+}
 
 constexpr int kGT = 256;        // tile edge
 constexpr int kGK = 32;         // rows per stage
@@ -55,8 +66,10 @@ __device__ __forceinline__ void split_tile_origin(int t, int n, int col0, int* c
 
 __device__ __forceinline__ int swizzled_slot(int c, int q) { return 4 * c + (q ^ ((c >> 2) & 3)); }
 
-__global__ __launch_bounds__(kGThreads) void syrk_split_kernel(const __bf16* __restrict__ hi,
-                                                               const __bf16* __restrict__ lo, long long ldt, int m,
+// V: the 8-element operand vector, bf16x8 or f16x8 (16-bit elements either way).
+template <typename V>
+__global__ __launch_bounds__(kGThreads) void syrk_split_kernel(const uint16_t* __restrict__ hi,
+                                                               const uint16_t* __restrict__ lo, long long ldt, int m,
                                                                int n, int col0, int per_split,
                                                                float* __restrict__ partial) {
     extern __shared__ uint4 lds[];  // [2 buffers][4 parts: A hi, A lo, B hi, B lo][kGSlots]
@@ -75,7 +88,7 @@ __global__ __launch_bounds__(kGThreads) void syrk_split_kernel(const __bf16* __r
     // q = (u & 3) ^ ((c >> 2) & 3) of column c = u >> 2; columns past n read column n - 1 (in
     // bounds; those tile entries are never stored). Every part is loaded, also on a diagonal
     // tile (whose B panel is its A panel, read twice through L2).
-    const __bf16* src[4][2];
+    const uint16_t* src[4][2];
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
@@ -118,27 +131,27 @@ __global__ __launch_bounds__(kGThreads) void syrk_split_kernel(const __bf16* __r
 #pragma unroll
         for (int kk = 0; kk < 2; ++kk) {
             const int q = 2 * kk + h;
-            bf16x8 ah[4], al[4], bh[2], bl[2];
+            V ah[4], al[4], bh[2], bl[2];
 #pragma unroll
             for (int x = 0; x < 4; ++x) {
                 const int s = swizzled_slot(wi * 128 + x * 32 + l32, q);
-                ah[x] = __builtin_bit_cast(bf16x8, base[s]);
-                al[x] = __builtin_bit_cast(bf16x8, base[kGSlots + s]);
+                ah[x] = __builtin_bit_cast(V, base[s]);
+                al[x] = __builtin_bit_cast(V, base[kGSlots + s]);
             }
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
                 const int s = swizzled_slot(wj * 64 + y * 32 + l32, q);
-                bh[y] = __builtin_bit_cast(bf16x8, base[pb * kGSlots + s]);
-                bl[y] = __builtin_bit_cast(bf16x8, base[(pb + 1) * kGSlots + s]);
+                bh[y] = __builtin_bit_cast(V, base[pb * kGSlots + s]);
+                bl[y] = __builtin_bit_cast(V, base[(pb + 1) * kGSlots + s]);
             }
 #pragma unroll
             for (int x = 0; x < 4; ++x)
 #pragma unroll
                 for (int y = 0; y < 2; ++y) {
                     f32x16 t = acc[x][y];
-                    t = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[x], bh[y], t, 0, 0, 0);
-                    t = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[x], bl[y], t, 0, 0, 0);
-                    t = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[x], bh[y], t, 0, 0, 0);
+                    t = mma_split(ah[x], bh[y], t);
+                    t = mma_split(ah[x], bl[y], t);
+                    t = mma_split(al[x], bh[y], t);
                     acc[x][y] = t;
                 }
         }
@@ -161,10 +174,12 @@ __global__ __launch_bounds__(kGThreads) void syrk_split_kernel(const __bf16* __r
         }
 }
 
-// out[gi][gj] += sum over splits in float64, fixed order: gi <= gj of out [n, n] for the
-// triangle, every entry of out [n, n - col0] (column gj - col0) for the rectangle.
+// out[gi][gj] += sum over splits in float64, fixed order, times 2^-(e_gi + e_gj) when the
+// operands carry column exponents: gi <= gj of out [n, n] for the triangle, every entry of
+// out [n, n - col0] (column gj - col0) for the rectangle.
 __global__ __launch_bounds__(256) void syrk_split_reduce(const float* __restrict__ partial, int n, int col0,
-                                                         int tiles, int splits, float* __restrict__ out) {
+                                                         int tiles, int splits, const float* __restrict__ col_exp,
+                                                         float* __restrict__ out) {
     int ci, cj;
     split_tile_origin(blockIdx.x, n, col0, &ci, &cj);
     const int width = col0 < 0 ? n : n - col0, shift = col0 < 0 ? 0 : col0;
@@ -174,6 +189,7 @@ __global__ __launch_bounds__(256) void syrk_split_reduce(const float* __restrict
         if (gi >= n || gj >= n || (col0 < 0 && gi > gj)) continue;
         double sum = 0.0;
         for (int s = 0; s < splits; ++s) sum += partial[((size_t)s * tiles + blockIdx.x) * kGT * kGT + e];
+        if (col_exp) sum = ldexp(sum, -(int)col_exp[gi] - (int)col_exp[gj]);
         out[(size_t)gi * width + (gj - shift)] += (float)sum;
     }
 }
@@ -200,19 +216,26 @@ size_t syrk_split_workspace(int m, int n, int col0) {
     return sizeof(float) * (size_t)s.splits * s.tiles * kGT * kGT;
 }
 
-hipError_t launch_syrk_split(const __bf16* hi, const __bf16* lo, long long ldt, int m, int n, int col0, float* out,
-                             void* workspace, hipStream_t st) {
+hipError_t launch_syrk_split(int format, const void* hi, const void* lo, long long ldt, int m, int n, int col0,
+                             const float* col_exp, float* out, void* workspace, hipStream_t st) {
     if (m <= 0 || n <= 0 || col0 >= n) return hipSuccess;
     const SplitShape s = syrk_split_shape(m, n, col0);
     float* partial = static_cast<float*>(workspace);
     const size_t smem = 2 * 4 * kGSlots * sizeof(uint4);
-    hipError_t e = allow_lds((const void*)syrk_split_kernel, smem);
+    const void* kernel = format == 0 ? (const void*)syrk_split_kernel<bf16x8> : (const void*)syrk_split_kernel<f16x8>;
+    hipError_t e = allow_lds(kernel, smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(syrk_split_kernel, dim3(s.tiles, s.splits), dim3(kGThreads), smem, st, hi, lo, ldt, m, n, col0,
-                       s.per_split, partial);
+    const uint16_t* h = static_cast<const uint16_t*>(hi);
+    const uint16_t* l = static_cast<const uint16_t*>(lo);
+    if (format == 0)
+        hipLaunchKernelGGL(syrk_split_kernel<bf16x8>, dim3(s.tiles, s.splits), dim3(kGThreads), smem, st, h, l, ldt, m,
+                           n, col0, s.per_split, partial);
+    else
+        hipLaunchKernelGGL(syrk_split_kernel<f16x8>, dim3(s.tiles, s.splits), dim3(kGThreads), smem, st, h, l, ldt, m,
+                           n, col0, s.per_split, partial);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(syrk_split_reduce, dim3(s.tiles, 16), dim3(256), 0, st, partial, n, col0, s.tiles, s.splits,
-                       out);
+                       col_exp, out);
     return hipGetLastError();
 }
 

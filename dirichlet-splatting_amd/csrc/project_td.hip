@@ -63,22 +63,42 @@ __device__ __forceinline__ f32x16 mfma_f16(f16x8 a, f16x8 b, f32x16 c) {
 }
 
 // Where a projected block goes. Row-major float32 out [rows * A, S] (row j * A + a), or
-// (split) BF16 hi / lo with hi + lo ~ value in 32-row column tiles, element (k, s) at
+// (split) BF16 or FP16 hi / lo with hi + lo ~ value (FP16: value times 2^col_exp[s]) in
+// 32-row column tiles, element (k, s) at
 // ((k / 32) ld + s) 32 + k % 32, where projected row j of aperture a is k = a * rows_pad + j,
 // rows_pad = rows rounded up to 32 (rows j >= rows hold zeros): the operand form of
 // syrk_split_kernel.
 struct ProjectOut {
     float* out;
     unsigned* colmax;
-    __bf16* hi;
-    __bf16* lo;
+    void* hi;              // split: BF16 or FP16 [m / 32][ld][32]
+    void* lo;
     long long ld;
     int rows_pad;
+    const float* col_exp;  // split FP16: column s is scaled by 2^col_exp[s] before the split
 };
 
-// One aperture of one unit: v, z = v^16 and the weight w / S of this lane's pair. Returns
-// the largest |value| this lane stored (0 for a padding column).
-template <int J, int NB, bool SPLIT>
+// Output forms of project_td_kernel.
+constexpr int kOutFloat = 0;  // row-major float32
+constexpr int kOutBf16 = 1;   // split BF16 hi / lo
+constexpr int kOutFp16 = 2;   // split FP16 hi / lo of the column-scaled value
+
+template <typename T>
+__device__ __forceinline__ void split_store(float x[4], void* hi, void* lo, size_t at) {
+    T vh[4], vl[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        vh[e] = (T)x[e];
+        vl[e] = (T)(x[e] - (float)vh[e]);
+    }
+    *reinterpret_cast<uint2*>(static_cast<T*>(hi) + at) = __builtin_bit_cast(uint2, vh);
+    *reinterpret_cast<uint2*>(static_cast<T*>(lo) + at) = __builtin_bit_cast(uint2, vl);
+}
+
+// One aperture of one unit: v, z = v^16 and the weight w / S of this lane's pair (times the
+// column's power-of-two scale in the FP16 split form). Returns the largest |value| this lane
+// stored (0 for a padding column).
+template <int J, int NB, int OUT>
 __device__ __forceinline__ float project_aperture(const _Float16* Bh, const _Float16* Bl, int bs, int h, int l32,
                                                  float2 v, float2 z, float2 w, int row0, int rows, int a, int A,
                                                  int s, int S, const ProjectOut& o) {
@@ -140,7 +160,7 @@ __device__ __forceinline__ float project_aperture(const _Float16* Bh, const _Flo
     // (Re, Im) row pairs times w; C[row][col] -> acc[r] with row = 8 (r / 4) + 4 h + r % 4, so
     // rows 2j and 2j + 1 are acc[r], acc[r + 1] for even r
     if (s >= S) return 0.0f;
-    if constexpr (SPLIT) {
+    if constexpr (OUT != kOutFloat) {
         // rows 4 g .. 4 g + 3 of a 32-row block are (re, im) of two projected rows: 8 bytes of
         // hi and of lo per group; the two lane halves fill 16 contiguous bytes, the four groups
         // the column's 64 bytes of the tile
@@ -157,15 +177,11 @@ __device__ __forceinline__ float project_aperture(const _Float16* Bh, const _Flo
                     val[e] = w.x * hr - w.y * hm;
                     val[e + 1] = w.x * hm + w.y * hr;
                 }
-                __bf16 vh[4], vl[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    vh[e] = (__bf16)val[e];
-                    vl[e] = (__bf16)(val[e] - (float)vh[e]);
-                }
                 const size_t at = ((tile0 + x) * o.ld + s) * 32 + 8 * g + 4 * h;
-                *reinterpret_cast<uint2*>(o.hi + at) = __builtin_bit_cast(uint2, vh);
-                *reinterpret_cast<uint2*>(o.lo + at) = __builtin_bit_cast(uint2, vl);
+                if constexpr (OUT == kOutBf16)
+                    split_store<__bf16>(val, o.hi, o.lo, at);
+                else
+                    split_store<_Float16>(val, o.hi, o.lo, at);
             }
         }
         return 0.0f;
@@ -203,7 +219,7 @@ __device__ __forceinline__ void flush_column_max(unsigned* colmax, float top, in
     if (colmax && h == 0 && s < S) atomicMax(colmax + s, __float_as_uint(fmaxf(top, other)));
 }
 
-template <int J, bool FRESNEL, int ROWS, bool SPLIT>
+template <int J, bool FRESNEL, int ROWS, int OUT>
 __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, const float* __restrict__ pos, int A,
                                                                 const float* __restrict__ ctr,
                                                                 const float* __restrict__ nrm,
@@ -242,6 +258,7 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
     const int u_end = min(units, (int)blockIdx.x * per + per);
     int tile = -1, s = 0;
     float cx = 0.f, cy = 0.f, cz = 0.f, nx = 0.f, ny = 0.f, nz = 0.f, root_area = 0.f;
+    float column_scale = 1.0f;  // FP16 split: 2^col_exp of this lane's column
     float top = 0.0f;  // largest |value| of this lane's column in the current tile
     for (int u = blockIdx.x * per; u < u_end; ++u) {
         if (u / pairs != tile) {
@@ -253,6 +270,7 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
             cx = ctr[3 * sc], cy = ctr[3 * sc + 1], cz = ctr[3 * sc + 2];
             nx = nrm[3 * sc], ny = nrm[3 * sc + 1], nz = nrm[3 * sc + 2];
             root_area = sqrtf(areas[sc]);
+            if constexpr (OUT == kOutFp16) column_scale = ldexpf(1.0f, (int)o.col_exp[sc]);
             __builtin_amdgcn_s_waitcnt(0);  // here, not at the first use that every unit shares
         }
         const int a0 = 2 * (u - tile * pairs), a1 = min(a0 + 1, A - 1);
@@ -263,7 +281,7 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
         const Pair q = pair_geometry_t<FRESNEL>(c, px, py, pz, cx, cy, cz, nx, ny, nz);
         const float2 v = rotation(c, E, q.lobe, q.delta, 1);
         const float2 z = rotation(c, E, q.lobe, q.delta, 16);
-        const float2 w = (q.amp * root_area * c.inv_S) * q.rot;
+        const float2 w = (q.amp * root_area * c.inv_S * column_scale) * q.rot;
         // every lane gets both apertures' values: f0 from half 0 (a0), f1 from half 1 (a0 + 1)
         float f0[6] = {v.x, v.y, z.x, z.y, w.x, w.y}, f1[6];
 #pragma unroll
@@ -273,7 +291,7 @@ __global__ __launch_bounds__(kPtThreads) void project_td_kernel(const Chirp c, c
         }
 #pragma unroll 1
         for (int e = 0; e < 2 && a0 + e < A; ++e)
-            top = fmaxf(top, project_aperture<J, NB, SPLIT>(Bh, Bl, BS, h, l32,
+            top = fmaxf(top, project_aperture<J, NB, OUT>(Bh, Bl, BS, h, l32,
                                                             make_float2(e ? f1[0] : f0[0], e ? f1[1] : f0[1]),
                                                             make_float2(e ? f1[2] : f0[2], e ? f1[3] : f0[3]),
                                                             make_float2(e ? f1[4] : f0[4], e ? f1[5] : f0[5]), row0,
@@ -296,11 +314,11 @@ bool project_td_supported(const Chirp& c, int rows) {
     return use_td(c) && rows >= 2 && rows % 2 == 0 && project_td_tile(c, rows) > 0;
 }
 
-template <int J, bool F, int ROWS, bool SPLIT>
+template <int J, bool F, int ROWS, int OUT>
 static hipError_t launch_ptd(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                              const float* areas, int S, const float* basis, int rows, const ProjectOut& o,
                              hipStream_t st) {
-    const void* kernel = (const void*)project_td_kernel<J, F, ROWS, SPLIT>;
+    const void* kernel = (const void*)project_td_kernel<J, F, ROWS, OUT>;
     const size_t smem = project_td_lds(c, ROWS);
     hipError_t e = allow_lds(kernel, smem);
     if (e != hipSuccess) return e;
@@ -313,32 +331,32 @@ static hipError_t launch_ptd(const Chirp& c, const float* pos, int A, const floa
     long long blocks = (long long)cus * (occ > 0 ? occ : 1) / passes;
     if (blocks < 1) blocks = 1;
     if (blocks > units) blocks = units;
-    hipLaunchKernelGGL((project_td_kernel<J, F, ROWS, SPLIT>), dim3((unsigned)blocks, passes), dim3(kPtThreads),
+    hipLaunchKernelGGL((project_td_kernel<J, F, ROWS, OUT>), dim3((unsigned)blocks, passes), dim3(kPtThreads),
                        smem, st, c, pos, A, ctr, nrm, areas, S, basis, rows, o);
     return hipGetLastError();
 }
 
-template <int J, bool F, bool SPLIT>
+template <int J, bool F, int OUT>
 static hipError_t launch_ptd_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                const float* areas, int S, const float* basis, int rows, const ProjectOut& o,
                                hipStream_t st) {
     if constexpr (J == 8) {
         if (project_td_tile(c, rows) == 128)
-            return launch_ptd<J, F, 128, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+            return launch_ptd<J, F, 128, OUT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
     }
-    return launch_ptd<J, F, 64, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+    return launch_ptd<J, F, 64, OUT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
 }
 
-template <bool SPLIT>
+template <int OUT>
 static hipError_t launch_ptd_any(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                                  const float* areas, int S, const float* basis, int rows, const ProjectOut& o,
                                  hipStream_t st) {
     const bool fresnel = c.fresnel_n > 1.0f;
     if (c.S == 128)
-        return fresnel ? launch_ptd_j<8, true, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st)
-                       : launch_ptd_j<8, false, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
-    return fresnel ? launch_ptd_j<16, true, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st)
-                   : launch_ptd_j<16, false, SPLIT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+        return fresnel ? launch_ptd_j<8, true, OUT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st)
+                       : launch_ptd_j<8, false, OUT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+    return fresnel ? launch_ptd_j<16, true, OUT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st)
+                   : launch_ptd_j<16, false, OUT>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
 }
 
 hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, float* out, hipStream_t st) {
@@ -350,17 +368,18 @@ hipError_t launch_sample_basis(const Chirp& c, const float* basis, int rows, flo
 hipError_t launch_project_td(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                              const float* areas, int S, const float* basis, int rows, float* out, unsigned* colmax,
                              hipStream_t st) {
-    ProjectOut o{out, colmax, nullptr, nullptr, 0, 0};
-    return launch_ptd_any<false>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+    ProjectOut o{out, colmax, nullptr, nullptr, 0, 0, nullptr};
+    return launch_ptd_any<kOutFloat>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
 }
 
 int project_rows_pad(int rows) { return (rows + 31) / 32 * 32; }
 
 hipError_t launch_project_split(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                                const float* areas, int S, const float* basis, int rows, __bf16* hi, __bf16* lo,
-                                long long ld, hipStream_t st) {
-    ProjectOut o{nullptr, nullptr, hi, lo, ld, project_rows_pad(rows)};
-    return launch_ptd_any<true>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+                                const float* areas, int S, const float* basis, int rows, int format,
+                                const float* col_exp, void* hi, void* lo, long long ld, hipStream_t st) {
+    ProjectOut o{nullptr, nullptr, hi, lo, ld, project_rows_pad(rows), col_exp};
+    if (format == 0) return launch_ptd_any<kOutBf16>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
+    return launch_ptd_any<kOutFp16>(c, pos, A, ctr, nrm, areas, S, basis, rows, o, st);
 }
 
 }  // namespace dsplat

@@ -136,11 +136,13 @@ SIGNATURES = {
     ),
     "dsplat_project_split_rows": (c_int, [c_int]),
     "dsplat_project_sampled_split": (
-        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_ptr, c_ptr, ctypes.c_int64, c_ptr]
+        c_int,
+        [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_int, c_int, c_ptr, c_ptr, c_ptr, ctypes.c_int64,
+         c_ptr],
     ),
     "dsplat_syrk_split_workspace": (c_size, [c_int, c_int, c_int]),
     "dsplat_syrk_split": (
-        c_int, [c_ptr, c_ptr, ctypes.c_int64, c_int, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]
+        c_int, [c_int, c_ptr, c_ptr, ctypes.c_int64, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
     ),
     "dsplat_qn_workspace": (c_size, [c_ptr, c_int]),
     "dsplat_qn_search": (

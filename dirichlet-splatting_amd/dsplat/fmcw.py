@@ -282,21 +282,27 @@ class _LeastSquaresStream:
 
         return (block(first) for first in self._aperture_steps(chunk))
 
-    def split_blocks(self, params, chunk):
+    def split_blocks(self, params, chunk, format="bf16"):
         """The same blocks in the operand form of the split SYRK (kernels.blas.SplitBlock:
-        BF16 hi / lo in 32-row column tiles, written by the projection kernel itself), or None
-        where the sample-domain projection does not serve the chirp."""
+        16-bit hi / lo in 32-row column tiles, written by the projection kernel itself; FP16
+        with column scales that hold for every aperture of the model), or None where the
+        sample-domain projection does not serve the chirp."""
         if self.basis is None:
             return None
         config, positions = self.sensor
         sampled = self.basis.sampled(config, self.fresnel_n)
         if sampled is None:
             return None
+        from .kernels.blas import projection_exponents
+
+        exponents = (projection_exponents(config, positions, params["centers"], params["areas"])
+                     if format == "fp16" else None)  # fmt: skip
         step = max(1, chunk // self.shape[1])
         rows = 2 * self.basis.rank
         return (
             surfel.kernel.project_split(config, positions[first : first + step], params["centers"],
-                                        params["normals"], params["areas"], sampled, rows, self.fresnel_n)
+                                        params["normals"], params["areas"], sampled, rows, self.fresnel_n, format,
+                                        exponents)
             for first in self._aperture_steps(chunk)
         )  # fmt: skip
 

@@ -58,38 +58,51 @@ def gemm_tn(rows, col0, out, mode="fp32", column_max=None):
     return out
 
 
-class SplitBlock:
-    """A block P [m, n] in the operand form of syrk_split: BF16 hi and lo [m / 32, ld, 32]
-    (32-row column tiles: element (k, c) at [k // 32, c, k % 32]; ld >= n) with P ~ hi + lo."""
+SPLIT_FORMATS = {"bf16": (0, torch.bfloat16), "fp16": (1, torch.float16)}
 
-    def __init__(self, hi, lo, m, columns=None):
-        if hi.dtype != torch.bfloat16 or lo.dtype != torch.bfloat16 or hi.shape != lo.shape:
-            raise TypeError("a split block is two bfloat16 [m / 32, ld, 32] tensors")
+
+class SplitBlock:
+    """A block P [m, n] in the operand form of syrk_split: 16-bit hi and lo [m / 32, ld, 32]
+    (32-row column tiles: element (k, c) at [k // 32, c, k % 32]; ld >= n) with hi + lo ~ P in
+    BF16, or hi + lo ~ P times 2^exponents[c] per column in FP16 (exponents float [n])."""
+
+    def __init__(self, hi, lo, m, columns=None, format="bf16", exponents=None):
+        code, dtype = SPLIT_FORMATS[format]
+        if hi.dtype != dtype or lo.dtype != dtype or hi.shape != lo.shape:
+            raise TypeError(f"a {format} split block is two {dtype} [m / 32, ld, 32] tensors")
         if not (hi.is_contiguous() and lo.is_contiguous()) or hi.ndim != 3 or hi.shape[2] != 32:
             raise ValueError("split block: contiguous [m / 32, ld, 32] tensors")
         if m % 32 or hi.shape[0] != m // 32:
             raise ValueError("split block: m is a multiple of 32 and hi has m / 32 tiles")
+        if format == "fp16" and (exponents is None or exponents.dtype != torch.float32):
+            raise ValueError("an fp16 split block carries float32 column exponents")
         self.hi, self.lo, self.m = hi, lo, int(m)
         self.columns = hi.shape[1] if columns is None else int(columns)
+        self.format, self.code, self.exponents = format, code, exponents
 
     @staticmethod
-    def of(rows):
+    def of(rows, format="bf16", exponents=None):
         """The split form of a float32 [m, n] matrix (m a multiple of 32)."""
         m, n = rows.shape
         t = rows.float().reshape(m // 32, 32, n).transpose(1, 2).contiguous()
-        hi = t.bfloat16()
-        lo = (t - hi.float()).bfloat16()
-        return SplitBlock(hi, lo, m)
+        if exponents is not None:
+            t = t * torch.exp2(exponents)[None, :, None]
+        dtype = SPLIT_FORMATS[format][1]
+        hi = t.to(dtype)
+        lo = (t - hi.float()).to(dtype)
+        return SplitBlock(hi, lo, m, format=format, exponents=exponents)
 
     def dense(self):
-        """float32 [m, n] of hi + lo."""
+        """float32 [m, n] of hi + lo (unscaled)."""
         t = self.hi[:, : self.columns].float() + self.lo[:, : self.columns].float()
+        if self.exponents is not None:
+            t = t * torch.exp2(-self.exponents[: self.columns])[None, :, None]
         return t.transpose(1, 2).reshape(self.m, self.columns)
 
 
 def syrk_split(block, out, col0=None):
     """Upper triangle of out [n, n] += P^T P (col0 None), or out [n, n - col0] +=
-    P^T P[:, col0:], for a SplitBlock P: hi hi + hi lo + lo hi on the BF16 matrix cores with
+    P^T P[:, col0:], for a SplitBlock P: hi hi + hi lo + lo hi on the 16-bit matrix cores with
     FP32 accumulation (kernels/cublas.py:31-44 SSYRK; replacement.py:178-199 cross products)."""
     n = block.columns
     tail = n if col0 is None else n - col0
@@ -98,9 +111,26 @@ def syrk_split(block, out, col0=None):
     c0 = -1 if col0 is None else int(col0)
     size = _lib.library().dsplat_syrk_split_workspace(block.m, n, c0)
     workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=out.device)
-    _lib.call("dsplat_syrk_split", _lib.ptr(block.hi), _lib.ptr(block.lo), block.hi.shape[1], block.m, n, c0,
-              _lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(out.device))  # fmt: skip
+    _lib.call("dsplat_syrk_split", block.code, _lib.ptr(block.hi), _lib.ptr(block.lo), block.hi.shape[1], block.m, n,
+              c0, _lib.ptr(block.exponents), _lib.ptr(out), _lib.ptr(workspace), size,
+              _lib.stream(out.device))  # fmt: skip
     return out
+
+
+def projection_exponents(config, positions, centers, areas, target=2.0**14):
+    """Power-of-two column scales for FP16 split blocks of the row-projected design: 2^e_s
+    times any projected entry of column s stays below target. A projected entry is the
+    weight sqrt(area) amp rot of its pair times U^H d(k0), where |amp| <= lambda / (8 pi r)
+    (cos^2 and the Fresnel factor are at most 1) and |U^H d| <= |d|_2 <= sqrt(pad) (U has
+    orthonormal columns; the unit-peak Dirichlet response has energy at most n_fft /
+    adc_samples over the bins); r is at least the distance from the centre to the bounding
+    box of the apertures."""
+    lower, upper = positions.min(0).values.double(), positions.max(0).values.double()
+    c = centers.double()
+    nearest = torch.minimum(torch.maximum(c, lower), upper)
+    r_min = (c - nearest).norm(dim=1).clamp(min=1e-9)
+    bound = areas.double().sqrt() * config.lambda_m / (8 * torch.pi * r_min) * config.pad_factor**0.5
+    return torch.floor(torch.log2(target / bound.clamp(min=1e-300))).clamp(-120, 120).float()
 
 
 GRAM_MODES = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}

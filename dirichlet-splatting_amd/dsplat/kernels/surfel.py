@@ -322,25 +322,29 @@ def project_design(config, positions, centers, normals, areas, basis, fresnel_n,
     return out
 
 
-def project_split(config, positions, centers, normals, areas, sampled, rows, fresnel_n):
+def project_split(config, positions, centers, normals, areas, sampled, rows, fresnel_n, format="bf16", exponents=None):
     """The sample-domain projected block of project_design in the operand form of
     blas.syrk_split: a SplitBlock of m = A * rows_pad rows (rows_pad = rows rounded up to 32;
-    projected row j of aperture a is row a * rows_pad + j, padding rows zero), S columns."""
-    from .blas import SplitBlock
+    projected row j of aperture a is row a * rows_pad + j, padding rows zero), S columns, in
+    BF16, or in FP16 with every column scaled by 2^exponents[s] (blas.projection_exponents)."""
+    from .blas import SPLIT_FORMATS, SplitBlock
 
     _check(positions, centers, normals)
+    code, dtype = SPLIT_FORMATS[format]
     cfg = _lib.fmcw(config, fresnel_n)
     pad = _lib.library().dsplat_project_split_rows(rows)
     m = pad * len(positions)
-    hi = torch.empty((m // 32, len(centers), 32), dtype=torch.bfloat16, device=centers.device)
+    hi = torch.empty((m // 32, len(centers), 32), dtype=dtype, device=centers.device)
     lo = torch.empty_like(hi)
+    if exponents is not None:
+        exponents = exponents.float().contiguous()
     _lib.call(
         "dsplat_project_sampled_split", _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
         _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()),
-        len(centers), _lib.ptr(sampled.float().contiguous()), rows, _lib.ptr(hi), _lib.ptr(lo), len(centers),
-        _lib.stream(centers.device),
+        len(centers), _lib.ptr(sampled.float().contiguous()), rows, code, _lib.ptr(exponents), _lib.ptr(hi),
+        _lib.ptr(lo), len(centers), _lib.stream(centers.device),
     )  # fmt: skip
-    return SplitBlock(hi, lo, m)
+    return SplitBlock(hi, lo, m, format=format, exponents=exponents)
 
 
 def orient_select(config, positions, centers, normals, areas, bank, residual, fresnel_n, table=None, real_part=False):

@@ -15,7 +15,9 @@ The real normal equations run on the MFMA products of kernels.blas, chosen by DS
   bf16x3  with the batched search on a sample-domain chirp: blocks written by the projection
           kernel itself as BF16 hi / lo operands, one split SYRK per block (hi hi + hi lo +
           lo hi); otherwise the float32 blocks split inside the SYRK
-  fp16x3  float32 blocks with per-column power-of-two scales, FP16 hi / lo inside the SYRK
+  fp16x3  the same in FP16 (22 bits per operand instead of 17), every column scaled by a
+          power of two from an a-priori bound of its entries so that FP16 holds it;
+          otherwise float32 blocks with their column maxima, split inside the SYRK
 
 Block products accumulate in float32 and fold into a float64 total every FOLD blocks.
 """
@@ -141,8 +143,8 @@ def block_products(model, params, target, config, out, col0=None):
     chunk = row_chunk(model, count, config)
     mode = SYRK_MODE[getattr(config, "gram", "plain")]
     batched = getattr(config, "batched", False) and hasattr(model, "correlate")
-    if batched and mode == "bf16x3" and hasattr(model, "split_blocks"):
-        split = model.split_blocks(params, chunk)
+    if batched and mode in ("bf16x3", "fp16x3") and hasattr(model, "split_blocks"):
+        split = model.split_blocks(params, chunk, {"bf16x3": "bf16", "fp16x3": "fp16"}[mode])
         if split is not None:
             for block in split:
                 syrk_split(block, out.part, col0)

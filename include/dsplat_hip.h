@@ -308,27 +308,32 @@ int dsplat_project_sampled(const dsplat_fmcw* cfg, const float* positions, int32
 /* Rows per aperture of a split projected block: rows rounded up to 32. */
 int32_t dsplat_project_split_rows(int32_t rows);
 
-/* dsplat_project_sampled's block in the operand form of dsplat_syrk_split: BF16 hi and lo
+/* dsplat_project_sampled's block in the operand form of dsplat_syrk_split: 16-bit hi and lo
  * arrays (16-byte aligned) with hi + lo ~ the float32 value (hi = RNE(value), lo =
- * RNE(value - hi)) in 32-row column tiles: element (k, s) at ((k / 32) ld + s) 32 + k % 32,
- * ld >= S, m = A * rows_pad rows (rows_pad = dsplat_project_split_rows(rows)); projected row j
- * of aperture a is row k = a * rows_pad + j, and rows j >= rows are written as zeros. Replaces the design + reduce_major product
+ * RNE(value - hi)), BF16 for format 0, FP16 of the value times 2^column_exponents[s] for
+ * format 1 (float, integral values [S]: they keep the column inside the FP16 range), in
+ * 32-row column tiles: element (k, s) at ((k / 32) ld + s) 32 + k % 32, ld >= S,
+ * m = A * rows_pad rows (rows_pad = dsplat_project_split_rows(rows)); projected row j of
+ * aperture a is row k = a * rows_pad + j, and rows j >= rows are written as zeros. Replaces the design + reduce_major product
  * of the streamed blocks (fmcw.py:243-261, surfel.slang:313-378) for the split SYRK. */
 int dsplat_project_sampled_split(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
                                  const float* centers, const float* normals, const float* areas,
-                                 int32_t num_surfels, const float* sample_basis, int32_t rows, void* hi, void* lo,
-                                 int64_t ld, void* stream);
+                                 int32_t num_surfels, const float* sample_basis, int32_t rows, int32_t format,
+                                 const float* column_exponents, void* hi, void* lo, int64_t ld, void* stream);
 
 /* Upper triangle of out [n, n] += P^T P (col0 < 0), or out [n, n - col0] += P^T P[:, col0:]
  * (col0 >= 0: the cross products of the trailing columns with all columns), for P [m, n]
- * given as BF16 hi / lo arrays in 32-row column tiles (element (k, c) at
- * ((k / 32) ld + c) 32 + k % 32, ld >= n, m a multiple of 32, 16-byte aligned). Products hi hi + hi lo + lo hi on MFMA with FP32 accumulation, split-K
- * partials summed in float64 in a fixed order. Deterministic. Workspace from
+ * given as 16-bit hi / lo arrays in 32-row column tiles (element (k, c) at
+ * ((k / 32) ld + c) 32 + k % 32, ld >= n, m a multiple of 32, 16-byte aligned): BF16 for
+ * format 0; FP16 of column c scaled by 2^column_exponents[c] for format 1 (the products are
+ * unscaled by 2^-(e_i + e_j)). Products hi hi + hi lo + lo hi on MFMA with FP32 accumulation,
+ * split-K partials summed in float64 in a fixed order. Deterministic. Workspace from
  * dsplat_syrk_split_workspace. Replaces kernels/cublas.py:31-44 (cublasSsyrk_v2) and the
  * streamed addmm_ of replacement.py:178-199 on projected blocks. */
 size_t dsplat_syrk_split_workspace(int32_t m, int32_t n, int32_t col0);
-int dsplat_syrk_split(const void* hi, const void* lo, int64_t ld, int32_t m, int32_t n, int32_t col0, float* out,
-                      void* workspace, size_t workspace_bytes, void* stream);
+int dsplat_syrk_split(int32_t format, const void* hi, const void* lo, int64_t ld, int32_t m, int32_t n, int32_t col0,
+                      const float* column_exponents, float* out, void* workspace, size_t workspace_bytes,
+                      void* stream);
 
 /* ---- Batched quasi-Newton certificate search. */
 

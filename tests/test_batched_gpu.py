@@ -246,9 +246,29 @@ def test_fused_normal_equations_match_the_plain_stream(fmcw):  # noqa: F811
     model, params, _ = fmcw
     model = model.prepare(fmcw_specs())
     plain = normal_equations(model, params, model.scan.data, DSFWConfig(row_chunk=2 * model.shape[1]))
-    for gram in ("plain", "fp16x3"):
+    # bf16x3 in the batched form streams the projection kernel's split BF16 blocks into the
+    # split SYRK (the chirp has 128 samples, so the sample-domain projection serves it)
+    assert model.split_blocks(params, 3 * model.shape[1]) is not None
+    for gram in ("plain", "fp16x3", "bf16x3"):
         fused = normal_equations(
             model, params, model.scan.data, DSFWConfig(row_chunk=3 * model.shape[1], gram=gram, batched=True)
         )
         close(fused[0], plain[0], 1e-5)
         close(fused[1], plain[1], 1e-5)
+
+
+def test_pooled_cross_products_of_the_split_blocks_match_the_streamed_ones(fmcw):  # noqa: F811
+    """The cross products of new columns with all columns (the pooled statistics) from the
+    split BF16 blocks against float64 products of the float32 projected design."""
+    from dsplat import DSFWConfig
+    from dsplat.varpro import _Folded, block_products
+
+    model, params, _ = fmcw
+    model = model.prepare(fmcw_specs())
+    count = len(params["centers"])
+    col0 = count // 2
+    acc = _Folded(count, count - col0, "cuda")
+    config = DSFWConfig(row_chunk=3 * model.shape[1], gram="bf16x3", batched=True)
+    assert block_products(model, params, model.scan.data, config, acc, col0=col0) is None
+    dense = torch.cat([a for a in model.projected_blocks(params, 3 * model.shape[1])]).double()
+    close(acc.result(), dense.T @ dense[:, col0:], 1e-5)
