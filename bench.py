@@ -542,8 +542,8 @@ DSFW_FORMS = {
     "plain_fp32": ("bins", dict()),
     # mixed-precision batched: sample-domain MFMA splat, the projection writing split BF16
     # blocks for the split SYRK (hi hi + hi lo + lo hi), one-launch orientation choice,
-    # quasi-Newton search and elimination, 2^29-element design blocks
-    "mixed_batched": ("samples", dict(gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22)),
+    # quasi-Newton search and elimination, one design block per normal-equation pass (2^32 elements)
+    "mixed_batched": ("samples", dict(gram="fp16x3", batched=True, dictionary_elements=2**32, row_chunk=2**24)),
 }
 
 

@@ -12,7 +12,7 @@
 // Only the lower triangle of the input is read. info = 1-based index of the first pivot
 // that is not positive (0 if none), as LAPACK potrf reports it, in the factor's own precision
 // (the float32 form is the first attempt of varpro.py:62-79); the factor is then not
-// meaningful. The solve is one workgroup: blocked forward and backward substitution.
+// meaningful, and the launches of the later block columns return at once. The solve is one workgroup: blocked forward and backward substitution.
 #include "dsplat_internal.h"
 
 namespace dsplat {
@@ -63,6 +63,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void chol_panel_kernel(T* __restrict__ A, int n, int lda, int kb,
                                                         int* __restrict__ info) {
     __shared__ T D[kCholBlock * kCholLd];
+    // an earlier block column failed: the factor is not meaningful, info is final
+    if (*info <= n) return;
     const int tid = threadIdx.x;
     const int c0 = kb * kCholBlock, w = min(kCholBlock, n - c0);
     const int ib = kb + blockIdx.x;
@@ -117,9 +119,11 @@ __global__ __launch_bounds__(256) void chol_panel_kernel(T* __restrict__ A, int 
 
 // A_ij -= L_ik L_jk^T for the trailing lower blocks kb < jb <= ib.
 template <typename T>
-__global__ __launch_bounds__(256) void chol_update_kernel(T* __restrict__ A, int n, int lda, int kb) {
+__global__ __launch_bounds__(256) void chol_update_kernel(T* __restrict__ A, int n, int lda, int kb,
+                                                         const int* __restrict__ info) {
     __shared__ T Li[kCholBlock * (kCholBlock + 1)];  // [p][r]
     __shared__ T Lj[kCholBlock * (kCholBlock + 1)];
+    if (*info <= n) return;  // a pivot failed: nothing left to compute
     const int tid = threadIdx.x;
     // linear index -> (ti, tj) with tj <= ti over the trailing blocks
     int t = blockIdx.x, ti = 0;
@@ -257,7 +261,7 @@ static hipError_t launch_cholesky_t(T* A, int n, int lda, int* info, hipStream_t
         const int trail = nblk - kb - 1;
         if (trail > 0) {
             hipLaunchKernelGGL(chol_update_kernel<T>, dim3(trail * (trail + 1) / 2), dim3(256), 0, st, A, n, lda,
-                               kb);
+                               kb, info);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
         }

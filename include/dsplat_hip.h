@@ -376,7 +376,8 @@ int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num
 /* Float64 Cholesky factor of the K x K systems, in place: on return the lower triangle of a
  * (row-major, leading dimension lda) holds L with A = L L^T and the strictly upper part is
  * zero; only the lower triangle is read. *info (device int) = 1-based index of the first
- * non-positive pivot, 0 on success. Replaces torch.linalg.cholesky_ex (cuSOLVER potrf) of
+ * non-positive pivot, 0 on success; after a failure the factor is not meaningful (the later
+ * block columns are skipped). Replaces torch.linalg.cholesky_ex (cuSOLVER potrf) of
  * varpro.py:44-79 (solve, shifted_cholesky). */
 int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stream);
 

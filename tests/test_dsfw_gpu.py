@@ -23,7 +23,7 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
-MIXED = dict(gram="fp16x3", batched=True, dictionary_elements=2**29, row_chunk=2**22)
+MIXED = dict(gram="fp16x3", batched=True, dictionary_elements=2**32, row_chunk=2**24)
 
 
 def say(*args):
