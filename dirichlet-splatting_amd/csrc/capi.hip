@@ -687,6 +687,25 @@ int dsplat_scan_batched(const dsplat_fmcw* cfg, int32_t num_scans, const float* 
                        (hipStream_t)stream);
 }
 
+static_assert(DSPLAT_TOP_MAX == kTopMax, "the header's selection limit is the kernel's LDS capacity");
+
+int dsplat_certificate_scores(const float* scan_out, int32_t num_scans, int32_t num_candidates, const float* areas,
+                              const double* power, int32_t real_part, double* scores, void* stream) {
+    if (num_scans < 1 || num_scans > 65535 || num_candidates < 0) return hipErrorInvalidValue;
+    if (num_candidates == 0) return hipSuccess;
+    if (!scan_out || !areas || !power || !scores) return hipErrorInvalidValue;
+    return launch_certificate_scores(scan_out, num_scans, num_candidates, areas, power, real_part ? 1 : 0, scores,
+                                     (hipStream_t)stream);
+}
+
+int dsplat_top_descending(const double* scores, int32_t rows, int32_t count, int32_t top, int64_t* out,
+                          void* stream) {
+    if (rows < 0 || count < 0 || top < 1 || top > kTopMax) return hipErrorInvalidValue;
+    if (rows == 0 || count == 0) return hipSuccess;
+    if (!scores || !out) return hipErrorInvalidValue;
+    return launch_top_descending(scores, rows, count, top, out, (hipStream_t)stream);
+}
+
 int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stream) {
     if (n < 0 || lda < n || !info) return hipErrorInvalidValue;
     if (n == 0) return fill_zero(reinterpret_cast<float*>(info), 1, (hipStream_t)stream);

@@ -45,6 +45,11 @@ bool td_scan_supported(const Chirp& c);
 size_t td_scan_workspace(const Chirp& c, const ScanSet& set, int C);
 hipError_t launch_td_scan(const Chirp& c, const ScanSet& set, const float* ctr, const float* nrm, int C, int chunk,
                           void* ws, float* out, hipStream_t st);
+// Certificate scores of a scan's (corr, norm) rows, and the top of each row of scores.
+constexpr int kTopMax = 4096;  // LDS holds the selection
+hipError_t launch_certificate_scores(const float* scan, int n, int C, const float* areas, const double* power,
+                                     int real_part, double* scores, hipStream_t st);
+hipError_t launch_top_descending(const double* scores, int rows, int count, int top, int64_t* out, hipStream_t st);
 
 hipError_t launch_design(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float* areas, int S, int complex_layout, float* out, hipStream_t st);

@@ -160,6 +160,8 @@ SIGNATURES = {
     "dsplat_scan_batched": (
         c_int, [c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_int, c_ptr, c_ptr, c_size, c_ptr]
     ),
+    "dsplat_certificate_scores": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
+    "dsplat_top_descending": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr]),
     "dsplat_gram_workspace": (c_size, [c_int, c_int]),
     "dsplat_gram": (
         c_int,

@@ -254,20 +254,19 @@ def search_certificate(
 @torch.no_grad()
 def certificate_scan(models, residuals, candidates, mode="complex", top=256, chunk=None):
     """Certificates of every candidate against several scans of one chirp in one launch, and
-    the top best per scan (stable: ties keep the lower index).
+    the top best per scan (stable: ties keep the lower index), both on the device: the scan,
+    a float64 scoring launch and one top selection per scan.
 
     models are FMCWModels sharing the chirp and material; residuals [A_i, K]. Returns
-    (scores [n, C] float32, top ids [n, top] int64)."""
-    from .kernels.surfel import scan_batched
+    (scores [n, C] float32, top ids [n, min(top, C)] int64)."""
+    from .kernels.search import certificate_scores, top_descending
+    from .kernels.surfel import scan_rows
 
     config = models[0].scan.fmcw_config
     scans = [(m.scan.scan_positions_m, r) for m, r in zip(models, residuals, strict=True)]
-    corr, norm = scan_batched(config, scans, candidates["centers"], candidates["normals"], models[0].fresnel_n, chunk)
-    area = candidates["areas"].double()
-    corr = corr.to(torch.complex128) * area.sqrt()
-    norm = norm.double() * area
-    power = torch.stack([r.abs().square().sum().double() for r in residuals]).clamp(min=1e-30)[:, None]
-    s = corr.real if mode == "real" else corr
-    scores = s.abs().square() / norm.clamp(min=1e-60) / power
-    best = torch.stack([torch.argsort(row, descending=True, stable=True)[:top] for row in scores])
-    return scores.float(), best
+    rows = scan_rows(config, scans, candidates["centers"], candidates["normals"], models[0].fresnel_n, chunk)
+    power = torch.stack([r.abs().square().sum().double() for r in residuals]).clamp(min=1e-30)
+    scores = certificate_scores(rows, candidates["areas"], power, mode == "real")
+    if scores.shape[1] == 0:
+        return scores.float(), scores.new_zeros((len(scans), 0), dtype=torch.int64)
+    return scores.float(), top_descending(scores, top)

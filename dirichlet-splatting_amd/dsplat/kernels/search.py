@@ -40,6 +40,37 @@ def peaks(scores, centers, count, separation):
     return output[output >= 0].long()
 
 
+TOP_MAX = 4096  # DSPLAT_TOP_MAX
+
+
+def certificate_scores(rows, areas, power, real_part):
+    """float64 certificates [n, C] of scan rows [n, C, 3] (corr.re, corr.im, norm) with area:
+    |s|^2 area / max(norm area, 1e-60) / power (certificate.py:11-31)."""
+    n, count = rows.shape[:2]
+    scores = torch.empty((n, count), dtype=torch.float64, device=rows.device)
+    _lib.call(
+        "dsplat_certificate_scores", _lib.ptr(rows.float().contiguous()), n, count,
+        _lib.ptr(areas.float().contiguous()), _lib.ptr(power.double().contiguous()), int(real_part),
+        _lib.ptr(scores), _lib.stream(rows.device),
+    )  # fmt: skip
+    return scores
+
+
+def top_descending(scores, top):
+    """Indices [rows, min(top, count)] of each row's largest float64 scores [rows, count],
+    descending, ties to the lower index: argsort(descending=True, stable=True)[:, :top] by one
+    selection launch."""
+    rows, count = scores.shape
+    if not 1 <= top <= TOP_MAX:
+        raise ValueError(f"top must lie in [1, {TOP_MAX}], got {top}")
+    out = torch.empty((rows, min(top, count)), dtype=torch.int64, device=scores.device)
+    _lib.call(
+        "dsplat_top_descending", _lib.ptr(scores.double().contiguous()), rows, count, int(top), _lib.ptr(out),
+        _lib.stream(scores.device),
+    )  # fmt: skip
+    return out
+
+
 def direction(inverse, gradient, normals, mask, radius):
     """Quasi-Newton descent per candidate: tangent to unit normals, inside the trust radius
     (kernels/search.py:42-56)."""

@@ -410,6 +410,12 @@ def scan_batched(config, scans, centers, normals, fresnel_n, chunk=None):
     """corr [n, C] complex and norm [n, C] of every candidate against each of n scans
     (positions [A_i, 3], residual [A_i, K]) in one launch over uneven aperture counts,
     without area. chunk None autotunes the candidates per workgroup."""
+    out = scan_rows(config, scans, centers, normals, fresnel_n, chunk)
+    return torch.complex(out[..., 0], out[..., 1]), out[..., 2]
+
+
+def scan_rows(config, scans, centers, normals, fresnel_n, chunk=None):
+    """The launch of scan_batched: rows [n, C, 3] float32 = (corr.re, corr.im, norm)."""
     import ctypes
 
     from .. import autotune
@@ -440,4 +446,4 @@ def scan_batched(config, scans, centers, normals, fresnel_n, chunk=None):
                f"|S{config.adc_samples}|F{cfg.splat_form}")
         chunk = autotune.tune("scan_batched", key, list(SCAN_CHUNKS), run, centers.device)
     run(chunk)
-    return torch.complex(out[..., 0], out[..., 1]), out[..., 2]
+    return out

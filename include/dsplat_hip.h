@@ -175,6 +175,25 @@ int dsplat_scan_batched(const dsplat_fmcw* cfg, int32_t num_scans, const float* 
                         const float* normals, int32_t num_candidates, int32_t chunk, float* out,
                         void* workspace, size_t workspace_bytes, void* stream);
 
+/* Certificates of a batched scan with area (certificate.py:11-31): scores [num_scans, C]
+ * float64 = |s|^2 area / max(norm area, 1e-60) / power[i], s = corr (real_part 0) or
+ * Re corr (real_part 1), from scan_out [num_scans, C, 3] of dsplat_scan_batched, areas [C]
+ * and power [num_scans] (device, float64: the residual energies, floored by the caller). */
+int dsplat_certificate_scores(const float* scan_out, int32_t num_scans, int32_t num_candidates,
+                              const float* areas, const double* power, int32_t real_part, double* scores,
+                              void* stream);
+
+/* DSPLAT_TOP_MAX: the largest `top` of dsplat_top_descending. */
+#define DSPLAT_TOP_MAX 4096
+
+/* Indices of the min(top, count) largest entries of every row of scores [rows, count]
+ * float64 in descending order, ties to the lower index, NaN first: the stable descending
+ * argsort cut to its head that ranks certificates (certificate.py:52-56, 196-201), one
+ * workgroup per row instead of a full sort. out [rows, min(top, count)] int64;
+ * 1 <= top <= DSPLAT_TOP_MAX. */
+int dsplat_top_descending(const double* scores, int32_t rows, int32_t count, int32_t top, int64_t* out,
+                          void* stream);
+
 /* ---- Spectral atoms: separable windowed Dirichlet kernels on a 1-3-D grid, bin units.
  * dim axes with samples[dim], nfft[dim]; window 0 rect, 1 hann, 2 hamming, 3 kaiser
  * (then taps [dim, taps_stride] are the normalized window taps); radius > 0 truncates.

@@ -47,6 +47,12 @@ def test_rejected_configurations_return_invalid_value_without_launching():
     bad.num_bins = 100  # K must be n_fft / 2
     args = [None, 1, None, None, None, 1, None, None, None, None, None, 0, None]
     assert lib.dsplat_surfel_vjp(ctypes.byref(bad), *args) == 1
+    # the top selection holds at most DSPLAT_TOP_MAX (4096) in LDS and takes at least one
+    for top in (0, 4097):
+        assert lib.dsplat_top_descending(None, 1, 10, top, None, None) == 1
+    assert lib.dsplat_top_descending(None, 1, 10, 256, None, None) == 1  # null buffers
+    assert lib.dsplat_top_descending(None, 0, 10, 256, None, None) == 0  # nothing to rank
+    assert lib.dsplat_certificate_scores(None, 1, 10, None, None, 0, None, None) == 1
 
 
 def test_vjp_workspace_per_splat_form():

@@ -50,3 +50,49 @@ def test_gram_matches_the_oracle_design():
     for mode in ("fp32", "fp16x3"):
         g = blas.gram(cfg, positions, centers, normals, areas, mode).cpu().numpy()
         assert rel_l2(g, dense) < BARS[mode]
+
+
+# (G, coefficients) bars at 2^21 rows, about twice to three times the measured errors:
+# fp32 2.2e-5 / 1.2e-3, bf16x3 3.0e-6 / 1.8e-4, fp16x3 1.6e-6 / 7.0e-5
+FULL_BARS = {"fp32": (5e-5, 3e-3), "bf16x3": (1e-5, 5e-4), "fp16x3": (5e-6, 2e-4)}
+
+
+def test_full_size_gram_and_solve_against_float64():
+    """The Gram as benchmarked: 128^2 apertures x 128 bins (2^21 rows) x the 2048-surfel
+    ellipsoid dictionary, seed-3 right-hand side. Each mode's G = A^H A / M against float64
+    products of the same float32 atoms, A^H r by the correlation kernel, and the coefficients
+    of a ridge-1e-2 solve (ridge relative to the mean diagonal) against the float64 solve."""
+    from dsplat import workloads
+    from dsplat.kernels import blas, surfel
+    from dsplat.varpro import solve
+
+    work = workloads.ellipsoid_scene()
+    cfg = work["config"]
+    pos, c, n, areas = (work[k].cuda() for k in ("positions", "centers", "normals", "areas"))
+    A, K, S = len(pos), cfg.num_range_bins, len(c)
+    M = A * K
+    residual = workloads.complex_normal((M,), torch.Generator().manual_seed(3)).cuda()
+    g64 = torch.zeros((S, S), dtype=torch.complex128, device="cuda")
+    q64 = torch.zeros(S, dtype=torch.complex128, device="cuda")
+    for start in range(0, A, 256):
+        a = surfel.design(cfg, pos[start : start + 256], c, n, areas, 0.0, False).to(torch.complex128)
+        g64 += a.H @ a
+        q64 += a.H @ residual[start * K : (start + 256) * K].to(torch.complex128)
+        del a
+    g64, q64 = g64 / M, q64 / M
+    corr, _ = surfel.correlate(cfg, pos, c, n, residual.reshape(A, K), 0.0)
+    q = (corr * areas.sqrt()).to(torch.complex128) / M
+    q_err = float((q - q64).norm() / q64.norm())
+    print(f"rhs: rel {q_err:.3e}", flush=True)
+    assert q_err < 1e-5
+    eye = torch.eye(S, dtype=torch.complex128, device="cuda")
+    ridge = 1e-2 * float(g64.diagonal().real.mean())
+    b64 = solve(g64 + ridge * eye, q64)
+    for mode in ("fp32", "bf16x3", "fp16x3"):
+        g = blas.gram(cfg, pos, c, n, areas, mode, scale=1.0 / M).to(torch.complex128)
+        err = float((g - g64).norm() / g64.norm())
+        b = solve(g + ridge * eye, q)
+        b_err = float((b - b64).norm() / b64.norm())
+        print(f"gram {mode}: rel Frobenius {err:.3e}, coefficients {b_err:.3e}", flush=True)
+        assert err < FULL_BARS[mode][0]
+        assert b_err < FULL_BARS[mode][1]
