@@ -706,6 +706,20 @@ int dsplat_top_descending(const double* scores, int32_t rows, int32_t count, int
     return launch_top_descending(scores, rows, count, top, out, (hipStream_t)stream);
 }
 
+int dsplat_pcg_start(const float* rhs, const float* diagonal, int32_t n, float tolerance, float* x, float* residual,
+                     float* direction, float* state, void* stream) {
+    if (n < 0 || !state) return hipErrorInvalidValue;
+    if (n > 0 && (!rhs || !diagonal || !x || !residual || !direction)) return hipErrorInvalidValue;
+    return launch_pcg_start(rhs, diagonal, n, tolerance, x, residual, direction, state, (hipStream_t)stream);
+}
+
+int dsplat_pcg_step(const float* image, const float* diagonal, int32_t n, float* x, float* residual,
+                    float* direction, float* state, void* stream) {
+    if (n < 0 || !state) return hipErrorInvalidValue;
+    if (n > 0 && (!image || !diagonal || !x || !residual || !direction)) return hipErrorInvalidValue;
+    return launch_pcg_step(image, diagonal, n, x, residual, direction, state, (hipStream_t)stream);
+}
+
 int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stream) {
     if (n < 0 || lda < n || !info) return hipErrorInvalidValue;
     if (n == 0) return fill_zero(reinterpret_cast<float*>(info), 1, (hipStream_t)stream);

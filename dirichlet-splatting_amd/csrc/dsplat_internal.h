@@ -50,6 +50,11 @@ constexpr int kTopMax = 4096;  // LDS holds the selection
 hipError_t launch_certificate_scores(const float* scan, int n, int C, const float* areas, const double* power,
                                      int real_part, double* scores, hipStream_t st);
 hipError_t launch_top_descending(const double* scores, int rows, int count, int top, int64_t* out, hipStream_t st);
+// Conjugate gradients with the stopping tests on the device (one workgroup per launch).
+hipError_t launch_pcg_start(const float* rhs, const float* diagonal, int n, float tolerance, float* x, float* r,
+                            float* d, float* state, hipStream_t st);
+hipError_t launch_pcg_step(const float* image, const float* diagonal, int n, float* x, float* r, float* d,
+                           float* state, hipStream_t st);
 
 hipError_t launch_design(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float* areas, int S, int complex_layout, float* out, hipStream_t st);

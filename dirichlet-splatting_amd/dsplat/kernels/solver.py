@@ -57,3 +57,26 @@ def cholesky_solve(rhs, factor):
         _lib.stream(factor.device),
     )  # fmt: skip
     return x.t()
+
+
+def pcg_start(rhs, diagonal, tolerance):
+    """(x, residual, direction, state) of conjugate gradients from zero on float32 device
+    vectors: x = 0, residual = rhs, direction = rhs / diagonal; state [4] float32 = (rz,
+    running flag, goal = tolerance |rhs|, 0) (varpro.py:206-214)."""
+    rhs, diagonal = rhs.float().contiguous(), diagonal.float().contiguous()
+    x, residual, direction = (torch.empty_like(rhs) for _ in range(3))
+    state = rhs.new_empty(4)
+    _lib.call(
+        "dsplat_pcg_start", _lib.ptr(rhs), _lib.ptr(diagonal), len(rhs), float(tolerance), _lib.ptr(x),
+        _lib.ptr(residual), _lib.ptr(direction), _lib.ptr(state), _lib.stream(rhs.device),
+    )  # fmt: skip
+    return x, residual, direction, state
+
+
+def pcg_step(image, diagonal, x, residual, direction, state):
+    """One conjugate-gradient iteration in place after image = H direction, the stopping
+    tests on the device (varpro.py:215-228); a stopped iteration changes nothing."""
+    _lib.call(
+        "dsplat_pcg_step", _lib.ptr(image.float().contiguous()), _lib.ptr(diagonal), len(x), _lib.ptr(x),
+        _lib.ptr(residual), _lib.ptr(direction), _lib.ptr(state), _lib.stream(x.device),
+    )  # fmt: skip

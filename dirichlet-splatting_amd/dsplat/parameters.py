@@ -38,6 +38,9 @@ def _onto_unit_sphere(value, reference, free):
     return frozen_part + direction * budget / length.clamp(min=1e-30)
 
 
+_MASKS: dict = {}
+
+
 @dataclass(frozen=True)
 class ParameterConfig:
     trainable: bool | tuple[bool, ...] = True  # one flag for the field, or one per component
@@ -48,9 +51,14 @@ class ParameterConfig:
     normalize: bool = False  # the field holds unit vectors (surfel normals)
 
     def mask(self, value):
-        """bool [D]: the components a step may change (D = 1 for a scalar field)."""
+        """bool [D]: the components a step may change (D = 1 for a scalar field). Masks are
+        made once per (flags, width, device) and shared read-only, so a solver iteration
+        copies nothing from the host (and can be captured in a HIP graph)."""
         width = value.shape[-1] if value.ndim == 2 else 1
-        return torch.as_tensor(self.trainable, dtype=torch.bool, device=value.device).expand(width)
+        key = (self.trainable, width, str(value.device))
+        if key not in _MASKS:
+            _MASKS[key] = torch.as_tensor(self.trainable, dtype=torch.bool, device=value.device).expand(width)
+        return _MASKS[key]
 
     def project(self, proposed, reference):
         """The feasible value nearest proposed: wrapped, clamped to the box, frozen components

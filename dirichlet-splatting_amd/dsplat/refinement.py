@@ -102,11 +102,14 @@ class _JointSystem:
         self.complex_mode = not config.real
         count = len(self.b)
         self.count = count
+        # (key, free components as indices or None for all, slice of the unknowns): the
+        # indices are formed once, so no solver iteration waits on a boolean mask
         self.sections, offset = [], 0
         for key, spec in trainable(params, specs):
             free = spec.mask(params[key])
+            columns = None if bool(free.all()) else free.nonzero().flatten()
             width = count * int(free.sum())
-            self.sections.append((key, free, slice(offset, offset + width)))
+            self.sections.append((key, columns, slice(offset, offset + width)))
             offset += width
         self.geometry = offset
         self.size = offset + (2 * count if self.complex_mode else count)
@@ -140,13 +143,21 @@ class _JointSystem:
         diagonal = total / self.config.cg_probes + self.ridge_diagonal
         return diagonal.clamp(min=diagonal.max().clamp(min=1e-30) * 1e-6)
 
+    @staticmethod
+    def _free(grid, columns):
+        return grid if columns is None else grid[:, columns]
+
     def unpack(self, vector):
         """A dimensionless vector as (parameter tangents, coefficient change)."""
         tangent = {key: torch.zeros_like(value) for key, value in self.params.items()}
-        for key, free, section in self.sections:
+        for key, columns, section in self.sections:
             value = tangent[key]
             grid = value[:, None] if value.ndim == 1 else value
-            grid[:, free] = vector[section].reshape(self.count, -1) * self.specs[key].scale
+            block = vector[section].reshape(self.count, -1) * self.specs[key].scale
+            if columns is None:
+                grid.copy_(block)
+            else:
+                grid[:, columns] = block
             tangent[key] = self.specs[key].tangent(value, self.params[key])
         tail = vector[self.geometry :]
         if self.complex_mode:
@@ -157,10 +168,10 @@ class _JointSystem:
         """J^H upstream in the dimensionless unknowns (the model's VJP, chained)."""
         grads, by_coefficient = self.model.vjp(self.params, self.b, upstream)
         out = self.amplitude.new_zeros(self.size)
-        for key, free, section in self.sections:
+        for key, columns, section in self.sections:
             value = self.specs[key].tangent(grads[key], self.params[key])
             grid = value[:, None] if value.ndim == 1 else value
-            out[section] = (grid[:, free] * self.specs[key].scale).flatten()
+            out[section] = (self._free(grid, columns) * self.specs[key].scale).flatten()
         tail = by_coefficient * self.coefficient_scale
         out[self.geometry :] = torch.view_as_real(tail).flatten() if self.complex_mode else tail.real
         return out
@@ -175,9 +186,9 @@ class _JointSystem:
         atom leaves the trust radius, then projected onto the constraints."""
         tangent, _ = self.unpack(vector)
         length2 = self.amplitude.new_zeros(self.count)
-        for key, free, _ in self.sections:
+        for key, columns, _ in self.sections:
             grid = tangent[key][:, None] if tangent[key].ndim == 1 else tangent[key]
-            length2 += (grid[:, free] / self.specs[key].scale).square().sum(1)
+            length2 += (self._free(grid, columns) / self.specs[key].scale).square().sum(1)
         shrink = (self.config.trust_radius / length2.sqrt().max().clamp(min=1e-30)).clamp(max=1)
         return {key: self.specs[key].project(value + tangent[key] * (shrink * fraction), value)
                 for key, value in self.params.items()}  # fmt: skip
@@ -203,7 +214,11 @@ def joint_slide(state, config, screen=None):
         matvec, gradient, diagonal, proposal = joint_system(model, state.params, state.coefficients, target,
                                                             state.parameters, config,
                                                             config.seed + state.iteration + inner)  # fmt: skip
-        delta = pcg(matvec, -gradient, diagonal, iterations=config.cg_iterations, tolerance=config.cg_tolerance)
+        # the batched form runs the iteration's algebra as one kernel and replays the
+        # iteration from a HIP graph
+        device_loop = getattr(config, "batched", False) and gradient.is_cuda
+        delta = pcg(matvec, -gradient, diagonal, iterations=config.cg_iterations, tolerance=config.cg_tolerance,
+                    fused=device_loop, graph=device_loop)  # fmt: skip
         if not (torch.isfinite(delta).all() and delta @ gradient < 0):
             break
         bar = screen_bar(screen, state.params, config)

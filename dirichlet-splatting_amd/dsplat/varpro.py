@@ -267,27 +267,73 @@ def accept(state, proposal, config, screen=None):
     return replace(state, params=proposal, **fit._asdict())
 
 
-def pcg(operator, rhs, diagonal, *, iterations=32, tolerance=1e-3):
+def pcg(operator, rhs, diagonal, *, iterations=32, tolerance=1e-3, fused=False, graph=False):
     """Jacobi-preconditioned conjugate gradients from zero: stops at the first direction of
     non-positive curvature or once the residual is tolerance times the right-hand side
-    (varpro.py:206-228)."""
+    (varpro.py:206-228).
+
+    The two stopping tests stay on the device: a flag turns every later update into a no-op
+    (torch.where, so values after a stop, NaN included, never leak), and the loop always runs
+    `iterations` operator applications. The result equals the early-exit loop's bit for bit,
+    and nothing waits for the host between iterations.
+
+    fused runs each iteration's vector algebra as one kernel (float32 vectors, float64 sums,
+    on float32 device vectors); graph then also replays the operator and that kernel from a
+    HIP graph captured once per call (the operator must launch only on the current stream,
+    allocate only through torch and never synchronize)."""
+    if fused:
+        return _pcg_device(operator, rhs, diagonal, iterations, tolerance, graph)
     x = torch.zeros_like(rhs)
     residual = rhs.clone()
     goal = tolerance * rhs.norm()
     z = residual / diagonal
     direction = z.clone()
     rz = residual @ z
+    active = torch.ones((), dtype=torch.bool, device=rhs.device)
     for _ in range(iterations):
         image = operator(direction)
         curvature = direction @ image
-        if not curvature > 0:
-            break
-        step = rz / curvature
-        x += step * direction
-        residual -= step * image
-        if residual.norm() <= goal:
-            break
+        moving = active & (curvature > 0)
+        step = torch.where(moving, rz / curvature, 0)
+        x = torch.where(moving, x + step * direction, x)
+        residual = torch.where(moving, residual - step * image, residual)
+        active = moving & ~(residual.norm() <= goal)
         z = residual / diagonal
-        rz, previous = residual @ z, rz
-        direction = z + (rz / previous) * direction
+        rz_next = residual @ z
+        direction = torch.where(active, z + (rz_next / rz) * direction, direction)
+        rz = torch.where(active, rz_next, rz)
+    return x
+
+
+def _pcg_device(operator, rhs, diagonal, iterations, tolerance, graph):
+    """pcg on the fused iteration kernel: the first iteration eagerly, then, with graph, the
+    next one captured on a side stream and replayed for the rest."""
+    from .kernels.solver import pcg_start, pcg_step
+
+    diagonal = diagonal.float().contiguous()
+    x, residual, direction, state = pcg_start(rhs, diagonal, tolerance)
+
+    def iteration():
+        pcg_step(operator(direction), diagonal, x, residual, direction, state)
+
+    if iterations < 1:
+        return x
+    iteration()
+    if not graph or iterations == 1:
+        for _ in range(iterations - 1):
+            iteration()
+        return x
+    device = rhs.device
+    side = torch.cuda.Stream(device)
+    side.wait_stream(torch.cuda.current_stream(device))
+    captured = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        captured.capture_begin()
+        try:
+            iteration()
+        finally:
+            captured.capture_end()
+    torch.cuda.current_stream(device).wait_stream(side)
+    for _ in range(iterations - 1):
+        captured.replay()
     return x

@@ -183,6 +183,20 @@ int dsplat_certificate_scores(const float* scan_out, int32_t num_scans, int32_t 
                               const float* areas, const double* power, int32_t real_part, double* scores,
                               void* stream);
 
+/* Jacobi-preconditioned conjugate gradients from zero with both stopping tests on the
+ * device (varpro.py:206-228 pcg, as refinement.py:193-227 calls it): the reference tests
+ * curvature <= 0 and |r| <= tolerance |rhs| on the host every iteration; here a flag in
+ * state makes every update after a stop a no-op, so the caller runs a fixed number of
+ * iterations and nothing waits for the host (the operator and dsplat_pcg_step can be
+ * captured in a HIP graph). float32 vectors [n] on the device, float64 sums.
+ * dsplat_pcg_start: x = 0, residual = rhs, direction = rhs / diagonal; state [4] floats
+ * (rz, running flag, goal, reserved). dsplat_pcg_step, after image = H direction: one
+ * iteration's updates of x, residual, direction and state. */
+int dsplat_pcg_start(const float* rhs, const float* diagonal, int32_t n, float tolerance, float* x, float* residual,
+                     float* direction, float* state, void* stream);
+int dsplat_pcg_step(const float* image, const float* diagonal, int32_t n, float* x, float* residual,
+                    float* direction, float* state, void* stream);
+
 /* DSPLAT_TOP_MAX: the largest `top` of dsplat_top_descending. */
 #define DSPLAT_TOP_MAX 4096
 

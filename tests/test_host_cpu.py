@@ -84,6 +84,56 @@ def test_pcg_solves_a_spd_system():
     torch.testing.assert_close(solution, torch.linalg.solve(h, b), atol=1e-9, rtol=1e-9)
 
 
+def _pcg_early_exit(operator, rhs, diagonal, iterations, tolerance):
+    """Conjugate gradients that leave the loop at either test, for comparison."""
+    x, residual = torch.zeros_like(rhs), rhs.clone()
+    goal = tolerance * rhs.norm()
+    z = residual / diagonal
+    direction, rz = z.clone(), residual @ z
+    for _ in range(iterations):
+        image = operator(direction)
+        curvature = direction @ image
+        if not curvature > 0:
+            break
+        step = rz / curvature
+        x = x + step * direction
+        residual = residual - step * image
+        if residual.norm() <= goal:
+            break
+        z = residual / diagonal
+        rz, previous = residual @ z, rz
+        direction = z + (rz / previous) * direction
+    return x
+
+
+@pytest.mark.parametrize("case", ["converges", "indefinite", "limit", "nan_after_stop"])
+def test_pcg_with_device_side_stops_equals_the_early_exit_loop(case):
+    """The stopping tests run as masks on the device; the result is the early-exit loop's bit
+    for bit, whether it stops on the tolerance, on non-positive curvature, at the iteration
+    limit, or before an operator that would turn to NaN."""
+    g = torch.Generator().manual_seed(11)
+    x = torch.randn(40, 12, dtype=torch.float64, generator=g)
+    h = x.T @ x + 0.05 * torch.eye(12, dtype=torch.float64)
+    if case == "indefinite":
+        h[5, 5] = -50.0
+    b = torch.randn(12, dtype=torch.float64, generator=g)
+    calls = []
+
+    def operator(v):
+        calls.append(1)
+        out = h @ v
+        return out * float("nan") if case == "nan_after_stop" and len(calls) > 3 else out
+
+    iterations, tolerance = {"converges": (200, 1e-6), "indefinite": (50, 1e-12), "limit": (4, 1e-12),
+                             "nan_after_stop": (50, 0.5)}[case]  # fmt: skip
+    got = pcg(operator, b, h.diag().abs(), iterations=iterations, tolerance=tolerance)
+    assert len(calls) == iterations
+    calls.clear()
+    want = _pcg_early_exit(operator, b, h.diag().abs(), iterations, tolerance)
+    assert len(calls) < iterations or case == "limit"
+    assert torch.equal(got, want)
+
+
 def test_unit_vectors_with_a_frozen_component_stay_unit():
     spec = ParameterConfig(trainable=(True, True, False), normalize=True)
     reference = torch.nn.functional.normalize(torch.tensor([[0.3, 0.4, -0.6]]), dim=1)

@@ -237,6 +237,67 @@ def test_joint_system_is_symmetric_and_pcg_matches_a_dense_solve(spectral):
     close(solution, torch.linalg.solve(dense, -gradient.double()), 1e-3)
 
 
+def test_fused_pcg_matches_a_dense_solve_and_the_tensor_iteration(spectral):
+    """The one-kernel iteration (float64 sums) converges to the dense solve and follows the
+    tensor-op iteration to rounding at the solver's 16 iterations."""
+    from dsplat import DSFWConfig, ParameterConfig
+    from dsplat.refinement import joint_system
+    from dsplat.varpro import pcg
+
+    model, params, b = spectral
+    target = model.render(params, b)
+    trial = {"centers": params["centers"] + 0.1}
+    config = DSFWConfig(coefficients="complex", ridge=1e-4)
+    matvec, gradient, diagonal, _ = joint_system(model, trial, b, target, {"centers": ParameterConfig()}, config, 0)
+    dense = torch.stack([matvec(column) for column in torch.eye(len(gradient), device="cuda")], 1).double()
+    solution = pcg(matvec, -gradient, diagonal, iterations=200, tolerance=1e-7, fused=True)
+    close(solution, torch.linalg.solve(dense, -gradient.double()), 1e-3)
+    close(pcg(matvec, -gradient, diagonal, iterations=16, fused=True), pcg(matvec, -gradient, diagonal, iterations=16),
+          1e-4)  # fmt: skip
+
+
+@pytest.mark.parametrize("case", ["indefinite", "converges"])
+def test_fused_pcg_stops_where_the_tensor_iteration_stops(case):
+    """Non-positive curvature stops before the update and a small residual after it, as in
+    the tensor-op iteration; later iterations change nothing."""
+    from dsplat.varpro import pcg
+
+    g = torch.Generator().manual_seed(5)
+    x = torch.randn(400, 120, generator=g)
+    h = (x.T @ x / 400 + 0.05 * torch.eye(120)).cuda()
+    if case == "indefinite":
+        h[7, 7] = -30.0
+    rhs = torch.randn(120, generator=g).cuda()
+    tolerance = 1e-12 if case == "indefinite" else 1e-2
+    want = pcg(lambda v: h @ v, rhs, h.diag().abs(), iterations=60, tolerance=tolerance)
+    got = pcg(lambda v: h @ v, rhs, h.diag().abs(), iterations=60, tolerance=tolerance, fused=True)
+    close(got, want, 1e-4)
+    longer = pcg(lambda v: h @ v, rhs, h.diag().abs(), iterations=90, tolerance=tolerance, fused=True)
+    assert torch.equal(longer, got)
+
+
+@pytest.mark.parametrize("iterations", [2, 5, 16])
+def test_pcg_graph_replays_equal_the_eager_iterations(fmcw, iterations):
+    """The FMCW joint system (JVP and VJP kernels, tangent projections, area chain): one
+    iteration captured in a HIP graph and replayed gives the eager fused iterations bit for bit,
+    run after run, with eager work and synchronizations between the runs."""
+    from dsplat import DSFWConfig
+    from dsplat.refinement import joint_system
+    from dsplat.varpro import pcg
+
+    model, params, b = fmcw
+    trial = dict(params, centers=params["centers"] + 2e-4)
+    matvec, gradient, diagonal, _ = joint_system(model, trial, b, model.scan.data, fmcw_specs(), DSFWConfig(), 0)
+    eager = pcg(matvec, -gradient, diagonal, iterations=iterations, fused=True)
+    assert torch.isfinite(eager).all() and eager.abs().max() > 0
+    for _ in range(3):
+        graphed = pcg(matvec, -gradient, diagonal, iterations=iterations, fused=True, graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(graphed, eager)
+        pcg(matvec, -gradient, diagonal, iterations=3)  # eager kernels and syncs between graph runs
+        torch.cuda.synchronize()
+
+
 def test_dense_lm_moves_tones_back(spectral):
     from dsplat import DSFWConfig, ParameterConfig, initialize
     from dsplat.refinement import dense_slide
