@@ -85,11 +85,12 @@ def parse():
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--cpu-seconds", type=float, default=8.0, help="target float64-oracle sample time")
     parser.add_argument(
-        "--workload", choices=("splat", "gram", "scan", "dsfw", "synth", "copy"), default="splat",
+        "--workload", choices=("splat", "gram", "scan", "dsfw", "synth", "syrk", "copy"), default="splat",
         help="splat: the fwd+VJP metric (default); gram: the split-precision Gram of the same scene; "
         "scan: the batched certificate scan of 64^3 candidates against 64^2, 96^2 and 128^2 scans; "
         "dsfw: DSFW steps, plain FP32 form against the mixed-precision batched form; "
         "synth: the float64 truth scan of the DSFW fit by the ADC signal chain on the FP64 matrix cores; "
+        "syrk: the split-operand SYRK of the mixed DSFW normal equations at two row counts; "
         "copy: a plain 1 GiB device copy, the reference size for the HBM byte counters",
     )  # fmt: skip
     parser.add_argument("--adc-samples", type=int, default=256, help="dsfw: ADC samples (bins = samples)")
@@ -630,8 +631,12 @@ def dsfw_compare(steps, warmup=1, adc_samples=256, truth_draws=4000, front_count
         record, _ = run(name, perturbed)
         control[name] = _relative_deviation(out[name]["nmse"], record["nmse"])
         out[name + "_perturbed"] = record
-    # identical geometries, fitted by both forms
-    fits = {name: setup(name, target) for name in DSFW_FORMS}
+    # identical geometries, fitted by both forms on the models the fits use (range subspace
+    # of the parameter box, as initialize prepares them)
+    fits = {}
+    for name in DSFW_FORMS:
+        model, config = setup(name, target)
+        fits[name] = (model.prepare(problem["parameters"]), config)
     power = float(target.abs().square().mean())
     same = []
     for params in states[::same_state_every] + states[-1:]:
@@ -650,6 +655,7 @@ def dsfw_compare(steps, warmup=1, adc_samples=256, truth_draws=4000, front_count
         "control_nmse_max_relative_deviation": control,
         "control_note": "same form, target + 1e-7 RMS complex noise: the search's own sensitivity",
         "same_state_nmse_max_relative_deviation": max(same),
+        "same_state_nmse_relative_deviation": same,
         "same_state_count": len(same),
         "forms": out,
     }  # fmt: skip
@@ -676,6 +682,57 @@ def dsfw_workload(args):
         "config": {"workload": result["workload"], "surfels": result["surfels"], "bins": result["bins"],
                    "parallelism": "single GPU"},
         "dsfw": result,
+    }
+
+
+SYRK_SHAPES = ((118784, 2048), (1048576, 2035))  # (rows, columns): 29 rows per aperture of 128^2 and 4096
+
+
+def syrk_workload(args):
+    """The split-operand SYRK of the mixed DSFW normal equations (upper triangle of A^T A,
+    FP16 hi + lo operands in 32-row column tiles, column scales 2^e_s, three products on the
+    FP16 matrix cores, float64 reduction), on seeded Gaussian blocks of 118784 x 2048 and
+    1048576 x 2035; beside it the FP32 SYRK of the same float32 rows. Executed work: 3 products
+    of m n (n + 1) FLOP over the upper tiles."""
+    import torch
+
+    from dsplat.kernels import blas
+
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream(dev)
+    shapes = {}
+    for m, n in SYRK_SHAPES:
+        rows = torch.randn(m, n, device=dev, generator=torch.Generator(device=dev).manual_seed(1))
+        exponents = torch.floor(torch.log2(2.0**14 / rows.abs().amax(0))).float()
+        block = blas.SplitBlock.of(rows, "fp16", exponents)
+        out = torch.zeros(n, n, device=dev)
+        split_ms, _ = timed(lambda block=block, out=out: blas.syrk_split(block, out, None), max(1, args.steps), stream)
+        fp32_ms, _ = timed(lambda rows=rows, out=out: blas.syrk(rows, out, "fp32"), 2, stream)
+        flop = float(m) * n * (n + 1)  # one product of the upper triangle
+        shapes[f"{m}x{n}"] = {
+            "split_fp16_ms": split_ms,
+            "split_fp16_executed_tflops": 3 * flop / split_ms / 1e9,
+            "split_fp16_frac_of_fp16_peak": 3 * flop / split_ms / 1e9 / 2516.6,
+            "fp32_ms": fp32_ms,
+            "fp32_frac_of_fp32_peak": flop / fp32_ms / 1e9 / 157.3,
+        }
+        del rows, block, out
+        torch.cuda.empty_cache()
+    big = shapes[f"{SYRK_SHAPES[1][0]}x{SYRK_SHAPES[1][1]}"]
+    return {
+        "metric": "split-operand SYRK ms (1048576 x 2035, FP16 hi+lo x3)",
+        "value": big["split_fp16_ms"],
+        "unit": "ms",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "fp16 hi+lo x3 MFMA, fp32 accumulate, f64 split-K reduction",
+        "data": "synthetic",
+        "config": {"workload": "upper-triangle A^T A of seeded Gaussian float32 blocks, pre-split FP16 hi/lo",
+                   "shapes": [list(x) for x in SYRK_SHAPES]},
+        "shapes": shapes,
+        "roofline": {"kernel": "syrk_split_kernel", "bound": "mfma",
+                     "achieved": big["split_fp16_executed_tflops"], "peak": 2516.6, "unit": "TFLOP/s",
+                     "frac": big["split_fp16_frac_of_fp16_peak"], "traffic": None},
     }
 
 
@@ -727,6 +784,9 @@ def main():
         return
     if args.workload == "synth":
         print(json.dumps(synth_workload(args)), flush=True)
+        return
+    if args.workload == "syrk":
+        print(json.dumps(syrk_workload(args)), flush=True)
         return
     if args.workload == "copy":
         print(json.dumps(copy_workload(args)), flush=True)

@@ -44,6 +44,8 @@ constexpr int kGK = 32;         // rows per stage
 constexpr int kGThreads = 512;  // 8 waves
 constexpr int kGSlots = kGT * kGK * 2 / 16;  // 16-byte slots per panel part (1024)
 constexpr int kGMinSplitRows = 2048;
+constexpr int kGMaxSplitRows = 32768;  // rows per float32 partial; the reduction beyond is float64
+constexpr int kGCUs = 256;             // one workgroup per CU at a time (128 KB of LDS)
 
 // First output row and column of tile t: the upper triangle (col0 < 0), or the rectangle
 // of columns col0 .. n - 1, row-major over tiles.
@@ -198,12 +200,18 @@ SplitShape syrk_split_shape(int m, int n, int col0) {
     SplitShape s;
     const int nb = (n + kGT - 1) / kGT;
     s.tiles = col0 < 0 ? nb * (nb + 1) / 2 : nb * ((n - col0 + kGT - 1) / kGT);
-    // one workgroup per CU (128 KB of LDS): as many row ranges as fill the 256 CUs once
-    int splits = s.tiles >= 256 ? 1 : 256 / s.tiles;
+    // One workgroup per CU: as many row ranges as fill whole rounds of the 256 CUs. Float32
+    // accumulators sum at most kGMaxSplitRows rows, so long blocks take more rounds; the
+    // ranges then fill those rounds.
+    const long long fold = ((long long)m + kGMaxSplitRows - 1) / kGMaxSplitRows;
+    const long long rounds = (s.tiles * (fold > 1 ? fold : 1) + kGCUs - 1) / kGCUs;
+    long long splits = rounds * kGCUs / s.tiles;
+    if (splits < fold) splits = fold;
     const int most = (m + kGMinSplitRows - 1) / kGMinSplitRows;
-    if (splits > most) splits = most;
+    if (fold <= 1 && splits > most) splits = most;
+    if (splits > 65535) splits = 65535;  // grid y
     if (splits < 1) splits = 1;
-    s.per_split = (m + splits - 1) / splits;
+    s.per_split = (int)((m + splits - 1) / splits);
     s.per_split = (s.per_split + kGK - 1) / kGK * kGK;
     s.splits = (m + s.per_split - 1) / s.per_split;
     if (s.splits < 1) s.splits = 1;

@@ -35,6 +35,7 @@ constexpr int kStrideF = kChunk + 4;  // floats per LDS column (MODE 0)
 constexpr int kStrideH = kChunkH + 8; // 16-bit values per LDS column (MODES 1, 2): 48 B, so the 16-byte
                                       // accesses of 16 consecutive columns fall on distinct bank groups
 constexpr int kMinSplitRows = 512;
+constexpr int kMaxSplitRows = 16384;  // rows per float32 partial; the reduction beyond is float64
 
 template <int MODE>
 struct Half16;
@@ -343,6 +344,10 @@ static SyrkShape syrk_shape(int m, int n, int col0) {
     int splits = (1024 + s.tiles - 1) / s.tiles;
     const int most = (m + kMinSplitRows - 1) / kMinSplitRows;
     if (splits > most) splits = most;
+    // float32 accumulators sum at most kMaxSplitRows rows: long blocks get more ranges
+    const int fold = (int)(((long long)m + kMaxSplitRows - 1) / kMaxSplitRows);
+    if (splits < fold) splits = fold;
+    if (splits > 65535) splits = 65535;  // grid y
     if (splits < 1) splits = 1;
     s.per_split = (m + splits - 1) / splits;
     s.per_split = (s.per_split + kChunk - 1) / kChunk * kChunk;

@@ -44,7 +44,7 @@ def fit(problem, form, extra, steps, batch, target=None):
     for _ in range(steps):
         state = step(state, config, SpatialSearchConfig())
         states.append(state.params)
-    return state, states, (model, config)
+    return state, states, (state.model, config)  # the prepared model the fit ran on
 
 
 @pytest.mark.parametrize("samples", [128, 256])
