@@ -173,6 +173,31 @@ def test_correlations_equal_dictionary_products():
     np.testing.assert_array_equal(real[(2 * 7 + 1) * 16 + 3], a[3 * 128 + 7].imag)
 
 
+def test_certificate_closed_forms():
+    """The certificate is a squared cosine (Cauchy-Schwarz): 1 for a residual along the atom's
+    own column, cos^2(phi) in the real mode for that column turned by e^(i phi), 0 for a
+    residual orthogonal to it, and in [0, 1] for any residual."""
+    cfg = config()
+    positions = aperture_grid(4)
+    p = surfels(3)
+    a = O.columns(positions, p["centers"], p["normals"], cfg)  # [A, K, S]
+    rng = np.random.default_rng(5)
+    for phi in (0.0, 0.4, 1.2, math.pi / 2):
+        r = 2.5 * np.exp(1j * phi) * a[..., 1]
+        corr, norm = O.correlate(positions, p["centers"][1:2], p["normals"][1:2], r, cfg)
+        np.testing.assert_allclose(O.certificate(corr, norm, r, "complex"), 1.0, rtol=1e-12)
+        np.testing.assert_allclose(O.certificate(corr, norm, r, "real"), math.cos(phi) ** 2, rtol=1e-10, atol=1e-15)
+    column = a[..., 0].ravel()
+    noise = rng.standard_normal(column.shape) + 1j * rng.standard_normal(column.shape)
+    orthogonal = (noise - column * (column.conj() @ noise) / (column.conj() @ column)).reshape(a.shape[:2])
+    corr, norm = O.correlate(positions, p["centers"][:1], p["normals"][:1], orthogonal, cfg)
+    assert O.certificate(corr, norm, orthogonal, "complex")[0] < 1e-24
+    r = noise.reshape(a.shape[:2])
+    corr, norm = O.correlate(positions, p["centers"], p["normals"], r, cfg)
+    value = O.certificate(corr, norm, r, "complex")
+    assert np.all((value >= 0) & (value <= 1))
+
+
 def test_spectral_atoms_equal_a_brute_force_windowed_dft():
     """A 2-D atom is the product of per-axis tapped DFT sums (pins oracle.spectral)."""
     from oracle import spectral as Sp
