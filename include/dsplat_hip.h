@@ -359,8 +359,8 @@ int dsplat_project_sampled_split(const dsplat_fmcw* cfg, const float* positions,
  * given as 16-bit hi / lo arrays in 32-row column tiles (element (k, c) at
  * ((k / 32) ld + c) 32 + k % 32, ld >= n, m a multiple of 32, 16-byte aligned): BF16 for
  * format 0; FP16 of column c scaled by 2^column_exponents[c] for format 1 (the products are
- * unscaled by 2^-(e_i + e_j)). Products hi hi + hi lo + lo hi on MFMA with FP32 accumulation,
- * split-K partials summed in float64 in a fixed order. Deterministic. Workspace from
+ * unscaled by 2^-(e_i + e_j)). Products hi hi + hi lo + lo hi on MFMA with FP32 accumulation
+ * over at most 32768 rows, split-K partials summed in float64 in a fixed order. Deterministic. Workspace from
  * dsplat_syrk_split_workspace. Replaces kernels/cublas.py:31-44 (cublasSsyrk_v2) and the
  * streamed addmm_ of replacement.py:178-199 on projected blocks. */
 size_t dsplat_syrk_split_workspace(int32_t m, int32_t n, int32_t col0);
