@@ -26,7 +26,10 @@ struct CorShape {
     int groups;  // candidate chunks
     int chunk;   // candidates per chunk
 };
-CorShape cor_shape(int A, int S);
+// Aperture blocks and candidate chunks for about `target` workgroups (the workspace depends on
+// the blocks only). Correlations aim at one round of their LDS-bound workgroups.
+constexpr int kCorRound = 512;
+CorShape cor_shape(int A, int S, int target = 4096);
 
 constexpr int kMaxScans = 8;
 // A set of scans of one chirp with uneven aperture counts, launched together.

@@ -313,10 +313,10 @@ hipError_t launch_reduce_partials(const float* partial, int rows, int nblk, int 
 }
 
 // ---------------------------------------------------------------------------------------
-CorShape cor_shape(int A, int S) {
+CorShape cor_shape(int A, int S, int target) {
     CorShape v;
     v.nblk = (A + kCorApertures - 1) / kCorApertures;
-    int groups = (4096 + v.nblk - 1) / v.nblk;
+    int groups = (target + v.nblk - 1) / v.nblk;
     const int max_groups = (S + 7) / 8;
     groups = groups < 1 ? 1 : (groups > max_groups ? max_groups : groups);
     const int chunk = (S + groups - 1) / groups;
@@ -332,7 +332,9 @@ size_t cor_smem(const Chirp& c) {
 template <int P, int W>
 static hipError_t launch_cor_pw(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, int S,
                                 const float2* res, float* partial, float* out, hipStream_t st) {
-    const CorShape v = cor_shape(A, S);
+    // one round of workgroups (two per CU: ~70 KB of LDS each), candidates split only as far
+    // as that needs, so each workgroup loads its residual tile once for many candidates
+    const CorShape v = cor_shape(A, S, kCorRound);
     const size_t smem = cor_smem(c);
     hipError_t e = allow_lds((const void*)correlate_kernel<P, W>, smem);
     if (e != hipSuccess) return e;

@@ -280,6 +280,12 @@ def correlate_orientations(config, positions, centers, normals, bank, residual, 
     return torch.complex(output[..., 0], output[..., 1]), output[..., 2]
 
 
+def sample_domain(config, fresnel_n=0.0):
+    """Whether the sample-domain kernels serve this chirp (its splat form and ADC samples)."""
+    cfg = _lib.fmcw(config, fresnel_n)
+    return _lib.library().dsplat_project_form(_lib.ctypes.byref(cfg), 2) == 0
+
+
 def sample_basis(config, basis, fresnel_n=0.0):
     """Sample-domain form [rows, 2 * adc_samples] of an interleaved real row basis [rows, 2K]
     (B = U^H F, F[k][n] = e^{-2 pi i n k / N}, float64 sums) for project_design's `sampled`

@@ -202,14 +202,14 @@ def pooled_statistics(state, pool, config):
     gram = state.gram.new_empty((total, total))
     rhs = state.rhs.new_empty(total)
     gram[:count, :count], rhs[:count] = state.gram, state.rhs
-    if config.real and getattr(config, "batched", False) and hasattr(model, "correlate"):
+    if config.real and getattr(config, "batched", False) and hasattr(model, "adjoint_correlation"):
         # cross products of the new columns with all of them on the MFMA block products, their
         # right-hand side from one correlation
         acc = _Folded(total, added, state.target.device)
         block_products(model, pool, state.target, config, acc, col0=count)
         cross = (acc.result() / model.size).float()
         fresh = {key: value[count:] for key, value in pool.items()}
-        right = (model.correlate(fresh, state.target)[0].real.double() / model.size).float()
+        right = (model.adjoint_correlation(fresh, state.target).real.double() / model.size).float()
     else:
         cross = state.gram.new_zeros((total, added))
         right = state.rhs.new_zeros(added)

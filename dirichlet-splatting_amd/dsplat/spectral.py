@@ -129,6 +129,10 @@ class SpectralModel:
                              flat, bins)  # fmt: skip
         return corr, self._norms(centers)
 
+    def adjoint_correlation(self, params, residual):
+        """A^H r of every atom (correlate's first output)."""
+        return self.correlate(params, residual)[0]
+
     def blocks(self, params, target, real, chunk):
         """Least-squares blocks (design rows, observations) of `chunk` bins; the real form
         stacks the real and imaginary rows of each bin."""

@@ -13,6 +13,8 @@ bin r / spacing. The kernels take one complex weight per surfel, w = b sqrt(area
 module maps (b, area) to w and chains every derivative back through it.
 """
 
+import torch
+
 from .kernels import surfel as kernel
 
 
@@ -94,6 +96,14 @@ def correlate(config, positions, params, residual, fresnel_n=0.0):
     corr, norm = kernel.correlate(config, positions, *_geometry(params), residual, fresnel_n)
     foot = _Footprint(params)
     return corr * foot.root, norm * foot.area
+
+
+def adjoint_correlation(config, positions, params, residual, fresnel_n=0.0):
+    """A^H r of every surfel's column (sqrt(area) included): correlate's first output, as the
+    coefficient gradient of Re<r, A b> at b = 1, which the VJP kernel forms on the matrix cores
+    in the sample domain (kernels/surfel.py:123-137 run for kernels/surfel.py:242-260)."""
+    ones = params["areas"].new_ones(len(params["areas"]), dtype=torch.complex64)
+    return vjp(config, positions, params, ones, residual, fresnel_n)[1]
 
 
 def correlate_gradient(config, positions, params, residual, fresnel_n=0.0, include_normals=False):

@@ -142,7 +142,7 @@ def block_products(model, params, target, config, out, col0=None):
     count = len(params["centers"])
     chunk = row_chunk(model, count, config)
     mode = SYRK_MODE[getattr(config, "gram", "plain")]
-    batched = getattr(config, "batched", False) and hasattr(model, "correlate")
+    batched = getattr(config, "batched", False) and hasattr(model, "adjoint_correlation")
     if batched and mode in ("bf16x3", "fp16x3") and hasattr(model, "split_blocks"):
         split = model.split_blocks(params, chunk, {"bf16x3": "bf16", "fp16x3": "fp16"}[mode])
         if split is not None:
@@ -189,9 +189,9 @@ def normal_equations(model, params, target, config):
     total = acc.result()
     gram = (torch.triu(total) + torch.triu(total, 1).T) / model.size
     if rhs is None:
-        # A^H y by one correlation over every aperture and bin: the range subspace holds every
+        # A^H y by one adjoint over every aperture and bin: the range subspace holds every
         # atom, so projecting y first changes nothing
-        rhs = model.correlate(params, target)[0].real.double()
+        rhs = model.adjoint_correlation(params, target).real.double()
     return gram.float(), (rhs / model.size).float()
 
 
