@@ -29,6 +29,7 @@ struct CorShape {
 // Aperture blocks and candidate chunks for about `target` workgroups (the workspace depends on
 // the blocks only). Correlations aim at one round of their LDS-bound workgroups.
 constexpr int kCorRound = 512;
+constexpr int kCorFewCandidates = 8192;
 CorShape cor_shape(int A, int S, int target = 4096);
 
 constexpr int kMaxScans = 8;

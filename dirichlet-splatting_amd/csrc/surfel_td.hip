@@ -1242,14 +1242,19 @@ struct TdScanSet {
     int per_range;
 };
 
-// Operands as vjp_operands_kernel (symmetric half-integer split, one K half) for the rows of
-// both mats: S = U+ + U-, D = U+ - U- of the residual (mat 0) and of psi (mat 1).
-__global__ __launch_bounds__(64) void scan_operands_kernel(const float2* __restrict__ U, int S, int N, int K,
-                                                           uint4* __restrict__ out, float* __restrict__ scale_exp) {
-    const int a = blockIdx.x, L = threadIdx.x;
+// Operands as vjp_operands_kernel<J> (symmetric half-integer split, one K half, T = J / 8
+// tiles) for the rows of both mats: S = U+ + U-, D = U+ - U- of the residual (mat 0, scaled
+// per aperture) and of psi (mat 1, the fixed norm-polynomial rows).
+template <int J>
+__global__ __launch_bounds__(64 * (J / 8)) void scan_operands_kernel(const float2* __restrict__ U, int S, int N,
+                                                                     int K, uint4* __restrict__ out,
+                                                                     float* __restrict__ scale_exp) {
+    constexpr int T = J / 8;
+    __shared__ float red[T];
+    const int a = blockIdx.x, tid = threadIdx.x, tile = tid >> 6, L = tid & 63;
     const int row = L & 31, h = L >> 5;
     int mat, j;
-    td_row<8>(0, row >> 3, (row >> 2) & 1, row & 3, &mat, &j);
+    td_row<J>(tile, row >> 3, (row >> 2) & 1, row & 3, &mat, &j);
     const int part = row & 1;
     // psi_m = conj(phi_m), scaled by 2^kScanPsiExp
     auto psi = [&](int m) {
@@ -1280,10 +1285,13 @@ __global__ __launch_bounds__(64) void scan_operands_kernel(const float2* __restr
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if (L == 0) red[tile] = mx;
+    __syncthreads();
+    for (int t = 0; t < T; ++t) mx = fmaxf(mx, red[t]);  // the residual rows of every tile
     int ex = 0;
     if (mx > 0.0f) frexpf(mx, &ex);
     ex -= 12;
-    if (L == 0) scale_exp[a] = (float)ex;
+    if (tid == 0) scale_exp[a] = (float)ex;
     const float sc = mat == 0 ? ldexpf(1.0f, -ex) : 1.0f;
     f16x8 hi, lo;
 #pragma unroll
@@ -1293,24 +1301,25 @@ __global__ __launch_bounds__(64) void scan_operands_kernel(const float2* __restr
         hi[e] = xh;
         lo[e] = (_Float16)(x - (float)xh);
     }
-    uint4* dst = out + ((size_t)a * 2) * 64 + L;
+    uint4* dst = out + (((size_t)a * T + tile) * 2) * 64 + L;
     dst[0] = __builtin_bit_cast(uint4, hi);
     dst[64] = __builtin_bit_cast(uint4, lo);
 }
 
 // grid (candidate tiles of 128, aperture ranges of every scan back to back); partial
 // [C][ranges][3] = (corr.re, corr.im, norm) of the range.
-template <bool FRESNEL>
+template <int J, bool FRESNEL>
 __global__ __launch_bounds__(256) void scan_td_kernel(const Chirp c, const TdScanSet set,
                                                       const float* __restrict__ ctr, const float* __restrict__ nrm,
                                                       int C, const uint4* __restrict__ ops,
                                                       const float* __restrict__ scale_exp,
                                                       float* __restrict__ partial) {
-    constexpr int J = 8, G = kTdVjpGroup, PER = 2 * 64;
+    constexpr int T = J / 8;  // 32-row tiles per aperture
+    constexpr int G = kTdVjpGroup / T, PER = T * 2 * 64;
     extern __shared__ float smem[];
     float2* E = reinterpret_cast<float2*>(smem);     // [N]
     float* Sc = reinterpret_cast<float*>(E + c.N);   // [G] scales 2^e_a
-    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);  // [G][2][64]
+    uint4* Aop = reinterpret_cast<uint4*>(Sc + 16);  // [G][T][2][64]
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, m = lane & 31;
     const int s = blockIdx.x * kScanCandidates + 32 * wave + m;
@@ -1346,32 +1355,36 @@ __global__ __launch_bounds__(256) void scan_td_kernel(const Chirp c, const TdSca
                 const int a = g0 + ga + h;  // this lane's pair: (a, s)
                 vjp_stage<J, FRESNEL>(c, E, pos, min(a, A - 1), a < a_end && s_ok, cx, cy, cz, nx, ny, nz, cur);
             }
-            f32x16 acc[2];
+            f32x16 acc[2][T];
 #pragma unroll
-            for (int ap = 0; ap < 2; ++ap) {
+            for (int ap = 0; ap < 2; ++ap)
 #pragma unroll
-                for (int r = 0; r < 16; ++r) acc[ap][r] = 0.f;
-                const uint4* src = Aop + (ga + ap) * 2 * 64 + lane;
-                const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
-                const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
-                acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap], acc[ap], 0, 0, 0);
-                acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap], acc[ap], 0, 0, 0);
-                acc[ap] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap], acc[ap], 0, 0, 0);
-            }
+                for (int tile = 0; tile < T; ++tile) {
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) acc[ap][tile][r] = 0.f;
+                    const uint4* src = Aop + ((ga + ap) * T + tile) * 2 * 64 + lane;
+                    const f16x8 ah = __builtin_bit_cast(f16x8, src[0]);
+                    const f16x8 al = __builtin_bit_cast(f16x8, src[64]);
+                    acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bh[ap], acc[ap][tile], 0, 0, 0);
+                    acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cur.Bl[ap], acc[ap][tile], 0, 0, 0);
+                    acc[ap][tile] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, cur.Bh[ap], acc[ap][tile], 0, 0, 0);
+                }
             float2 P[2][2];  // [aperture a / a + 1][mat 0: residual, mat 1: psi]
 #pragma unroll
             for (int x = 0; x < 2; ++x) P[x][0] = P[x][1] = make_float2(0.f, 0.f);
 #pragma unroll
-            for (int rr = 0; rr < 16; rr += 2) {
-                int mat, j0, j1;
-                td_row<J>(0, rr >> 2, 0, rr & 3, &mat, &j0);
-                td_row<J>(0, rr >> 2, 1, rr & 3, &mat, &j1);
-                float2 za0 = cur.Z[j0], za1 = cur.Z[j1];
-                swap_halves(za0.x, za1.x);
-                swap_halves(za0.y, za1.y);
-                P[0][mat] = cmac(P[0][mat], za0, make_float2(acc[0][rr], acc[0][rr + 1]));
-                P[1][mat] = cmac(P[1][mat], za1, make_float2(acc[1][rr], acc[1][rr + 1]));
-            }
+            for (int tile = 0; tile < T; ++tile)
+#pragma unroll
+                for (int rr = 0; rr < 16; rr += 2) {
+                    int mat, j0, j1;
+                    td_row<J>(tile, rr >> 2, 0, rr & 3, &mat, &j0);
+                    td_row<J>(tile, rr >> 2, 1, rr & 3, &mat, &j1);
+                    float2 za0 = cur.Z[j0], za1 = cur.Z[j1];
+                    swap_halves(za0.x, za1.x);
+                    swap_halves(za0.y, za1.y);
+                    P[0][mat] = cmac(P[0][mat], za0, make_float2(acc[0][tile][rr], acc[0][tile][rr + 1]));
+                    P[1][mat] = cmac(P[1][mat], za1, make_float2(acc[1][tile][rr], acc[1][tile][rr + 1]));
+                }
             P[0][0] = Sc[ga] * P[0][0];
             P[1][0] = Sc[ga + 1] * P[1][0];
 #pragma unroll
@@ -1422,7 +1435,7 @@ __global__ __launch_bounds__(256) void scan_td_reduce(const float* __restrict__ 
     }
 }
 
-bool td_scan_supported(const Chirp& c) { return c.form == 0 && c.S == 128 && td_supported(c); }
+bool td_scan_supported(const Chirp& c) { return use_td(c); }
 
 static TdScanSet td_scan_set(const ScanSet& set, int chunk) {
     TdScanSet t;
@@ -1453,7 +1466,7 @@ static TdScanWorkspace td_scan_layout(const Chirp& c, const ScanSet& set, int C)
     w.partial = 0;
     w.u = up(sizeof(float) * 3 * (size_t)C * t.range_first[set.count]);
     w.ops = w.u + up(sizeof(float2) * apertures * c.S);
-    w.scales = w.ops + up(sizeof(uint4) * apertures * 128);
+    w.scales = w.ops + up(sizeof(uint4) * apertures * 128 * (c.S / 128));  // T = S / 128 tiles
     w.total = w.scales + up(sizeof(float) * apertures);
     return w;
 }
@@ -1479,18 +1492,34 @@ hipError_t launch_td_scan(const Chirp& c, const ScanSet& set, const float* ctr, 
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     const int apertures = t.ap_off[set.count];
-    hipLaunchKernelGGL(scan_operands_kernel, dim3(apertures), dim3(64), 0, st, U, c.S, c.N, c.K, ops, scales);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     const int nrange = t.range_first[set.count];
     const dim3 grid((C + kScanCandidates - 1) / kScanCandidates, nrange);
+    // the G x T operand tiles of a group: 16 x 128 uint4 for either sample count
     const size_t smem = sizeof(float2) * c.N + sizeof(float) * 16 + sizeof(uint4) * (size_t)kTdVjpGroup * 128;
     const bool fres = c.fresnel_n > 1.0f;
-    e = allow_lds(fres ? (const void*)scan_td_kernel<true> : (const void*)scan_td_kernel<false>, smem);
-    if (e != hipSuccess) return e;
-    if (fres)
-        hipLaunchKernelGGL(scan_td_kernel<true>, grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales, partial);
-    else
-        hipLaunchKernelGGL(scan_td_kernel<false>, grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales, partial);
+    if (c.S == 128) {
+        hipLaunchKernelGGL(scan_operands_kernel<8>, dim3(apertures), dim3(64), 0, st, U, c.S, c.N, c.K, ops, scales);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        const void* k = fres ? (const void*)scan_td_kernel<8, true> : (const void*)scan_td_kernel<8, false>;
+        if ((e = allow_lds(k, smem)) != hipSuccess) return e;
+        if (fres)
+            hipLaunchKernelGGL((scan_td_kernel<8, true>), grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales,
+                               partial);
+        else
+            hipLaunchKernelGGL((scan_td_kernel<8, false>), grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales,
+                               partial);
+    } else {
+        hipLaunchKernelGGL(scan_operands_kernel<16>, dim3(apertures), dim3(128), 0, st, U, c.S, c.N, c.K, ops, scales);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        const void* k = fres ? (const void*)scan_td_kernel<16, true> : (const void*)scan_td_kernel<16, false>;
+        if ((e = allow_lds(k, smem)) != hipSuccess) return e;
+        if (fres)
+            hipLaunchKernelGGL((scan_td_kernel<16, true>), grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales,
+                               partial);
+        else
+            hipLaunchKernelGGL((scan_td_kernel<16, false>), grid, dim3(256), smem, st, c, t, ctr, nrm, C, ops, scales,
+                               partial);
+    }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(scan_td_reduce, dim3((C + 3) / 4, set.count), dim3(256), 0, st, partial, C, nrange, t, out);
     return hipGetLastError();

@@ -310,11 +310,24 @@ class _LeastSquaresStream:
 class _SearchSupport:
     """Correlations, batched scoring and the proposal filters of the spatial search."""
 
-    def correlate(self, params, residual):
-        return surfel.correlate(*self.sensor, params, residual.reshape(self.shape), self.fresnel_n)
-
-    # atoms from which A^H r runs faster as the sample-domain VJP than as the correlation
+    # atoms from which A^H r runs faster as the sample-domain VJP than as the correlation, and
+    # candidates from which (A^H r, |a|^2) run faster as the sample-domain scan
     ADJOINT_MIN_ATOMS = 128
+    SCAN_MIN_ATOMS = 512
+    SCAN_RANGE = 512  # apertures per range of the scan launch
+
+    def correlate(self, params, residual):
+        """(a^H r, |a|^2) of every atom, area included: the sample-domain scan (one scan, on the
+        matrix cores) where it serves the chirp and the candidates are many, else the
+        correlation kernel."""
+        config, positions = self.sensor
+        residual = residual.reshape(self.shape)
+        if len(params["centers"]) >= self.SCAN_MIN_ATOMS and surfel.kernel.sample_domain(config, self.fresnel_n):
+            corr, norm = surfel.kernel.scan_batched(config, [(positions, residual)], params["centers"],
+                                                    params["normals"], self.fresnel_n, self.SCAN_RANGE)  # fmt: skip
+            root = params["areas"].sqrt()
+            return corr[0] * root, norm[0] * params["areas"]
+        return surfel.correlate(config, positions, params, residual, self.fresnel_n)
 
     def adjoint_correlation(self, params, residual):
         """A^H r of every atom (correlate's first output) for the right-hand sides: through the

@@ -32,13 +32,14 @@ def scan_set(bins, sides=(5, 6, 3), seed=0):
     return out
 
 
-@pytest.mark.parametrize("pad,fresnel_n", [(2, 0.0), (2, 1.6), (1, 0.0), (4, 0.0), (8, 0.0)])
+@pytest.mark.parametrize("pad,fresnel_n,samples", [(2, 0.0, 128), (2, 1.6, 128), (1, 0.0, 128), (4, 0.0, 128),
+                                                   (8, 0.0, 128), (2, 0.0, 256), (2, 1.6, 256), (4, 0.0, 256)])  # fmt: skip
 @pytest.mark.parametrize("chunk", [8, 24, 0])
-def test_batched_scan_matches_the_oracle_per_scan(pad, fresnel_n, chunk):
+def test_batched_scan_matches_the_oracle_per_scan(pad, fresnel_n, samples, chunk):
     from dsplat import FMCWConfig
     from dsplat.kernels import surfel
 
-    cfg = FMCWConfig(pad_factor=pad)
+    cfg = FMCWConfig(pad_factor=pad, adc_samples=samples)
     p = surfels(37, seed=pad)
     scans = scan_set(cfg.num_range_bins, seed=pad)
     corr, norm = surfel.scan_batched(
@@ -213,3 +214,25 @@ def test_certificate_scan_at_full_size_selects_the_float64_top():
             corr, norm = O.correlate(x.cpu().numpy(), c64_[window], n64_[window], r.cpu().numpy(), cfg)
             truth = O.certificate(corr * np.sqrt(area), norm * area, r.cpu().numpy(), "complex")
             assert truth[inside].min() >= truth[~inside].max()
+
+
+@pytest.mark.parametrize("samples", [128, 256])
+def test_model_correlation_of_many_candidates_takes_the_scan(samples):
+    """FMCWModel.correlate of at least SCAN_MIN_ATOMS candidates on a sample-domain chirp runs as
+    one scan; it equals the bin-domain correlation kernel of the same model within 1e-5."""
+    from dataclasses import replace
+
+    from dsplat import FMCWConfig, FMCWModel, Scan
+
+    cfg = FMCWConfig(adc_samples=samples)
+    p = surfels(600, seed=9, spread=0.03)
+    (x, r), = scan_set(cfg.num_range_bins, sides=(6,), seed=9)
+    areas = p["areas"] * np.linspace(0.5, 2.0, 600)
+    params = dict(centers=f32(p["centers"]), normals=f32(p["normals"]), areas=f32(areas))
+    scan = FMCWModel(Scan(c64(r), f32(x), cfg), fresnel_n=1.6)
+    bins = FMCWModel(Scan(c64(r), f32(x), replace(cfg, splat_form="bins")), fresnel_n=1.6)
+    assert len(areas) >= scan.SCAN_MIN_ATOMS
+    corr, norm = scan.correlate(params, scan.scan.data)
+    want_corr, want_norm = bins.correlate(params, bins.scan.data)
+    assert rel_l2(corr.cpu().numpy(), want_corr.cpu().numpy()) < TOL
+    assert rel_l2(norm.cpu().numpy(), want_norm.cpu().numpy()) < TOL
