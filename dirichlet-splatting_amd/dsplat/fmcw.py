@@ -316,13 +316,14 @@ class _SearchSupport:
     SCAN_MIN_ATOMS = 512
     SCAN_RANGE = 512  # apertures per range of the scan launch
 
-    def correlate(self, params, residual):
+    def correlate(self, params, residual, scan=True):
         """(a^H r, |a|^2) of every atom, area included: the sample-domain scan (one scan, on the
-        matrix cores) where it serves the chirp and the candidates are many, else the
-        correlation kernel."""
+        matrix cores) where it serves the chirp and the candidates are many, else (and with
+        scan=False) the correlation kernel, which correlate_gradient shares."""
         config, positions = self.sensor
         residual = residual.reshape(self.shape)
-        if len(params["centers"]) >= self.SCAN_MIN_ATOMS and surfel.kernel.sample_domain(config, self.fresnel_n):
+        if scan and len(params["centers"]) >= self.SCAN_MIN_ATOMS and surfel.kernel.sample_domain(config,
+                                                                                                 self.fresnel_n):
             corr, norm = surfel.kernel.scan_batched(config, [(positions, residual)], params["centers"],
                                                     params["normals"], self.fresnel_n, self.SCAN_RANGE)  # fmt: skip
             root = params["areas"].sqrt()

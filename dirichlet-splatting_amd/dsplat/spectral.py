@@ -112,8 +112,9 @@ class SpectralModel:
             norms += kernel.design(self.grid, centers, bins[first : first + step]).abs().square().sum(0)
         return norms
 
-    def correlate(self, params, residual):
-        """a^H r and |a|^2 for every atom (spectral.py:90-106 of the reference)."""
+    def correlate(self, params, residual, scan=True):
+        """a^H r and |a|^2 for every atom (spectral.py:90-106 of the reference); scan is
+        accepted for the FMCW model's signature and has no effect here."""
         centers = params["centers"]
         bins = self._bins(params)
         flat = residual.reshape(-1)

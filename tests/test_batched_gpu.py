@@ -272,3 +272,4 @@ def test_pooled_cross_products_of_the_split_blocks_match_the_streamed_ones(fmcw)
     assert block_products(model, params, model.scan.data, config, acc, col0=col0) is None
     dense = torch.cat([a for a in model.projected_blocks(params, 3 * model.shape[1])]).double()
     close(acc.result(), dense.T @ dense[:, col0:], 1e-5)
+
