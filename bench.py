@@ -98,6 +98,8 @@ def parse():
                         help="dsfw: area-uniform ellipsoid draws whose front faces are the truth")
     parser.add_argument("--front-count", type=int, default=0,
                         help="dsfw: exactly this many front faces as the truth instead (0: use --truth-draws)")
+    parser.add_argument("--dsfw-form", choices=("both", "plain_fp32", "mixed_batched"), default="both",
+                        help="--workload dsfw: compare both forms, or time one form's steps alone")
     parser.add_argument("--dsfw-steps", type=int, default=100,
                         help="default run: timed DSFW steps per form (0 skips the DSFW comparison)")
     return parser.parse_args()
@@ -553,7 +555,7 @@ def _relative_deviation(a, b):
 
 
 def dsfw_compare(steps, warmup=1, adc_samples=256, truth_draws=4000, front_count=None, controls=("mixed_batched",),
-                 same_state_every=5):
+                 same_state_every=5, only=None):
     """The full DSFW fit in the plain FP32 and the mixed-precision batched forms from the same
     problem and initial state: `warmup` untimed steps, then `steps` steps each timed between
     device synchronizations (median and mean), the NMSE after every step, the final median
@@ -623,6 +625,10 @@ def dsfw_compare(steps, warmup=1, adc_samples=256, truth_draws=4000, front_count
         }
         return record, states
 
+    if only is not None:  # one form's steps alone (for kernel traces of that form)
+        record, _ = run(only, target)
+        del record["ms"]
+        return {"steps": steps, "warmup": warmup, "surfels": S, "bins": adc_samples, "forms": {only: record}}
     plain, states = run("plain_fp32", target, keep_states=True)
     mixed, _ = run("mixed_batched", target)
     out = {"plain_fp32": plain, "mixed_batched": mixed}
@@ -664,8 +670,15 @@ def dsfw_compare(steps, warmup=1, adc_samples=256, truth_draws=4000, front_count
 def dsfw_workload(args):
     """The DSFW comparison on its own (see dsfw_compare); both forms get a perturbed control."""
     draws = args.truth_draws or 4000
+    only = None if args.dsfw_form == "both" else args.dsfw_form
     result = dsfw_compare(args.steps, max(1, args.warmup), args.adc_samples, draws,
-                          front_count=args.front_count or None, controls=("plain_fp32", "mixed_batched"))  # fmt: skip
+                          front_count=args.front_count or None, controls=("plain_fp32", "mixed_batched"),
+                          only=only)  # fmt: skip
+    if only is not None:
+        record = result["forms"][only]
+        return {"metric": f"DSFW step ms (median), {only} form", "value": record["ms_per_step_median"], "unit": "ms",
+                "n_gpus": 1, "steps": result["steps"], "warmup": result["warmup"], "higher_is_better": False,
+                "data": "synthetic", "dsfw": result}  # fmt: skip
     mixed = result["forms"]["mixed_batched"]
     return {
         "metric": "DSFW step ms (median), mixed-precision batched form",
@@ -676,8 +689,8 @@ def dsfw_workload(args):
         "warmup": result["warmup"],
         "ms_per_step": mixed["ms_per_step_median"],
         "higher_is_better": False,
-        "dtype": "f32 solver (f64 shifted Cholesky fallback), bf16 hi+lo x3 MFMA normal equations, fp16 hi+lo x3 "
-        "MFMA splat at 128 samples, fp32 MFMA splat at 256 samples (mixed form)",
+        "dtype": "mixed form: fp16 hi+lo x3 MFMA split-operand normal equations and scan, fp32 MFMA splat at 256 "
+        "samples, f32 Cholesky first with the f64 shifted fallback",
         "data": "synthetic",
         "config": {"workload": result["workload"], "surfels": result["surfels"], "bins": result["bins"],
                    "parallelism": "single GPU"},
