@@ -706,6 +706,42 @@ int dsplat_top_descending(const double* scores, int32_t rows, int32_t count, int
     return launch_top_descending(scores, rows, count, top, out, (hipStream_t)stream);
 }
 
+size_t dsplat_certificate_operands_bytes(const dsplat_fmcw* cfg, int32_t num_apertures) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures <= 0) return 0;
+    return cert_operands_bytes(c, num_apertures);
+}
+
+int dsplat_certificate_operands(const dsplat_fmcw* cfg, const float* residual, int32_t num_apertures,
+                                void* operands, size_t operands_bytes, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures <= 0 || !residual || !operands) return hipErrorInvalidValue;
+    const size_t need = cert_operands_bytes(c, num_apertures);
+    if (need == 0 || operands_bytes < need) return hipErrorInvalidValue;
+    return launch_cert_operands(c, reinterpret_cast<const float2*>(residual), num_apertures, operands,
+                                (hipStream_t)stream);
+}
+
+size_t dsplat_certificate_gradient_workspace(const dsplat_fmcw* cfg, int32_t num_apertures, int32_t num_candidates) {
+    Chirp c;
+    if (!make_chirp(cfg, &c)) return 0;
+    return cert_grad_workspace(c, num_apertures, num_candidates);
+}
+
+int dsplat_certificate_gradient(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                                const float* centers, const float* normals, int32_t num_candidates,
+                                const void* operands, float* out, void* workspace, size_t workspace_bytes,
+                                void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures <= 0 || num_candidates < 0) return hipErrorInvalidValue;
+    if (num_candidates == 0) return hipSuccess;
+    if (!positions || !centers || !normals || !operands || !out || !workspace) return hipErrorInvalidValue;
+    const size_t need = cert_grad_workspace(c, num_apertures, num_candidates);
+    if (need == 0 || workspace_bytes < need) return hipErrorInvalidValue;
+    return launch_cert_grad(c, positions, num_apertures, centers, normals, num_candidates, operands, out, workspace,
+                            (hipStream_t)stream);
+}
+
 int dsplat_pcg_start(const float* rhs, const float* diagonal, int32_t n, float tolerance, float* x, float* residual,
                      float* direction, float* state, void* stream) {
     if (n < 0 || !state) return hipErrorInvalidValue;

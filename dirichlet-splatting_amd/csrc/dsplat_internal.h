@@ -178,6 +178,15 @@ hipError_t launch_td_render(const Chirp& c, const float* pos, int A, const float
 hipError_t launch_td_jvp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, const float* dctr, const float* dnrm, const float2* dw, int S, float2* out,
                          hipStream_t st);
+// U = DFT of K-bin rows to the S samples (sign +1), the sample-domain form of a residual.
+hipError_t launch_dft_rows_up(const Chirp& c, const float2* in, int rows, float2* out, hipStream_t st);
+// Sample-domain certificate sums and their gradients (cert_td.hip).
+constexpr int kCgValues = 21;  // corr (2), norm, dcorr re / im (6 + 6), dnorm (6)
+size_t cert_operands_bytes(const Chirp& c, int A);
+hipError_t launch_cert_operands(const Chirp& c, const float2* residual, int A, void* operands, hipStream_t st);
+size_t cert_grad_workspace(const Chirp& c, int A, int C);
+hipError_t launch_cert_grad(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, int C,
+                            const void* operands, float* out, void* ws, hipStream_t st);
 hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, const float2* up, void* ws, float* gc, float* gn, float* gw,
                          hipStream_t st);

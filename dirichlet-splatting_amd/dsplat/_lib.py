@@ -162,6 +162,12 @@ SIGNATURES = {
     ),
     "dsplat_certificate_scores": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr]),
     "dsplat_top_descending": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr]),
+    "dsplat_certificate_operands_bytes": (c_size, [c_ptr, c_int]),
+    "dsplat_certificate_operands": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_size, c_ptr]),
+    "dsplat_certificate_gradient_workspace": (c_size, [c_ptr, c_int, c_int]),
+    "dsplat_certificate_gradient": (
+        c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
+    ),
     "dsplat_pcg_start": (c_int, [c_ptr, c_ptr, c_int, c_float, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
     "dsplat_pcg_step": (c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr]),
     "dsplat_gram_workspace": (c_size, [c_int, c_int]),

@@ -280,6 +280,44 @@ def correlate_orientations(config, positions, centers, normals, bank, residual, 
     return torch.complex(output[..., 0], output[..., 1]), output[..., 2]
 
 
+class CertificateOperands:
+    """The sample-domain operand tiles of one residual [A, K] (dsplat_certificate_operands),
+    built once and shared by every certificate_gradient launch against that residual."""
+
+    def __init__(self, config, residual, fresnel_n=0.0):
+        self.cfg = _lib.fmcw(config, fresnel_n)
+        self.apertures = residual.shape[0]
+        size = _lib.library().dsplat_certificate_operands_bytes(_lib.ctypes.byref(self.cfg), self.apertures)
+        if size == 0:
+            raise ValueError("the sample-domain certificate kernels do not serve this chirp")
+        self.buffer = torch.empty(size, dtype=torch.uint8, device=residual.device)
+        _lib.call(
+            "dsplat_certificate_operands", _lib.ctypes.byref(self.cfg),
+            _lib.ptr(interleaved(residual.reshape(self.apertures, config.num_range_bins))), self.apertures,
+            _lib.ptr(self.buffer), size, _lib.stream(residual.device),
+        )  # fmt: skip
+
+
+def certificate_gradient(positions, centers, normals, operands):
+    """[C, 21] = (corr, norm, dcorr re / im [6], dnorm [6]) of C candidates against the residual
+    of `operands` (CertificateOperands), without area: correlate_gradient's columns with the
+    normals, on the sample-domain matrix-core kernel."""
+    _check(positions, centers, normals)
+    count = len(centers)
+    out = centers.new_empty((count, 21))
+    if count == 0:
+        return out
+    cfg = operands.cfg
+    size = _lib.library().dsplat_certificate_gradient_workspace(_lib.ctypes.byref(cfg), len(positions), count)
+    workspace = centers.new_empty(max(1, size // 4))
+    _lib.call(
+        "dsplat_certificate_gradient", _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
+        _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), count, _lib.ptr(operands.buffer),
+        _lib.ptr(out), _lib.ptr(workspace), size, _lib.stream(centers.device),
+    )  # fmt: skip
+    return out
+
+
 def sample_domain(config, fresnel_n=0.0):
     """Whether the sample-domain kernels serve this chirp (its splat form and ADC samples)."""
     cfg = _lib.fmcw(config, fresnel_n)

@@ -197,6 +197,30 @@ int dsplat_pcg_start(const float* rhs, const float* diagonal, int32_t n, float t
 int dsplat_pcg_step(const float* image, const float* diagonal, int32_t n, float* x, float* residual,
                     float* direction, float* state, void* stream);
 
+/* Sample-domain certificate operands of one residual [A, K] complex: its rows taken to the
+ * samples (DFT) and split into the FP16 hi / lo operand tiles of four accumulator sets per
+ * aperture (the residual sums g0 and g1, the norm polynomial and its derivative), in a device
+ * buffer of dsplat_certificate_operands_bytes (0: the chirp's splat form or sample count is
+ * not served by the sample-domain kernels). Built once per residual and shared by every
+ * dsplat_certificate_gradient launch against it; no counterpart in the reference, whose
+ * correlation kernels read the residual bins directly. */
+size_t dsplat_certificate_operands_bytes(const dsplat_fmcw* cfg, int32_t num_apertures);
+int dsplat_certificate_operands(const dsplat_fmcw* cfg, const float* residual, int32_t num_apertures,
+                                void* operands, size_t operands_bytes, void* stream);
+
+/* Correlations, norms and their centre and normal derivatives of C candidates against the
+ * residual of `operands`, in the sample domain on the FP16 matrix cores: out [C, 21] =
+ * (corr.re, corr.im, norm, dcorr.re[6], dcorr.im[6], dnorm[6]), the columns of
+ * dsplat_surfel_correlate_gradient with include_normals (without area). Deterministic:
+ * per-range partials summed in float64. Workspace from
+ * dsplat_certificate_gradient_workspace. Replaces kernels/surfel.py:293-315
+ * (correlate_gradient) for chirps the sample-domain form serves. */
+size_t dsplat_certificate_gradient_workspace(const dsplat_fmcw* cfg, int32_t num_apertures, int32_t num_candidates);
+int dsplat_certificate_gradient(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures,
+                                const float* centers, const float* normals, int32_t num_candidates,
+                                const void* operands, float* out, void* workspace, size_t workspace_bytes,
+                                void* stream);
+
 /* DSPLAT_TOP_MAX: the largest `top` of dsplat_top_descending. */
 #define DSPLAT_TOP_MAX 4096
 

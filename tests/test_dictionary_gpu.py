@@ -108,3 +108,23 @@ def test_adjoint_correlation_matches_the_oracle(samples, fresnel_n):
     a = O.columns(positions, centers, normals, cfg, fresnel_n) * np.sqrt(areas)
     want = a.reshape(-1, len(centers)).conj().T @ r.reshape(-1)
     assert rel_l2(got, want) < TOL
+
+
+@pytest.mark.parametrize("samples,pad,fresnel_n", [(128, 2, 0.0), (128, 4, 1.6), (256, 2, 0.0), (256, 2, 1.6)])
+def test_sample_domain_certificate_gradient_matches_the_oracle(samples, pad, fresnel_n):
+    """Correlations, norms and their centre and normal derivatives from the sample-domain
+    certificate operands (two candidate tiles, ragged) against the float64 oracle."""
+    from dsplat import FMCWConfig
+    from dsplat.kernels import surfel
+
+    cfg = FMCWConfig(adc_samples=samples, pad_factor=pad)
+    positions, centers, normals, _ = scene(count=150, side=6)
+    rng = np.random.default_rng(samples + pad)
+    r = rng.standard_normal((36, cfg.num_range_bins)) + 1j * rng.standard_normal((36, cfg.num_range_bins))
+    ops = surfel.CertificateOperands(cfg, torch.as_tensor(r, dtype=torch.complex64).cuda(), fresnel_n)
+    out = surfel.certificate_gradient(f32(positions), f32(centers), f32(normals), ops).double().cpu().numpy()
+    ec, en, edc, edn = O.correlate_gradient(positions, centers, normals, r, cfg, fresnel_n, True)
+    assert rel_l2(out[:, 0] + 1j * out[:, 1], ec) < TOL
+    assert rel_l2(out[:, 2], en) < TOL
+    assert rel_l2(out[:, 3:9] + 1j * out[:, 9:15], edc) < TOL
+    assert rel_l2(out[:, 15:21], edn) < TOL
