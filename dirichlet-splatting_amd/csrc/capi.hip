@@ -550,6 +550,29 @@ int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num
                             areas, count, *spec, centers_out, normals_out, values_out, workspace, (hipStream_t)stream);
 }
 
+size_t dsplat_qn_lockstep_workspace(const dsplat_fmcw* cfg, int32_t num_apertures, int32_t count) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 1 || count < 1) return 0;
+    return qn_lockstep_workspace(c, num_apertures, count);
+}
+
+int dsplat_qn_lockstep(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const void* operands,
+                       const float* centers, const float* normals, const float* areas, int32_t count,
+                       const dsplat_qn_spec* spec, float* centers_out, float* normals_out, float* values_out,
+                       void* workspace, size_t workspace_bytes, void* stream) {
+    Chirp c;
+    if (!make_chirp(cfg, &c) || num_apertures < 1 || count < 0 || !spec) return hipErrorInvalidValue;
+    if (!(spec->width == 3 || spec->width == 6) || spec->iterations < 0 || spec->backtracks < 0)
+        return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    const size_t need = qn_lockstep_workspace(c, num_apertures, count);
+    if (need == 0 || !positions || !operands || !centers || !normals || !areas || !centers_out || !normals_out ||
+        !values_out || !workspace || workspace_bytes < need)
+        return hipErrorInvalidValue;
+    return launch_qn_lockstep(c, positions, num_apertures, operands, centers, normals, areas, count, *spec,
+                              centers_out, normals_out, values_out, workspace, (hipStream_t)stream);
+}
+
 size_t dsplat_elimination_workspace(int32_t n, int32_t count) { return elimination_workspace(n, count); }
 
 int dsplat_backward_elimination(const double* inverse, const double* b, int32_t n, int32_t count,

@@ -101,13 +101,14 @@ __global__ __launch_bounds__(64 * (J / 4)) void cert_operands_kernel(const float
     dst[64] = __builtin_bit_cast(uint4, lo);
 }
 
-// grid (candidate tiles of 128, aperture ranges); partial [C][ranges][21].
+// grid (candidate tiles of 128, aperture ranges); partial [C][ranges][21]. With live given, a
+// tile none of whose candidates is live (live[s] == 0) writes nothing.
 template <int J, bool FRESNEL>
 __global__ __launch_bounds__(256) void cert_grad_kernel(const Chirp c, const float* __restrict__ pos, int A,
                                                         const float* __restrict__ ctr, const float* __restrict__ nrm,
                                                         int C, const uint4* __restrict__ ops,
                                                         const float* __restrict__ scale_exp, int per_range,
-                                                        float* __restrict__ partial) {
+                                                        const int* __restrict__ live, float* __restrict__ partial) {
     constexpr int TT = J / 4;               // tiles per aperture
     constexpr int G = kTdVjpGroup / TT;     // apertures staged per LDS fill (32 KB)
     constexpr int PER = TT * 2 * 64;        // uint4 per aperture
@@ -124,6 +125,7 @@ __global__ __launch_bounds__(256) void cert_grad_kernel(const Chirp c, const flo
     const bool s_ok = s < C;
     const int sc = min(s, C - 1);
     const int a_begin = range * per_range, a_end = min(A, a_begin + per_range);
+    if (live != nullptr && !__syncthreads_or(s_ok && live[sc] != 0)) return;  // uniform
     fill_twiddles(E, c.N, tid, 256);
 
     const float cx = ctr[3 * sc], cy = ctr[3 * sc + 1], cz = ctr[3 * sc + 2];
@@ -235,12 +237,14 @@ __global__ __launch_bounds__(256) void cert_grad_kernel(const Chirp c, const flo
     }
 }
 
-// out[s][k] = sum over ranges of partial[s][r][k] in float64, fixed order.
+// out[s][k] = sum over ranges of partial[s][r][k] in float64, fixed order (live candidates only
+// when live is given).
 __global__ __launch_bounds__(256) void cert_grad_reduce(const float* __restrict__ partial, int C, int nrange,
-                                                        float* __restrict__ out) {
+                                                        const int* __restrict__ live, float* __restrict__ out) {
     const int e = blockIdx.x * 256 + threadIdx.x;
     if (e >= C * kCgValues) return;
     const int s = e / kCgValues, k = e - s * kCgValues;
+    if (live != nullptr && live[s] == 0) return;
     double sum = 0.0;
     for (int r = 0; r < nrange; ++r) sum += partial[((size_t)s * nrange + r) * kCgValues + k];
     out[e] = (float)sum;
@@ -307,7 +311,8 @@ size_t cert_grad_workspace(const Chirp& c, int A, int C) {
 
 template <int J>
 static hipError_t launch_cert_grad_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
-                                     int C, const void* operands, float* out, void* ws, hipStream_t st) {
+                                     int C, const void* operands, float* out, void* ws, hipStream_t st,
+                                     const int* live) {
     const CertOps o = cert_ops_layout(c, A);
     const char* base = static_cast<const char*>(operands);
     const uint4* ops = reinterpret_cast<const uint4*>(base + o.ops);
@@ -321,20 +326,21 @@ static hipError_t launch_cert_grad_j(const Chirp& c, const float* pos, int A, co
     if (e != hipSuccess) return e;
     if (fres)
         hipLaunchKernelGGL((cert_grad_kernel<J, true>), dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A, ctr,
-                           nrm, C, ops, scales, v.per_range, partial);
+                           nrm, C, ops, scales, v.per_range, live, partial);
     else
         hipLaunchKernelGGL((cert_grad_kernel<J, false>), dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A,
-                           ctr, nrm, C, ops, scales, v.per_range, partial);
+                           ctr, nrm, C, ops, scales, v.per_range, live, partial);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const int total = C * kCgValues;
-    hipLaunchKernelGGL(cert_grad_reduce, dim3((total + 255) / 256), dim3(256), 0, st, partial, C, v.ranges, out);
+    hipLaunchKernelGGL(cert_grad_reduce, dim3((total + 255) / 256), dim3(256), 0, st, partial, C, v.ranges, live,
+                       out);
     return hipGetLastError();
 }
 
 hipError_t launch_cert_grad(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, int C,
-                            const void* operands, float* out, void* ws, hipStream_t st) {
-    if (c.S == 128) return launch_cert_grad_j<8>(c, pos, A, ctr, nrm, C, operands, out, ws, st);
-    return launch_cert_grad_j<16>(c, pos, A, ctr, nrm, C, operands, out, ws, st);
+                            const void* operands, float* out, void* ws, hipStream_t st, const int* live) {
+    if (c.S == 128) return launch_cert_grad_j<8>(c, pos, A, ctr, nrm, C, operands, out, ws, st, live);
+    return launch_cert_grad_j<16>(c, pos, A, ctr, nrm, C, operands, out, ws, st, live);
 }
 
 }  // namespace dsplat

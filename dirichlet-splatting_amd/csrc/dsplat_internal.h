@@ -155,6 +155,12 @@ size_t qn_workspace(int A, int P, int K);
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,
                             const float* nrm, const float* areas, int count, const QnSpec& sp, float* oc, float* on,
                             float* ov, void* workspace, hipStream_t st);
+// Lockstep ascent on sample-domain chirps: rounds of certificate_gradient evaluations of every
+// live candidate and one state-machine step each (qn_search.hip).
+size_t qn_lockstep_workspace(const Chirp& c, int A, int count);
+hipError_t launch_qn_lockstep(const Chirp& c, const float* pos, int A, const void* operands, const float* ctr,
+                              const float* nrm, const float* areas, int count, const QnSpec& sp, float* oc,
+                              float* on, float* ov, void* workspace, hipStream_t st);
 size_t elimination_workspace(int n, int count);
 hipError_t launch_elimination(const double* W, const double* b, int n, int count, const int* limited, int limit,
                               long long* removed, void* workspace, hipStream_t st);
@@ -185,8 +191,9 @@ constexpr int kCgValues = 21;  // corr (2), norm, dcorr re / im (6 + 6), dnorm (
 size_t cert_operands_bytes(const Chirp& c, int A);
 hipError_t launch_cert_operands(const Chirp& c, const float2* residual, int A, void* operands, hipStream_t st);
 size_t cert_grad_workspace(const Chirp& c, int A, int C);
+// live (optional, [C]): candidates with live[s] == 0 may be skipped, their rows of out left as they were.
 hipError_t launch_cert_grad(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm, int C,
-                            const void* operands, float* out, void* ws, hipStream_t st);
+                            const void* operands, float* out, void* ws, hipStream_t st, const int* live = nullptr);
 hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, const float2* up, void* ws, float* gc, float* gn, float* gw,
                          hipStream_t st);

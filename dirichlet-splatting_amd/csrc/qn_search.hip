@@ -156,6 +156,75 @@ __device__ __forceinline__ float qn_certificate(const QnSpec& sp, const float* t
     return (float)value;
 }
 
+// The damped quasi-Newton direction from the inverse-Hessian estimate H [6][6] and the ascent
+// gradient g at normals n: -H g, or -g when that does not ascend; masked, the normal part
+// tangent to n, shortened to the trust radius (search.slang:66-88).
+template <int W>
+__device__ __forceinline__ void qn_direction(const float* H, const float* g, const float* n, const QnSpec& sp,
+                                             float* d) {
+    float descent = 0.0f;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        float v = 0.0f;
+#pragma unroll
+        for (int k = 0; k < W; ++k) v -= H[j * 6 + k] * g[k];
+        d[j] = v;
+        descent += v * g[j];
+    }
+#pragma unroll
+    for (int j = 0; j < W; ++j) d[j] = (descent >= 0.0f ? -g[j] : d[j]) * sp.direction_mask[j];
+    if constexpr (W == 6) {
+        float proj = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) proj += d[j + 3] * n[j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) d[j + 3] -= proj * n[j];
+    }
+    float len2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < W; ++j) len2 += d[j] * d[j];
+    const float shrink = fminf(1.0f, sp.radius / fmaxf(sqrtf(len2), 1e-30f));
+#pragma unroll
+    for (int j = 0; j < W; ++j) d[j] *= shrink;
+}
+
+// The trial point at `fraction` of direction d from (oc, on): centres clamped to the box,
+// normals renormalized (search.slang:90-109).
+template <int W>
+__device__ __forceinline__ void qn_trial(const float* oc, const float* on, const float* d, float fraction,
+                                         const QnSpec& sp, float* tc, float* tn) {
+    float nl2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const float x = oc[j] + fraction * sp.step_scale[j] * d[j];
+        tc[j] = sp.direction_mask[j] > 0.0f ? fminf(fmaxf(x, sp.lower[j]), sp.upper[j]) : oc[j];
+        tn[j] = on[j] + (W == 6 ? fraction * sp.step_scale[j + 3] * d[j + 3] : 0.0f);
+        nl2 += tn[j] * tn[j];
+    }
+    const float inv = W == 6 ? 1.0f / sqrtf(nl2) : 1.0f;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) tn[j] = W == 6 ? tn[j] * inv : on[j];
+}
+
+// Step and gradient change of an accepted move in scaled coordinates, and whether the
+// curvature condition s . y > 1e-8 |s| |y| (and > 1e-20) admits the BFGS update; rho = 1 / s . y.
+template <int W>
+__device__ __forceinline__ bool qn_secant(const QnSpec& sp, const float* c, const float* n, const float* oc,
+                                          const float* on, const float* g_new, const float* g, float* step,
+                                          float* change, float* rho) {
+    float sy = 0.0f, ss = 0.0f, yy = 0.0f;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        step[j] = j < 3 ? (c[j] - oc[j]) / sp.center_scale : (n[j - 3] - on[j - 3]) / sp.normal_scale;
+        change[j] = g_new[j] - g[j];
+        sy += step[j] * change[j];
+        ss += step[j] * step[j];
+        yy += change[j] * change[j];
+    }
+    *rho = 1.0f / fmaxf(sy, 1e-30f);
+    return sy > 1e-8f * sqrtf(ss) * sqrtf(yy) && sy > 1e-20f;
+}
+
 template <int P, int W>
 __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, const float* __restrict__ pos, int A,
                                                                const float2* __restrict__ res,
@@ -187,32 +256,8 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
     st.value = qn_certificate<W>(sp, t, area, st.n, st.g);
 
     for (int it = 0; it < sp.iterations; ++it) {
-        // direction: -H g, or -g when that does not ascend; masked, tangent, within the radius
         float d[6];
-        float descent = 0.0f;
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            float v = 0.0f;
-#pragma unroll
-            for (int k = 0; k < W; ++k) v -= H[j * 6 + k] * st.g[k];
-            d[j] = v;
-            descent += v * st.g[j];
-        }
-#pragma unroll
-        for (int j = 0; j < W; ++j) d[j] = (descent >= 0.0f ? -st.g[j] : d[j]) * sp.direction_mask[j];
-        if constexpr (W == 6) {
-            float proj = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) proj += d[j + 3] * st.n[j];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) d[j + 3] -= proj * st.n[j];
-        }
-        float len2 = 0.0f;
-#pragma unroll
-        for (int j = 0; j < W; ++j) len2 += d[j] * d[j];
-        const float shrink = fminf(1.0f, sp.radius / fmaxf(sqrtf(len2), 1e-30f));
-#pragma unroll
-        for (int j = 0; j < W; ++j) d[j] *= shrink;
+        qn_direction<W>(H, st.g, st.n, sp, d);
 
         // backtracking along the direction
         float oc[3], on[3];
@@ -225,17 +270,7 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
         float fraction = 1.0f;
         for (int b = 0; b < sp.backtracks; ++b, fraction *= 0.5f) {
             float tc[3], tn[3];
-            float nl2 = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const float x = oc[j] + fraction * sp.step_scale[j] * d[j];
-                tc[j] = sp.direction_mask[j] > 0.0f ? fminf(fmaxf(x, sp.lower[j]), sp.upper[j]) : oc[j];
-                tn[j] = on[j] + (W == 6 ? fraction * sp.step_scale[j + 3] * d[j + 3] : 0.0f);
-                nl2 += tn[j] * tn[j];
-            }
-            const float inv = W == 6 ? 1.0f / sqrtf(nl2) : 1.0f;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) tn[j] = W == 6 ? tn[j] * inv : on[j];
+            qn_trial<W>(oc, on, d, fraction, sp, tc, tn);
             qn_sums<P, 0>(c, T, pos, A, res, us, tc, tn, red, t);
             float unused[6];
             const float trial = qn_certificate<0>(sp, t, area, tn, unused);
@@ -256,19 +291,9 @@ __global__ __launch_bounds__(kQnThreads) void qn_search_kernel(const Chirp c, co
         const float v_new = qn_certificate<W>(sp, t, area, st.n, g_new);
 
         // BFGS update of the inverse with step and gradient change in scaled coordinates
-        float step[6], change[6];
-        float sy = 0.0f, ss = 0.0f, yy = 0.0f;
-#pragma unroll
-        for (int j = 0; j < W; ++j) {
-            step[j] = j < 3 ? (st.c[j] - oc[j]) / sp.center_scale : (st.n[j - 3] - on[j - 3]) / sp.normal_scale;
-            change[j] = g_new[j] - st.g[j];
-            sy += step[j] * change[j];
-            ss += step[j] * step[j];
-            yy += change[j] * change[j];
-        }
-        const bool valid = sy > 1e-8f * sqrtf(ss) * sqrtf(yy) && sy > 1e-20f;
+        float step[6], change[6], rho;
+        const bool valid = qn_secant<W>(sp, st.c, st.n, oc, on, g_new, st.g, step, change, &rho);
         if (valid) {  // uniform; H <- L H L^T + rho s s^T, L = I - rho s y^T, one element per thread
-            const float rho = 1.0f / fmaxf(sy, 1e-30f);
             const int i = tid / W, j = tid - (tid / W) * W;
             if (tid < W * W) {
                 float v = 0.0f;
@@ -334,6 +359,230 @@ hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float
         case 8: return launch_qn_p<8>(c, pos, A, res, us, ctr, nrm, areas, count, sp, oc, on, ov, st);
         default: return hipErrorInvalidValue;
     }
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Lockstep ascent on sample-domain chirps
+// ---------------------------------------------------------------------------------------
+// The same per-candidate ascent as qn_search_kernel, taken apart into rounds: one
+// certificate_gradient launch evaluates the certificate sums and their derivatives (21
+// values) of every live candidate at its query point on the matrix cores, then one thread
+// per candidate moves its state machine:
+//   round 0          the starting point's value and gradient; the first direction;
+//   a trial rises    accept it: its gradient came with it, the BFGS update, the next direction;
+//   a trial fails    halve the fraction, or stop after `backtracks` trials;
+// and writes its next query point. An accepted trial costs one round (its value and gradient
+// come from one evaluation), so an iteration whose full step rises takes one round where the
+// one-launch kernel makes two passes over the apertures. Decisions compare values of the one
+// kernel, as there.
+struct QnLockState {
+    float c[3], n[3];    // current point
+    float oc[3], on[3];  // start of the line search
+    float d[6];          // its direction
+    float g[6];          // ascent gradient at the current point
+    float H[36];         // inverse-Hessian estimate [6][6]
+    float value, fraction;
+    int backtrack, iteration;
+};
+
+__global__ __launch_bounds__(256) void qn_lock_init_kernel(const float* __restrict__ ctr, const float* __restrict__ nrm,
+                                                           int C, float* __restrict__ qc, float* __restrict__ qn,
+                                                           int* __restrict__ live, int* __restrict__ counters) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s == 0) counters[0] = counters[1] = 0;
+    if (s >= C) return;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        qc[3 * s + j] = ctr[3 * s + j];
+        qn[3 * s + j] = nrm[3 * s + j];
+    }
+    live[s] = 1;
+}
+
+// One round's step (first: round 0); counters[parity] counts the candidates still live after
+// it, and the other counter is cleared for the next round.
+template <int W>
+__global__ __launch_bounds__(256) void qn_lock_step_kernel(const float* __restrict__ sums,
+                                                           const float* __restrict__ areas, int C, const QnSpec sp,
+                                                           QnLockState* __restrict__ states, float* __restrict__ qc,
+                                                           float* __restrict__ qn, int* __restrict__ live,
+                                                           int* __restrict__ counters, int first, int parity) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s == 0) counters[parity ^ 1] = 0;
+    if (s >= C || live[s] == 0) return;
+    QnLockState st = states[s];
+    // certificate_gradient's columns (corr, norm, dcorr re [6], im [6], dnorm [6]) in the
+    // layout of qn_certificate for W coordinates
+    const float* v = sums + (size_t)s * kCgValues;
+    float t[kQnMaxValues];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) t[j] = v[j];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        t[3 + j] = v[3 + j];
+        t[3 + W + j] = v[9 + j];
+        t[3 + 2 * W + j] = v[15 + j];
+    }
+    float pc[3], pn[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        pc[j] = qc[3 * s + j];
+        pn[j] = qn[3 * s + j];
+    }
+    float g[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    const float value = qn_certificate<W>(sp, t, areas[s], pn, g);
+    bool going, begin;
+    if (first) {  // the starting point
+        st.value = value;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) st.g[j] = g[j];
+#pragma unroll
+        for (int i = 0; i < 36; ++i) st.H[i] = i / 6 == i % 6 ? 1.0f : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            st.c[j] = pc[j];
+            st.n[j] = pn[j];
+        }
+        st.iteration = 0;
+        going = begin = sp.iterations > 0 && sp.backtracks > 0;
+    } else if (isfinite(value) && value > st.value) {  // the trial rises
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            st.c[j] = pc[j];
+            st.n[j] = pn[j];
+        }
+        float step[6], change[6], rho;
+        if (qn_secant<W>(sp, st.c, st.n, st.oc, st.on, g, st.g, step, change, &rho)) {
+            // H <- L H L^T + rho s s^T, L = I - rho s y^T, in the order of qn_search_kernel
+            float LH[36];
+#pragma unroll
+            for (int i = 0; i < W; ++i)
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                    float x = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) x += ((i == k ? 1.0f : 0.0f) - rho * step[i] * change[k]) * st.H[k * 6 + j];
+                    LH[i * 6 + j] = x;
+                }
+#pragma unroll
+            for (int i = 0; i < W; ++i)
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                    float x = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < W; ++k) x += LH[i * 6 + k] * ((j == k ? 1.0f : 0.0f) - rho * step[j] * change[k]);
+                    st.H[i * 6 + j] = x + rho * step[i] * step[j];
+                }
+        }
+        st.value = value;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) st.g[j] = g[j];
+        going = begin = ++st.iteration < sp.iterations;
+    } else {  // the trial fails: a shorter one, or stop
+        going = ++st.backtrack < sp.backtracks;
+        begin = false;
+        st.fraction *= 0.5f;
+    }
+    if (begin) {
+        qn_direction<W>(st.H, st.g, st.n, sp, st.d);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            st.oc[j] = st.c[j];
+            st.on[j] = st.n[j];
+        }
+        st.fraction = 1.0f;
+        st.backtrack = 0;
+    }
+    if (going) {
+        qn_trial<W>(st.oc, st.on, st.d, st.fraction, sp, pc, pn);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            qc[3 * s + j] = pc[j];
+            qn[3 * s + j] = pn[j];
+        }
+        atomicAdd(&counters[parity], 1);
+    } else {
+        live[s] = 0;
+    }
+    states[s] = st;
+}
+
+__global__ __launch_bounds__(256) void qn_lock_finish_kernel(const QnLockState* __restrict__ states, int C,
+                                                             float* __restrict__ ctr_out, float* __restrict__ nrm_out,
+                                                             float* __restrict__ value_out) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= C) return;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        ctr_out[3 * s + j] = states[s].c[j];
+        nrm_out[3 * s + j] = states[s].n[j];
+    }
+    value_out[s] = states[s].value;
+}
+
+struct QnLockLayout {
+    size_t states, qc, qn, sums, live, counters, grad, total;  // byte offsets
+};
+
+static QnLockLayout qn_lock_layout(const Chirp& c, int A, int C) {
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    QnLockLayout o;
+    o.states = 0;
+    o.qc = up(sizeof(QnLockState) * (size_t)C);
+    o.qn = o.qc + up(sizeof(float) * 3 * (size_t)C);
+    o.sums = o.qn + up(sizeof(float) * 3 * (size_t)C);
+    o.live = o.sums + up(sizeof(float) * kCgValues * (size_t)C);
+    o.counters = o.live + up(sizeof(int) * (size_t)C);
+    o.grad = o.counters + 256;
+    o.total = o.grad + up(cert_grad_workspace(c, A, C));
+    return o;
+}
+
+size_t qn_lockstep_workspace(const Chirp& c, int A, int count) {
+    if (cert_grad_workspace(c, A, count) == 0) return 0;
+    return qn_lock_layout(c, A, count).total;
+}
+
+constexpr int kQnLockCheck = 4;  // rounds between reads of the live count
+
+hipError_t launch_qn_lockstep(const Chirp& c, const float* pos, int A, const void* operands, const float* ctr,
+                              const float* nrm, const float* areas, int C, const QnSpec& sp, float* oc, float* on,
+                              float* ov, void* workspace, hipStream_t st) {
+    const QnLockLayout o = qn_lock_layout(c, A, C);
+    char* base = static_cast<char*>(workspace);
+    QnLockState* states = reinterpret_cast<QnLockState*>(base + o.states);
+    float* qc = reinterpret_cast<float*>(base + o.qc);
+    float* qn = reinterpret_cast<float*>(base + o.qn);
+    float* sums = reinterpret_cast<float*>(base + o.sums);
+    int* live = reinterpret_cast<int*>(base + o.live);
+    int* counters = reinterpret_cast<int*>(base + o.counters);
+    const dim3 grid((C + 255) / 256);
+    hipLaunchKernelGGL(qn_lock_init_kernel, grid, dim3(256), 0, st, ctr, nrm, C, qc, qn, live, counters);
+    hipError_t e = hipGetLastError();
+    // round 0 evaluates the starting points; each later round one trial per live candidate,
+    // at most `backtracks` per iteration
+    const long long rounds = 1 + (long long)sp.iterations * sp.backtracks;
+    for (long long r = 0; r < rounds && e == hipSuccess; ++r) {
+        e = launch_cert_grad(c, pos, A, qc, qn, C, operands, sums, base + o.grad, st, live);
+        if (e != hipSuccess) break;
+        if (sp.width == 6)
+            hipLaunchKernelGGL(qn_lock_step_kernel<6>, grid, dim3(256), 0, st, sums, areas, C, sp, states, qc, qn, live,
+                               counters, (int)(r == 0), (int)(r & 1));
+        else
+            hipLaunchKernelGGL(qn_lock_step_kernel<3>, grid, dim3(256), 0, st, sums, areas, C, sp, states, qc, qn, live,
+                               counters, (int)(r == 0), (int)(r & 1));
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        if (r % kQnLockCheck == kQnLockCheck - 1 && r + 1 < rounds) {
+            int left = 0;
+            e = hipMemcpyAsync(&left, counters + (r & 1), sizeof(int), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess || left == 0) break;
+        }
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(qn_lock_finish_kernel, grid, dim3(256), 0, st, states, C, oc, on, ov);
+    return hipGetLastError();
 }
 
 }  // namespace dsplat

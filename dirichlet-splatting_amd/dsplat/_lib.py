@@ -148,6 +148,11 @@ SIGNATURES = {
     "dsplat_qn_search": (
         c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
     ),
+    "dsplat_qn_lockstep_workspace": (c_size, [c_ptr, c_int, c_int]),
+    "dsplat_qn_lockstep": (
+        c_int,
+        [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],
+    ),
     "dsplat_elimination_workspace": (c_size, [c_int, c_int]),
     "dsplat_backward_elimination": (c_int, [c_ptr, c_ptr, c_int, c_int, c_ptr, c_int, c_ptr, c_ptr, c_size, c_ptr]),
     "dsplat_syrk_workspace": (c_size, [c_int, c_int, c_int]),

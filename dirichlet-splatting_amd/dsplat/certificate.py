@@ -244,8 +244,10 @@ def search_certificate(
     ids = select_peaks(coarse_scores, space.start["centers"], config.peaks, config.separation)
     coarse = {key: value[ids] for key, value in space.start.items()}
     # one workgroup per candidate fills the chip only with enough candidates; a few dozen
-    # (the full-aperture refinement) run the host loop, whose launches spread every aperture
-    if batched and hasattr(model, "qn_search") and not center_spec.normalize and len(ids) >= QN_MIN_BATCH:
+    # (the full-aperture refinement) run the host loop, whose launches spread every aperture,
+    # unless the model's ascent runs in lockstep rounds whose launches do that too
+    if (batched and hasattr(model, "qn_search") and not center_spec.normalize
+            and (len(ids) >= QN_MIN_BATCH or getattr(model, "lockstep_search", False))):  # fmt: skip
         centers, normals, value = model.qn_search(coarse, residual, space.kernel_spec(config, mode, residual))
         refined = dict(coarse, centers=centers, normals=normals)
     else:

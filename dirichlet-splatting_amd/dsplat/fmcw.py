@@ -351,11 +351,22 @@ class _SearchSupport:
         return surfel.orient_select(*self.sensor, params, bank, residual.reshape(self.shape), self.fresnel_n, table,
                                     real_part)  # fmt: skip
 
-    def qn_search(self, params, residual, spec):
-        """Quasi-Newton certificate ascent of every candidate in one launch."""
+    @property
+    def lockstep_search(self):
+        """Whether qn_search runs in lockstep rounds of matrix-core evaluations (chirps the
+        sample-domain kernels serve), which fill the chip at any candidate count."""
+        return surfel.kernel.sample_domain(self.sensor[0], self.fresnel_n)
+
+    def qn_search(self, params, residual, spec, lockstep=None):
+        """Quasi-Newton certificate ascent of every candidate on the device: lockstep rounds
+        (lockstep_search by default) or one launch with a workgroup per candidate."""
         config, positions = self.sensor
-        return search.qn_search(config, positions, residual.reshape(self.shape), params["centers"],
-                                params["normals"], params["areas"], spec, self.fresnel_n)  # fmt: skip
+        residual = residual.reshape(self.shape)
+        if self.lockstep_search if lockstep is None else lockstep:
+            operands = surfel.kernel.CertificateOperands(config, residual, self.fresnel_n)
+            return search.qn_lockstep(positions, params["centers"], params["normals"], params["areas"], spec, operands)
+        return search.qn_search(config, positions, residual, params["centers"], params["normals"], params["areas"],
+                                spec, self.fresnel_n)  # fmt: skip
 
     @torch.no_grad()
     def range_gate(self, points, measurement, relative_power, margin_cells, min_fraction):

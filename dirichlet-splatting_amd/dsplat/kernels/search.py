@@ -119,3 +119,24 @@ def qn_search(config, positions, residual, centers, normals, areas, spec, fresne
         _lib.ptr(values), _lib.ptr(workspace), size, _lib.stream(centers.device),
     )  # fmt: skip
     return out_c, out_n, values
+
+
+def qn_lockstep(positions, centers, normals, areas, spec, operands):
+    """The same ascent as qn_search in lockstep rounds of matrix-core evaluations against the
+    sample-domain operands of the residual (surfel.CertificateOperands): final (centers,
+    normals, values) (certificate.py:106-201)."""
+    count = len(centers)
+    out_c, out_n = torch.empty_like(centers), torch.empty_like(normals)
+    values = torch.empty(count, dtype=torch.float32, device=centers.device)
+    if count == 0:
+        return out_c, out_n, values
+    lib, cfg = _lib.library(), operands.cfg
+    size = lib.dsplat_qn_lockstep_workspace(_lib.ctypes.byref(cfg), len(positions), count)
+    workspace = torch.empty(max(1, size // 4), dtype=torch.float32, device=centers.device)
+    _lib.call(
+        "dsplat_qn_lockstep", _lib.ctypes.byref(cfg), _lib.ptr(positions.float().contiguous()), len(positions),
+        _lib.ptr(operands.buffer), _lib.ptr(centers.float().contiguous()), _lib.ptr(normals.float().contiguous()),
+        _lib.ptr(areas.float().contiguous()), count, _lib.ctypes.byref(spec), _lib.ptr(out_c), _lib.ptr(out_n),
+        _lib.ptr(values), _lib.ptr(workspace), size, _lib.stream(centers.device),
+    )  # fmt: skip
+    return out_c, out_n, values

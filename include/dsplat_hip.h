@@ -422,6 +422,21 @@ int dsplat_qn_search(const dsplat_fmcw* cfg, const float* positions, int32_t num
                      int32_t count, const dsplat_qn_spec* spec, float* centers_out, float* normals_out,
                      float* values_out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* The same ascent, with the same outputs and stopping rules, on chirps the sample-domain
+ * kernels serve, in lockstep rounds: each round evaluates the certificate and its gradient of
+ * every live candidate at its next point in one dsplat_certificate_gradient launch against
+ * `operands` (dsplat_certificate_operands of the residual), and one thread per candidate
+ * accepts the point (with the BFGS update and the next direction) or halves the step. An
+ * iteration whose first trial rises costs one round. The call reads a live count from the
+ * device every few rounds, so it synchronizes `stream` and cannot be graph-captured.
+ * Workspace from dsplat_qn_lockstep_workspace (0: chirp not served). Replaces the host loop
+ * of certificate.py:106-201 (search_certificate), like dsplat_qn_search. */
+size_t dsplat_qn_lockstep_workspace(const dsplat_fmcw* cfg, int32_t num_apertures, int32_t count);
+int dsplat_qn_lockstep(const dsplat_fmcw* cfg, const float* positions, int32_t num_apertures, const void* operands,
+                       const float* centers, const float* normals, const float* areas, int32_t count,
+                       const dsplat_qn_spec* spec, float* centers_out, float* normals_out, float* values_out,
+                       void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- Dense products. */
 
 /* Upper triangle of out [n, n] (row-major) += rows^T rows, rows [m, n] row-major float32,

@@ -53,6 +53,21 @@ def test_rejected_configurations_return_invalid_value_without_launching():
     assert lib.dsplat_top_descending(None, 1, 10, 256, None, None) == 1  # null buffers
     assert lib.dsplat_top_descending(None, 0, 10, 256, None, None) == 0  # nothing to rank
     assert lib.dsplat_certificate_scores(None, 1, 10, None, None, 0, None, None) == 1
+    # the lockstep ascent serves sample-domain chirps only, and checks its spec and workspace
+    samples, bins = _lib.fmcw(FMCWConfig()), _lib.fmcw(FMCWConfig(splat_form="bins"))
+    assert lib.dsplat_qn_lockstep_workspace(ctypes.byref(bins), 64, 16) == 0
+    size = lib.dsplat_qn_lockstep_workspace(ctypes.byref(samples), 64, 16)
+    assert size > 0
+    spec = _lib.QnSpec()
+    spec.width, spec.iterations, spec.backtracks = 6, 4, 2
+    fake = ctypes.c_void_p(256)  # never dereferenced: the checks come first
+    args = [fake, 64, fake, fake, fake, fake, 16, ctypes.byref(spec), fake, fake, fake, fake]
+    assert lib.dsplat_qn_lockstep(ctypes.byref(samples), *args, size - 1, None) == 1  # short workspace
+    assert lib.dsplat_qn_lockstep(ctypes.byref(bins), *args, size, None) == 1  # bin-domain chirp
+    spec.width = 4
+    assert lib.dsplat_qn_lockstep(ctypes.byref(samples), *args, size, None) == 1  # width 3 or 6 only
+    spec.width = 6
+    assert lib.dsplat_qn_lockstep(ctypes.byref(samples), *args[:6], 0, *args[7:], size, None) == 0  # no candidates
 
 
 def test_vjp_workspace_per_splat_form():

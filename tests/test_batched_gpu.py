@@ -95,14 +95,19 @@ def test_batched_normal_equations_take_the_rhs_from_one_correlation(fmcw):  # no
     close(fused[1], plain[1], 1e-5)
 
 
+@pytest.mark.parametrize("lockstep", [False, True])
 @pytest.mark.parametrize("mode,normals", [("real", True), ("complex", True), ("complex", False)])
-def test_qn_kernel_follows_the_host_ascent(fmcw, mode, normals):  # noqa: F811
-    """Same candidates, same settings: the one-launch ascent and the host loop end at the
-    same certificates (their sums are rounded in different orders, so trajectories agree
-    to float32 accuracy rather than bit for bit), and both reach the true reflector."""
+def test_qn_kernel_follows_the_host_ascent(fmcw, mode, normals, lockstep, monkeypatch):  # noqa: F811
+    """Same candidates, same settings: the device ascent (one launch, or lockstep rounds on
+    the sample-domain kernel) and the host loop end at the same certificates (their sums are
+    rounded in different orders, so trajectories agree to float32 accuracy rather than bit
+    for bit), and both reach the true reflector."""
+    from dsplat import certificate
     from dsplat.certificate import CertificateSearchConfig, search_certificate
 
     model, params, coefficients = fmcw
+    monkeypatch.setattr(certificate, "QN_MIN_BATCH", 1)
+    monkeypatch.setattr(type(model), "lockstep_search", property(lambda self: lockstep))
     one = {k: v[:1] for k, v in params.items()}
     residual = model.render(one, coefficients[:1])
     start = {k: v.repeat(24, *([1] * (v.ndim - 1))) for k, v in one.items()}
@@ -123,6 +128,93 @@ def test_qn_kernel_follows_the_host_ascent(fmcw, mode, normals):  # noqa: F811
     assert float((host["centers"][0] - one["centers"][0]).norm()) < 5e-5
     if normals:
         torch.testing.assert_close(fused["normals"].norm(dim=1), torch.ones(len(fused["normals"]), device="cuda"))
+
+
+class _OneKernel:
+    """The model's certificate sums (with area), all from the sample-domain certificate
+    gradient kernel, so that the host ascent compares values of the kernel the lockstep
+    rounds evaluate."""
+
+    def __init__(self, model, residual):
+        from dsplat.kernels import surfel
+
+        self.positions = model.sensor[1]
+        self.operands = surfel.CertificateOperands(model.sensor[0], residual.reshape(model.shape), model.fresnel_n)
+
+    def _sums(self, params):
+        from dsplat.kernels import surfel
+
+        out = surfel.certificate_gradient(self.positions, params["centers"], params["normals"], self.operands)
+        area = params["areas"].float()
+        scale = area.sqrt()
+        corr = torch.complex(out[:, 0] * scale, out[:, 1] * scale)
+        d_corr = torch.complex(out[:, 3:9] * scale[:, None], out[:, 9:15] * scale[:, None])
+        return corr, out[:, 2] * area, d_corr, out[:, 15:21] * area[:, None]
+
+    def correlate(self, params, residual, scan=True):
+        return self._sums(params)[:2]
+
+    def correlate_gradient(self, params, residual, include_normals=False):
+        corr, norm, d_corr, d_norm = self._sums(params)
+        return (corr, norm, d_corr, d_norm) if include_normals else (corr, norm, d_corr[:, :3], d_norm[:, :3])
+
+
+@pytest.mark.parametrize(
+    "mode,normals,fresnel_n,iterations,backtracks",
+    [("complex", True, 0.0, 1, 8), ("complex", True, 0.0, 3, 8), ("real", True, 1.6, 3, 8),
+     ("complex", False, 0.0, 3, 8), ("complex", True, 0.0, 8, 8), ("complex", True, 0.0, 0, 8),
+     ("complex", True, 1.6, 3, 1)],
+)  # fmt: skip
+def test_lockstep_ascent_makes_the_host_loops_decisions(fmcw, mode, normals, fresnel_n, iterations, backtracks):  # noqa: F811
+    """The lockstep rounds against the host loop scoring on the same evaluation kernel: every
+    acceptance and halving compares values of one kernel, and the first iteration ends at
+    the same points bit for bit. Later iterations differ by the rounding of the BFGS update
+    (a batched matmul on the host, sequential sums here), which the ascent carries forward:
+    about 1e-7 after two iterations, a few 1e-6 after eight. Also the stopping rules: no
+    iterations, and a single trial per iteration."""
+    from dataclasses import replace
+
+    from dsplat.certificate import CertificateSearchConfig, _ascend_on_host, _SearchSpace, certificate
+    from dsplat.kernels import search
+
+    model, params, coefficients = fmcw
+    model = replace(model, fresnel_n=fresnel_n)
+    residual = model.render(params, coefficients) * 0.5 + model.render({k: v[:1] for k, v in params.items()},
+                                                                         coefficients[:1])  # fmt: skip
+    g = torch.Generator("cuda").manual_seed(3)
+    pick = torch.arange(40, device="cuda") % len(params["centers"])
+    start = {k: v[pick].clone() for k, v in params.items()}
+    start["centers"] = start["centers"] + 8e-4 * torch.randn(start["centers"].shape, device="cuda", generator=g)
+    start["normals"] = torch.nn.functional.normalize(
+        start["normals"] + 0.3 * torch.randn(start["normals"].shape, device="cuda", generator=g), dim=1
+    )
+    specs = fmcw_specs()
+    config = CertificateSearchConfig(peaks=40, separation=0, iterations=iterations, backtracks=backtracks)
+    space = _SearchSpace(start, specs["centers"], specs["normals"] if normals else None)
+    one = _OneKernel(model, residual)
+
+    def score(p, gradient=False, scan=True):
+        return certificate(one, p, residual, mode, gradient=gradient, normals=space.with_normals, scan=scan)
+
+    host, host_values = _ascend_on_host(space, {k: v.clone() for k, v in space.start.items()}, score, config)
+    c, n, values = search.qn_lockstep(model.sensor[1], space.start["centers"], space.start["normals"],
+                                      space.start["areas"], space.kernel_spec(config, mode, residual), one.operands)
+    begin = score(space.start)
+    assert torch.all(values >= begin * (1 - 1e-6))
+    if iterations == 0:
+        assert torch.equal(c, space.start["centers"]) and torch.equal(n, space.start["normals"])
+        torch.testing.assert_close(values, begin, rtol=1e-6, atol=0)
+        return
+    assert float((values / begin).max()) > 1.001  # the ascent moved
+    if iterations == 1:
+        assert torch.equal(values, host_values) and torch.equal(c, host["centers"])
+        return
+    value_tol, center_tol = (1e-6, 1e-8) if iterations <= 3 else (3e-5, 2e-6)
+    torch.testing.assert_close(values, host_values, rtol=value_tol, atol=0)
+    torch.testing.assert_close(c, host["centers"], rtol=0, atol=center_tol)
+    torch.testing.assert_close(n, host["normals"], rtol=0, atol=400 * center_tol)
+    if not normals:
+        assert torch.equal(n, space.start["normals"])
 
 
 @pytest.mark.parametrize("gram", ["plain", "fp16x3", "bf16x3"])
