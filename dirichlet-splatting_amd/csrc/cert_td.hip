@@ -237,17 +237,33 @@ __global__ __launch_bounds__(256) void cert_grad_kernel(const Chirp c, const flo
     }
 }
 
-// out[s][k] = sum over ranges of partial[s][r][k] in float64, fixed order (live candidates only
-// when live is given).
+// out[s][k] = sum over ranges of partial[s][r][k] in float64 (live candidates only when live
+// is given): one workgroup per candidate, thread t summing ranges t, t + 256, ..., then a
+// butterfly over each wave and the four waves in order, so the result does not depend on
+// timing.
 __global__ __launch_bounds__(256) void cert_grad_reduce(const float* __restrict__ partial, int C, int nrange,
                                                         const int* __restrict__ live, float* __restrict__ out) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= C * kCgValues) return;
-    const int s = e / kCgValues, k = e - s * kCgValues;
-    if (live != nullptr && live[s] == 0) return;
-    double sum = 0.0;
-    for (int r = 0; r < nrange; ++r) sum += partial[((size_t)s * nrange + r) * kCgValues + k];
-    out[e] = (float)sum;
+    __shared__ double red[4][kCgValues];
+    const int s = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (live != nullptr && live[s] == 0) return;  // uniform
+    double acc[kCgValues];
+#pragma unroll
+    for (int k = 0; k < kCgValues; ++k) acc[k] = 0.0;
+    const float* src = partial + (size_t)s * nrange * kCgValues;
+    for (int r = tid; r < nrange; r += 256) {
+#pragma unroll
+        for (int k = 0; k < kCgValues; ++k) acc[k] += src[(size_t)r * kCgValues + k];
+    }
+#pragma unroll
+    for (int k = 0; k < kCgValues; ++k) {
+        double d = acc[k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) d += __shfl_xor(d, o, 64);
+        if (lane == 0) red[wave][k] = d;
+    }
+    __syncthreads();
+    if (tid < kCgValues)
+        out[(size_t)s * kCgValues + tid] = (float)(((red[0][tid] + red[1][tid]) + red[2][tid]) + red[3][tid]);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -331,9 +347,7 @@ static hipError_t launch_cert_grad_j(const Chirp& c, const float* pos, int A, co
         hipLaunchKernelGGL((cert_grad_kernel<J, false>), dim3(v.tiles, v.ranges), dim3(256), smem, st, c, pos, A,
                            ctr, nrm, C, ops, scales, v.per_range, live, partial);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    const int total = C * kCgValues;
-    hipLaunchKernelGGL(cert_grad_reduce, dim3((total + 255) / 256), dim3(256), 0, st, partial, C, v.ranges, live,
-                       out);
+    hipLaunchKernelGGL(cert_grad_reduce, dim3(C), dim3(256), 0, st, partial, C, v.ranges, live, out);
     return hipGetLastError();
 }
 
