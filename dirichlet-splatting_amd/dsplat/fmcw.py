@@ -219,7 +219,8 @@ class _LeastSquaresStream:
             if band is not None:
                 low, high = float(band[0].min()), float(band[1].max())
                 bands = (band[0].cpu().numpy(), band[1].cpu().numpy())
-        return replace(self, basis=response_basis(config, low, high, positions.device), bands=bands, block_bases={})
+        return replace(self, basis=response_basis(config, low, high, positions.device), bands=bands, block_bases={},
+                       split_cache=SplitCache())
 
     def block_basis(self, rows):
         """The range subspace of a block of whole apertures: the band of its own apertures,
@@ -282,29 +283,119 @@ class _LeastSquaresStream:
 
         return (block(first) for first in self._aperture_steps(chunk))
 
-    def split_blocks(self, params, chunk, format="bf16"):
+    def split_blocks(self, params, chunk, format="bf16", known=None):
         """The same blocks in the operand form of the split SYRK (kernels.blas.SplitBlock:
         16-bit hi / lo in 32-row column tiles, written by the projection kernel itself; FP16
         with column scales that hold for every aperture of the model), or None where the
-        sample-domain projection does not serve the chirp."""
+        sample-domain projection does not serve the chirp.
+
+        A model with a split_cache projects every aperture into it at once and returns views of
+        it: columns the cache already holds for `known` (the leading atoms of params, e.g. the
+        support before pooled candidates) are not projected again. The values are the same
+        either way: every column is projected on its own."""
         if self.basis is None:
             return None
         config, positions = self.sensor
         sampled = self.basis.sampled(config, self.fresnel_n)
         if sampled is None:
             return None
-        from .kernels.blas import projection_exponents
+        from .kernels.blas import SplitBlock, projection_exponents
 
-        exponents = (projection_exponents(config, positions, params["centers"], params["areas"])
-                     if format == "fp16" else None)  # fmt: skip
         step = max(1, chunk // self.shape[1])
         rows = 2 * self.basis.rank
+        centers, normals, areas = params["centers"], params["normals"], params["areas"]
+
+        def exponents_of(first=0):
+            if format != "fp16":
+                return None
+            return projection_exponents(config, positions, centers[first:], areas[first:])
+
+        cache = self.split_cache
+        if cache is None:
+            return (
+                surfel.kernel.project_split(config, positions[first : first + step], centers, normals, areas, sampled,
+                                            rows, self.fresnel_n, format, exponents_of())
+                for first in self._aperture_steps(chunk)
+            )  # fmt: skip
+        count = len(centers)
+        pad = surfel.kernel.project_split_rows(rows)
+        first = cache.reserve(self.apertures * pad // 32, count, format, positions.device, known)
+        if first < count:
+            part = exponents_of(first)
+            surfel.kernel.project_split(config, positions, centers[first:], normals[first:], areas[first:], sampled,
+                                        rows, self.fresnel_n, format, part, out=(cache.hi, cache.lo), col0=first)
+            if part is not None:
+                cache.exponents[first:count] = part
+        cache.holding(params, count, known)
+        exponents = cache.exponents[:count] if format == "fp16" else None
+        tiles = step * pad // 32
         return (
-            surfel.kernel.project_split(config, positions[first : first + step], params["centers"],
-                                        params["normals"], params["areas"], sampled, rows, self.fresnel_n, format,
-                                        exponents)
-            for first in self._aperture_steps(chunk)
+            SplitBlock(cache.hi[t : t + tiles], cache.lo[t : t + tiles], min(tiles, len(cache.hi) - t) * 32,
+                       columns=count, format=format, exponents=exponents)
+            for t in range(0, len(cache.hi), tiles)
         )  # fmt: skip
+
+
+class SplitCache:
+    """The split SYRK operands of the whole row-projected design (every aperture of one model),
+    kept in HBM between the normal-equation passes of a fit: hi / lo [A rows_pad / 32, ld, 32]
+    and the float32 column exponents of the FP16 form. At 16384 apertures x 64 rows x 2304
+    columns that is 9.7 GB.
+
+    `held` lists (atoms, count): columns 0 .. count - 1 are the projections of those atoms
+    (centres, normals and areas, matched by tensor identity and in-place version, the tensors
+    kept alive here so an identity is never reused). Pooled statistics then project only the
+    pooled candidates, and a merged support adopts the columns of the candidates it took."""
+
+    KEYS = ("centers", "normals", "areas")
+
+    def __init__(self):
+        self.hi = self.lo = self.exponents = None
+        self.format = None
+        self.held = []
+
+    @classmethod
+    def _token(cls, params):
+        return tuple((params[key], params[key]._version) for key in cls.KEYS)
+
+    def _holds(self, params, count):
+        token = self._token(params)
+        return any(n >= count and all(a is b and u == v for (a, u), (b, v) in zip(t, token)) for t, n in self.held)
+
+    def reserve(self, tiles, count, format, device, known=None):
+        """Room for count columns; returns the first column to project (the columns of
+        `known` are kept when held)."""
+        from .kernels.blas import SPLIT_FORMATS
+
+        dtype = SPLIT_FORMATS[format][1]
+        if self.hi is None or self.hi.shape[0] != tiles or self.hi.shape[1] < count or self.format != format:
+            ld = -(-count // 256) * 256
+            self.hi = self.lo = None
+            self.hi = torch.empty((tiles, ld, 32), dtype=dtype, device=device)
+            self.lo = torch.empty_like(self.hi)
+            self.exponents = torch.zeros(ld, dtype=torch.float32, device=device)
+            self.format, self.held = format, []
+        if known is not None and self._holds(known, len(known["centers"])):
+            return len(known["centers"])
+        return 0
+
+    def holding(self, params, count, known=None):
+        """Columns 0 .. count - 1 now hold params (and their leading columns `known`)."""
+        self.held = [(self._token(params), count)]
+        if known is not None:
+            self.held.append((self._token(known), len(known["centers"])))
+
+    def adopt(self, pool, params, slots, sources):
+        """params took pool column sources[i] into slot slots[i] (the other slots keep the
+        pool's columns): move those columns, if the cache holds the pool."""
+        count = len(params["centers"])
+        if not self._holds(pool, len(pool["centers"])):
+            return
+        if len(slots):
+            for part in (self.hi, self.lo):
+                part.index_copy_(1, slots, part.index_select(1, sources))
+            self.exponents[slots] = self.exponents[sources]
+        self.held = [(self._token(params), count)]
 
 
 class _SearchSupport:
@@ -418,6 +509,7 @@ class FMCWModel(_SplatProducts, _LeastSquaresStream, _SearchSupport):
     basis: RangeSubspace | None = None
     bands: tuple | None = None  # per aperture, on the host: lowest and highest peak bin
     block_bases: dict = field(default_factory=dict, repr=False)
+    split_cache: SplitCache | None = field(default=None, repr=False, compare=False)
 
     @property
     def shape(self):
@@ -445,7 +537,7 @@ class FMCWModel(_SplatProducts, _LeastSquaresStream, _SearchSupport):
         ids = ids.sort().values.to(self.scan.data.device)
         subset = replace(self.scan, data=self.scan.data[ids].contiguous(),
                          scan_positions_m=self.scan.scan_positions_m[ids].contiguous())  # fmt: skip
-        return ids, replace(self, scan=subset, bands=None, block_bases={})
+        return ids, replace(self, scan=subset, bands=None, block_bases={}, split_cache=None)
 
     def sample_rows(self, count, seed, device):
         """Every row of a seeded random set of count // K whole apertures."""

@@ -366,29 +366,51 @@ def project_design(config, positions, centers, normals, areas, basis, fresnel_n,
     return out
 
 
-def project_split(config, positions, centers, normals, areas, sampled, rows, fresnel_n, format="bf16", exponents=None):
+def project_split_rows(rows):
+    """Rows per aperture of a split block: rows rounded up to 32."""
+    return int(_lib.library().dsplat_project_split_rows(rows))
+
+
+def project_split(config, positions, centers, normals, areas, sampled, rows, fresnel_n, format="bf16", exponents=None,
+                  out=None, col0=0):
     """The sample-domain projected block of project_design in the operand form of
     blas.syrk_split: a SplitBlock of m = A * rows_pad rows (rows_pad = rows rounded up to 32;
     projected row j of aperture a is row a * rows_pad + j, padding rows zero), S columns, in
-    BF16, or in FP16 with every column scaled by 2^exponents[s] (blas.projection_exponents)."""
+    BF16, or in FP16 with every column scaled by 2^exponents[s] (blas.projection_exponents).
+
+    out = (hi, lo), existing [m / 32, ld, 32] arrays, takes the S columns at columns
+    col0 .. col0 + S - 1 instead (ld >= col0 + S; the other columns are left as they are);
+    nothing is returned then."""
     from .blas import SPLIT_FORMATS, SplitBlock
 
     _check(positions, centers, normals)
     code, dtype = SPLIT_FORMATS[format]
     cfg = _lib.fmcw(config, fresnel_n)
-    pad = _lib.library().dsplat_project_split_rows(rows)
-    m = pad * len(positions)
-    hi = torch.empty((m // 32, len(centers), 32), dtype=dtype, device=centers.device)
-    lo = torch.empty_like(hi)
+    m = project_split_rows(rows) * len(positions)
+    if out is None:
+        hi = torch.empty((m // 32, len(centers), 32), dtype=dtype, device=centers.device)
+        lo = torch.empty_like(hi)
+        first_hi, first_lo = _lib.ptr(hi), _lib.ptr(lo)
+    else:
+        hi, lo = out
+        if hi.dtype != dtype or lo.dtype != dtype or hi.shape != lo.shape or hi.shape[0] != m // 32:
+            raise ValueError(f"project_split: out is two {dtype} [{m // 32}, ld, 32] arrays")
+        if col0 < 0 or col0 + len(centers) > hi.shape[1]:
+            raise ValueError("project_split: columns col0 .. col0 + S - 1 must lie within ld")
+        step = 32 * hi.element_size() * col0  # column col0 of tile 0; ld stays the tiles' stride
+        first_hi = _lib.ctypes.c_void_p(_lib.ptr(hi).value + step)
+        first_lo = _lib.ctypes.c_void_p(_lib.ptr(lo).value + step)
     if exponents is not None:
         exponents = exponents.float().contiguous()
     _lib.call(
         "dsplat_project_sampled_split", _lib.ctypes.byref(cfg), _lib.ptr(positions.contiguous()), len(positions),
         _lib.ptr(centers.contiguous()), _lib.ptr(normals.contiguous()), _lib.ptr(areas.float().contiguous()),
-        len(centers), _lib.ptr(sampled.float().contiguous()), rows, code, _lib.ptr(exponents), _lib.ptr(hi),
-        _lib.ptr(lo), len(centers), _lib.stream(centers.device),
+        len(centers), _lib.ptr(sampled.float().contiguous()), rows, code, _lib.ptr(exponents), first_hi, first_lo,
+        hi.shape[1], _lib.stream(centers.device),
     )  # fmt: skip
-    return SplitBlock(hi, lo, m, format=format, exponents=exponents)
+    if out is None:
+        return SplitBlock(hi, lo, m, format=format, exponents=exponents)
+    return None
 
 
 def orient_select(config, positions, centers, normals, areas, bank, residual, fresnel_n, table=None, real_part=False):

@@ -206,7 +206,7 @@ def pooled_statistics(state, pool, config):
         # cross products of the new columns with all of them on the MFMA block products, their
         # right-hand side from one correlation
         acc = _Folded(total, added, state.target.device)
-        block_products(model, pool, state.target, config, acc, col0=count)
+        block_products(model, pool, state.target, config, acc, col0=count, known=state.params)
         cross = (acc.result() / model.size).float()
         fresh = {key: value[count:] for key, value in pool.items()}
         right = (model.adjoint_correlation(fresh, state.target).real.double() / model.size).float()
@@ -281,6 +281,9 @@ def merge_support(state, config, pool, scores=None):
         ids[owners] = (~gone[count:]).nonzero().flatten() + count
         trial, fit = _settle(state, config, pooled, gram, rhs, ids, owners)
         if fit.objective < state.objective:
+            cache = getattr(state.model, "split_cache", None)
+            if cache is not None:
+                cache.adopt(pooled, trial, owners, ids[owners])
             return replace(state, params=trial, **fit._asdict()), len(owners)
         limit = len(owners) // 2
         if limit < 1:

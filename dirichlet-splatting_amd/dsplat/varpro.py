@@ -134,17 +134,18 @@ class _Folded:
         return self.total
 
 
-def block_products(model, params, target, config, out, col0=None):
+def block_products(model, params, target, config, out, col0=None, known=None):
     """Accumulate the real block products of the streamed design into `out` (a _Folded of
     [S, S] upper triangle, or [S, S - col0] cross products of the trailing columns) with the
     configured backend. Returns the float64 A^T y of the streamed blocks, or None for the cross
-    products and in the batched form (which takes A^H y from one correlation)."""
+    products and in the batched form (which takes A^H y from one correlation). known: the
+    leading atoms of params whose projections a model's split cache may already hold."""
     count = len(params["centers"])
     chunk = row_chunk(model, count, config)
     mode = SYRK_MODE[getattr(config, "gram", "plain")]
     batched = getattr(config, "batched", False) and hasattr(model, "adjoint_correlation")
     if batched and mode in ("bf16x3", "fp16x3") and hasattr(model, "split_blocks"):
-        split = model.split_blocks(params, chunk, {"bf16x3": "bf16", "fp16x3": "fp16"}[mode])
+        split = model.split_blocks(params, chunk, {"bf16x3": "bf16", "fp16x3": "fp16"}[mode], known=known)
         if split is not None:
             for block in split:
                 syrk_split(block, out.part, col0)

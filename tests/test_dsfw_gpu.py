@@ -76,3 +76,37 @@ def test_mixed_and_plain_fits_agree(samples):
         f"final {p[-1]:.4f} / {m[-1]:.4f}")  # fmt: skip
     assert max(same) < 1e-4
     assert deviation < max(0.01, 2 * control)
+
+
+@pytest.mark.parametrize("samples", [128, 256])
+def test_split_cache_changes_no_value(samples, monkeypatch):
+    """The mixed form keeps the split operands of the support's projected design in HBM, so a
+    merge projects only the pooled candidates and a merged support adopts their columns: the
+    fit must be bit for bit the one that projects every column every time, with far fewer
+    projected columns."""
+    from dsplat import fmcw, workloads
+    from dsplat.kernels import surfel as K
+
+    say(f"split cache: 32x32 apertures, {samples} samples, 128 front faces, 6 mixed steps with and without")
+    problem = workloads.dsfw_problem(adc_samples=samples, front_count=128, side=32)
+    projected, project = [], K.project_split
+
+    def counting(*args, **kwargs):
+        projected.append(len(args[1]) * len(args[2]))  # apertures x columns of the call
+        return project(*args, **kwargs)
+
+    monkeypatch.setattr(K, "project_split", counting)
+    cached, cached_states, (model, _) = fit(problem, "samples", MIXED, 6, 64)
+    assert isinstance(model.split_cache, fmcw.SplitCache)
+    with_cache = sum(projected)
+    projected.clear()
+    monkeypatch.setattr(fmcw, "SplitCache", lambda: None)
+    fresh, fresh_states, (model, _) = fit(problem, "samples", MIXED, 6, 64)
+    assert model.split_cache is None
+    without = sum(projected)
+    say(f"  projected aperture-columns: {with_cache} with the cache, {without} without")
+    assert [h["objective"] for h in cached.history] == [h["objective"] for h in fresh.history]
+    for a, b in zip(cached_states, fresh_states, strict=True):
+        assert all(torch.equal(a[key], b[key]) for key in a)
+    assert torch.equal(cached.coefficients, fresh.coefficients)
+    assert with_cache < 0.75 * without
