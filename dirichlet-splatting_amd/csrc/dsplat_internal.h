@@ -199,6 +199,12 @@ hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* c
                          hipStream_t st);
 hipError_t launch_vjp_reduce(const float* partial, int S, int nblk, float* gc, float* gn, float* gw,
                              hipStream_t st);
+// Sample-domain VJP by bin moments (vjp_moments.hip): chirps with 128 or 256 samples and pad >= 2.
+bool vjp_moments_supported(const Chirp& c);
+size_t vjp_moments_workspace(int A, int S);
+hipError_t launch_vjp_moments(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
+                              const float2* w, int S, const float2* up, void* ws, float* gc, float* gn, float* gw,
+                              hipStream_t st);
 
 constexpr int kCholMaxSolve = 8192;  // LDS holds the solution vector
 hipError_t launch_cholesky(double* A, int n, int lda, int* info, hipStream_t st);

@@ -39,39 +39,6 @@
 
 namespace dsplat {
 
-// out[r][q] = scale sum_{p < Lin} e^{sign 2 pi i p q / N} X[r][p] for the 16 rows of X (LDS,
-// row stride ldx float2) and q < Lout (multiple of 8), one 16 x 8-bin tile per wave at a time
-// on v_mfma_f32_16x16x4f32. Lane layouts (gfx950): A[i][k] -> lane i + 16 k, B[k][j] -> lane
-// j + 16 k, C[4 (lane / 16) + r][lane % 16] in register r. K index k = 2 dp + pin: sample
-// p = 2 t + dp, input part pin; column x = 2 (bin offset) + output part.
-__device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const float2* E, int N, float sign,
-                                      float scale, int Lout, float2* out, int ldo, int rows, int wave,
-                                      int lane) {
-    const int kk = lane >> 4, x = lane & 15;
-    const int pin = kk & 1, dp = kk >> 1, pout = x & 1;
-    const float* Xf = reinterpret_cast<const float*>(X);
-    for (int ct = wave; ct * 8 < Lout; ct += 4) {
-        const int q = ct * 8 + (x >> 1);
-        f32x4 d = {0.f, 0.f, 0.f, 0.f};
-        int m = (dp * q) & (N - 1);
-        const int step = (2 * q) & (N - 1);
-        const float* xrow = Xf + 2 * (x * ldx + dp) + pin;
-        for (int t = 0; t < Lin / 2; ++t) {
-            const float a = xrow[4 * t];
-            const float2 e = E[m];
-            const float fr = e.x * scale, fi = sign * e.y * scale;
-            const float b = pin ? (pout ? fr : -fi) : (pout ? fi : fr);
-            d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
-            m = (m + step) & (N - 1);
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = 4 * kk + r;
-            if (row < rows) reinterpret_cast<float*>(out + (size_t)row * ldo + q)[pout] = d[r];
-        }
-    }
-}
-
 // dft16 for 8 rows: the 16 MFMA rows take the two halves of the samples, row i the samples
 // p < Lin / 2 of X row i % 8 (i < 8) or p + Lin / 2 (i >= 8), so the K loop is half as long.
 // The second half's result is turned by e^{sign 2 pi i (Lin / 2) q / N} (Re and Im of a bin
@@ -1051,7 +1018,9 @@ static TdVjpWorkspace td_vjp_layout(const Chirp& c, int A, int S) {
     return w;
 }
 
-size_t td_vjp_workspace(const Chirp& c, int A, int S) { return td_vjp_layout(c, A, S).total; }
+size_t td_vjp_workspace(const Chirp& c, int A, int S) {
+    return vjp_moments_supported(c) ? vjp_moments_workspace(A, S) : td_vjp_layout(c, A, S).total;
+}
 
 template <int J>
 static hipError_t launch_td_vjp_j(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
@@ -1113,6 +1082,7 @@ hipError_t launch_dft_rows_up(const Chirp& c, const float2* in, int rows, float2
 hipError_t launch_td_vjp(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, const float2* up, void* ws, float* gc, float* gn, float* gw,
                          hipStream_t st) {
+    if (vjp_moments_supported(c)) return launch_vjp_moments(c, pos, A, ctr, nrm, w, S, up, ws, gc, gn, gw, st);
     if (c.S == 128) return launch_td_vjp_j<8>(c, pos, A, ctr, nrm, w, S, up, ws, gc, gn, gw, st);
     if (c.S == 256) return launch_td_vjp_j<16>(c, pos, A, ctr, nrm, w, S, up, ws, gc, gn, gw, st);
     return hipErrorInvalidValue;

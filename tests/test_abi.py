@@ -76,15 +76,17 @@ def test_vjp_workspace_per_splat_form():
     lib = _lib.library()
     cfg = ctypes.byref(_lib.fmcw(FMCWConfig()))
     assert lib.dsplat_surfel_vjp_workspace(cfg, 0, 10) == 0
-    # sample-domain form at 128 samples: partials of 8 floats per (surfel, 128-aperture
-    # range), the FP16 hi + lo MFMA operands of the symmetric split (one K half: 2 KiB per
-    # aperture) and one scale per aperture; the cotangent's sample-domain rows stay in LDS
-    assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == 2048 * 128 * 8 * 4 + 16384 * 2048 + 16384 * 4
-    # 256 samples: the sample-domain rows U [A, 256] complex go through the workspace
+    # sample-domain form at 128 and 256 samples (pad >= 2): partials of 8 floats per (surfel,
+    # 64-aperture range), one slot of 32 bins x 10 complex moments per aperture (2560 bytes)
+    # and its band (int2)
+    moments = 2048 * 256 * 8 * 4 + 16384 * 2560 + 16384 * 8
+    assert lib.dsplat_surfel_vjp_workspace(cfg, 16384, 2048) == moments
     wide = ctypes.byref(_lib.fmcw(FMCWConfig(adc_samples=256)))
-    assert lib.dsplat_surfel_vjp_workspace(wide, 16384, 2048) == (
-        2048 * 128 * 8 * 4 + 16384 * 256 * 8 + 16384 * 2 * 2048 + 16384 * 4
-    )
+    assert lib.dsplat_surfel_vjp_workspace(wide, 16384, 2048) == moments
+    # pad 1 keeps the FP16 hi + lo MFMA form: partials per 128-aperture range, the symmetric
+    # split's operands (one K half: 2 KiB per aperture) and one scale per aperture
+    pad1 = ctypes.byref(_lib.fmcw(FMCWConfig(pad_factor=1)))
+    assert lib.dsplat_surfel_vjp_workspace(pad1, 16384, 2048) == 2048 * 128 * 8 * 4 + 16384 * 2048 + 16384 * 4
     # bin-domain form, chosen per call: one partial per surfel and 32-aperture block
     bins = ctypes.byref(_lib.fmcw(FMCWConfig(splat_form="bins")))
     assert lib.dsplat_surfel_vjp_workspace(bins, 16384, 2048) == 2048 * 512 * 8 * 4
