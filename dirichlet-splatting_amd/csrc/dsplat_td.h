@@ -42,6 +42,40 @@ __device__ __forceinline__ void stage_to_lds(uint4* dst, int tid, Src src_of) {
 // on v_mfma_f32_16x16x4f32. Lane layouts (gfx950): A[i][k] -> lane i + 16 k, B[k][j] -> lane
 // j + 16 k, C[4 (lane / 16) + r][lane % 16] in register r. K index k = 2 dp + pin: sample
 // p = 2 t + dp, input part pin; column x = 2 (bin offset) + output part.
+// The K loop of one dft16 / dft8_split tile, sum over t < count of a(t) b(t) on
+// v_mfma_f32_16x16x4f32 with a(t) = xrow[4 t] and b(t) from E[m + t step]: the LDS reads of 8
+// steps are issued before their products and two accumulators alternate, so a step does not
+// wait for its own reads and the previous product.
+__device__ __forceinline__ f32x4 dft_tile(const float* xrow, const float2* E, int m, int step, int N, float scale,
+                                          float sign, int pin, int pout, int count) {
+    auto bval = [&](float2 e) {
+        const float fr = e.x * scale, fi = sign * e.y * scale;
+        return pin ? (pout ? fr : -fi) : (pout ? fi : fr);
+    };
+    f32x4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
+    int t = 0;
+    for (; t + 8 <= count; t += 8) {
+        float a[8];
+        float2 e[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a[u] = xrow[4 * (t + u)];
+            e[u] = E[m];
+            m = (m + step) & (N - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], bval(e[u]), d0, 0, 0, 0);
+            d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u + 1], bval(e[u + 1]), d1, 0, 0, 0);
+        }
+    }
+    for (; t < count; ++t) {
+        d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(xrow[4 * t], bval(E[m]), d0, 0, 0, 0);
+        m = (m + step) & (N - 1);
+    }
+    return d0 + d1;
+}
+
 __device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const float2* E, int N, float sign,
                                       float scale, int Lout, float2* out, int ldo, int rows, int wave,
                                       int lane) {
@@ -50,18 +84,8 @@ __device__ __forceinline__ void dft16(const float2* X, int ldx, int Lin, const f
     const float* Xf = reinterpret_cast<const float*>(X);
     for (int ct = wave; ct * 8 < Lout; ct += 4) {
         const int q = ct * 8 + (x >> 1);
-        f32x4 d = {0.f, 0.f, 0.f, 0.f};
-        int m = (dp * q) & (N - 1);
-        const int step = (2 * q) & (N - 1);
         const float* xrow = Xf + 2 * (x * ldx + dp) + pin;
-        for (int t = 0; t < Lin / 2; ++t) {
-            const float a = xrow[4 * t];
-            const float2 e = E[m];
-            const float fr = e.x * scale, fi = sign * e.y * scale;
-            const float b = pin ? (pout ? fr : -fi) : (pout ? fi : fr);
-            d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
-            m = (m + step) & (N - 1);
-        }
+        const f32x4 d = dft_tile(xrow, E, (dp * q) & (N - 1), (2 * q) & (N - 1), N, scale, sign, pin, pout, Lin / 2);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int row = 4 * kk + r;

@@ -54,17 +54,7 @@ __device__ __forceinline__ void dft8_split(const float2* X, int ldx, int Lin, co
     const float* xrow = Xf + 2 * ((x & 7) * ldx + dp + (x >> 3) * half) + pin;
     for (int ct = wave; ct * 8 < Lout; ct += 4) {
         const int q = ct * 8 + (x >> 1);
-        f32x4 d = {0.f, 0.f, 0.f, 0.f};
-        int m = (dp * q) & (N - 1);
-        const int step = (2 * q) & (N - 1);
-        for (int t = 0; t < half / 2; ++t) {
-            const float a = xrow[4 * t];
-            const float2 e = E[m];
-            const float fr = e.x * scale, fi = sign * e.y * scale;
-            const float b = pin ? (pout ? fr : -fi) : (pout ? fi : fr);
-            d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
-            m = (m + step) & (N - 1);
-        }
+        const f32x4 d = dft_tile(xrow, E, (dp * q) & (N - 1), (2 * q) & (N - 1), N, scale, sign, pin, pout, half / 2);
         const float2 rho = E[(half * q) & (N - 1)];
         const float rr = rho.x, ri = sign * rho.y;
         float res[4];

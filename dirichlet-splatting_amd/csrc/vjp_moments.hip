@@ -82,9 +82,8 @@ static MomLayout mom_layout(int A, int S) {
 size_t vjp_moments_workspace(int A, int S) { return mom_layout(A, S).total; }
 
 // 16 apertures per workgroup: the cotangent rows go to the sample domain in LDS (dft16, sign
-// +1), each aperture's band comes from the box of the centres, and each of the 16 x 16
-// threads forms the moments of rows j = rj, rj + 16 < span of its aperture (or, for a wide
-// aperture, copies U to the slot).
+// +1), each aperture's band comes from the box of the centres, and the moments of its band
+// rows come from a small product on the matrix cores (below); a wide aperture's slot gets U.
 __global__ __launch_bounds__(256) void vjp_moments_prep(const Chirp c, const float* __restrict__ pos, int A,
                                                         const float* __restrict__ ctr, int S,
                                                         const float2* __restrict__ up, float2* __restrict__ slots,
@@ -147,45 +146,76 @@ __global__ __launch_bounds__(256) void vjp_moments_prep(const Chirp c, const flo
         bd[tid] = make_int2((int)m0 & (c.N - 1), span <= (double)kMomBand ? (int)span : 0);  // 0: wide
     }
     __syncthreads();
-    const int q = tid >> 4, rj = tid & 15, a = r0 + q;
-    if (a >= A) return;
-    const int2 b = bd[q];
-    if (rj == 0) bands[a] = b;
-    const float2* Ua = U + q * (c.S + 1);
-    float2* slot = slots + (size_t)a * kMomSlot;
-    if (b.y == 0) {
-        for (int n = rj; n < c.S; n += 16) slot[n] = Ua[n];
-        return;
-    }
-    const float n0 = 0.5f * (float)(c.S - 1), inv_h = 2.0f / (float)c.S;
-    for (int j = rj; j < b.y; j += 16) {
-        const int m = (b.x + j) & (c.N - 1);
-        float2 acc[kMomQ];
+    // Xin is free after the DFT: the powers X[n][q] = x_n^q / q! (q < 10; columns 10..15 zero)
+    float* X = reinterpret_cast<float*>(Xin);
+    for (int n = tid; n < c.S; n += 256) {
+        const float x = ((float)n - 0.5f * (float)(c.S - 1)) * (2.0f / (float)c.S);
+        float pw = 1.0f;
 #pragma unroll
-        for (int k = 0; k < kMomQ; ++k) acc[k] = make_float2(0.f, 0.f);
-        int idx = 0;  // m n mod N
-        for (int n = 0; n < c.S; ++n) {
-            const float2 e = cmulc(E[idx], Ua[n]);  // e^{-2 pi i m n / N} U[n]
-            idx = (idx + m) & (c.N - 1);
-            const float x = ((float)n - n0) * inv_h;
-            float p = 1.0f;
-#pragma unroll
-            for (int k = 0; k < kMomQ; ++k) {
-                acc[k].x = fmaf(p, e.x, acc[k].x);
-                acc[k].y = fmaf(p, e.y, acc[k].y);
-                p *= x;
-            }
+        for (int q = 0; q < 16; ++q) {
+            X[n * 16 + q] = q < kMomQ ? pw : 0.0f;
+            pw = pw * x / (float)(q + 1);
         }
-        // times (-i)^k / k!
-        float2* dst = slot + j * kMomQ;
-        float f = 1.0f;
+    }
+    if (tid < 16 && r0 + tid < A) bands[r0 + tid] = bd[tid];
+    __syncthreads();
+    // Per aperture the moments are a real product on v_mfma_f32_16x16x4f32 (A[i][k] -> lane
+    // i + 16 k, B[k][q] -> lane q + 16 k, C[4 (lane / 16) + r][lane % 16] in register r):
+    // rows i = (j, part) of 8 band rows j per tile, K = samples n = 4 t + k,
+    //   A[(j, part)][n] = Re / Im of e^{-2 pi i m_j n / N} U[n],  B[n][q] = x_n^q / q!,
+    // and C[(j, part)][q] = the real or imaginary part of M_q[m_j] before (-i)^q. Each wave takes
+    // every fourth aperture of the group.
+    const int kk = L >> 4, col = L & 15;
+    for (int qa = wave; qa < 16; qa += 4) {
+        const int a = r0 + qa;
+        if (a >= A) break;  // uniform over the wave
+        const int2 b = bd[qa];
+        const float2* Ua = U + qa * (c.S + 1);
+        float2* slot = slots + (size_t)a * kMomSlot;
+        if (b.y == 0) {
+            for (int n = L; n < c.S; n += 64) slot[n] = Ua[n];
+            continue;
+        }
+        for (int T = 0; 8 * T < b.y; ++T) {
+            const int j = 8 * T + (col >> 1), part = col & 1;  // this lane's A row
+            const int m = (b.x + j) & (c.N - 1);
+            const int step = (4 * m) & (c.N - 1);
+            int idx = (m * kk) & (c.N - 1);  // m n mod N at n = 4 t + kk
+            // the reads of 8 steps go ahead of their products; two accumulators alternate
+            f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < c.S / 4; t += 8) {  // S / 4 is a multiple of 8
+                float2 ev[8], uv[8];
+                float xv[8];
 #pragma unroll
-        for (int k = 0; k < kMomQ; ++k) {
-            const float2 v = f * acc[k];
-            const int r = k & 3;
-            dst[k] = r == 0 ? v : (r == 1 ? make_float2(v.y, -v.x) : (r == 2 ? make_float2(-v.x, -v.y)
-                                                                            : make_float2(-v.y, v.x)));
-            f /= (float)(k + 1);
+                for (int u = 0; u < 8; ++u) {
+                    const int n = 4 * (t + u) + kk;
+                    ev[u] = E[idx];
+                    uv[u] = Ua[n];
+                    xv[u] = X[n * 16 + col];
+                    idx = (idx + step) & (c.N - 1);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u += 2) {
+                    const float2 e0 = cmulc(ev[u], uv[u]), e1 = cmulc(ev[u + 1], uv[u + 1]);  // e^{-2 pi i m n / N} U[n]
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(part ? e0.y : e0.x, xv[u], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(part ? e1.y : e1.x, xv[u + 1], acc1, 0, 0, 0);
+                }
+            }
+            const f32x4 acc = acc0 + acc1;
+            // row 4 kk + r of C: band row j = 8 T + 2 kk + r / 2, part r % 2; column col = moment q
+            if (col < kMomQ) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int jj = 8 * T + 2 * kk + h;
+                    if (jj < b.y) {
+                        const float2 v = make_float2(acc[2 * h], acc[2 * h + 1]);
+                        const int r = col & 3;  // times (-i)^q
+                        slot[jj * kMomQ + col] = r == 0 ? v : (r == 1 ? make_float2(v.y, -v.x)
+                                                                    : (r == 2 ? make_float2(-v.x, -v.y)
+                                                                              : make_float2(-v.y, v.x)));
+                    }
+                }
+            }
         }
     }
 }
