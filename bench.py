@@ -41,7 +41,9 @@ sys.path.insert(0, str(ROOT / "dirichlet-splatting_amd"))
 # these sums, FP32-accurate (FP16 hi + lo operands, three products, FP32 accumulation) and so
 # priced against the FP32 peak.
 #   forward  4.5 FLOP: per (a, s, row of 16 samples) 9 W columns x 4 real products
-#   VJP      8 FLOP: per (a, s, row) 8 W_i x (Re, Im) x 2 output parts x 2 sums (w^n U, n w^n U)
+#   VJP      8 FLOP: per (a, s, row) 8 W_i x (Re, Im) x 2 output parts x 2 sums (w^n U, n w^n U),
+#            the symmetric-split kernel of pad 1; at pad >= 2 the VJP runs by bin moments, with no
+#            per-(a, n, s) sums, and is measured by its VALU issue
 #   scan     8 FLOP: as the VJP, the residual's sum and the norm polynomial's
 FWD_FLOP, VJP_FLOP, SCAN_FLOP = 13, 19, 11
 TD_FWD_FLOP, TD_VJP_FLOP = 4.5, 8
@@ -53,9 +55,11 @@ PEAK_FP16_TFLOPS = 2516.6  # MI355X published dense FP16 MFMA peak
 PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 peak (vector and dense MFMA)
 PEAK_HBM_GBS = 8000.0
 TRAFFIC = ROOT / "profiles" / "splat_traffic_128x128x2048.json"
-# VALU instructions per 64 (aperture, surfel) pairs (rocprofv3 SQ_INSTS_VALU per dispatch / pairs * 64)
-VALU_PROFILE = ROOT / "profiles" / "pmc_valu_td_render_h16_128x128x2048.md"
-VALU_PER_64_PAIRS = {"td_render_h16_kernel<false, false>": 352, "td_vjp_kernel<8, false>": 415}
+# VALU instructions per launch of the headline scene (rocprofv3 SQ_INSTS_VALU per dispatch)
+VALU_PROFILE = ROOT / "profiles" / "pmc_valu_splat_128x128x2048.md"
+VALU_PER_LAUNCH = {"td_render_h16_kernel<false, false>": 184_378_736, "vjp_moments_kernel<false>": 111_960_064}
+# the published FP32 peak is one wave64 FP32 VALU instruction per 2 cycles on each SIMD
+VALU_PEAK_PER_S = PEAK_FP32_TFLOPS * 1e12 / (64 * 2)
 SIMDS, CLOCK_GHZ = 1024, 2.4  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
 
 
@@ -85,13 +89,14 @@ def parse():
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     parser.add_argument("--cpu-seconds", type=float, default=8.0, help="target float64-oracle sample time")
     parser.add_argument(
-        "--workload", choices=("splat", "gram", "scan", "dsfw", "synth", "syrk", "copy"), default="splat",
+        "--workload", choices=("splat", "gram", "scan", "dsfw", "synth", "syrk", "copy", "valu"), default="splat",
         help="splat: the fwd+VJP metric (default); gram: the split-precision Gram of the same scene; "
         "scan: the batched certificate scan of 64^3 candidates against 64^2, 96^2 and 128^2 scans; "
         "dsfw: DSFW steps, plain FP32 form against the mixed-precision batched form; "
         "synth: the float64 truth scan of the DSFW fit by the ADC signal chain on the FP64 matrix cores; "
         "syrk: the split-operand SYRK of the mixed DSFW normal equations at two row counts; "
-        "copy: a plain 1 GiB device copy, the reference size for the HBM byte counters",
+        "copy: a plain 1 GiB device copy, the reference size for the HBM byte counters; "
+        "valu: the FP32 and FP64 VALU FMA issue rate that bounds the splat kernels",
     )  # fmt: skip
     parser.add_argument("--adc-samples", type=int, default=256, help="dsfw: ADC samples (bins = samples)")
     parser.add_argument("--truth-draws", type=int, default=4000,
@@ -749,6 +754,45 @@ def syrk_workload(args):
     }
 
 
+def valu_workload(args):
+    """The VALU issue rate: eight independent FMA chains per lane on every SIMD, FP32 and FP64
+    (dsplat_calibrate_valu), as wave64 instructions per second and FLOP/s (2 per lane FMA)."""
+    import torch
+
+    from dsplat import _lib
+
+    dev = torch.device("cuda", 0)
+    blocks, iters = 256 * 16, 4096  # 16 workgroups of 4 waves per CU
+    stream = torch.cuda.current_stream(dev)
+    rates = {}
+    for name, fp64 in (("fp32", 0), ("fp64", 1)):
+        out = torch.empty(blocks * 256, dtype=torch.float64 if fp64 else torch.float32, device=dev)
+
+        def launch():
+            _lib.call("dsplat_calibrate_valu", fp64, blocks, iters, _lib.ptr(out), _lib.stream(dev))
+
+        for _ in range(max(1, args.warmup)):
+            launch()
+        ms, _ = timed(launch, max(3, args.steps), stream)
+        instructions = blocks * 4 * 8 * iters  # wave64 FMA instructions
+        rates[name] = {"ms": ms, "wave_instructions_per_s": instructions / (ms * 1e-3),
+                       "tflops": instructions * 64 * 2 / (ms * 1e-3) / 1e12,
+                       "cycles_per_instruction_per_simd_at_2_4_ghz": SIMDS * CLOCK_GHZ * 1e9 / (instructions / (ms * 1e-3))}
+    return {
+        "metric": "VALU FMA issue rate (wave64 instructions/s), FP32",
+        "value": rates["fp32"]["wave_instructions_per_s"],
+        "unit": "wave64 instructions/s",
+        "n_gpus": 1,
+        "steps": max(3, args.steps),
+        "higher_is_better": True,
+        "dtype": "f32 and f64",
+        "data": "synthetic",
+        "config": {"workload": f"{blocks} workgroups x 256 lanes x 8 independent FMA chains x {iters} steps"},
+        "rates": rates,
+        "published_fp32_peak_tflops": PEAK_FP32_TFLOPS,
+    }
+
+
 def copy_workload(args):
     """A plain device-to-device copy of 2^28 float32 (1 GiB read, 1 GiB written) per step:
     the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -804,6 +848,9 @@ def main():
     if args.workload == "copy":
         print(json.dumps(copy_workload(args)), flush=True)
         return
+    if args.workload == "valu":
+        print(json.dumps(valu_workload(args)), flush=True)
+        return
     import torch
 
     from dsplat import workloads
@@ -857,54 +904,70 @@ def main():
     jvp_ms, _ = timed(lambda: surfel.jvp(cfg, positions, centers, normals, weights, dc, dn, dw, 0.0), 5, stream)
 
     sample_domain = cfg.splat_form == "samples" and cfg.adc_samples in (128, 256)
-    fwd_flop, vjp_flop = (TD_FWD_FLOP, TD_VJP_FLOP) if sample_domain else (FWD_FLOP, VJP_FLOP)
+    moments = sample_domain and cfg.pad_factor >= 2
     groups = {  # the launches each timed call makes
         "render": ["td_render_h16_kernel<false, false>"] if sample_domain else ["render_kernel<2, 8>"],
-        "vjp": ["vjp_prep_kernel", "td_vjp_kernel<8, false>", "vjp_reduce"] if sample_domain
-        else ["vjp_kernel<2>", "vjp_reduce"],
+        "vjp": ["vjp_moments_prep", "vjp_moments_kernel<false>", "vjp_reduce"] if moments
+        else (["vjp_prep_kernel", "td_vjp_kernel<8, false>", "vjp_reduce"] if sample_domain
+              else ["vjp_kernel<2>", "vjp_reduce"]),
     }  # fmt: skip
-    kernels = {
-        "render": (fwd_ms, fwd_flop * evals, FWD_FLOP * evals),
-        "vjp": (vjp_ms, vjp_flop * evals, VJP_FLOP * evals),
-    }
-    name = max(kernels, key=lambda k: kernels[k][0])
-    ms, flop, cot_flop = kernels[name]
-    achieved = flop / (ms * 1e-3) / 1e12
     pairs = A * S
-    roofline = {
-        "kernel": name,
-        "launches": groups[name],
-        "bound": "valu-issue" if sample_domain else "valu",
-        "achieved": achieved,
-        "peak": PEAK_FP32_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP32_TFLOPS,
-        "traffic": measured_traffic(groups[name]),
-        "traffic_source": f"{TRAFFIC.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch, "
-        "scaled by the same counters on a 1 GiB copy)",
-        "flop_per_eval": fwd_flop if name == "render" else vjp_flop,
-        "peak_basis": "FP32 peak: results are FP32-accurate, executed as FP16 hi+lo x3 MFMA products"
-        if sample_domain else "FP32 vector",
-        # the cotangent-form count per (aperture, bin, surfel): 13 FLOP forward, 19 VJP
-        "cotangent_form_count": {"flop_per_eval": FWD_FLOP if name == "render" else VJP_FLOP,
-                                 "tflops": cot_flop / (ms * 1e-3) / 1e12,
-                                 "frac": cot_flop / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS},
-    }  # fmt: skip
-    if sample_domain and cfg.adc_samples == 128:
-        kernel = groups[name][0] if name == "render" else "td_vjp_kernel<8, false>"
-        issue_ms = VALU_PER_64_PAIRS[kernel] * pairs / 64 * 4 / SIMDS / CLOCK_GHZ / 1e6
-        roofline["valu_issue"] = {
-            "kernel": kernel, "instructions_per_64_pairs": VALU_PER_64_PAIRS[kernel],
-            "source": f"{VALU_PROFILE.relative_to(ROOT)} (rocprofv3 --pmc SQ_INSTS_VALU)",
-            "issue_ms_at_peak": issue_ms, "frac": issue_ms / ms,
-            "basis": f"one wave64 VALU instruction per 4 cycles on each of {SIMDS} SIMDs at {CLOCK_GHZ} GHz",
+    valu_rate = valu_workload(argparse.Namespace(warmup=1, steps=3))["rates"] if sample_domain else None
+
+    def valu_issue(kernel, ms):
+        if kernel not in VALU_PER_LAUNCH:
+            return None
+        count = VALU_PER_LAUNCH[kernel]
+        out = {"kernel": kernel, "instructions_per_launch": count, "instructions_per_64_pairs": count * 64 / pairs,
+               "source": f"{VALU_PROFILE.relative_to(ROOT)} (rocprofv3 --pmc SQ_INSTS_VALU)",
+               "issue_ms_at_published_peak": count / VALU_PEAK_PER_S * 1e3,
+               "frac_of_published_peak": count / VALU_PEAK_PER_S * 1e3 / ms,
+               "basis": "published FP32 peak: one wave64 FP32 instruction per 2 cycles on each of 1024 SIMDs "
+                        "at 2.4 GHz"}  # fmt: skip
+        if valu_rate is not None:
+            measured = valu_rate["fp32"]["wave_instructions_per_s"]
+            out["measured_fma_rate"] = measured
+            out["frac_of_measured_rate"] = count / measured * 1e3 / ms
+        return out
+
+    if sample_domain:
+        # forward: the real multiply-adds of the symmetric sample sums per (a, n, s)
+        name, ms = "render", fwd_ms
+        achieved = TD_FWD_FLOP * evals / (ms * 1e-3) / 1e12
+        roofline = {
+            "kernel": name,
+            "dominant": fwd_ms >= vjp_ms,  # the render is the longer launch group of the step
+            "launches": groups[name],
+            "bound": "valu-issue",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": measured_traffic(groups[name]),
+            "traffic_source": f"{TRAFFIC.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch, "
+            "scaled by the same counters on a 1 GiB copy)",
+            "flop_per_eval": TD_FWD_FLOP,
+            "peak_basis": "FP32 peak: results are FP32-accurate, executed as FP16 hi+lo x3 MFMA products",
+            "cotangent_form_count": {"flop_per_eval": FWD_FLOP, "tflops": FWD_FLOP * evals / (ms * 1e-3) / 1e12,
+                                     "frac": FWD_FLOP * evals / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS},
+            "valu_issue": valu_issue(groups[name][0], ms),
+            "executed_fp16_mfma": {
+                "tflops": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12,
+                "frac_of_fp16_peak": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12 / PEAK_FP16_TFLOPS,
+            },
+            # the VJP by bin moments: no per-(a, n, s) sums; its bound is the per-pair VALU work
+            "vjp": {"launches": groups["vjp"], "ms": vjp_ms, "traffic": measured_traffic(groups["vjp"]),
+                    "valu_issue": valu_issue(groups["vjp"][1], vjp_ms) if moments else None},
         }  # fmt: skip
-        roofline["executed_fp16_mfma"] = {
-            "tflops": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12,
-            "frac_of_fp16_peak": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12 / PEAK_FP16_TFLOPS,
-            "note": "the matrix products co-issue with the per-pair VALU work (geometry, powers, FP16 splits), "
-            "which bounds these kernels",
-        }
+    else:
+        kernels = {"render": (fwd_ms, FWD_FLOP * evals), "vjp": (vjp_ms, VJP_FLOP * evals)}
+        name = max(kernels, key=lambda k: kernels[k][0])
+        ms, flop = kernels[name]
+        achieved = flop / (ms * 1e-3) / 1e12
+        roofline = {"kernel": name, "launches": groups[name], "bound": "valu", "achieved": achieved,
+                    "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
+                    "traffic": measured_traffic(groups[name]), "flop_per_eval": FWD_FLOP if name == "render" else VJP_FLOP,
+                    "peak_basis": "FP32 vector"}  # fmt: skip
     result = {
         "metric": "fwd+VJP splat evals/s (aperture·bin·surfel)",
         "value": evals / (ms_per_step * 1e-3),

@@ -148,6 +148,7 @@ SIGNATURES = {
     "dsplat_qn_search": (
         c_int, [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr]
     ),
+    "dsplat_calibrate_valu": (c_int, [c_int, c_int, c_int, c_ptr, c_ptr]),
     "dsplat_qn_lockstep_workspace": (c_size, [c_ptr, c_int, c_int]),
     "dsplat_qn_lockstep": (
         c_int,

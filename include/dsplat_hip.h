@@ -513,6 +513,13 @@ int dsplat_gram(const dsplat_fmcw* cfg, const float* positions, int32_t num_aper
                 int32_t mode, int32_t real_only, float inv_rows, const float* column_exponents,
                 float* out, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- Measurement aid (no counterpart in the reference). */
+
+/* VALU issue-rate calibration: blocks x 256 lanes each run iters steps of eight independent
+ * FMA chains in FP32 (fp64 = 0) or FP64 (fp64 = 1), 8 x iters wave64 FMA instructions per
+ * wave; out receives one float (or double) per lane, blocks x 256. For bench.py's valu workload. */
+int dsplat_calibrate_valu(int32_t fp64, int32_t blocks, int32_t iters, void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
