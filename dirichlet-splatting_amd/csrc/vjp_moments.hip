@@ -81,26 +81,29 @@ static MomLayout mom_layout(int A, int S) {
 
 size_t vjp_moments_workspace(int A, int S) { return mom_layout(A, S).total; }
 
-// 16 apertures per workgroup: the cotangent rows go to the sample domain in LDS (dft16, sign
-// +1), each aperture's band comes from the box of the centres, and the moments of its band
-// rows come from a small product on the matrix cores (below); a wide aperture's slot gets U.
+// Apertures per pre-pass workgroup: the FFT buffer holds G rows of N points (32 KB).
+__host__ __device__ constexpr int mom_prep_rows(int N) { return N <= 256 ? 16 : 4096 / N; }
+
+// G apertures per workgroup (mom_prep_rows). The cotangent rows go to the sample domain by a
+// radix-2 FFT in LDS: U[n] = sum_{k<K} e^{2 pi i n k / N} u[k] for n < N, in bit-reversed order
+// (decimation in frequency; u is zero past K = N / 2, which the first stage folds in). Each
+// aperture's band comes from the box of the centres, and the moments of its band rows from a
+// small product on the matrix cores (below); a wide aperture's slot gets U.
 __global__ __launch_bounds__(256) void vjp_moments_prep(const Chirp c, const float* __restrict__ pos, int A,
                                                         const float* __restrict__ ctr, int S,
                                                         const float2* __restrict__ up, float2* __restrict__ slots,
                                                         int2* __restrict__ bands) {
     extern __shared__ float smem[];
+    const int G = mom_prep_rows(c.N), NH = c.N >> 1;
     float2* E = reinterpret_cast<float2*>(smem);  // [N]
-    float2* Xin = E + c.N;                        // [16][K + 1]
-    float2* U = Xin + 16 * (c.K + 1);             // [16][S + 1]
+    float2* F = E + c.N;                          // [G][N], U in bit-reversed order
+    float* X = reinterpret_cast<float*>(F + G * c.N);  // [S][10]: x_n^q / q!
     __shared__ float box[4][6];
     __shared__ int2 bd[16];
     const int tid = threadIdx.x, wave = tid >> 6, L = tid & 63;
-    const int r0 = blockIdx.x * 16;
+    const int r0 = blockIdx.x * G;
+    const int bits = 31 - __builtin_clz(c.N);  // log2 N
     fill_twiddles(E, c.N, tid, 256);
-    for (int i = tid; i < 16 * c.K; i += 256) {
-        const int r = i / c.K, p = i - r * c.K;
-        Xin[r * (c.K + 1) + p] = r0 + r < A ? up[(size_t)(r0 + r) * c.K + p] : make_float2(0.f, 0.f);
-    }
     // box of the centres: lower (0..2) and upper (3..5) corner
     float bx[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
     for (int s = tid; s < S; s += 256) {
@@ -123,9 +126,39 @@ __global__ __launch_bounds__(256) void vjp_moments_prep(const Chirp c, const flo
 #pragma unroll
         for (int d = 0; d < 6; ++d) box[wave][d] = bx[d];
     }
+    // powers X[n][q] = x_n^q / q! (q < 10; the product's columns 10..15 are zero)
+    for (int n = tid; n < c.S; n += 256) {
+        const float x = ((float)n - 0.5f * (float)(c.S - 1)) * (2.0f / (float)c.S);
+        float pw = 1.0f;
+#pragma unroll
+        for (int q = 0; q < kMomQ; ++q) {
+            X[n * kMomQ + q] = pw;
+            pw = pw * x / (float)(q + 1);
+        }
+    }
+    __syncthreads();  // twiddles written
+    // first FFT stage with the zero upper half: F[k] = u[k], F[k + N / 2] = u[k] e^{2 pi i k / N}
+    for (int i = tid; i < G * NH; i += 256) {
+        const int r = i / NH, k = i - r * NH;
+        const float2 u = r0 + r < A ? up[(size_t)(r0 + r) * c.K + k] : make_float2(0.f, 0.f);
+        float2* row = F + r * c.N;
+        row[k] = u;
+        row[k + NH] = cmul(u, E[k]);
+    }
     __syncthreads();
-    dft16(Xin, c.K + 1, c.K, E, c.N, 1.0f, 1.0f, c.S, U, c.S + 1, 16, wave, L);
-    if (tid < 16) {
+    for (int half = c.N >> 2; half >= 1; half >>= 1) {
+        const int tw = NH / half;  // twiddle index step: e^{2 pi i j / (2 half)} = E[j N / (2 half)]
+        for (int i = tid; i < G * NH; i += 256) {
+            const int r = i / NH, t = i - r * NH;
+            const int j = t & (half - 1), at = 2 * (t - j) + j;
+            float2* row = F + r * c.N;
+            const float2 a = row[at], b = row[at + half];
+            row[at] = a + b;
+            row[at + half] = cmul(a - b, E[j * tw]);
+        }
+        __syncthreads();
+    }
+    if (tid < G) {
         // distance interval from the aperture to the box, in float64; lobes lie within one
         // bin of [rint(k_lo), rint(k_hi)], so the band starts at rint(k_lo) - 1
         const int a = min(r0 + tid, A - 1);
@@ -144,20 +177,8 @@ __global__ __launch_bounds__(256) void vjp_moments_prep(const Chirp c, const flo
         const double span = m_hi - m_lo + 1.0;
         const double m0 = m_lo - (double)c.N * floor(m_lo / (double)c.N);
         bd[tid] = make_int2((int)m0 & (c.N - 1), span <= (double)kMomBand ? (int)span : 0);  // 0: wide
+        if (r0 + tid < A) bands[r0 + tid] = bd[tid];
     }
-    __syncthreads();
-    // Xin is free after the DFT: the powers X[n][q] = x_n^q / q! (q < 10; columns 10..15 zero)
-    float* X = reinterpret_cast<float*>(Xin);
-    for (int n = tid; n < c.S; n += 256) {
-        const float x = ((float)n - 0.5f * (float)(c.S - 1)) * (2.0f / (float)c.S);
-        float pw = 1.0f;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            X[n * 16 + q] = q < kMomQ ? pw : 0.0f;
-            pw = pw * x / (float)(q + 1);
-        }
-    }
-    if (tid < 16 && r0 + tid < A) bands[r0 + tid] = bd[tid];
     __syncthreads();
     // Per aperture the moments are a real product on v_mfma_f32_16x16x4f32 (A[i][k] -> lane
     // i + 16 k, B[k][q] -> lane q + 16 k, C[4 (lane / 16) + r][lane % 16] in register r):
@@ -166,14 +187,14 @@ __global__ __launch_bounds__(256) void vjp_moments_prep(const Chirp c, const flo
     // and C[(j, part)][q] = the real or imaginary part of M_q[m_j] before (-i)^q. Each wave takes
     // every fourth aperture of the group.
     const int kk = L >> 4, col = L & 15;
-    for (int qa = wave; qa < 16; qa += 4) {
+    for (int qa = wave; qa < G; qa += 4) {
         const int a = r0 + qa;
         if (a >= A) break;  // uniform over the wave
         const int2 b = bd[qa];
-        const float2* Ua = U + qa * (c.S + 1);
+        const float2* Fa = F + qa * c.N;  // U[n] = Fa[rev(n)]
         float2* slot = slots + (size_t)a * kMomSlot;
         if (b.y == 0) {
-            for (int n = L; n < c.S; n += 64) slot[n] = Ua[n];
+            for (int n = L; n < c.S; n += 64) slot[n] = Fa[__builtin_bitreverse32(n) >> (32 - bits)];
             continue;
         }
         for (int T = 0; 8 * T < b.y; ++T) {
@@ -190,8 +211,8 @@ __global__ __launch_bounds__(256) void vjp_moments_prep(const Chirp c, const flo
                 for (int u = 0; u < 8; ++u) {
                     const int n = 4 * (t + u) + kk;
                     ev[u] = E[idx];
-                    uv[u] = Ua[n];
-                    xv[u] = X[n * 16 + col];
+                    uv[u] = Fa[__builtin_bitreverse32(n) >> (32 - bits)];
+                    xv[u] = col < kMomQ ? X[n * kMomQ + col] : 0.0f;
                     idx = (idx + step) & (c.N - 1);
                 }
 #pragma unroll
@@ -258,10 +279,12 @@ __global__ __launch_bounds__(256) void vjp_moments_kernel(const Chirp c, const f
         if (tid < kMomGroup) bd[tid] = bands[min(g0 + tid, A - 1)];
         __syncthreads();
         const int na = min(kMomGroup, a_end - g0);
+        // each aperture's position is loaded one aperture ahead, so no step waits for its load
+        float px = pos[3 * g0], py = pos[3 * g0 + 1], pz = pos[3 * g0 + 2];
         for (int g = 0; g < na; ++g) {
-            const int a = g0 + g;
-            const Pair q = pair_geometry_t<FRESNEL>(c, pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], cx, cy, cz, nx,
-                                                    ny, nz);
+            const int an = min(g0 + g + 1, A - 1);
+            const float qx = pos[3 * an], qy = pos[3 * an + 1], qz = pos[3 * an + 2];
+            const Pair q = pair_geometry_t<FRESNEL>(c, px, py, pz, cx, cy, cz, nx, ny, nz);
             const float2* Ta = T + g * kMomSlot;
             const int2 b = bd[g];
             float2 G0, G1;
@@ -322,6 +345,9 @@ __global__ __launch_bounds__(256) void vjp_moments_kernel(const Chirp c, const f
             gacc[5] += gcos * q.uz;
             gacc[6] += amp * z0.x;
             gacc[7] += amp * z0.y;
+            px = qx;
+            py = qy;
+            pz = qz;
         }
     }
     if (s_ok) {
@@ -340,10 +366,11 @@ hipError_t launch_vjp_moments(const Chirp& c, const float* pos, int A, const flo
     float* partial = reinterpret_cast<float*>(base + lay.partial);
     float2* slots = reinterpret_cast<float2*>(base + lay.slots);
     int2* bands = reinterpret_cast<int2*>(base + lay.bands);
-    const size_t smem = sizeof(float2) * (c.N + 16 * (size_t)(c.K + 1) + 16 * (size_t)(c.S + 1));
+    const int G = mom_prep_rows(c.N);
+    const size_t smem = sizeof(float2) * (c.N + (size_t)G * c.N) + sizeof(float) * kMomQ * (size_t)c.S;
     hipError_t e = allow_lds((const void*)vjp_moments_prep, smem);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(vjp_moments_prep, dim3((A + 15) / 16), dim3(256), smem, st, c, pos, A, ctr, S, up, slots,
+    hipLaunchKernelGGL(vjp_moments_prep, dim3((A + G - 1) / G), dim3(256), smem, st, c, pos, A, ctr, S, up, slots,
                        bands);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
