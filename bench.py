@@ -914,11 +914,14 @@ def main():
     pairs = A * S
     valu_rate = valu_workload(argparse.Namespace(warmup=1, steps=3))["rates"] if sample_domain else None
 
-    def valu_issue(kernel, ms):
+    def valu_issue(kernel, ms, over):
+        """The kernel's VALU instruction count against the issue rates, over `ms` of the
+        launch set named by `over` (the kernel alone, or its whole launch group)."""
         if kernel not in VALU_PER_LAUNCH:
             return None
         count = VALU_PER_LAUNCH[kernel]
         out = {"kernel": kernel, "instructions_per_launch": count, "instructions_per_64_pairs": count * 64 / pairs,
+               "time_basis": over, "ms": ms,
                "source": f"{VALU_PROFILE.relative_to(ROOT)} (rocprofv3 --pmc SQ_INSTS_VALU)",
                "issue_ms_at_published_peak": count / VALU_PEAK_PER_S * 1e3,
                "frac_of_published_peak": count / VALU_PEAK_PER_S * 1e3 / ms,
@@ -950,14 +953,15 @@ def main():
             "peak_basis": "FP32 peak: results are FP32-accurate, executed as FP16 hi+lo x3 MFMA products",
             "cotangent_form_count": {"flop_per_eval": FWD_FLOP, "tflops": FWD_FLOP * evals / (ms * 1e-3) / 1e12,
                                      "frac": FWD_FLOP * evals / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS},
-            "valu_issue": valu_issue(groups[name][0], ms),
+            "valu_issue": valu_issue(groups[name][0], ms, "the render launch"),
             "executed_fp16_mfma": {
                 "tflops": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12,
                 "frac_of_fp16_peak": TD_EXEC_FLOP_PER_PAIR * pairs / (ms * 1e-3) / 1e12 / PEAK_FP16_TFLOPS,
             },
             # the VJP by bin moments: no per-(a, n, s) sums; its bound is the per-pair VALU work
             "vjp": {"launches": groups["vjp"], "ms": vjp_ms, "traffic": measured_traffic(groups["vjp"]),
-                    "valu_issue": valu_issue(groups["vjp"][1], vjp_ms) if moments else None},
+                    "valu_issue": valu_issue(groups["vjp"][1], vjp_ms, "the VJP launch group: pre-pass, moments "
+                                             "kernel and reduction") if moments else None},
         }  # fmt: skip
     else:
         kernels = {"render": (fwd_ms, FWD_FLOP * evals), "vjp": (vjp_ms, VJP_FLOP * evals)}
