@@ -7,55 +7,157 @@
 //   panel:  every workgroup factors the diagonal block in LDS (the same few microseconds of
 //           work in each, which saves a launch), then solves its own block row of the panel,
 //           X L_kk^T = A_ik, and zeroes the mirrored block of the upper triangle;
-//   update: one workgroup per lower trailing block, A_ij -= L_ik L_jk^T, float64 FMA with
-//           a 4 x 4 register tile per thread.
+//   update: one workgroup per lower trailing block, A_ij -= L_ik L_jk^T on the matrix cores.
+// The panel is a chain of 64 dependent pivots, so its cost is latency: the diagonal block is
+// factored in four 16-column steps (LAPACK's blocked order at a 16-column inner block): the
+// 16 x 16 leaf by one wave with the pivot and column entries passed by lane reads, then the
+// rows below the leaf (of the diagonal block and of the workgroup's panel block together)
+// by forward substitution, one row per thread, then the trailing columns of both blocks by
+// the leaf's 16 columns. Pivot reciprocals scale as LAPACK's potf2 / trsm do (a product by
+// 1 / l_jj, here from the hardware reciprocal square root refined by Newton steps). Loops stay
+// rolled where an unrolled body would not fit the instruction cache.
 // Only the lower triangle of the input is read. info = 1-based index of the first pivot
 // that is not positive (0 if none), as LAPACK potrf reports it, in the factor's own precision
 // (the float32 form is the first attempt of varpro.py:62-79); the factor is then not
-// meaningful, and the launches of the later block columns return at once. The solve is one workgroup: blocked forward and backward substitution.
+// meaningful, and the launches of the later block columns return at once. The solve is one
+// workgroup: blocked forward and backward substitution.
 #include "dsplat_internal.h"
 
 namespace dsplat {
 
 constexpr int kCholBlock = 64;
-constexpr int kCholLd = kCholBlock + 1;  // LDS row stride (doubles): conflict-free columns
+constexpr int kCholLd = kCholBlock + 1;  // LDS row stride of row-major blocks: conflict-free columns
+constexpr int kCholLeaf = 16;            // inner block of the panel factorization
 
-// Unblocked Cholesky of the w x w lower block D (LDS, stride kCholLd) by wave 0 alone: lane i
-// keeps row i in registers; at step j every lane takes the pivot from LDS, lane i > j scales
-// its entry and publishes it in the LDS column buffer, and every lane subtracts
-// l_ij l_kj from its entries k > j. One wave's LDS accesses complete in order, so the steps
-// need no barrier; the other waves wait at one barrier at the end. Entries above the
-// diagonal of a row are updated too and never read. Returns the first failing pivot index
-// (or -1) on every thread.
+// 1 / sqrt(d): the hardware estimate, then Newton steps (two in float64, one in float32),
+// y <- y + y (1 - d y^2) / 2.
 template <typename T>
-__device__ int factor_block(T* D, int w) {
+__device__ __forceinline__ T rsqrt_newton(T d) {
+    if constexpr (sizeof(T) == 8) {
+        double y = __builtin_amdgcn_rsq(d);
+        y = fma(0.5 * y, fma(-d * y, y, 1.0), y);
+        return fma(0.5 * y, fma(-d * y, y, 1.0), y);
+    } else {
+        const float y = __builtin_amdgcn_rsqf(d);
+        return fmaf(0.5f * y, fmaf(-d * y, y, 1.0f), y);
+    }
+}
+
+// Cholesky of the 16 x 16 leaf at (c, c) of D (row-major, stride kCholLd) by wave 0: lane
+// i < 16 keeps row c + i in registers. At step j each lane publishes its entry of column j
+// in the wave's column buffer and reads the pivot and the column back (one wave's LDS
+// accesses complete in order), so the step is one LDS round trip and the lane's own FMAs.
+// Writes the leaf (its lower triangle is the factor) and the pivot reciprocals rd[c + j]; returns the first
+// failing step (or -1), uniform over the wave.
+template <typename T>
+__device__ __forceinline__ int factor_leaf(T* D, T* rd, T* col, int c, int lane) {
+    // entries above the diagonal are carried along unread: the leaf's upper triangle is never
+    // used, and the diagonal block's final store writes zeros there
+    const int i = lane & (kCholLeaf - 1);
+    T x[kCholLeaf];
+#pragma unroll
+    for (int m = 0; m < kCholLeaf; ++m) x[m] = D[(c + i) * kCholLd + c + m];
+    int first = -1;
+#pragma unroll
+    for (int j = 0; j < kCholLeaf; ++j) {
+        if (lane < kCholLeaf) col[lane] = x[j];  // a_kj, k = lane (entries k < j unused)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        T a[kCholLeaf];
+#pragma unroll
+        for (int k = j; k < kCholLeaf; ++k) a[k] = col[k];
+        const T d = a[j];
+        first = (first < 0 && !(d > T(0))) ? j : first;
+        const T r = rsqrt_newton(d);
+        const T l = x[j] * r;  // l_ij; on lane j x[j] = d, so l_jj = d / sqrt(d)
+        const T t = l * r;
+#pragma unroll
+        for (int k = j + 1; k < kCholLeaf; ++k) x[k] = fma(-t, a[k], x[k]);  // a_ik -= l_ij l_kj
+        x[j] = l;
+        if (lane == 0) rd[c + j] = r;
+        // the next step's write follows this step's reads in the wave's LDS order
+    }
+    if (lane < kCholLeaf) {
+#pragma unroll
+        for (int m = 0; m < kCholLeaf; ++m) D[(c + i) * kCholLd + c + m] = x[m];
+    }
+    return __builtin_amdgcn_readfirstlane(first);
+}
+
+// Row y (16 values at columns c..c+15 of a row-major LDS block) solved against the leaf:
+// y L_leaf^T = b by forward substitution.
+template <typename T>
+__device__ __forceinline__ void solve_leaf_row(T* y, const T* D, const T* rd, int c) {
+    T v[kCholLeaf];
+#pragma unroll
+    for (int m = 0; m < kCholLeaf; ++m) v[m] = y[c + m];
+#pragma unroll
+    for (int j = 0; j < kCholLeaf; ++j) {
+        T s = v[j];
+#pragma unroll
+        for (int m = 0; m < j; ++m) s = fma(-v[m], D[(c + j) * kCholLd + c + m], s);
+        v[j] = s * rd[c + j];
+    }
+#pragma unroll
+    for (int m = 0; m < kCholLeaf; ++m) y[c + m] = v[m];
+}
+
+// Factor the 64 x 64 diagonal block D (lower triangle; rows >= w padded with the identity)
+// and, when B is given, solve its h rows: B <- B L^-T. rd receives the pivot reciprocals.
+// Returns the first failing pivot index (or -1) on every thread.
+template <typename T>
+__device__ int factor_panel(T* D, T* B, T* rd, int w) {
     __shared__ int fail;
-    __shared__ T col[kCholBlock];
-    const int tid = threadIdx.x;
-    if (tid < 64) {
-        const int i = tid;
-        T x[kCholBlock];
+    __shared__ T col[kCholLeaf];
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid == 0) fail = -1;
+    for (int c = 0; c < kCholBlock; c += kCholLeaf) {
+        if (tid < 64) {
+            const int f = factor_leaf(D, rd, col, c, lane);
+            if (tid == 0 && f >= 0 && fail < 0 && c + f < w) fail = c + f;
+        }
+        __syncthreads();
+        // rows below the leaf: D rows c + 16 .. 63 (threads 64..), B rows 0..63 (threads 0..63)
+        const int below = kCholBlock - c - kCholLeaf;
+        if (tid < 64 && B) {
+            solve_leaf_row(B + tid * kCholLd, D, rd, c);
+        } else if (tid >= 64 && tid < 64 + below) {
+            solve_leaf_row(D + (c + kCholLeaf + tid - 64) * kCholLd, D, rd, c);
+        }
+        __syncthreads();
+        if (below > 0) {
+            // trailing columns s in [c + 16, 64) of D rows c + 16.. and of B rows:
+            // row index rho = tid % 128 (B rows, then D rows), columns of parity tid / 128
+            const int rho = tid & 127, par = tid >> 7;
+            T* row = nullptr;
+            if (rho < 64) {
+                if (B) row = B + rho * kCholLd;
+            } else if (rho - 64 < below) {
+                row = D + (c + kCholLeaf + rho - 64) * kCholLd;
+            }
+            if (row) {
+                T u[kCholLeaf];
 #pragma unroll
-        for (int k = 0; k < kCholBlock; ++k) x[k] = D[i * kCholLd + k];
-        int first = -1;
+                for (int m = 0; m < kCholLeaf; ++m) u[m] = row[c + m];
+                // two columns per pass: two independent FMA chains
+                for (int s = c + kCholLeaf + par; s < kCholBlock; s += 4) {
+                    const bool two = s + 2 < kCholBlock;
+                    const T* l0 = D + s * kCholLd + c;
+                    const T* l1 = D + (two ? s + 2 : s) * kCholLd + c;
+                    T a0 = row[s], a1 = two ? row[s + 2] : T(0);
 #pragma unroll
-        for (int j = 0; j < kCholBlock; ++j) {
-            if (j < w) {  // uniform
-                col[i] = x[j];  // column j after the updates of steps < j
-                const T d = col[j];
-                if (!(d > 0.0) && first < 0) first = j;
-                const T ljj = sqrt(d);
-                x[j] = i == j ? ljj : x[j] / ljj;
-                col[i] = x[j];
-#pragma unroll
-                for (int k = j + 1; k < kCholBlock; ++k) x[k] = fma(-x[j], col[k], x[k]);
+                    for (int m = 0; m < kCholLeaf; ++m) {
+                        a0 = fma(-u[m], l0[m], a0);
+                        a1 = fma(-u[m], l1[m], a1);
+                    }
+                    row[s] = a0;
+                    if (two) row[s + 2] = a1;
+                }
             }
         }
-#pragma unroll
-        for (int k = 0; k < kCholBlock; ++k) D[i * kCholLd + k] = x[k];
-        if (tid == 0) fail = first;
+        __syncthreads();
     }
-    __syncthreads();
     return fail;
 }
 
@@ -63,68 +165,67 @@ template <typename T>
 __global__ __launch_bounds__(256) void chol_panel_kernel(T* __restrict__ A, int n, int lda, int kb,
                                                         int* __restrict__ info) {
     __shared__ T D[kCholBlock * kCholLd];
+    __shared__ T Bs[kCholBlock * kCholLd];
+    __shared__ T rd[kCholBlock];
     // an earlier block column failed: the factor is not meaningful, info is final
     if (*info <= n) return;
     const int tid = threadIdx.x;
     const int c0 = kb * kCholBlock, w = min(kCholBlock, n - c0);
     const int ib = kb + blockIdx.x;
     const int r0 = ib * kCholBlock, h = min(kCholBlock, n - r0);
-    for (int e = tid; e < kCholBlock * kCholBlock; e += 256) {
-        const int i = e >> 6, k = e & 63;
-        D[i * kCholLd + k] = (i < w && k <= i) ? A[(size_t)(c0 + i) * lda + c0 + k] : 0.0;
+    const bool diag = blockIdx.x == 0;
+    // all loads in flight before the first LDS store
+    constexpr int kPer = kCholBlock * kCholBlock / 256;
+    T vd[kPer], vb[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + 256 * q, i = e >> 6, k = e & 63;
+        vd[q] = (i < w && k <= i) ? A[(size_t)(c0 + i) * lda + c0 + k] : T(i >= w && k == i ? 1 : 0);
+        vb[q] = (!diag && i < h && k < w) ? A[(size_t)(r0 + i) * lda + c0 + k] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + 256 * q, i = e >> 6, k = e & 63;
+        D[i * kCholLd + k] = vd[q];
+        Bs[i * kCholLd + k] = vb[q];
     }
     __syncthreads();
-    const int fail = factor_block(D, w);
-    if (blockIdx.x == 0) {
+    const int fail = factor_panel(D, diag ? nullptr : Bs, rd, w);
+    if (diag) {
         if (tid == 0 && fail >= 0) atomicMin(info, c0 + fail + 1);
         for (int e = tid; e < w * kCholBlock; e += 256) {
             const int i = e >> 6, k = e & 63;
-            if (k < w) A[(size_t)(c0 + i) * lda + c0 + k] = k <= i ? D[i * kCholLd + k] : 0.0;
+            if (k < w) A[(size_t)(c0 + i) * lda + c0 + k] = k <= i ? D[i * kCholLd + k] : T(0);
         }
         return;
     }
-    // X L^T = B for the h x w block below the diagonal. Rows are independent: the 4 lanes
-    // of row i = tid / 4 (one wave holds 16 rows) keep its 16 columns k = tid % 4 + 4 q in
-    // registers and pass X[i][j] by a lane shuffle, without barriers.
-    const int i = tid >> 2, k0 = tid & 3;
-    T x[kCholBlock / 4];
-#pragma unroll
-    for (int q = 0; q < kCholBlock / 4; ++q) {
-        const int k = k0 + 4 * q;
-        x[q] = (i < h && k < w) ? A[(size_t)(r0 + i) * lda + c0 + k] : 0.0;
+    for (int e = tid; e < h * kCholBlock; e += 256) {
+        const int i = e >> 6, k = e & 63;
+        if (k < w) A[(size_t)(r0 + i) * lda + c0 + k] = Bs[i * kCholLd + k];
     }
-    const int base = (threadIdx.x & 63) & ~3;  // lane of this row's first owner
-#pragma unroll
-    for (int j = 0; j < kCholBlock; ++j) {
-        if (j < w) {  // uniform
-            T xj = x[j >> 2] / D[j * kCholLd + j];  // valid on the owner lane k0 == j % 4
-            xj = __shfl(xj, base + (j & 3), 64);
-            if (k0 == (j & 3)) x[j >> 2] = xj;
-#pragma unroll
-            for (int q = 0; q < kCholBlock / 4; ++q) {
-                const int k = k0 + 4 * q;
-                if (k > j) x[q] -= xj * D[k * kCholLd + j];
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < kCholBlock / 4; ++q) {
-        const int k = k0 + 4 * q;
-        if (i < h && k < w) {
-            A[(size_t)(r0 + i) * lda + c0 + k] = x[q];
-            A[(size_t)(c0 + k) * lda + r0 + i] = 0.0;  // mirrored block of the upper triangle
-        }
+    // the mirrored block of the upper triangle: rows c0.., columns r0.. (coalesced along i)
+    for (int e = tid; e < w * kCholBlock; e += 256) {
+        const int k = e >> 6, i = e & 63;
+        if (i < h) A[(size_t)(c0 + k) * lda + r0 + i] = T(0);
     }
 }
 
-// A_ij -= L_ik L_jk^T for the trailing lower blocks kb < jb <= ib.
+// A_ij -= L_ik L_jk^T for the trailing lower blocks kb < jb <= ib on the matrix cores
+// (v_mfma_f64_16x16x4f64, or v_mfma_f32_16x16x4f32 for float32): each wave computes a 32 x 32
+// quarter of the 64 x 64 tile as 2 x 2 products of 16 x 16, K = the panel's 64 columns in
+// steps of 4. Operand lane l reads row l % 16, column l / 16 of a step: A[i][k] and
+// B[k][j] = L_j[j][k] alike; result register r holds column l % 16 of row 4 (l / 16) + r
+// (float32) or row l / 16 + 4 r (float64). The panel
+// blocks are staged row-major with a row stride of 68 words (banks of a 16-row x 4-column read
+// spread), a diagonal tile stages one block.
 template <typename T>
 __global__ __launch_bounds__(256) void chol_update_kernel(T* __restrict__ A, int n, int lda, int kb,
                                                          const int* __restrict__ info) {
-    __shared__ T Li[kCholBlock * (kCholBlock + 1)];  // [p][r]
-    __shared__ T Lj[kCholBlock * (kCholBlock + 1)];
+    constexpr int LD = kCholBlock + 4;
+    __shared__ T Li[kCholBlock * LD];
+    __shared__ T Lj[kCholBlock * LD];
     if (*info <= n) return;  // a pivot failed: nothing left to compute
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // linear index -> (ti, tj) with tj <= ti over the trailing blocks
     int t = blockIdx.x, ti = 0;
     while (t > ti) {
@@ -135,38 +236,75 @@ __global__ __launch_bounds__(256) void chol_update_kernel(T* __restrict__ A, int
     const int c0 = kb * kCholBlock, w = min(kCholBlock, n - c0);
     const int ri = (kb + 1 + ti) * kCholBlock, hi = min(kCholBlock, n - ri);
     const int rj = (kb + 1 + tj) * kCholBlock, hj = min(kCholBlock, n - rj);
-    constexpr int LD = kCholBlock + 1;
-    for (int e = tid; e < kCholBlock * kCholBlock; e += 256) {
-        const int r = e / kCholBlock, p = e % kCholBlock;
-        Li[p * LD + r] = (r < hi && p < w) ? A[(size_t)(ri + r) * lda + c0 + p] : 0.0;
-        Lj[p * LD + r] = (r < hj && p < w) ? A[(size_t)(rj + r) * lda + c0 + p] : 0.0;
+    const bool same = ti == tj;
+    // all loads in flight before the first LDS store
+    constexpr int kPer = kCholBlock * kCholBlock / 256;
+    T vi[kPer], vj[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + 256 * q, r = e >> 6, p = e & 63;
+        vi[q] = (r < hi && p < w) ? A[(size_t)(ri + r) * lda + c0 + p] : T(0);
+        vj[q] = (!same && r < hj && p < w) ? A[(size_t)(rj + r) * lda + c0 + p] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + 256 * q, r = e >> 6, p = e & 63;
+        Li[r * LD + p] = vi[q];
+        Lj[r * LD + p] = vj[q];
     }
     __syncthreads();
-    const int r4 = 4 * (tid >> 4), c4 = 4 * (tid & 15);
-    T acc[4][4];
+    const T* Lb = same ? Li : Lj;
+    using V4 = T __attribute__((ext_vector_type(4)));
+    const int row0 = 32 * (wave >> 1), col0 = 32 * (wave & 1);
+    const int i16 = lane & 15, k4 = lane >> 4;
+    V4 acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int p = 0; p < w; ++p) {
-        T x[4], y[4];
+        for (int b = 0; b < 2; ++b) acc[a][b] = V4{T(0), T(0), T(0), T(0)};
+    // columns p >= w are staged as zeros: all 16 steps run
+#pragma unroll 4
+    for (int s4 = 0; s4 < kCholBlock / 4; ++s4) {
+        const int k = 4 * s4 + k4;
+        T x[2], y[2];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            x[a] = Li[p * LD + r4 + a];
-            y[a] = Lj[p * LD + c4 + a];
-        }
+        for (int a = 0; a < 2; ++a) x[a] = Li[(row0 + 16 * a + i16) * LD + k];
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 2; ++b) y[b] = Lb[(col0 + 16 * b + i16) * LD + k];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                if constexpr (sizeof(T) == 8)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[a], y[b], acc[a][b], 0, 0, 0);
+                else
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[a], y[b], acc[a][b], 0, 0, 0);
+            }
     }
+    // the tile's current values are all loaded before the first store (a store to A would
+    // otherwise order every later load of A behind it)
+    T old[2][2][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int r = r4 + a, col = c4 + b;
-            if (r < hi && col < hj) A[(size_t)(ri + r) * lda + rj + col] -= acc[a][b];
-        }
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                // result rows: 4 (l / 16) + r for the float32 product, l / 16 + 4 r for float64
+                const int rr = sizeof(T) == 8 ? k4 + 4 * r : 4 * k4 + r;
+                const int row = row0 + 16 * a + rr, cc = col0 + 16 * b + i16;
+                old[a][b][r] = (row < hi && cc < hj) ? A[(size_t)(ri + row) * lda + rj + cc] : T(0);
+            }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = sizeof(T) == 8 ? k4 + 4 * r : 4 * k4 + r;
+                const int row = row0 + 16 * a + rr, cc = col0 + 16 * b + i16;
+                if (row < hi && cc < hj) A[(size_t)(ri + row) * lda + rj + cc] = old[a][b][r] - acc[a][b][r];
+            }
 }
 
 // Solve L L^T x = b in place for nrhs columns of B (column j at B + j * ldb), one
