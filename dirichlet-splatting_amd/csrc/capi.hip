@@ -793,12 +793,24 @@ int dsplat_cholesky_f32(float* a, int32_t n, int32_t lda, int32_t* info, void* s
     return launch_cholesky_f32(a, n, lda, info, (hipStream_t)stream);
 }
 
+size_t dsplat_cholesky_solve_workspace(int32_t n, int32_t nrhs) {
+    if (n < 0 || nrhs < 0 || n > kCholMaxSolve) return 0;
+    return cholesky_solve_workspace(n, nrhs);
+}
+
 int dsplat_cholesky_solve(const double* l, int32_t n, int32_t lda, double* b, int32_t nrhs, int32_t ldb,
-                          void* stream) {
+                          void* workspace, size_t workspace_bytes, void* stream) {
     if (n < 0 || nrhs < 0 || lda < n || ldb < n || n > kCholMaxSolve) return hipErrorInvalidValue;
     if (n == 0 || nrhs == 0) return hipSuccess;
-    if (!l || !b) return hipErrorInvalidValue;
-    return launch_cholesky_solve(l, n, lda, b, nrhs, ldb, (hipStream_t)stream);
+    if (!l || !b || !workspace || workspace_bytes < cholesky_solve_workspace(n, nrhs)) return hipErrorInvalidValue;
+    return launch_cholesky_solve(l, n, lda, b, nrhs, ldb, static_cast<double*>(workspace), (hipStream_t)stream);
+}
+
+int dsplat_power_iteration(const double* h, int32_t n, int32_t ld, int32_t steps, const double* start,
+                           double* out, double* work, void* stream) {
+    if (n < 1 || ld < n || steps < 1 || n > kPowerMaxN || !h || !start || !out || !work)
+        return hipErrorInvalidValue;
+    return launch_power_iteration(h, n, ld, steps, start, out, work, (hipStream_t)stream);
 }
 
 }  // extern "C"

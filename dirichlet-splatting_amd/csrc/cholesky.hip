@@ -19,8 +19,8 @@
 // Only the lower triangle of the input is read. info = 1-based index of the first pivot
 // that is not positive (0 if none), as LAPACK potrf reports it, in the factor's own precision
 // (the float32 form is the first attempt of varpro.py:62-79); the factor is then not
-// meaningful, and the launches of the later block columns return at once. The solve is one
-// workgroup: blocked forward and backward substitution.
+// meaningful, and the launches of the later block columns return at once. The solve runs one
+// launch per 64-row block in each direction.
 #include "dsplat_internal.h"
 
 namespace dsplat {
@@ -307,77 +307,152 @@ __global__ __launch_bounds__(256) void chol_update_kernel(T* __restrict__ A, int
             }
 }
 
-// Solve L L^T x = b in place for nrhs columns of B (column j at B + j * ldb), one
-// workgroup of 1024 threads; x stays in LDS. Per 64-row block: the product with the solved
-// part by all threads, then the 64 x 64 triangle by wave 0 alone (lane = row, the pivot
-// value broadcast by a shuffle), so each block costs two barriers.
-__global__ __launch_bounds__(1024) void chol_solve_kernel(const double* __restrict__ L, int n, int lda,
-                                                         double* __restrict__ B, int nrhs, int ldb) {
+// value of v on lane k (k uniform)
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)b, k);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(b >> 32), k);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Solve L L^T X = B in place for nrhs columns of B (column j at B + j * ldb): forward
+// L Y = B into the workspace W, then backward L^T X = Y into B, one launch per 64-row block in
+// each direction. Block b's launch: every workgroup stages the diagonal block and solves its
+// triangle for the block's rows (one wave: lane = row; at step m the value of unknown m reaches
+// every lane by a lane read, and lanes subtract their entry of column m times it; the staged
+// triangle holds zeros off its strict part, so no lane needs a mask); workgroup 0 stores the
+// solved rows in the other array and workgroup g >= 1 subtracts their product with its 64-row
+// slice of the off-diagonal part from the rows still to be solved. Rows a launch reads are
+// written by none of its workgroups. The slice is staged while the triangle is solved.
+constexpr int kTriMaxRhs = 8;
+
+template <bool BACKWARD>
+__global__ __launch_bounds__(256) void tri_block_kernel(const double* __restrict__ L, int n, int lda,
+                                                        double* __restrict__ R, int ldr, double* __restrict__ X,
+                                                        int ldx, int nrhs, int blk) {
+    __shared__ double T[kCholBlock * kCholLd];  // strict triangle, row = unknown's lane
+    __shared__ double S[kCholBlock * kCholLd];  // the workgroup's off-diagonal slice
+    __shared__ double rinv[kCholBlock];
+    __shared__ double Y[kTriMaxRhs][kCholBlock];
+    __shared__ double part[4][kCholBlock];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = blk * kCholBlock, h = min(kCholBlock, n - r0);
+    const int g = blockIdx.x;
+    // slice rows: below the block (forward) or above it (backward)
+    const int c0 = BACKWARD ? (g - 1) * kCholBlock : r0 + g * kCholBlock;
+    const int hs = g == 0 ? 0 : min(kCholBlock, BACKWARD ? r0 - c0 : n - c0);
+    constexpr int kPer = kCholBlock * kCholBlock / 256;
+    double vt[kPer], vs[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + 256 * q, i = e >> 6, m = e & 63;
+        // forward: T[i][m] = L[r0 + i][r0 + m] (m < i); backward: T[i][m] = L[r0 + m][r0 + i] (m > i)
+        const bool strict = BACKWARD ? (m > i && m < h) : (m < i && i < h);
+        vt[q] = strict ? (BACKWARD ? L[(size_t)(r0 + m) * lda + r0 + i] : L[(size_t)(r0 + i) * lda + r0 + m]) : 0.0;
+        // forward slice S[i][m] = L[c0 + i][r0 + m]; backward S[m][c] = L[r0 + m][c0 + c] (i = m, m = c)
+        vs[q] = 0.0;
+        if (BACKWARD) {
+            if (i < h && m < hs) vs[q] = L[(size_t)(r0 + i) * lda + c0 + m];
+        } else {
+            if (i < hs && m < h) vs[q] = L[(size_t)(c0 + i) * lda + r0 + m];
+        }
+    }
+    if (tid < kCholBlock) rinv[tid] = tid < h ? 1.0 / L[(size_t)(r0 + tid) * lda + r0 + tid] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int e = tid + 256 * q, i = e >> 6, m = e & 63;
+        T[i * kCholLd + m] = vt[q];
+        S[i * kCholLd + m] = vs[q];
+    }
+    __syncthreads();
+    if (wave == 0) {
+        const double ri = rinv[lane];
+        for (int j = 0; j < nrhs; ++j) {
+            double sv = lane < h ? R[(size_t)j * ldr + r0 + lane] : 0.0;
+            if (BACKWARD) {
+#pragma unroll
+                for (int m = kCholBlock - 1; m >= 0; --m) {
+                    const double um = lane_value(sv * ri, m);
+                    sv = fma(-T[lane * kCholLd + m], um, sv);
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < kCholBlock; ++m) {
+                    const double um = lane_value(sv * ri, m);
+                    sv = fma(-T[lane * kCholLd + m], um, sv);
+                }
+            }
+            Y[j][lane] = sv * ri;  // lane i's value is final once step i has passed
+        }
+    }
+    __syncthreads();
+    if (g == 0) {
+        for (int e = tid; e < nrhs * kCholBlock; e += 256) {
+            const int j = e >> 6, i = e & 63;
+            if (i < h) X[(size_t)j * ldx + r0 + i] = Y[j][i];
+        }
+        return;
+    }
+    // R[c] -= sum_m S(c, m) Y[m] over the slice's rows c; thread (c = lane, quarter = wave)
+    for (int j = 0; j < nrhs; ++j) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < kCholBlock / 4; ++k) {
+            const int m = wave + 4 * k;
+            const double sv = BACKWARD ? S[m * kCholLd + lane] : S[lane * kCholLd + m];
+            acc = fma(sv, Y[j][m], acc);
+        }
+        part[wave][lane] = acc;
+        __syncthreads();
+        if (tid < hs) {
+            const double sub = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+            R[(size_t)j * ldr + c0 + tid] -= sub;
+        }
+        __syncthreads();
+    }
+}
+
+// One power-iteration product for this workgroup's rows: yout = h x with x = yin /
+// max(|yin|, tiny) (x = yin on the first step, the start vector). Every workgroup forms |yin|
+// in the same fixed order, so all normalize identically; one wave per row pair, lanes over
+// the columns.
+constexpr int kPowerRows = 8;
+__global__ __launch_bounds__(256) void power_step_kernel(const double* __restrict__ h, int n, int ld,
+                                                         const double* __restrict__ yin, int normalize,
+                                                         double* __restrict__ yout) {
     extern __shared__ double xs[];  // [n]
-    __shared__ double part[1024];
-    __shared__ double T[kCholBlock * kCholLd];  // diagonal block of L
-    const int tid = threadIdx.x, row = tid >> 4, sub = tid & 15;
-    const int nblk = (n + kCholBlock - 1) / kCholBlock;
-    for (int col = 0; col < nrhs; ++col) {
-        double* x = B + (size_t)col * ldb;
-        for (int i = tid; i < n; i += 1024) xs[i] = x[i];
+    __shared__ double red[256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double top = 1.0;
+    if (normalize) {
+        double sq = 0.0;
+        for (int c = tid; c < n; c += 256) sq = fma(yin[c], yin[c], sq);
+        red[tid] = sq;
         __syncthreads();
-        // forward: L y = b
-        for (int bb = 0; bb < nblk; ++bb) {
-            const int r0 = bb * kCholBlock, h = min(kCholBlock, n - r0);
-            double s = 0.0;
-            if (row < h)
-                for (int p = sub; p < r0; p += 16) s = fma(L[(size_t)(r0 + row) * lda + p], xs[p], s);
-            part[tid] = s;
-            for (int e = tid; e < h * kCholBlock; e += 1024) {
-                const int a = e >> 6, b = e & 63;
-                if (b < h) T[a * kCholLd + b] = L[(size_t)(r0 + a) * lda + r0 + b];
-            }
-            __syncthreads();
-            if (tid < 64) {
-                double y = 0.0;
-                if (tid < h) {
-                    y = xs[r0 + tid];
-                    for (int q = 0; q < 16; ++q) y -= part[tid * 16 + q];
-                }
-                for (int j = 0; j < h; ++j) {
-                    const double yj = __shfl(y / T[j * kCholLd + j], j, 64);
-                    if (tid == j) y = yj;
-                    if (tid > j && tid < h) y -= T[tid * kCholLd + j] * yj;
-                }
-                if (tid < h) xs[r0 + tid] = y;
-            }
+        for (int w = 128; w >= 1; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
             __syncthreads();
         }
-        // backward: L^T x = y
-        for (int bb = nblk - 1; bb >= 0; --bb) {
-            const int r0 = bb * kCholBlock, h = min(kCholBlock, n - r0);
-            double s = 0.0;
-            if (row < h)
-                for (int p = r0 + h + sub; p < n; p += 16) s = fma(L[(size_t)p * lda + r0 + row], xs[p], s);
-            part[tid] = s;
-            for (int e = tid; e < h * kCholBlock; e += 1024) {
-                const int a = e >> 6, b = e & 63;
-                if (b < h) T[a * kCholLd + b] = L[(size_t)(r0 + a) * lda + r0 + b];
-            }
-            __syncthreads();
-            if (tid < 64) {
-                double y = 0.0;
-                if (tid < h) {
-                    y = xs[r0 + tid];
-                    for (int q = 0; q < 16; ++q) y -= part[tid * 16 + q];
-                }
-                for (int j = h - 1; j >= 0; --j) {
-                    const double yj = __shfl(y / T[j * kCholLd + j], j, 64);
-                    if (tid == j) y = yj;
-                    if (tid < j) y -= T[j * kCholLd + tid] * yj;
-                }
-                if (tid < h) xs[r0 + tid] = y;
-            }
-            __syncthreads();
+        top = fmax(sqrt(red[0]), 2.2250738585072014e-308);
+    }
+    for (int c = tid; c < n; c += 256) xs[c] = normalize ? yin[c] / top : yin[c];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = blockIdx.x * kPowerRows + 2 * wave + q;
+        if (r >= n) break;  // uniform over the wave
+        const double* row = h + (size_t)r * ld;
+        double acc0 = 0.0, acc1 = 0.0;
+        int c = lane;
+        for (; c + 64 < n; c += 128) {
+            acc0 = fma(row[c], xs[c], acc0);
+            acc1 = fma(row[c + 64], xs[c + 64], acc1);
         }
-        for (int i = tid; i < n; i += 1024) x[i] = xs[i];
-        __syncthreads();
+        if (c < n) acc0 = fma(row[c], xs[c], acc0);
+        double acc = acc0 + acc1;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (lane == 0) yout[r] = acc;
     }
 }
 
@@ -417,13 +492,44 @@ hipError_t launch_cholesky_f32(float* A, int n, int lda, int* info, hipStream_t 
     return launch_cholesky_t<float>(A, n, lda, info, st);
 }
 
-hipError_t launch_cholesky_solve(const double* L, int n, int lda, double* B, int nrhs, int ldb, hipStream_t st) {
+hipError_t launch_cholesky_solve(const double* L, int n, int lda, double* B, int nrhs, int ldb, double* W,
+                                 hipStream_t st) {
+    const int nblk = (n + kCholBlock - 1) / kCholBlock;
+    for (int j0 = 0; j0 < nrhs; j0 += kTriMaxRhs) {
+        const int m = min(kTriMaxRhs, nrhs - j0);
+        double* Bj = B + (size_t)j0 * ldb;
+        for (int b = 0; b < nblk; ++b) {
+            hipLaunchKernelGGL(tri_block_kernel<false>, dim3(nblk - b), dim3(256), 0, st, L, n, lda, Bj, ldb, W, n, m,
+                               b);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        for (int b = nblk - 1; b >= 0; --b) {
+            hipLaunchKernelGGL(tri_block_kernel<true>, dim3(1 + b), dim3(256), 0, st, L, n, lda, W, n, Bj, ldb, m, b);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+    }
+    return hipSuccess;
+}
+
+size_t cholesky_solve_workspace(int n, int nrhs) {
+    return sizeof(double) * (size_t)n * (size_t)min(nrhs, kTriMaxRhs);
+}
+
+hipError_t launch_power_iteration(const double* h, int n, int ld, int steps, const double* start, double* out,
+                                  double* work, hipStream_t st) {
     const size_t smem = sizeof(double) * (size_t)n;
-    hipError_t e =
-        allow_lds((const void*)chol_solve_kernel, smem + sizeof(double) * (1024 + kCholBlock * kCholLd));
+    hipError_t e = allow_lds((const void*)power_step_kernel, smem + sizeof(double) * 256);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(chol_solve_kernel, dim3(1), dim3(1024), smem, st, L, n, lda, B, nrhs, ldb);
-    return hipGetLastError();
+    const dim3 grid((n + kPowerRows - 1) / kPowerRows);
+    for (int i = 0; i < steps; ++i) {
+        const double* in = i == 0 ? start : work + (size_t)((i - 1) & 1) * n;
+        double* o = i == steps - 1 ? out : work + (size_t)(i & 1) * n;
+        hipLaunchKernelGGL(power_step_kernel, grid, dim3(256), smem, st, h, n, ld, in, (int)(i > 0), o);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace dsplat

@@ -206,10 +206,15 @@ hipError_t launch_vjp_moments(const Chirp& c, const float* pos, int A, const flo
                               const float2* w, int S, const float2* up, void* ws, float* gc, float* gn, float* gw,
                               hipStream_t st);
 
-constexpr int kCholMaxSolve = 8192;  // LDS holds the solution vector
+constexpr int kCholMaxSolve = 1 << 20;  // rows of the blocked solve
+constexpr int kPowerMaxN = 16384;      // LDS holds the power-iteration vector
 hipError_t launch_cholesky(double* A, int n, int lda, int* info, hipStream_t st);
 hipError_t launch_cholesky_f32(float* A, int n, int lda, int* info, hipStream_t st);
-hipError_t launch_cholesky_solve(const double* L, int n, int lda, double* B, int nrhs, int ldb, hipStream_t st);
+hipError_t launch_cholesky_solve(const double* L, int n, int lda, double* B, int nrhs, int ldb, double* W,
+                                 hipStream_t st);
+size_t cholesky_solve_workspace(int n, int nrhs);
+hipError_t launch_power_iteration(const double* h, int n, int ld, int steps, const double* start, double* out,
+                                  double* work, hipStream_t st);
 
 hipError_t launch_render(const Chirp& c, const float* pos, int A, const float* ctr, const float* nrm,
                          const float2* w, int S, float2* out, hipStream_t st);

@@ -72,7 +72,9 @@ SIGNATURES = {
     "dsplat_surfel_vjp_workspace": (c_size, [c_ptr, c_int, c_int]),
     "dsplat_cholesky": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
     "dsplat_cholesky_f32": (c_int, [c_ptr, c_int, c_int, c_ptr, c_ptr]),
-    "dsplat_cholesky_solve": (c_int, [c_ptr, c_int, c_int, c_ptr, c_int, c_int, c_ptr]),
+    "dsplat_cholesky_solve_workspace": (c_size, [c_int, c_int]),
+    "dsplat_cholesky_solve": (c_int, [c_ptr, c_int, c_int, c_ptr, c_int, c_int, c_ptr, c_size, c_ptr]),
+    "dsplat_power_iteration": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr]),
     "dsplat_surfel_vjp": (
         c_int,
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],

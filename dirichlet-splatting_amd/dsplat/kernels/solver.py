@@ -45,18 +45,37 @@ def cholesky_ex(h):
 
 def cholesky_solve(rhs, factor):
     """torch.cholesky_solve(rhs, factor) (lower factor) for real float64 [n, m] right-hand
-    sides in one workgroup; other cases go to torch. (The solver path keeps
-    torch.cholesky_solve, whose multi-workgroup products are faster at K ~ 2,000.)"""
+    sides on the device: blocked forward and backward substitution, one launch per 64-row
+    block and direction; other cases go to torch."""
     n = len(factor)
-    if not (_native(factor) and rhs.dtype == torch.float64 and rhs.ndim == 2 and n <= 8192):
+    if not (_native(factor) and rhs.dtype == torch.float64 and rhs.ndim == 2 and rhs.is_cuda):
         return torch.cholesky_solve(rhs, factor)
     # [m, n], a copy (the kernel works in place): column j of the solution is row j
     x = rhs.t().clone(memory_format=torch.contiguous_format)
+    size = _lib.library().dsplat_cholesky_solve_workspace(n, rhs.shape[1])
+    workspace = torch.empty(max(size // 8, 1), dtype=torch.float64, device=factor.device)
     _lib.call(
         "dsplat_cholesky_solve", _lib.ptr(factor.contiguous()), n, n, _lib.ptr(x), rhs.shape[1], n,
-        _lib.stream(factor.device),
+        _lib.ptr(workspace), size, _lib.stream(factor.device),
     )  # fmt: skip
     return x.t()
+
+
+def power_iteration(h, start, steps):
+    """The last product y = h x of `steps` power-iteration steps from `start` (x normalized by
+    max(|y|, tiny) between steps) for a real float64 [n, n] device matrix, or None when the
+    kernel does not apply (varpro.py:31-41)."""
+    n = len(h)
+    if not (_native(h) and 1 <= n <= 16384 and steps >= 1):
+        return None
+    h = h.contiguous()
+    out = torch.empty(n, dtype=torch.float64, device=h.device)
+    work = torch.empty(2 * n, dtype=torch.float64, device=h.device)
+    _lib.call(
+        "dsplat_power_iteration", _lib.ptr(h), n, n, int(steps), _lib.ptr(start.to(torch.float64).contiguous()),
+        _lib.ptr(out), _lib.ptr(work), _lib.stream(h.device),
+    )  # fmt: skip
+    return out
 
 
 def pcg_start(rhs, diagonal, tolerance):

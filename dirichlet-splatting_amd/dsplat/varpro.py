@@ -29,7 +29,7 @@ import torch
 
 from .kernels.blas import gemm_tn, syrk, syrk_split
 from .kernels.reduce import energy
-from .kernels.solver import cholesky_ex
+from .kernels.solver import cholesky_ex, cholesky_solve, power_iteration
 
 
 class Fit(NamedTuple):
@@ -58,6 +58,9 @@ def largest_eigenvalue(h):
     the estimate is reproducible."""
     index = torch.arange(len(h), device=h.device, dtype=torch.float64)
     vector = (torch.remainder(index * 0.6180339887, 1) + 0.5).to(h.dtype)
+    last = power_iteration(h, vector, POWER_STEPS)  # real float64 on the device: one launch per step
+    if last is not None:
+        return last.norm()
     top = vector.norm()
     for _ in range(POWER_STEPS):
         vector = h @ vector
@@ -92,7 +95,7 @@ def solve(h, rhs):
         factor = shifted_cholesky(h, len(h) * torch.finfo(h.real.dtype).eps)
         if factor is None:
             return torch.zeros_like(rhs)
-    x = torch.cholesky_solve(columns.to(factor.dtype), factor).to(columns.dtype)
+    x = cholesky_solve(columns.to(factor.dtype), factor).to(columns.dtype)
     return x[:, 0] if rhs.ndim == 1 else x
 
 

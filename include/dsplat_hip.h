@@ -459,10 +459,21 @@ int dsplat_cholesky(double* a, int32_t n, int32_t lda, int32_t* info, void* stre
 int dsplat_cholesky_f32(float* a, int32_t n, int32_t lda, int32_t* info, void* stream);
 
 /* Solve L L^T X = B in place for nrhs right-hand sides, column j of B at b + j * ldb, with
- * L from dsplat_cholesky; n <= 8192. Replaces torch.cholesky_solve (cuSOLVER potrs) of
- * varpro.py:62-79. */
+ * L from dsplat_cholesky: forward and backward substitution by 64-row blocks, one launch per
+ * block and direction; the workspace (dsplat_cholesky_solve_workspace bytes) holds the forward
+ * solution. Replaces torch.cholesky_solve (cuSOLVER potrs) of varpro.py:62-79. */
+size_t dsplat_cholesky_solve_workspace(int32_t n, int32_t nrhs);
 int dsplat_cholesky_solve(const double* l, int32_t n, int32_t lda, double* b, int32_t nrhs, int32_t ldb,
-                          void* stream);
+                          void* workspace, size_t workspace_bytes, void* stream);
+
+/* Power iteration of the largest-eigenvalue estimate: `steps` products y_i = h x_i of the
+ * float64 [n, n] matrix h (row-major, leading dimension ld), x_0 = start, x_i = y_(i-1) /
+ * max(|y_(i-1)|, DBL_MIN) (the tiny of float64); out [n] = y_(steps-1), whose norm is the
+ * estimate. work holds 2 n doubles; n <= 16384. One launch per step: every workgroup
+ * normalizes the previous product itself. Replaces the host loop of varpro.py:31-41
+ * (largest_eigenvalue) for real float64 systems. */
+int dsplat_power_iteration(const double* h, int32_t n, int32_t ld, int32_t steps, const double* start,
+                           double* out, double* work, void* stream);
 
 /* Backward elimination of `count` unknowns from a least-squares solution, one launch per
  * step on the stream (no host synchronization): inverse [n, n] float64 (of the regularized

@@ -46,7 +46,7 @@ def test_info_is_the_first_non_positive_pivot(n, bad):
     assert int(info) == bad + 1 == int(ref)
 
 
-@pytest.mark.parametrize("n,m", [(1, 1), (64, 1), (129, 3), (2304, 1), (2304, 2)])
+@pytest.mark.parametrize("n,m", [(1, 1), (64, 1), (129, 3), (700, 9), (2304, 1), (2304, 2)])
 def test_solve_matches_lapack(n, m):
     from dsplat.kernels.solver import cholesky_ex, cholesky_solve
 
@@ -104,3 +104,36 @@ def test_float32_info_is_the_first_non_positive_pivot(n, bad):
     _, info = cholesky_ex(torch.tensor(h, device="cuda"))
     torch.cuda.synchronize()
     assert int(info) == bad + 1
+
+
+def _power_on_host(h, steps):
+    """The reference's power iteration (varpro.py:31-41) as a float64 tensor loop on the CPU."""
+    vector = (torch.arange(len(h), dtype=torch.float64) * 0.6180339887) % 1 + 0.5
+    top = vector.norm()
+    for _ in range(steps):
+        vector = h @ vector
+        top = vector.norm()
+        vector = vector / top.clamp(min=torch.finfo(top.dtype).tiny)
+    return float(top)
+
+
+@pytest.mark.parametrize("n", [1, 100, 2291])
+def test_power_iteration_matches_the_host_loop(n):
+    """One launch per step on the device against the tensor loop on the CPU: the same estimate
+    to 1e-12 relative (only the summation order differs), and within the power method's own
+    convergence of numpy's largest eigenvalue."""
+    from dsplat.varpro import POWER_STEPS, largest_eigenvalue
+
+    h = spd(n, 31 + n)
+    got = float(largest_eigenvalue(torch.tensor(h, device="cuda")))
+    want = _power_on_host(torch.tensor(h), POWER_STEPS)
+    print(f"n={n} power estimate {got:.15e} host {want:.15e}")
+    assert abs(got - want) <= 1e-12 * want
+    assert got <= np.linalg.eigvalsh(h)[-1] * (1 + 1e-12)
+
+
+def test_power_iteration_of_a_zero_matrix_is_zero():
+    """A zero matrix gives zero products; the clamp keeps the normalization finite."""
+    from dsplat.varpro import largest_eigenvalue
+
+    assert float(largest_eigenvalue(torch.zeros(70, 70, dtype=torch.float64, device="cuda"))) == 0.0
