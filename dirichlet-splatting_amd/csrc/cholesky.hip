@@ -437,22 +437,40 @@ __global__ __launch_bounds__(256) void power_step_kernel(const double* __restric
     }
     for (int c = tid; c < n; c += 256) xs[c] = normalize ? yin[c] / top : yin[c];
     __syncthreads();
+    // the wave's two rows together, four columns per lane and row in flight per pass
+    const int r0 = blockIdx.x * kPowerRows + 2 * wave;
+    if (r0 >= n) return;  // uniform over the wave; no barrier follows
+    const bool two = r0 + 1 < n;
+    const double* row0 = h + (size_t)r0 * ld;
+    const double* row1 = h + (size_t)(two ? r0 + 1 : r0) * ld;
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
+    int c = lane;
+    for (; c + 192 < n; c += 256) {
+        const double x0 = xs[c], x1 = xs[c + 64], x2 = xs[c + 128], x3 = xs[c + 192];
+        const double p0 = row0[c], p1 = row0[c + 64], p2 = row0[c + 128], p3 = row0[c + 192];
+        const double q0 = row1[c], q1 = row1[c + 64], q2 = row1[c + 128], q3 = row1[c + 192];
+        a0 = fma(p0, x0, a0);
+        a1 = fma(p1, x1, a1);
+        a0 = fma(p2, x2, a0);
+        a1 = fma(p3, x3, a1);
+        b0 = fma(q0, x0, b0);
+        b1 = fma(q1, x1, b1);
+        b0 = fma(q2, x2, b0);
+        b1 = fma(q3, x3, b1);
+    }
+    for (; c < n; c += 64) {
+        a0 = fma(row0[c], xs[c], a0);
+        b0 = fma(row1[c], xs[c], b0);
+    }
+    double ya = a0 + a1, yb = b0 + b1;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int r = blockIdx.x * kPowerRows + 2 * wave + q;
-        if (r >= n) break;  // uniform over the wave
-        const double* row = h + (size_t)r * ld;
-        double acc0 = 0.0, acc1 = 0.0;
-        int c = lane;
-        for (; c + 64 < n; c += 128) {
-            acc0 = fma(row[c], xs[c], acc0);
-            acc1 = fma(row[c + 64], xs[c + 64], acc1);
-        }
-        if (c < n) acc0 = fma(row[c], xs[c], acc0);
-        double acc = acc0 + acc1;
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
-        if (lane == 0) yout[r] = acc;
+    for (int o = 32; o >= 1; o >>= 1) {
+        ya += __shfl_xor(ya, o, 64);
+        yb += __shfl_xor(yb, o, 64);
+    }
+    if (lane == 0) {
+        yout[r0] = ya;
+        if (two) yout[r0 + 1] = yb;
     }
 }
 
