@@ -83,9 +83,11 @@ __device__ __forceinline__ unsigned long long readlane64(unsigned long long v, i
 }
 
 // Uniform grid over the first (up to 3) coordinates for the taken peaks: cells of edge
-// 1.01 separation, so points closer than the separation lie in the same or adjacent cells
-// even after rounding; buckets hold chains of taken indices (hash collisions are weeded out
-// by the distance test itself).
+// 2.02 separation. A peak closer than the separation to point p differs from it by less than
+// 0.495 cell edges per coordinate, so it lies in p's cell or in the neighbour on the side of
+// p's nearer cell face: 2 cells per coordinate (8 in three dimensions) even after rounding;
+// buckets hold chains of taken indices (hash collisions are weeded out by the distance test
+// itself).
 constexpr int kPeakBuckets = 4096;
 
 __device__ __forceinline__ int peak_bucket(int ix, int iy, int iz) {
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lds_cap = kPeakLdsFloats / dim;
     const bool grid = dim <= 3;  // taken lookups through the cell grid, else a linear scan
-    const float inv_h = 1.0f / (1.01f * sqrtf(sep2));
+    const float inv_h = 1.0f / (2.02f * sqrtf(sep2));
     if (tid == 0) {
         n_taken = 0;
         finished = 0;
@@ -126,10 +128,15 @@ __global__ __launch_bounds__(kPeakBlock) void peaks_kernel(const float* __restri
         if (!(sep2 > 0.0f)) return false;
         if (grid) {
             const int ix = cell(c, 0), iy = cell(c, 1), iz = cell(c, 2);
-            for (int dz = dim > 2 ? -1 : 0; dz <= (dim > 2 ? 1 : 0); ++dz)
-                for (int dy = dim > 1 ? -1 : 0; dy <= (dim > 1 ? 1 : 0); ++dy)
-                    for (int dx = -1; dx <= 1; ++dx)
-                        for (int q = head[peak_bucket(ix + dx, iy + dy, iz + dz)]; q >= 0;
+            // the second cell per coordinate: toward the nearer face (none past the dimension)
+            auto side = [&](int j, int i0) {
+                return j < dim ? (c[j] * inv_h - (float)i0 < 0.5f ? -1 : 1) : 0;
+            };
+            const int sx = side(0, ix), sy = side(1, iy), sz = side(2, iz);
+            for (int bz = 0; bz <= (sz != 0); ++bz)
+                for (int by = 0; by <= (sy != 0); ++by)
+                    for (int bx = 0; bx <= (sx != 0); ++bx)
+                        for (int q = head[peak_bucket(ix + bx * sx, iy + by * sy, iz + bz * sz)]; q >= 0;
                              q = q < kPeakChainLds ? chain_lds[q] : chain[q]) {
                             const float* t = q < lds_cap ? taken_lds + q * dim : spill + (size_t)(q - lds_cap) * dim;
                             if (peak_d2(c, t, dim) < sep2) return true;
