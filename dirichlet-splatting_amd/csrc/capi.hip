@@ -573,6 +573,36 @@ int dsplat_qn_lockstep(const dsplat_fmcw* cfg, const float* positions, int32_t n
                               centers_out, normals_out, values_out, workspace, (hipStream_t)stream);
 }
 
+static bool joint_layout_ok(int count, const dsplat_joint_layout* L) {
+    return L && count >= 0 && L->tail >= 0 && (L->centers >= -1) && (L->normals >= -1) && (L->areas >= -1);
+}
+
+int dsplat_joint_unpack(int32_t count, const float* vector, const dsplat_joint_layout* layout,
+                        const float* centers, const float* normals, const float* coefficients, const float* root,
+                        const float* coefficient_scale, float* dcenters, float* dnormals, float* dweights,
+                        void* stream) {
+    if (!joint_layout_ok(count, layout)) return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    if (!vector || !centers || !normals || !coefficients || !root || !coefficient_scale || !dcenters || !dnormals ||
+        !dweights)
+        return hipErrorInvalidValue;
+    return launch_joint_unpack(count, vector, *layout, centers, normals, coefficients, root, coefficient_scale,
+                               dcenters, dnormals, dweights, (hipStream_t)stream);
+}
+
+int dsplat_joint_pack(int32_t count, const float* gcenters, const float* gnormals, const float* gweights,
+                      const dsplat_joint_layout* layout, const float* centers, const float* normals,
+                      const float* coefficients, const float* root, const float* coefficient_scale,
+                      const float* vector, const float* damping, float normalization, float* out, void* stream) {
+    if (!joint_layout_ok(count, layout)) return hipErrorInvalidValue;
+    if (count == 0) return hipSuccess;
+    if (!gcenters || !gnormals || !gweights || !centers || !normals || !coefficients || !root ||
+        !coefficient_scale || !vector || !damping || !out)
+        return hipErrorInvalidValue;
+    return launch_joint_pack(count, gcenters, gnormals, gweights, *layout, centers, normals, coefficients, root,
+                             coefficient_scale, vector, damping, normalization, out, (hipStream_t)stream);
+}
+
 size_t dsplat_elimination_workspace(int32_t n, int32_t count) { return elimination_workspace(n, count); }
 
 int dsplat_backward_elimination(const double* inverse, const double* b, int32_t n, int32_t count,

@@ -151,6 +151,16 @@ size_t syrk_split_workspace(int m, int n, int col0);
 hipError_t launch_syrk_split(int format, const void* hi, const void* lo, long long ld, int m, int n, int col0,
                              const float* col_exp, float* out, void* workspace, hipStream_t st);
 typedef dsplat_qn_spec QnSpec;
+typedef dsplat_joint_layout JointLayout;
+hipError_t launch_joint_unpack(int count, const float* vector, const JointLayout& layout, const float* centers,
+                               const float* normals, const float* coefficients, const float* root,
+                               const float* coefficient_scale, float* dcenters, float* dnormals, float* dweights,
+                               hipStream_t st);
+hipError_t launch_joint_pack(int count, const float* gcenters, const float* gnormals, const float* gweights,
+                             const JointLayout& layout, const float* centers, const float* normals,
+                             const float* coefficients, const float* root, const float* coefficient_scale,
+                             const float* vector, const float* damping, float normalization, float* out,
+                             hipStream_t st);
 size_t qn_workspace(int A, int P, int K);
 hipError_t launch_qn_search(const Chirp& c, const float* pos, int A, const float2* res, const float* ctr,
                             const float* nrm, const float* areas, int count, const QnSpec& sp, float* oc, float* on,

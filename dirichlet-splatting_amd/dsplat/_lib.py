@@ -40,6 +40,23 @@ class Fmcw(ctypes.Structure):
 
 
 # name -> (restype, argtypes); one line per entry point of include/dsplat_hip.h
+class JointLayout(ctypes.Structure):
+    """dsplat_joint_layout of dsplat_hip.h."""
+
+    _fields_ = [
+        ("centers", ctypes.c_int32),
+        ("normals", ctypes.c_int32),
+        ("areas", ctypes.c_int32),
+        ("tail", ctypes.c_int32),
+        ("complex_mode", ctypes.c_int32),
+        ("centers_unit", ctypes.c_int32),
+        ("normals_unit", ctypes.c_int32),
+        ("center_scale", ctypes.c_float),
+        ("normal_scale", ctypes.c_float),
+        ("area_scale", ctypes.c_float),
+    ]
+
+
 class QnSpec(ctypes.Structure):
     """dsplat_qn_spec of dsplat_hip.h."""
 
@@ -75,6 +92,10 @@ SIGNATURES = {
     "dsplat_cholesky_solve_workspace": (c_size, [c_int, c_int]),
     "dsplat_cholesky_solve": (c_int, [c_ptr, c_int, c_int, c_ptr, c_int, c_int, c_ptr, c_size, c_ptr]),
     "dsplat_power_iteration": (c_int, [c_ptr, c_int, c_int, c_int, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "dsplat_joint_unpack": (c_int, [c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr,
+                                    c_ptr]),
+    "dsplat_joint_pack": (c_int, [c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr,
+                                  c_ptr, ctypes.c_float, c_ptr, c_ptr]),
     "dsplat_surfel_vjp": (
         c_int,
         [c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_int, c_ptr, c_ptr, c_ptr, c_ptr, c_ptr, c_size, c_ptr],

@@ -99,3 +99,48 @@ def pcg_step(image, diagonal, x, residual, direction, state):
         "dsplat_pcg_step", _lib.ptr(image.float().contiguous()), _lib.ptr(diagonal), len(x), _lib.ptr(x),
         _lib.ptr(residual), _lib.ptr(direction), _lib.ptr(state), _lib.stream(x.device),
     )  # fmt: skip
+
+
+def joint_layout(tail, complex_mode, centers=None, normals=None, areas=None):
+    """dsplat_joint_layout of the joint system's unknowns: each trainable field as (first
+    unknown, scale, unit vectors), None when it is not trainable; the coefficients from tail."""
+    layout = _lib.JointLayout(centers=-1, normals=-1, areas=-1, tail=int(tail), complex_mode=int(complex_mode))
+    for key, entry in (("centers", centers), ("normals", normals), ("areas", areas)):
+        if entry is None:
+            continue
+        first, scale, unit = entry
+        setattr(layout, key, int(first))
+        setattr(layout, key[:-1] + "_scale", float(scale))
+        if key != "areas":
+            setattr(layout, key + "_unit", int(unit))
+    return layout
+
+
+def joint_unpack(vector, layout, centers, normals, coefficients, root, coefficient_scale):
+    """(dcenters, dnormals, dweights complex) of the dimensionless unknowns vector
+    (refinement.py:136-146 with the weight chain of surfel.py:53-61), one launch."""
+    count = len(centers)
+    dc = torch.empty_like(centers)
+    dn = torch.empty_like(normals)
+    dw = torch.empty((count, 2), dtype=torch.float32, device=centers.device)
+    _lib.call(
+        "dsplat_joint_unpack", count, _lib.ptr(vector), _lib.ctypes.byref(layout), _lib.ptr(centers),
+        _lib.ptr(normals), _lib.ptr(torch.view_as_real(coefficients)), _lib.ptr(root), _lib.ptr(coefficient_scale),
+        _lib.ptr(dc), _lib.ptr(dn), _lib.ptr(dw), _lib.stream(centers.device),
+    )  # fmt: skip
+    return dc, dn, torch.view_as_complex(dw)
+
+
+def joint_pack(gcenters, gnormals, gweights, layout, centers, normals, coefficients, root, coefficient_scale, vector,
+               damping, normalization):
+    """The operator image adjoint / normalization + damping * vector from the VJP's gradients
+    (refinement.py:148-158, 170-174), one launch."""
+    out = torch.empty_like(vector)
+    _lib.call(
+        "dsplat_joint_pack", len(centers), _lib.ptr(gcenters.contiguous()), _lib.ptr(gnormals.contiguous()),
+        _lib.ptr(torch.view_as_real(gweights.contiguous())), _lib.ctypes.byref(layout), _lib.ptr(centers),
+        _lib.ptr(normals), _lib.ptr(torch.view_as_real(coefficients)), _lib.ptr(root), _lib.ptr(coefficient_scale),
+        _lib.ptr(vector), _lib.ptr(damping), float(normalization), _lib.ptr(out), _lib.stream(centers.device),
+    )  # fmt: skip
+    return out
+

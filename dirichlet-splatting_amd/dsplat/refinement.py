@@ -130,6 +130,39 @@ class _JointSystem:
         self.gradient = self.adjoint(model.render(params, self.b) - target) / self.normalization + self.ridge_gradient
         self.diagonal = self._probe_diagonal(target, seed)
         self.damping = config.damping * self.diagonal
+        self._fused = self._fused_maps()
+
+    def _fused_maps(self):
+        """The unpack / adjoint maps of matvec as two kernels around the splat's JVP and VJP
+        (kernels.solver.joint_unpack / joint_pack), when every trainable field is a fully free
+        centres, normals or areas field of a splat model on float32 device tensors; None
+        otherwise (matvec then runs the tensor form)."""
+        from .kernels.solver import joint_layout
+
+        model, params = self.model, self.params
+        if not (self.b.is_cuda and self.b.dtype == torch.complex64 and hasattr(model, "sensor")
+                and hasattr(model, "fresnel_n") and {"centers", "normals", "areas"} <= params.keys()):  # fmt: skip
+            return None
+        for key in ("centers", "normals", "areas"):
+            value = params[key]
+            if value.dtype != torch.float32 or not value.is_contiguous():
+                return None
+            if value.shape != ((self.count,) if key == "areas" else (self.count, 3)):
+                return None
+        fields = {}
+        for key, columns, section in self.sections:
+            spec = self.specs[key]
+            if columns is not None or key not in ("centers", "normals", "areas") or (key == "areas" and spec.normalize):
+                return None
+            fields[key] = (section.start, spec.scale, spec.normalize)
+        root = params["areas"].sqrt()
+        return {
+            "layout": joint_layout(self.geometry, self.complex_mode, **fields),
+            "root": root,
+            "weights": self.b * root,
+            "damping": self.ridge_diagonal + self.damping,
+            "normalization": float(self.normalization),
+        }
 
     def _probe_diagonal(self, target, seed):
         """Jacobi diagonal of J^H J from cg_probes Rademacher probes (complex, unit modulus
@@ -178,8 +211,24 @@ class _JointSystem:
 
     def matvec(self, vector):
         """(J^H J / normalization + ridge + damping) vector: the model's JVP, then its VJP."""
+        if self._fused is not None:
+            return self._matvec_fused(vector)
         return (self.adjoint(self.model.jvp(self.params, self.b, *self.unpack(vector))) / self.normalization
                 + (self.ridge_diagonal + self.damping) * vector)  # fmt: skip
+
+    def _matvec_fused(self, vector):
+        """matvec as unpack kernel, JVP, VJP and pack kernel (four to six launches)."""
+        from .kernels import surfel as splat
+        from .kernels.solver import joint_pack, joint_unpack
+
+        f = self._fused
+        centers, normals = self.params["centers"], self.params["normals"]
+        config, positions = self.model.sensor
+        dc, dn, dw = joint_unpack(vector, f["layout"], centers, normals, self.b, f["root"], self.coefficient_scale)
+        image = splat.jvp(config, positions, centers, normals, f["weights"], dc, dn, dw, self.model.fresnel_n)
+        gc, gn, gw = splat.vjp(config, positions, centers, normals, f["weights"], image, self.model.fresnel_n)
+        return joint_pack(gc, gn, gw, f["layout"], centers, normals, self.b, f["root"], self.coefficient_scale, vector,
+                          f["damping"], f["normalization"])  # fmt: skip
 
     def proposal(self, vector, fraction):
         """The geometry moved by fraction of vector, the whole move first shortened so that no

@@ -475,6 +475,36 @@ int dsplat_cholesky_solve(const double* l, int32_t n, int32_t lda, double* b, in
 int dsplat_power_iteration(const double* h, int32_t n, int32_t ld, int32_t steps, const double* start,
                            double* out, double* work, void* stream);
 
+/* ---- Joint Gauss-Newton operator: the parameter maps around the splat's JVP and VJP. */
+
+/* Layout of the dimensionless unknowns of the joint system (refinement.py:113-121): the
+ * first unknown of each trainable field's section (count x 3 for centers and normals, count
+ * for areas; -1 when the field is not trainable; every component of a field is free), then
+ * the coefficients from `tail` (one real value per atom, or (Re, Im) pairs). */
+typedef struct dsplat_joint_layout {
+    int32_t centers, normals, areas, tail;
+    int32_t complex_mode;               /* 1: (Re, Im) pair per coefficient */
+    int32_t centers_unit, normals_unit; /* the field holds unit vectors (tangent projection) */
+    float center_scale, normal_scale, area_scale; /* physical length of one solver unit */
+} dsplat_joint_layout;
+
+/* Unknowns -> splat tangents (refinement.py:136-146 unpack, surfel.py:53-61 weight chain):
+ * dcenters, dnormals [count, 3] float32 (zero for an untrained field), dweights [count] complex
+ * (interleaved float32) = db sqrt(area) + b d(area) / (2 sqrt(area)). coefficients [count]
+ * complex interleaved, root = sqrt(area), coefficient_scale [count]. */
+int dsplat_joint_unpack(int32_t count, const float* vector, const dsplat_joint_layout* layout,
+                        const float* centers, const float* normals, const float* coefficients, const float* root,
+                        const float* coefficient_scale, float* dcenters, float* dnormals, float* dweights,
+                        void* stream);
+
+/* The VJP's gradients (gcenters, gnormals [count, 3], gweights [count] complex interleaved) ->
+ * the operator's image (refinement.py:148-158 adjoint, 170-174 matvec): out = adjoint /
+ * normalization + damping * vector over every unknown; damping = ridge diagonal + damping. */
+int dsplat_joint_pack(int32_t count, const float* gcenters, const float* gnormals, const float* gweights,
+                      const dsplat_joint_layout* layout, const float* centers, const float* normals,
+                      const float* coefficients, const float* root, const float* coefficient_scale,
+                      const float* vector, const float* damping, float normalization, float* out, void* stream);
+
 /* Backward elimination of `count` unknowns from a least-squares solution, one launch per
  * step on the stream (no host synchronization): inverse [n, n] float64 (of the regularized
  * real Gram), b [n] = inverse q. Each step removes the unknown with the smallest

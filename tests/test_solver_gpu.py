@@ -298,6 +298,36 @@ def test_pcg_graph_replays_equal_the_eager_iterations(fmcw, iterations):
         torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("coefficients,normals,areas", [("real", True, False), ("complex", True, True),
+                                                      ("real", False, True)])
+def test_fused_joint_maps_match_the_tensor_operator(fmcw, coefficients, normals, areas):
+    """The joint operator as unpack kernel, JVP, VJP and pack kernel against its tensor form
+    (refinement.py:136-174) on the same system: the same image to float32 rounding, for real
+    and complex coefficients, unit normals trained or frozen, areas trained or frozen."""
+    from dsplat import DSFWConfig, ParameterConfig
+    from dsplat.refinement import joint_system
+
+    model, params, b = fmcw
+    trial = dict(params, centers=params["centers"] + 2e-4)
+    specs = fmcw_specs(trainable_normals=normals)
+    if areas:
+        specs["areas"] = ParameterConfig(scale=1e-5)
+    config = DSFWConfig(coefficients=coefficients)
+    matvec, gradient, _, _ = joint_system(model, trial, b, model.scan.data, specs, config, 0)
+    system = matvec.__self__
+    assert system._fused is not None
+    g = torch.Generator(device="cuda").manual_seed(3)
+    for _ in range(3):
+        vector = torch.randn(len(gradient), device="cuda", generator=g)
+        fused = matvec(vector)
+        maps, system._fused = system._fused, None
+        tensor = matvec(vector)
+        system._fused = maps
+        err = float((fused - tensor).norm() / tensor.norm())
+        print(f"{coefficients} normals={normals} areas={areas}: fused vs tensor rel L2 {err:.2e}")
+        assert err < 1e-5
+
+
 def test_dense_lm_moves_tones_back(spectral):
     from dsplat import DSFWConfig, ParameterConfig, initialize
     from dsplat.refinement import dense_slide
