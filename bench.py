@@ -57,7 +57,7 @@ PEAK_HBM_GBS = 8000.0
 TRAFFIC = ROOT / "profiles" / "splat_traffic_128x128x2048.json"
 # VALU instructions per launch of the headline scene (rocprofv3 SQ_INSTS_VALU per dispatch)
 VALU_PROFILE = ROOT / "profiles" / "pmc_valu_splat_128x128x2048.md"
-VALU_PER_LAUNCH = {"td_render_h16_kernel<false, false>": 184_378_736, "vjp_moments_kernel<false>": 111_960_064}
+VALU_PER_LAUNCH = {"td_render_h16_kernel<false, false>": 183_625_072, "vjp_moments_kernel<false>": 112_459_776}
 # the published FP32 peak is one wave64 FP32 VALU instruction per 2 cycles on each SIMD
 VALU_PEAK_PER_S = PEAK_FP32_TFLOPS * 1e12 / (64 * 2)
 SIMDS, CLOCK_GHZ = 1024, 2.4  # MI355X: 256 CUs x 4 SIMDs, peak engine clock
